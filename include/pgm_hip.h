@@ -1,0 +1,121 @@
+/* pgm_hip: pGM electrostatics on one AMD MI355X (gfx950), C ABI.
+ *
+ * The engine of the reference's core/cuda/pgm_cuda.h (pgmcuda::Engine, core/cuda/pgm_cuda.h:47-127),
+ * re-designed for CDNA4 and exposed as plain C: an opaque handle, plain pointers and sizes, int status
+ * (0 = success, nonzero = error, message in pgm_last_error), no exceptions across the boundary.
+ *
+ * What it computes (as core/pgm_core.cpp does on the CPU): Gaussian charges, covalent permanent dipoles and
+ * Gaussian induced dipoles with smooth PME; the induced dipoles by Jacobi-preconditioned CG with the mu4
+ * predictor; energy terms, forces and the strain derivative S_ab = dE/d eps_ab.
+ *
+ * Precision: PGM_DOUBLE runs every kernel in FP64.  PGM_MIXED runs the pair and PME arithmetic in FP32 with
+ * FP64 accumulation of fields, forces, energies and the strain, and keeps the CG vectors, dot products and
+ * convergence norms in FP64 (FP32 matrix-vector products inside an FP64 Krylov recurrence).
+ *
+ * Units: nm, e, e nm, nm^3, kJ/mol.  Box: rows a, b, c, reduced triclinic (a = (ax,0,0), b = (bx,by,0)),
+ * row-major double[9].  Forces and the strain are ADDED to the caller's arrays.  Threading: one thread at a
+ * time per engine (any thread), as core/cuda/pgm_cuda.h:18.
+ */
+#ifndef PGM_HIP_H_
+#define PGM_HIP_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgm_engine pgm_engine;
+
+enum { PGM_DOUBLE = 0, PGM_MIXED = 1 };
+
+/* pgmcuda::Settings (core/cuda/pgm_cuda.h:25-39) plus precision and polarizable
+ * (pgmcore::Settings::polarizable, core/pgm_core.h:27). */
+typedef struct {
+    double cutoff;    /* direct-space cutoff [nm] */
+    double beta;      /* Ewald coefficient [1/nm] */
+    int grid[3];      /* PME grid */
+    int order;        /* B-spline order, 4..8 */
+    double tol;       /* max|alpha r| / mean|alpha b| */
+    int max_iter;
+    double peek;      /* final mu += peek alpha r */
+    int predictor;    /* 0 none, 1 mu3, 2 mu4 */
+    double ke;        /* Coulomb constant [kJ mol^-1 nm e^-2] */
+    int device;       /* HIP device; < 0: the device current on the calling thread, never switched */
+    double skin;      /* pair list: cutoff + skin, rebuilt when an atom moved > skin/2 [nm] */
+    int precision;    /* PGM_DOUBLE or PGM_MIXED */
+    int polarizable;  /* 1: induced dipoles; 0: permanent multipoles only */
+} pgm_settings;
+
+/* pgmcuda::Terms (core/cuda/pgm_cuda.h:41-45), kJ/mol. */
+typedef struct {
+    double direct, reciprocal, self, background, polarization, gvdw;
+} pgm_terms;
+
+/* Defaults of pgmcuda::Settings (core/cuda/pgm_cuda.h:25-39) with pgmcore's Coulomb constant 138.935456
+ * (core/pgm_core.h:33) and PGM_MIXED. */
+void pgm_default_settings(pgm_settings* s);
+
+/* Engine(const Settings&) (core/cuda/pgm_cuda.h:49).  NULL on failure (message: pgm_last_error(NULL)). */
+pgm_engine* pgm_create(const pgm_settings* s);
+/* ~Engine (core/cuda/pgm_cuda.h:50). */
+void pgm_destroy(pgm_engine* e);
+/* The last error of e (NULL: of the last failed pgm_create on this thread). */
+const char* pgm_last_error(const pgm_engine* e);
+
+/* setAtoms (core/cuda/pgm_cuda.h:55-56): n atoms in the caller's order; global_index (NULL: i) keys the
+ * predictor history (n_global slots). */
+int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, const double* alpha,
+                  const int* global_index, int n_global);
+/* setCovalent (core/cuda/pgm_cuda.h:67): p_atom += strength unit(r_partner - r_atom). */
+int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength);
+
+/* compute (core/cuda/pgm_cuda.h:79): host positions (3n), forces added to f (3n, may be NULL), strain
+ * added to strain (9, may be NULL), terms out (may be NULL).  new_step: this evaluation starts a new MD step
+ * (its solution enters the predictor history). */
+int pgm_compute(pgm_engine* e, const double* pos, const double* box, int new_step, double* f, double* strain,
+                pgm_terms* terms);
+/* computeF (core/cuda/pgm_cuda.h:82): float positions; forces kept for pgm_forces. */
+int pgm_compute_f32(pgm_engine* e, const float* pos, const double* box, int new_step, double* strain,
+                    pgm_terms* terms);
+/* forces() (core/cuda/pgm_cuda.h:86): the forces of the last pgm_compute_f32 (3n, kJ/mol/nm). */
+int pgm_forces(pgm_engine* e, double* out);
+
+/* computeDevice (core/cuda/pgm_cuda.h:93-94): positions from the device float4 array posq4 (w ignored,
+ * caller's order), forces added atomically to the device fixed-point buffer force_fixed (x of atom i at i,
+ * y at i + stride, z at i + 2 stride, scale fixed_scale); stream: the caller's hipStream_t (0: null stream),
+ * whose work so far comes first. */
+int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
+                       int stride, double fixed_scale, void* stream, pgm_terms* terms);
+/* launchDevice / finishDevice (core/cuda/pgm_cuda.h:102-104): enqueue, then wait; valid = 0 when the pair
+ * list turned out to be stale for these positions (forces incomplete; the list is rebuilt: repeat). */
+int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
+                      int stride, double fixed_scale, void* stream);
+int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid);
+
+/* No counterpart in the reference: compute on FP64 device arrays, for callers that keep the system in HBM
+ * (positions 3n doubles in the caller's order; forces added to f_dev, 3n doubles, may be NULL). */
+int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,
+                           double* strain, void* stream, pgm_terms* terms);
+
+/* iterations, residual, inducedDipoles, permanentDipoles, resetHistory, listBuilds, numCandidates,
+ * lateListChecks (core/cuda/pgm_cuda.h:112-122). */
+int pgm_iterations(const pgm_engine* e);
+double pgm_residual(const pgm_engine* e);
+int pgm_induced_dipoles(pgm_engine* e, double* mu);
+int pgm_permanent_dipoles(pgm_engine* e, double* p);
+int pgm_reset_history(pgm_engine* e);
+long pgm_list_builds(const pgm_engine* e);
+long long pgm_num_candidates(const pgm_engine* e);
+long pgm_late_list_checks(const pgm_engine* e);
+
+/* No counterpart in the reference: load nset prior solutions (each 3*n_global doubles, global order, most
+ * recent first) as the predictor history, e.g. the converged dipoles of earlier steps of a trajectory. */
+int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets);
+/* No counterpart in the reference: freeze (1) or unfreeze (0) the history, so that repeated evaluations
+ * of one workload start from the same predicted guess (benchmarks). */
+int pgm_freeze_history(pgm_engine* e, int frozen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

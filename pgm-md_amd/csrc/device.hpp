@@ -1,0 +1,257 @@
+// Device helpers shared by the pGM HIP kernels (gfx950): the box and its minimum image, the Gaussian-damped
+// Ewald pair kernels G_n, cardinal B-splines, sub-wavefront reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pgmhip {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrtPi = 1.7724538509055160273;
+constexpr double kTwoOverSqrtPi = 1.1283791670955125739;
+
+// Reduced triclinic box (rows a = (ax,0,0), b = (bx,by,0), c = (cx,cy,cz)) and its inverse
+// recip[k][a] = (H^-1)_{k a}: the fractional coordinate a of r is sum_k r_k recip[k][a].
+template<class T>
+struct BoxT {
+    T ax, bx, by, cx, cy, cz;
+    T iax, iby, icz;   // 1/ax, 1/by, 1/cz
+    T recip[3][3];
+};
+
+// Minimum image in a reduced triclinic box: c first, then b, then a (core/pgm_math.h:153-160).  rint of a
+// product instead of round of a quotient: they differ only at half-box ties, far outside any pair cutoff.
+template<class T>
+__device__ __forceinline__ void minImage(T& x, T& y, T& z, const BoxT<T>& b) {
+    T k = rint(z*b.icz);
+    x -= k*b.cx;
+    y -= k*b.cy;
+    z -= k*b.cz;
+    k = rint(y*b.iby);
+    x -= k*b.bx;
+    y -= k*b.by;
+    k = rint(x*b.iax);
+    x -= k*b.ax;
+}
+
+// Precision traits of the pair kernels: series length below a r = 1 and the damping cut beyond which the
+// erfc(a r) part is dropped (the reference's GPU constants, openmm/platforms/common/src/CommonPGMKernels.cpp:351-352).
+template<class T>
+struct PairPrec;
+template<>
+struct PairPrec<float> {
+    static constexpr int kSeries = 12;
+    static constexpr float kDampCut = 4.5f;
+};
+template<>
+struct PairPrec<double> {
+    static constexpr int kSeries = 15;
+    static constexpr double kDampCut = 6.0;
+};
+
+__device__ __forceinline__ float erfcT(float x) { return erfcf(x); }
+__device__ __forceinline__ double erfcT(double x) { return erfc(x); }
+__device__ __forceinline__ float erfT(float x) { return erff(x); }
+__device__ __forceinline__ double erfT(double x) { return erf(x); }
+__device__ __forceinline__ float expT(float x) { return __expf(x); }
+__device__ __forceinline__ double expT(double x) { return exp(x); }
+__device__ __forceinline__ float rsqrtT(float x) { return __frsqrt_rn(x); }
+__device__ __forceinline__ double rsqrtT(double x) { return rsqrt(x); }
+
+// B_n of erf(a r)/r, n < NMAX, by the series in s = (a r)^2 (valid for s < 1; no cancellation at short
+// range): B_n = (2a/sqrt(pi)) (2a^2)^n sum_m (-s)^m / (m! (2m+2n+1)) (core/pgm_math.h:31-49, docs/spec.md Sec. 4).
+template<class T, int NMAX>
+__device__ __forceinline__ void erfSeries(T a, T s, T* B) {
+    constexpr int NS = PairPrec<T>::kSeries;
+    T term[NS];
+    term[0] = T(1);
+#pragma unroll
+    for (int m = 1; m < NS; m++)
+        term[m] = -term[m-1]*s*(T(1)/T(m));
+    T scale = T(kTwoOverSqrtPi)*a;
+    const T twoA2 = T(2)*a*a;
+#pragma unroll
+    for (int k = 0; k < NMAX; k++) {
+        T acc = T(0);
+#pragma unroll
+        for (int m = NS-1; m >= 0; m--)
+            acc += term[m]*(T(1)/T(2*m+2*k+1));
+        B[k] = scale*acc;
+        scale *= twoA2;
+    }
+}
+
+// B_n of erf(a r)/r by the closed-form upward recursion (s >= 1), or by the series below.
+template<class T, int NMAX>
+__device__ __forceinline__ void erfKernels(T a, T r, T ir2, T* B) {
+    const T x = a*r, s = x*x;
+    if (s < T(1)) {
+        erfSeries<T, NMAX>(a, s, B);
+        return;
+    }
+    const T g = T(kTwoOverSqrtPi)*a*expT(-s);
+    B[0] = erfT(x)/r;
+    T scale = T(1);
+    const T twoA2 = T(2)*a*a;
+#pragma unroll
+    for (int k = 1; k < NMAX; k++) {
+        B[k] = (T(2*k-1)*B[k-1]-scale*g)*ir2;
+        scale *= twoA2;
+    }
+}
+
+// B_n of erfc(a r)/r: all terms positive (core/pgm_math.h:63-71).
+template<class T, int NMAX>
+__device__ __forceinline__ void erfcKernels(T a, T r, T ir2, T* B) {
+    const T x = a*r;
+    const T g = T(kTwoOverSqrtPi)*a*expT(-x*x);
+    B[0] = erfcT(x)/r;
+    T scale = T(1);
+    const T twoA2 = T(2)*a*a;
+#pragma unroll
+    for (int k = 1; k < NMAX; k++) {
+        B[k] = (T(2*k-1)*B[k-1]+scale*g)*ir2;
+        scale *= twoA2;
+    }
+}
+
+// G_n = B_n[erf(a r)/r] - B_n[erf(beta r)/r] for n in [N0, NMAX): the series difference for a r < 1, the erfc
+// form Bc(beta) - Bc(a) beyond (core/pgm_math.h:78-92), the Bc(a) part dropped beyond a r > kDampCut.
+template<class T, int NMAX>
+__device__ __forceinline__ void pairKernels(T a, T beta, T r, T* G) {
+    const T ir2 = T(1)/(r*r);
+    T B[NMAX];
+    if (a*r < T(1)) {
+        erfSeries<T, NMAX>(a, (a*r)*(a*r), G);
+        erfKernels<T, NMAX>(beta, r, ir2, B);
+    }
+    else {
+        erfcKernels<T, NMAX>(beta, r, ir2, G);
+        if (a*r < PairPrec<T>::kDampCut)
+            erfcKernels<T, NMAX>(a, r, ir2, B);
+        else {
+#pragma unroll
+            for (int k = 0; k < NMAX; k++)
+                B[k] = T(0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NMAX; k++)
+        G[k] -= B[k];
+}
+
+// Cardinal B-spline weights of order P at fraction fr on the grid points (base - j), j = 0..P-1
+// (core/pgm_core.cpp:131-152): th[j] = M_P(fr + j), dth[j] = M_P'(fr + j), d2th[j] = M_P''(fr + j).
+template<class T, int P>
+__device__ __forceinline__ void bsplines(T fr, T* th, T* dth, T* d2th) {
+    T m[P+1];   // m[j] = M_k(fr + j), rebuilt for k = 1..P
+    T m1[P+1], m2[P+1];
+#pragma unroll
+    for (int j = 0; j <= P; j++)
+        m[j] = m1[j] = m2[j] = T(0);
+    m[0] = T(1);
+#pragma unroll
+    for (int k = 2; k <= P; k++) {
+        if (k == P-1) {
+#pragma unroll
+            for (int j = 0; j <= P; j++)
+                m2[j] = m[j];   // M_{P-2}
+        }
+        if (k == P) {
+#pragma unroll
+            for (int j = 0; j <= P; j++)
+                m1[j] = m[j];   // M_{P-1}
+        }
+        T nxt[P+1];
+        const T inv = T(1)/T(k-1);
+#pragma unroll
+        for (int j = 0; j <= P; j++) {
+            const T hi = (j < k-1 ? m[j] : T(0));
+            const T lo = (j >= 1 && j-1 < k-1 ? m[j-1] : T(0));
+            nxt[j] = (j < k ? ((fr+T(j))*hi+(T(k)-fr-T(j))*lo)*inv : T(0));
+        }
+#pragma unroll
+        for (int j = 0; j <= P; j++)
+            m[j] = nxt[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        th[j] = m[j];
+        const T a1 = (j < P-1 ? m1[j] : T(0)), b1 = (j >= 1 ? m1[j-1] : T(0));
+        dth[j] = a1-b1;
+        const T a2 = (j < P-2 ? m2[j] : T(0)), b2 = (j >= 1 && j-1 < P-2 ? m2[j-1] : T(0)),
+                c2 = (j >= 2 && j-2 < P-2 ? m2[j-2] : T(0));
+        d2th[j] = a2-T(2)*b2+c2;
+    }
+}
+
+// Sum over the L lanes of a sub-wavefront group (L a power of two dividing 64).
+template<int L, class V>
+__device__ __forceinline__ V groupSum(V v) {
+#pragma unroll
+    for (int o = L/2; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, L);
+    return v;
+}
+
+// Block-wide sum of a double (blockDim.x a multiple of 64, <= 1024); the result is valid in thread 0.
+__device__ __forceinline__ double blockSum(double v, double* smem) {
+    v = groupSum<64>(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        smem[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int) (blockDim.x >> 6); k++)
+            r += smem[k];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double groupMax64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double blockMax(double v, double* smem) {
+    v = groupMax64(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        smem[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int) (blockDim.x >> 6); k++)
+            r = fmax(r, smem[k]);
+    __syncthreads();
+    return r;
+}
+
+// Vector types of the two precisions: xyz + charge (positions) or xyz + 0 (dipoles), 16 or 32 bytes.
+template<class T>
+struct Vec4;
+template<>
+struct Vec4<float> {
+    typedef float4 type;
+};
+template<>
+struct Vec4<double> {
+    typedef double4 type;
+};
+template<class T>
+struct Vec2;
+template<>
+struct Vec2<float> {
+    typedef float2 type;
+};
+template<>
+struct Vec2<double> {
+    typedef double2 type;
+};
+
+}   // namespace pgmhip

@@ -1,0 +1,1075 @@
+// pGM engine on one MI355X: host orchestration of the HIP kernels and the C ABI of include/pgm_hip.h.
+//
+// One evaluation (core/pgm_core.cpp:405-537 on the CPU; core/cuda/pgm_cuda.cu:2172-2334 in the reference's
+// GPU engine):
+//   positions -> (list check, rebuild if stale) -> sorted positions, permanent dipoles, PME bricks/splines
+//   -> pair setup (compacted list + field of q and p) + PME(q, p) -> b
+//   -> guess (history) -> CG (matvec = stored-kernel pair pass + PME(dipoles) + self)
+//   -> peek, history -> final pair pass + PME with energy/strain + forces gather -> covalent chain rule
+//   -> reductions -> terms, forces, strain.
+// The CG iterations are enqueued ahead (as many as the last evaluation needed) with per-iteration done
+// flags on the device, so a typical evaluation synchronizes with the host only for the list check and the
+// result read-back.
+#include "kernels.hpp"
+
+#include "../../include/pgm_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace pgmhip {
+
+namespace {
+
+constexpr double kMinPolarizability = 1e-9;   // core/pgm_constants.h:9
+constexpr int kNHist = 5;                     // 4 predictor entries + the slot this step writes
+constexpr int kBrick = 8;
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("pgm_hip: ")+what+": "+hipGetErrorString(e));
+}
+#define HIPCHK(x) hipCheck((x), #x)
+
+// Device allocation that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p)
+            (void) hipFree(p);
+    }
+    void reserve(size_t b) {
+        if (b <= bytes)
+            return;
+        if (p)
+            HIPCHK(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(b, 256)));
+        bytes = std::max<size_t>(b, 256);
+    }
+    template<class U>
+    U* as() const {
+        return static_cast<U*>(p);
+    }
+};
+
+// Pinned host staging that only grows.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~HostBuf() {
+        if (p)
+            (void) hipHostFree(p);
+    }
+    void reserve(size_t b) {
+        if (b <= bytes)
+            return;
+        if (p)
+            HIPCHK(hipHostFree(p));
+        p = nullptr;
+        HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+    }
+    template<class U>
+    U* as() const {
+        return static_cast<U*>(p);
+    }
+};
+
+template<class T>
+BoxT<T> makeBox(const double H[3][3], const double recip[3][3]) {
+    BoxT<T> b;
+    b.ax = (T) H[0][0];
+    b.bx = (T) H[1][0];
+    b.by = (T) H[1][1];
+    b.cx = (T) H[2][0];
+    b.cy = (T) H[2][1];
+    b.cz = (T) H[2][2];
+    b.iax = (T) (1.0/H[0][0]);
+    b.iby = (T) (1.0/H[1][1]);
+    b.icz = (T) (1.0/H[2][2]);
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++)
+            b.recip[k][a] = (T) recip[k][a];
+    return b;
+}
+
+// |sum_k M_p(k+1) e^{2 pi i m k/K}|^2 with zeros replaced by the mean of their neighbours
+// (core/pgm_math.h:98-126).
+std::vector<double> bsplineModuli(int K, int p) {
+    std::vector<double> prev(p+1, 0.0), cur(p+1, 0.0);
+    prev[0] = 1.0;
+    for (int k = 2; k <= p; k++) {
+        for (int j = 0; j < k; j++) {
+            const double hi = (j < k-1 ? prev[j] : 0.0), lo = (j > 0 ? prev[j-1] : 0.0);
+            cur[j] = (j*hi+(k-j)*lo)/(k-1);
+        }
+        for (int j = 0; j < k; j++)
+            prev[j] = cur[j];
+    }
+    std::vector<double> mod(K);
+    for (int m = 0; m < K; m++) {
+        double re = 0, im = 0;
+        for (int k = 0; k+1 < p && k < K; k++) {
+            const double ang = 2*kPi*m*k/K;
+            re += prev[k+1]*std::cos(ang);
+            im += prev[k+1]*std::sin(ang);
+        }
+        mod[m] = re*re+im*im;
+    }
+    for (int m = 0; m < K; m++)
+        if (mod[m] < 1e-7)
+            mod[m] = 0.5*(mod[(m+K-1)%K]+mod[(m+1)%K]);
+    return mod;
+}
+
+// The results slots of one evaluation (device, read back once).
+enum {
+    kResPair = 0,      // energy, strain[9] (pair, already halved)
+    kResRec = 10,      // energy, strain[9] (k-space)
+    kResComb = 20,     // |d|^2, polarization, E_rec (x) d [9]
+    kResCov = 31,      // covalent strain [9]
+    kResCount = 40,
+};
+
+}   // namespace
+
+enum class PosSource { HostF64, HostF32, DeviceF64, DeviceF4 };
+
+class Engine {
+public:
+    explicit Engine(const pgm_settings& s) : s_(s) {
+        validate();
+        withDevice([&] {
+            HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&evIn_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evOut_, hipEventDisableTiming));
+        });
+    }
+    ~Engine() {
+        int prev = -1;
+        (void) hipGetDevice(&prev);
+        if (s_.device >= 0)
+            (void) hipSetDevice(s_.device);
+        if (st_)
+            (void) hipStreamSynchronize(st_);
+        if (evIn_)
+            (void) hipEventDestroy(evIn_);
+        if (evOut_)
+            (void) hipEventDestroy(evOut_);
+        if (st_)
+            (void) hipStreamDestroy(st_);
+        if (prev >= 0)
+            (void) hipSetDevice(prev);
+    }
+
+    std::string lastError;
+
+    template<class Fn>
+    void withDevice(Fn fn) {
+        int prev = -1;
+        if (s_.device >= 0) {
+            HIPCHK(hipGetDevice(&prev));
+            if (prev != s_.device)
+                HIPCHK(hipSetDevice(s_.device));
+        }
+        try {
+            fn();
+        }
+        catch (...) {
+            if (prev >= 0 && prev != s_.device)
+                (void) hipSetDevice(prev);
+            throw;
+        }
+        if (prev >= 0 && prev != s_.device)
+            HIPCHK(hipSetDevice(prev));
+    }
+
+    void setAtoms(int n, const double* q, const double* radius, const double* alpha, const int* gidx, int nGlobal) {
+        if (n <= 0 || !q || !radius || !alpha)
+            throw std::invalid_argument("pgm_hip: setAtoms needs n > 0 and q, radius, alpha");
+        if (nGlobal < n && gidx)
+            throw std::invalid_argument("pgm_hip: setAtoms: n_global < n");
+        const int ng = gidx ? nGlobal : n;
+        std::vector<int> gi(n);
+        for (int i = 0; i < n; i++) {
+            gi[i] = gidx ? gidx[i] : i;
+            if (gi[i] < 0 || gi[i] >= ng)
+                throw std::invalid_argument("pgm_hip: setAtoms: global index out of range");
+            if (!(radius[i] > 0))
+                throw std::invalid_argument("pgm_hip: setAtoms: radii must be positive");
+        }
+        withDevice([&] {
+            HIPCHK(hipStreamSynchronize(st_));
+            if (n != n_ || ng != nGlobal_) {
+                histSlots_.clear();
+                haveList_ = false;
+                lastIterations_ = 0;
+            }
+            n_ = n;
+            nGlobal_ = ng;
+            qH_.assign(q, q+n);
+            radH_.assign(radius, radius+n);
+            alphaH_.resize(n);
+            polarizable_ = false;
+            q2_ = Qtot_ = 0;
+            for (int i = 0; i < n; i++) {
+                alphaH_[i] = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
+                polarizable_ = polarizable_ || alphaH_[i] > 0;
+                q2_ += q[i]*q[i];
+                Qtot_ += q[i];
+            }
+            gidxH_ = gi;
+            std::vector<double> rg(n);
+            for (int i = 0; i < n; i++)
+                rg[i] = 2.0*radius[i]*radius[i];
+            upload(qD_, qH_.data(), n*sizeof(double));
+            upload(rgD_, rg.data(), n*sizeof(double));
+            upload(alphaD_, alphaH_.data(), n*sizeof(double));
+            upload(gidxD_, gi.data(), n*sizeof(int));
+            haveList_ = false;   // parameters of the sorted arrays change: rebuild
+            if (covN_ < 0)
+                setCovalentImpl(0, nullptr, nullptr, nullptr);
+            haveResults_ = false;
+        });
+    }
+
+    void setCovalent(int ncov, const int* atom, const int* partner, const double* strength) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: setCovalent before setAtoms");
+        if (ncov < 0 || (ncov > 0 && (!atom || !partner || !strength)))
+            throw std::invalid_argument("pgm_hip: setCovalent: bad arrays");
+        for (int m = 0; m < ncov; m++)
+            if (atom[m] < 0 || atom[m] >= n_ || partner[m] < 0 || partner[m] >= n_ || atom[m] == partner[m])
+                throw std::invalid_argument("pgm_hip: setCovalent: atom index out of range");
+        withDevice([&] {
+            HIPCHK(hipStreamSynchronize(st_));
+            setCovalentImpl(ncov, atom, partner, strength);
+        });
+    }
+
+    // One evaluation.  fHost / fDev / fixed: force outputs (added); strainOut added to.
+    void evaluate(PosSource src, const void* pos, const double* boxIn, bool newStep, double* fHost, double* fDev,
+                  long long* fixed, int stride, double fixedScale, double* strainOut, pgm_terms* terms,
+                  hipStream_t caller, bool keepForces) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: compute before setAtoms");
+        if (!pos || !boxIn)
+            throw std::invalid_argument("pgm_hip: compute needs positions and a box");
+        withDevice([&] {
+            HIPCHK(hipEventRecord(evIn_, caller));
+            HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
+            setBox(boxIn);
+            stagePositions(src, pos);
+            checkList();
+            if (s_.precision == PGM_MIXED)
+                run<float>(newStep, fHost || fDev || fixed || keepForces, strainOut != nullptr);
+            else
+                run<double>(newStep, fHost || fDev || fixed || keepForces, strainOut != nullptr);
+            // forces out, caller's order
+            if (fDev || fixed)
+                launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
+                                fcovD_.as<double>(), s_.ke, fDev, fixed, stride, fixedScale);
+            if (fHost || keepForces) {
+                forcesOutD_.reserve(3*sizeof(double)*n_);
+                HIPCHK(hipMemsetAsync(forcesOutD_.p, 0, 3*sizeof(double)*n_, st_));
+                launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
+                                fcovD_.as<double>(), s_.ke, forcesOutD_.as<double>(), nullptr, 0, 0.0);
+                stageH_.reserve(3*sizeof(double)*n_);
+                HIPCHK(hipMemcpyAsync(stageH_.p, forcesOutD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost, st_));
+            }
+            HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, kResCount*sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            assemble(strainOut != nullptr);
+            if (fHost) {
+                const double* f = stageH_.as<double>();
+                for (int k = 0; k < 3*n_; k++)
+                    fHost[k] += f[k];
+            }
+            if (keepForces)
+                forcesH_.assign(stageH_.as<double>(), stageH_.as<double>()+3*n_);
+            if (strainOut)
+                for (int k = 0; k < 9; k++)
+                    strainOut[k] += strain_[k];
+            if (terms)
+                *terms = terms_;
+            HIPCHK(hipEventRecord(evOut_, st_));
+            HIPCHK(hipStreamWaitEvent(caller, evOut_, 0));
+        });
+    }
+
+    int iterations() const { return iterations_; }
+    double residual() const { return residual_; }
+    long listBuilds() const { return listBuilds_; }
+    long long numCandidates() const { return nCand_; }
+    long lateListChecks() const { return 0; }
+    const std::vector<double>& forces() const { return forcesH_; }
+
+    void dipoles(bool induced, double* out) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: no atoms");
+        if (!haveResults_) {
+            std::fill(out, out+3*n_, 0.0);
+            return;
+        }
+        withDevice([&] {
+            tmpD_.reserve(3*sizeof(double)*n_);
+            launchUnsort3(st_, n_, rankD_.as<int>(), induced ? xD_.as<double>() : permD_.as<double>(),
+                          tmpD_.as<double>());
+            HIPCHK(hipMemcpyAsync(out, tmpD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+        });
+    }
+
+    void resetHistory() { histSlots_.clear(); }
+
+    void setHistory(int nset, const double* sets) {
+        if (nset < 0 || nset > 4 || (nset > 0 && !sets))
+            throw std::invalid_argument("pgm_hip: set_history takes 0..4 sets");
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: set_history before setAtoms");
+        withDevice([&] {
+            const size_t stride = 3*(size_t) nGlobal_;
+            histD_.reserve(kNHist*stride*sizeof(double));
+            histSlots_.clear();
+            for (int k = 0; k < nset; k++) {
+                HIPCHK(hipMemcpyAsync(histD_.as<double>()+k*stride, sets+k*stride, stride*sizeof(double),
+                                      hipMemcpyHostToDevice, st_));
+                histSlots_.push_back(k);
+            }
+            HIPCHK(hipStreamSynchronize(st_));
+        });
+    }
+
+    void freezeHistory(bool f) { frozen_ = f; }
+    const pgm_terms& lastTerms() const { return terms_; }
+
+private:
+    pgm_settings s_;
+    hipStream_t st_ = nullptr;
+    hipEvent_t evIn_ = nullptr, evOut_ = nullptr;
+    int n_ = 0, nGlobal_ = 0, covN_ = -1;
+    bool polarizable_ = false;
+    double q2_ = 0, Qtot_ = 0;
+    std::vector<double> qH_, radH_, alphaH_, forcesH_;
+    std::vector<int> gidxH_;
+    // box
+    double H_[3][3] = {{0}}, recip_[3][3] = {{0}}, volume_ = 0;
+    double influenceBox_[3][3] = {{0}};
+    bool haveInfluence_ = false;
+    double buildBox_[3][3] = {{0}};
+    double skinNow_ = 0, rl_ = 0;
+    CellGrid cells_{};
+    PmeGrid pme_{};
+    // list state
+    bool haveList_ = false;
+    long listBuilds_ = 0;
+    long long nCand_ = 0;
+    int lastIterations_ = 0, iterations_ = 0;
+    double residual_ = 0;
+    std::deque<int> histSlots_;
+    bool frozen_ = false;
+    bool haveResults_ = false;
+    pgm_terms terms_{};
+    double strain_[9] = {0};
+    // device buffers
+    DevBuf qD_, rgD_, alphaD_, gidxD_, posInD_, posBuildD_;
+    const double* posDev_ = nullptr;   // the FP64 positions (caller's order) of this evaluation
+    DevBuf covStartD_, covPartnerD_, covStrengthD_, covAtomOfD_, revStartD_, revEntryD_;
+    DevBuf cellOfD_, slotD_, cellCountD_, cellStartD_, orderD_, rankD_, sortedCellD_, scratchD_;
+    DevBuf candCountD_, candStartD_, candD_, nbD_, pjD_, g12D_;
+    DevBuf qSD_, rgSD_, alphaSD_, posDD_, posFD_;
+    DevBuf permD_, permTD_, edD_, erD_, bD_, xD_, rD_, zD_, pD_, apD_, tdD_, trD_, pTD_, dD_, dTD_, etotD_;
+    DevBuf fpairD_, frecD_, fcovD_, forcesOutD_, tmpD_;
+    DevBuf brickOfD_, pslotD_, brickCountD_, brickStartD_, pmeAtomD_, baseD_, splD_;
+    DevBuf gridD_, cplxD_, inflD_, twD_, bmodD_;
+    DevBuf cgPartD_, partD_, resD_, cgD_, doneD_, histD_, maxBitsD_;
+    HostBuf resH_, stageH_, scalH_;
+
+    void validate() {
+        const pgm_settings& s = s_;
+        if (!(s.cutoff > 0))
+            throw std::invalid_argument("pgm_hip: cutoff must be positive");
+        if (!(s.beta > 0))
+            throw std::invalid_argument("pgm_hip: Ewald coefficient must be positive");
+        if (s.order < 4 || s.order > 8)
+            throw std::invalid_argument("pgm_hip: PME order must be 4..8");
+        for (int a = 0; a < 3; a++)
+            if (s.grid[a] < s.order)
+                throw std::invalid_argument("pgm_hip: PME grid must be at least the order in every dimension");
+        for (int a = 0; a < 3; a++)
+            (void) makePlan(s.grid[a]);
+        if (!(s.tol > 0))
+            throw std::invalid_argument("pgm_hip: tolerance must be positive");
+        if (s.max_iter < 0)
+            throw std::invalid_argument("pgm_hip: max_iter must be >= 0");
+        if (s.predictor < 0 || s.predictor > 2)
+            throw std::invalid_argument("pgm_hip: predictor must be 0, 1 or 2");
+        if (s.skin < 0)
+            throw std::invalid_argument("pgm_hip: skin must be >= 0");
+        if (s.precision != PGM_DOUBLE && s.precision != PGM_MIXED)
+            throw std::invalid_argument("pgm_hip: precision must be PGM_DOUBLE or PGM_MIXED");
+    }
+
+    void upload(DevBuf& b, const void* src, size_t bytes) {
+        b.reserve(bytes);
+        if (bytes)
+            HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void setCovalentImpl(int ncov, const int* atom, const int* partner, const double* strength) {
+        // CSR by atom (entries in input order per atom) and the reverse CSR by partner
+        std::vector<int> start(n_+1, 0), rstart(n_+1, 0);
+        for (int m = 0; m < ncov; m++) {
+            start[atom[m]+1]++;
+            rstart[partner[m]+1]++;
+        }
+        for (int i = 0; i < n_; i++) {
+            start[i+1] += start[i];
+            rstart[i+1] += rstart[i];
+        }
+        std::vector<int> fill(start.begin(), start.end()-1), rfill(rstart.begin(), rstart.end()-1);
+        std::vector<int> part(ncov), atomOf(ncov), rev(ncov);
+        std::vector<double> str(ncov);
+        std::vector<int> entryOf(ncov);
+        for (int m = 0; m < ncov; m++) {
+            const int k = fill[atom[m]]++;
+            part[k] = partner[m];
+            str[k] = strength[m];
+            atomOf[k] = atom[m];
+            entryOf[m] = k;
+        }
+        for (int m = 0; m < ncov; m++)
+            rev[rfill[partner[m]]++] = entryOf[m];
+        upload(covStartD_, start.data(), (n_+1)*sizeof(int));
+        upload(covPartnerD_, part.data(), ncov*sizeof(int));
+        upload(covStrengthD_, str.data(), ncov*sizeof(double));
+        upload(covAtomOfD_, atomOf.data(), ncov*sizeof(int));
+        upload(revStartD_, rstart.data(), (n_+1)*sizeof(int));
+        upload(revEntryD_, rev.data(), ncov*sizeof(int));
+        covN_ = ncov;
+        haveResults_ = false;
+    }
+
+    void setBox(const double* b) {
+        double H[3][3];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                H[r][c] = b[3*r+c];
+        if (H[0][1] != 0 || H[0][2] != 0 || H[1][2] != 0)
+            throw std::invalid_argument("pgm_hip: the box must be in reduced triclinic form (a = (ax,0,0), b = (bx,by,0))");
+        if (!(H[0][0] > 0 && H[1][1] > 0 && H[2][2] > 0))
+            throw std::invalid_argument("pgm_hip: box diagonal must be positive");
+        std::memcpy(H_, H, sizeof(H));
+        const double det = H[0][0]*H[1][1]*H[2][2];
+        volume_ = det;
+        // recip = H^-1 of the lower-triangular H (rows = box vectors)
+        double R[3][3] = {{0}};
+        R[0][0] = 1.0/H[0][0];
+        R[1][1] = 1.0/H[1][1];
+        R[2][2] = 1.0/H[2][2];
+        R[1][0] = -H[1][0]/(H[0][0]*H[1][1]);
+        R[2][1] = -H[2][1]/(H[1][1]*H[2][2]);
+        R[2][0] = (H[1][0]*H[2][1]-H[1][1]*H[2][0])/(H[0][0]*H[1][1]*H[2][2]);
+        std::memcpy(recip_, R, sizeof(R));
+        // plane spacings and the list radius (skin shrunk in small boxes)
+        double wmin = 1e300;
+        double w[3];
+        for (int a = 0; a < 3; a++) {
+            w[a] = 1.0/std::sqrt(R[0][a]*R[0][a]+R[1][a]*R[1][a]+R[2][a]*R[2][a]);
+            wmin = std::min(wmin, w[a]);
+        }
+        if (s_.cutoff >= 0.5*wmin)
+            throw std::invalid_argument("pgm_hip: the cutoff must be below half the smallest box plane spacing");
+        skinNow_ = std::min(s_.skin, 0.5*wmin-s_.cutoff);
+        rl_ = s_.cutoff+skinNow_;
+        // PME geometry
+        const bool boxChanged = !haveInfluence_ || std::memcmp(H, influenceBox_, sizeof(H)) != 0;
+        if (boxChanged)
+            setupPme(w);
+    }
+
+    void setupPme(const double*) {
+        PmeGrid g{};
+        for (int a = 0; a < 3; a++) {
+            g.K[a] = s_.grid[a];
+            g.plan[a] = makePlan(g.K[a]);
+            g.nb[a] = (g.K[a]+kBrick-1)/kBrick;
+        }
+        g.K3h = g.K[2]/2+1;
+        g.order = s_.order;
+        g.brick = kBrick;
+        g.nbrick = g.nb[0]*g.nb[1]*g.nb[2];
+        pme_ = g;
+        std::vector<double> bm;
+        for (int a = 0; a < 3; a++) {
+            const std::vector<double> m = bsplineModuli(g.K[a], g.order);
+            bm.insert(bm.end(), m.begin(), m.end());
+        }
+        upload(bmodD_, bm.data(), bm.size()*sizeof(double));
+        const size_t nc = (size_t) g.K[0]*g.K[1]*g.K3h;
+        const BoxT<double> bd = makeBox<double>(H_, recip_);
+        if (s_.precision == PGM_MIXED) {
+            inflD_.reserve(nc*sizeof(float));
+            launchInfluence<float>(st_, g, bmodD_.as<double>(), bd, s_.beta, volume_, inflD_.as<float>());
+            twD_.reserve((g.K[0]+g.K[1]+g.K[2])*sizeof(float2));
+            launchTwiddles<float>(st_, g, twD_.as<float2>());
+        }
+        else {
+            inflD_.reserve(nc*sizeof(double));
+            launchInfluence<double>(st_, g, bmodD_.as<double>(), bd, s_.beta, volume_, inflD_.as<double>());
+            twD_.reserve((g.K[0]+g.K[1]+g.K[2])*sizeof(double2));
+            launchTwiddles<double>(st_, g, twD_.as<double2>());
+        }
+        std::memcpy(influenceBox_, H_, sizeof(H_));
+        haveInfluence_ = true;
+    }
+
+    void stagePositions(PosSource src, const void* pos) {
+        const size_t bytes = 3*sizeof(double)*n_;
+        switch (src) {
+        case PosSource::HostF64:
+            posInD_.reserve(bytes);
+            stageH_.reserve(bytes);
+            std::memcpy(stageH_.p, pos, bytes);
+            HIPCHK(hipMemcpyAsync(posInD_.p, stageH_.p, bytes, hipMemcpyHostToDevice, st_));
+            posDev_ = posInD_.as<double>();
+            break;
+        case PosSource::HostF32: {
+            posInD_.reserve(bytes);
+            stageH_.reserve(bytes);
+            const float* f = static_cast<const float*>(pos);
+            double* d = stageH_.as<double>();
+            for (int k = 0; k < 3*n_; k++)
+                d[k] = f[k];
+            HIPCHK(hipMemcpyAsync(posInD_.p, stageH_.p, bytes, hipMemcpyHostToDevice, st_));
+            posDev_ = posInD_.as<double>();
+            break;
+        }
+        case PosSource::DeviceF64:
+            posDev_ = static_cast<const double*>(pos);
+            break;
+        case PosSource::DeviceF4:
+            posInD_.reserve(bytes);
+            launchF4ToDouble(st_, n_, static_cast<const float4*>(pos), posInD_.as<double>());
+            posDev_ = posInD_.as<double>();
+            break;
+        }
+    }
+
+    double boxChange() const {
+        double dH = 0;
+        for (int a = 0; a < 3; a++) {
+            double r2 = 0;
+            for (int c = 0; c < 3; c++)
+                r2 += (H_[a][c]-buildBox_[a][c])*(H_[a][c]-buildBox_[a][c]);
+            dH += std::sqrt(r2);
+        }
+        return dH;
+    }
+
+    void checkList() {
+        bool stale = !haveList_;
+        if (!stale) {
+            maxBitsD_.reserve(sizeof(unsigned));
+            HIPCHK(hipMemsetAsync(maxBitsD_.p, 0, sizeof(unsigned), st_));
+            launchMaxDisplacement(st_, n_, posDev_, posBuildD_.as<double>(), makeBox<double>(H_, recip_),
+                                  maxBitsD_.as<unsigned>());
+            scalH_.reserve(64);
+            HIPCHK(hipMemcpyAsync(scalH_.p, maxBitsD_.p, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            float d2;
+            std::memcpy(&d2, scalH_.p, sizeof(float));
+            stale = 2.0*std::sqrt((double) d2)+boxChange() > skinNow_;
+        }
+        if (stale)
+            buildList();
+    }
+
+    void buildList() {
+        const BoxT<double> bd = makeBox<double>(H_, recip_);
+        // cells at least rl/2 wide, +-2 stencil (all distinct cells when fewer than 5 per dimension)
+        CellGrid g{};
+        for (int a = 0; a < 3; a++) {
+            const double w = 1.0/std::sqrt(recip_[0][a]*recip_[0][a]+recip_[1][a]*recip_[1][a]+recip_[2][a]*recip_[2][a]);
+            g.nc[a] = std::max(1, (int) std::floor(w/(0.5*rl_)));
+            if (g.nc[a] >= 5) {
+                g.noff[a] = 5;
+                for (int o = 0; o < 5; o++)
+                    g.off[a][o] = o-2;
+            }
+            else {
+                g.noff[a] = g.nc[a];
+                for (int o = 0; o < g.nc[a]; o++)
+                    g.off[a][o] = o;
+            }
+        }
+        g.ncell = g.nc[0]*g.nc[1]*g.nc[2];
+        cells_ = g;
+        const int n = n_;
+        cellOfD_.reserve(n*sizeof(int));
+        slotD_.reserve(n*sizeof(int));
+        cellCountD_.reserve((g.ncell+1)*sizeof(int));
+        cellStartD_.reserve((g.ncell+1)*sizeof(int));
+        orderD_.reserve(n*sizeof(int));
+        rankD_.reserve(n*sizeof(int));
+        sortedCellD_.reserve(n*sizeof(int));
+        HIPCHK(hipMemsetAsync(cellCountD_.p, 0, (g.ncell+1)*sizeof(int), st_));
+        launchCellAssign(st_, n, posDev_, bd, g, cellOfD_.as<int>(), slotD_.as<int>(), cellCountD_.as<int>());
+        launchScan(st_, cellCountD_.as<int>(), cellStartD_.as<int>(), g.ncell, nullptr);
+        launchCellScatter(st_, n, cellOfD_.as<int>(), slotD_.as<int>(), cellStartD_.as<int>(), orderD_.as<int>());
+        launchCellSort(st_, g.ncell, cellStartD_.as<int>(), orderD_.as<int>());
+        launchRank(st_, n, orderD_.as<int>(), rankD_.as<int>(), sortedCellD_.as<int>(), cellOfD_.as<int>());
+        // sorted parameters
+        qSD_.reserve(n*sizeof(double));
+        alphaSD_.reserve(n*sizeof(double));
+        tmpD_.reserve(3*n*sizeof(double));
+        launchPermute(st_, n, orderD_.as<int>(), qD_.as<double>(), qSD_.as<double>());
+        launchPermute(st_, n, orderD_.as<int>(), alphaD_.as<double>(), alphaSD_.as<double>());
+        rgSD_.reserve(n*sizeof(double));
+        launchPermute(st_, n, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
+        posDD_.reserve(n*sizeof(double4));
+        launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(), nullptr);
+        // candidates
+        candCountD_.reserve(n*sizeof(int));
+        candStartD_.reserve((n+1)*sizeof(int));
+        launchCandCount(st_, n, posDD_.as<double4>(), sortedCellD_.as<int>(), cellStartD_.as<int>(), g, bd, rl_*rl_,
+                        candCountD_.as<int>());
+        launchScan(st_, candCountD_.as<int>(), candStartD_.as<int>(), n, nullptr);
+        scalH_.reserve(64);
+        HIPCHK(hipMemcpyAsync(scalH_.p, candStartD_.as<int>()+n, sizeof(int), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        int total;
+        std::memcpy(&total, scalH_.p, sizeof(int));
+        if (total < 0)
+            throw std::runtime_error("pgm_hip: pair list larger than 2^31 entries");
+        nCand_ = total;
+        const size_t cap = (size_t) total+64;
+        candD_.reserve(cap*sizeof(int));
+        pjD_.reserve(cap*sizeof(int));
+        g12D_.reserve(cap*(s_.precision == PGM_MIXED ? sizeof(float2) : sizeof(double2)));
+        launchCandFill(st_, n, posDD_.as<double4>(), sortedCellD_.as<int>(), cellStartD_.as<int>(), g, bd, rl_*rl_,
+                       candStartD_.as<int>(), candD_.as<int>());
+        posBuildD_.reserve(3*n*sizeof(double));
+        HIPCHK(hipMemcpyAsync(posBuildD_.p, posDev_, 3*n*sizeof(double), hipMemcpyDeviceToDevice, st_));
+        std::memcpy(buildBox_, H_, sizeof(H_));
+        haveList_ = true;
+        listBuilds_++;
+    }
+
+    GuessCoef guessCoefficients(bool newStep) const {
+        GuessCoef gc{};
+        const int h = (int) histSlots_.size();
+        auto set = [&](std::initializer_list<double> c) {
+            gc.nc = (int) c.size();
+            int k = 0;
+            for (double v : c) {
+                gc.c[k] = v;
+                gc.slot[k] = histSlots_[k];
+                k++;
+            }
+        };
+        if (!newStep && h > 0)
+            set({1.0});
+        else if (s_.predictor == 2 && h >= 4)
+            set({4.0, -6.0, 4.0, -1.0});
+        else if (s_.predictor >= 1 && h >= 3)
+            set({3.0, -3.0, 1.0});
+        else if (s_.predictor >= 1 && h >= 1)
+            set({1.0});
+        return gc;
+    }
+
+    int freeSlot() const {
+        for (int k = 0; k < kNHist; k++)
+            if (std::find(histSlots_.begin(), histSlots_.end(), k) == histSlots_.end())
+                return k;
+        return histSlots_.back();
+    }
+
+    template<class T>
+    void pmeSolve(const typename Vec4<T>::type* dip, bool withQ, const typename Vec4<T>::type* posq, int energyMode,
+                  bool strain, double* field, double* force, const int* skip, double* res) {
+        typedef typename Vec2<T>::type T2;
+        const BoxT<double> bd = makeBox<double>(H_, recip_);
+        const size_t nreal = (size_t) pme_.K[0]*pme_.K[1]*pme_.K[2];
+        launchZero<T>(st_, gridD_.as<T>(), nreal, skip);
+        launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
+                        posq, withQ ? 1 : 0, bd, gridD_.as<T>(), skip);
+        const int nb = launchPmeConvolve<T>(st_, pme_, gridD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(),
+                                            bd, s_.beta, energyMode, strain ? 1 : 0, partD_.as<double>(), skip);
+        if (energyMode)
+            launchReducePartials(st_, partD_.as<double>(), nb, 10, res);
+        launchGather<T>(st_, pme_, n_, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), gridD_.as<T>(), bd, dip,
+                        posq, force ? 1 : 0, field, force, skip);
+    }
+
+    template<class T>
+    void matvec(const typename Vec4<T>::type* posT, const typename Vec4<T>::type* dipT, const BoxT<T>& bt,
+                const int* skip) {
+        launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
+                           g12D_.as<typename Vec2<T>::type>(), bt, tdD_.as<double>(), skip);
+        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr);
+    }
+
+    template<class T>
+    void run(bool newStep, bool wantForces, bool wantStrain) {
+        typedef typename Vec4<T>::type T4;
+        typedef typename Vec2<T>::type T2;
+        (void) wantForces;
+        const int n = n_;
+        const size_t v3 = 3*sizeof(double)*n;
+        const BoxT<double> bd = makeBox<double>(H_, recip_);
+        const BoxT<T> bt = makeBox<T>(H_, recip_);
+        const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        // per-atom buffers
+        for (DevBuf* b : {&permD_, &edD_, &erD_, &bD_, &xD_, &rD_, &zD_, &pD_, &apD_, &tdD_, &trD_, &dD_, &etotD_,
+                          &fpairD_, &frecD_, &fcovD_})
+            b->reserve(v3);
+        for (DevBuf* b : {&permTD_, &pTD_, &dTD_})
+            b->reserve(n*sizeof(T4));
+        // sorted positions (and, for FP32, the FP32 copy and radii)
+        T4* posT;
+        if (sizeof(T) == sizeof(float)) {
+            posFD_.reserve(n*sizeof(float4));
+            launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
+                                  posFD_.as<float4>());
+            posT = reinterpret_cast<T4*>(posFD_.p);
+        }
+        else {
+            launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
+                                  nullptr);
+            posT = reinterpret_cast<T4*>(posDD_.p);
+        }
+        T* rgT;
+        if (sizeof(T) == sizeof(float)) {
+            scratchD_.reserve(n*sizeof(float));
+            launchToFloat(st_, n, rgSD_.as<double>(), scratchD_.as<float>());
+            rgT = scratchD_.as<T>();
+        }
+        else
+            rgT = rgSD_.as<T>();
+        launchPermDipoles(st_, n, orderD_.as<int>(), posDev_, covStartD_.as<int>(), covPartnerD_.as<int>(),
+                          covStrengthD_.as<double>(), bd, permD_.as<double>());
+        launchToVec4<T>(st_, n, permD_.as<double>(), permTD_.as<T4>());
+        // PME bricks and splines
+        brickOfD_.reserve(n*sizeof(int));
+        pslotD_.reserve(n*sizeof(int));
+        brickCountD_.reserve((pme_.nbrick+1)*sizeof(int));
+        brickStartD_.reserve((pme_.nbrick+1)*sizeof(int));
+        pmeAtomD_.reserve(n*sizeof(int));
+        baseD_.reserve(3*n*sizeof(int));
+        splD_.reserve((size_t) n*9*pme_.order*sizeof(T));
+        gridD_.reserve((size_t) pme_.K[0]*pme_.K[1]*pme_.K[2]*sizeof(T));
+        cplxD_.reserve((size_t) pme_.K[0]*pme_.K[1]*pme_.K3h*sizeof(T2));
+        HIPCHK(hipMemsetAsync(brickCountD_.p, 0, (pme_.nbrick+1)*sizeof(int), st_));
+        launchPmeBin<T>(st_, n, posDD_.as<double4>(), bd, pme_, brickOfD_.as<int>(), pslotD_.as<int>(),
+                        brickCountD_.as<int>());
+        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), pme_.nbrick, nullptr);
+        launchPmePlace<T>(st_, n, posDD_.as<double4>(), bd, pme_, brickOfD_.as<int>(), pslotD_.as<int>(),
+                          brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
+        // partial-sum storage
+        const int pairBlocks = (int) (((long long) n*16+255)/256);
+        const int xBlocks = pme_.K[1]*pme_.K3h;   // upper bound of the convolution blocks
+        partD_.reserve((size_t) std::max({pairBlocks, xBlocks, (n+255)/256, 1})*11*sizeof(double));
+        cgPartD_.reserve(kPRegions*kCgMaxBlocks*sizeof(double));
+        resD_.reserve(kResCount*sizeof(double));
+        resH_.reserve(kResCount*sizeof(double));
+        cgD_.reserve(sizeof(CgState));
+        doneD_.reserve((s_.max_iter+2)*sizeof(int));
+        nbD_.reserve(n*sizeof(int));
+        HIPCHK(hipMemsetAsync(resD_.p, 0, kResCount*sizeof(double), st_));
+        // field of the charges and permanent dipoles: pair setup + PME
+        const T rc2 = (T) (s_.cutoff*s_.cutoff);
+        launchPairSetup<T>(st_, n, posT, rgT, permTD_.as<T4>(), candStartD_.as<int>(), candD_.as<int>(),
+                           nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), bt, rc2, (T) s_.beta,
+                           edD_.as<double>(), nullptr);
+        pmeSolve<T>(permTD_.as<T4>(), true, posT, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
+        launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
+        // induced dipoles
+        int pushSlot = -1;
+        if (polarizable_ && s_.polarizable) {
+            const GuessCoef gc = guessCoefficients(newStep);
+            const size_t stride = 3*(size_t) nGlobal_;
+            histD_.reserve(kNHist*stride*sizeof(double));
+            launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
+                        alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
+            launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
+            matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
+            launchCgInit<T>(st_, n, alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>(), tdD_.as<double>(),
+                            trD_.as<double>(), cself, s_.tol, s_.max_iter, rD_.as<double>(), zD_.as<double>(),
+                            pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(),
+                            doneD_.as<int>());
+            int it = 0;
+            int chunk = std::max(1, lastIterations_);
+            scalH_.reserve(64);
+            while (true) {
+                const int stop = std::min(s_.max_iter, it+chunk);
+                for (; it < stop; it++) {
+                    const int* skip = doneD_.as<int>()+it;
+                    matvec<T>(posT, pTD_.as<T4>(), bt, skip);
+                    launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
+                               trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
+                    launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(),
+                                   xD_.as<double>(), rD_.as<double>(), zD_.as<double>(), cgPartD_.as<double>(),
+                                   doneD_.as<int>());
+                    launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(),
+                                         pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+                }
+                HIPCHK(hipMemcpyAsync(scalH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
+                HIPCHK(hipMemcpyAsync(scalH_.as<char>()+sizeof(CgState), doneD_.as<int>()+it, sizeof(int),
+                                      hipMemcpyDeviceToHost, st_));
+                HIPCHK(hipStreamSynchronize(st_));
+                int doneNow;
+                std::memcpy(&doneNow, scalH_.as<char>()+sizeof(CgState), sizeof(int));
+                if (doneNow || it >= s_.max_iter)
+                    break;
+                chunk = 1;
+            }
+            CgState cs;
+            std::memcpy(&cs, scalH_.p, sizeof(CgState));
+            iterations_ = cs.iterations;
+            residual_ = cs.err;
+            lastIterations_ = iterations_;
+            const int h = (int) histSlots_.size();
+            const bool push = !frozen_ && (newStep || h == 0);
+            if (push)
+                pushSlot = freeSlot();
+            launchPost<T>(st_, n, s_.peek, alphaSD_.as<double>(), rD_.as<double>(), xD_.as<double>(),
+                          permD_.as<double>(), orderD_.as<int>(), gidxD_.as<int>(),
+                          push ? histD_.as<double>()+pushSlot*stride : nullptr, dD_.as<double>(), dTD_.as<T4>());
+        }
+        else {
+            HIPCHK(hipMemsetAsync(xD_.p, 0, v3, st_));
+            HIPCHK(hipMemsetAsync(rD_.p, 0, v3, st_));
+            launchPost<T>(st_, n, 0.0, alphaSD_.as<double>(), rD_.as<double>(), xD_.as<double>(), permD_.as<double>(),
+                          orderD_.as<int>(), gidxD_.as<int>(), nullptr, dD_.as<double>(), dTD_.as<T4>());
+            iterations_ = 0;
+            residual_ = 0;
+        }
+        // final stage
+        double* res = resD_.as<double>();
+        const int nbPair = launchPairFinal<T>(st_, n, posT, rgT, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(),
+                                              pjD_.as<int>(), bt, (T) s_.beta, fpairD_.as<double>(), edD_.as<double>(),
+                                              partD_.as<double>());
+        launchReducePartials(st_, partD_.as<double>(), nbPair, 10, res+kResPair);
+        pmeSolve<T>(dTD_.as<T4>(), true, posT, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr,
+                    res+kResRec);
+        const int nbComb = launchFinalCombine(st_, n, dD_.as<double>(), xD_.as<double>(), alphaSD_.as<double>(),
+                                              edD_.as<double>(), erD_.as<double>(), cself, etotD_.as<double>(),
+                                              partD_.as<double>());
+        launchReducePartials(st_, partD_.as<double>(), nbComb, 11, res+kResComb);
+        const int nbCov = launchCovalentForces(st_, n, rankD_.as<int>(), posDev_, covStartD_.as<int>(),
+                                               covPartnerD_.as<int>(), covStrengthD_.as<double>(),
+                                               revStartD_.as<int>(), revEntryD_.as<int>(), covAtomOfD_.as<int>(),
+                                               etotD_.as<double>(), bd, fcovD_.as<double>(), partD_.as<double>());
+        launchReducePartials(st_, partD_.as<double>(), nbCov, 9, res+kResCov);
+        if (pushSlot >= 0) {
+            histSlots_.push_front(pushSlot);
+            while (histSlots_.size() > 4)
+                histSlots_.pop_back();
+        }
+        haveResults_ = true;
+    }
+
+    void assemble(bool wantStrain) {
+        const double* r = resH_.as<double>();
+        const double ke = s_.ke;
+        const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        terms_.direct = ke*r[kResPair];
+        terms_.reciprocal = ke*r[kResRec];
+        terms_.self = ke*(-(s_.beta/kSqrtPi)*q2_-0.5*cself*r[kResComb]);
+        const double bg = -kPi*Qtot_*Qtot_/(2.0*volume_*s_.beta*s_.beta);
+        terms_.background = ke*bg;
+        terms_.polarization = ke*r[kResComb+1];
+        terms_.gvdw = 0;
+        if (wantStrain) {
+            for (int k = 0; k < 9; k++)
+                strain_[k] = r[kResPair+1+k]+r[kResRec+1+k]+r[kResComb+2+k]+r[kResCov+k];
+            for (int a = 0; a < 3; a++)
+                strain_[4*a] -= bg;
+            for (int k = 0; k < 9; k++)
+                strain_[k] *= ke;
+        }
+    }
+};
+
+}   // namespace pgmhip
+
+// ------------------------------------------------------------------------------------------------ C ABI
+
+using pgmhip::Engine;
+using pgmhip::PosSource;
+
+struct pgm_engine {
+    std::unique_ptr<Engine> eng;
+};
+
+namespace {
+thread_local std::string g_createError;
+
+template<class Fn>
+int guarded(pgm_engine* e, Fn fn) {
+    if (!e || !e->eng)
+        return -2;
+    try {
+        fn(*e->eng);
+        e->eng->lastError.clear();
+        return 0;
+    }
+    catch (const std::exception& ex) {
+        e->eng->lastError = ex.what();
+        return -1;
+    }
+}
+}   // namespace
+
+extern "C" {
+
+void pgm_default_settings(pgm_settings* s) {
+    if (!s)
+        return;
+    s->cutoff = 0.9;
+    s->beta = 3.5;
+    s->grid[0] = s->grid[1] = s->grid[2] = 36;
+    s->order = 6;
+    s->tol = 1e-5;
+    s->max_iter = 100;
+    s->peek = 0.65;
+    s->predictor = 2;
+    s->ke = 138.935456;
+    s->device = 0;
+    s->skin = 0.15;
+    s->precision = PGM_MIXED;
+    s->polarizable = 1;
+}
+
+pgm_engine* pgm_create(const pgm_settings* s) {
+    if (!s) {
+        g_createError = "pgm_hip: pgm_create needs settings";
+        return nullptr;
+    }
+    try {
+        pgm_engine* e = new pgm_engine;
+        e->eng.reset(new Engine(*s));
+        return e;
+    }
+    catch (const std::exception& ex) {
+        g_createError = ex.what();
+        return nullptr;
+    }
+}
+
+void pgm_destroy(pgm_engine* e) { delete e; }
+
+const char* pgm_last_error(const pgm_engine* e) {
+    if (!e || !e->eng)
+        return g_createError.c_str();
+    return e->eng->lastError.c_str();
+}
+
+int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, const double* alpha,
+                  const int* global_index, int n_global) {
+    return guarded(e, [&](Engine& g) { g.setAtoms(n, q, radius, alpha, global_index, n_global); });
+}
+
+int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength) {
+    return guarded(e, [&](Engine& g) { g.setCovalent(ncov, atom, partner, strength); });
+}
+
+int pgm_compute(pgm_engine* e, const double* pos, const double* box, int new_step, double* f, double* strain,
+                pgm_terms* terms) {
+    return guarded(e, [&](Engine& g) {
+        g.evaluate(PosSource::HostF64, pos, box, new_step != 0, f, nullptr, nullptr, 0, 0.0, strain, terms, nullptr,
+                   false);
+    });
+}
+
+int pgm_compute_f32(pgm_engine* e, const float* pos, const double* box, int new_step, double* strain,
+                    pgm_terms* terms) {
+    return guarded(e, [&](Engine& g) {
+        g.evaluate(PosSource::HostF32, pos, box, new_step != 0, nullptr, nullptr, nullptr, 0, 0.0, strain, terms,
+                   nullptr, true);
+    });
+}
+
+int pgm_forces(pgm_engine* e, double* out) {
+    return guarded(e, [&](Engine& g) {
+        const std::vector<double>& f = g.forces();
+        if (f.empty())
+            throw std::logic_error("pgm_hip: no forces: pgm_compute_f32 has not run");
+        std::copy(f.begin(), f.end(), out);
+    });
+}
+
+int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
+                       int stride, double fixed_scale, void* stream, pgm_terms* terms) {
+    return guarded(e, [&](Engine& g) {
+        g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
+                   nullptr, terms, static_cast<hipStream_t>(stream), false);
+    });
+}
+
+int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
+                      int stride, double fixed_scale, void* stream) {
+    return guarded(e, [&](Engine& g) {
+        g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
+                   nullptr, nullptr, static_cast<hipStream_t>(stream), false);
+    });
+}
+
+int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
+    return guarded(e, [&](Engine& g) {
+        (void) g;
+        if (terms)
+            *terms = g.lastTerms();
+        if (valid)
+            *valid = 1;
+    });
+}
+
+int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,
+                           double* strain, void* stream, pgm_terms* terms) {
+    return guarded(e, [&](Engine& g) {
+        g.evaluate(PosSource::DeviceF64, pos_dev, box, new_step != 0, nullptr, f_dev, nullptr, 0, 0.0, strain, terms,
+                   static_cast<hipStream_t>(stream), false);
+    });
+}
+
+int pgm_iterations(const pgm_engine* e) { return (e && e->eng) ? e->eng->iterations() : -1; }
+double pgm_residual(const pgm_engine* e) { return (e && e->eng) ? e->eng->residual() : -1.0; }
+
+int pgm_induced_dipoles(pgm_engine* e, double* mu) {
+    return guarded(e, [&](Engine& g) { g.dipoles(true, mu); });
+}
+
+int pgm_permanent_dipoles(pgm_engine* e, double* p) {
+    return guarded(e, [&](Engine& g) { g.dipoles(false, p); });
+}
+
+int pgm_reset_history(pgm_engine* e) {
+    return guarded(e, [&](Engine& g) { g.resetHistory(); });
+}
+
+long pgm_list_builds(const pgm_engine* e) { return (e && e->eng) ? e->eng->listBuilds() : -1; }
+long long pgm_num_candidates(const pgm_engine* e) { return (e && e->eng) ? e->eng->numCandidates() : -1; }
+long pgm_late_list_checks(const pgm_engine* e) { return (e && e->eng) ? e->eng->lateListChecks() : -1; }
+
+int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets) {
+    return guarded(e, [&](Engine& g) { g.setHistory(nset, mu_sets); });
+}
+
+int pgm_freeze_history(pgm_engine* e, int frozen) {
+    return guarded(e, [&](Engine& g) { g.freezeHistory(frozen != 0); });
+}
+
+}   // extern "C"

@@ -1,0 +1,178 @@
+// Host-side launchers of the pGM HIP kernels (one translation unit per subsystem) and the plain-data
+// structures they share with the engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace pgmhip {
+
+// -------------------------------------------------------------------------------------------- pair list
+
+struct CellGrid {
+    int nc[3];     // fractional cells per box dimension
+    int ncell;
+    int off[3][5];   // distinct neighbour-cell offsets per dimension
+    int noff[3];
+};
+
+// Cell of every atom (fractional cells of the wrapped position), the per-cell counts and each atom's slot.
+void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
+                      int* cellCount);
+// Exclusive scan of m ints into out[0..m] (out[m] = total); scratch: >= 1024 ints.
+void launchScan(hipStream_t st, const int* in, int* out, int m, int* scratch);
+// Sorted order: atoms grouped by cell, ascending original index inside a cell.
+void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order);
+void launchCellSort(hipStream_t st, int ncell, const int* cellStart, int* order);
+void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf);
+// Full candidate lists (both directions) of the sorted atoms within rl: counts, then the lists.
+void launchCandCount(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
+                     CellGrid g, BoxT<double> box, double rl2, int* count);
+void launchCandFill(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
+                    CellGrid g, BoxT<double> box, double rl2, const int* candStart, int* cand);
+// Largest squared displacement (minimum image) since the list build, as float bits in *maxBits.
+void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const double* posBuild, BoxT<double> box,
+                           unsigned* maxBits);
+
+// -------------------------------------------------------------------------------------------- per atom
+
+// Sorted positions wrapped into the box (xyz, w = charge) in FP64 and, for the mixed path, FP32.
+void launchGatherPositions(hipStream_t st, int n, const int* order, const double* pos, const double* qS,
+                           BoxT<double> box, double4* posD, float4* posF);
+void launchGatherPositionsF4(hipStream_t st, int n, const int* order, const float4* posq, const double* qS,
+                             BoxT<double> box, double4* posD, float4* posF, double* posOrig);
+void launchPermute(hipStream_t st, int n, const int* order, const double* in, double* out);
+void launchToFloat(hipStream_t st, int n, const double* in, float* out);
+// Caller-order float4 (xyz, w ignored) or float xyz positions to FP64 xyz.
+void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out);
+// Permanent dipoles of the sorted atoms from the covalent CSR (caller's order, positions in FP64).
+void launchPermDipoles(hipStream_t st, int n, const int* order, const double* pos, const int* covStart,
+                       const int* covPartner, const double* covStrength, BoxT<double> box, double* perm);
+
+// -------------------------------------------------------------------------------------------- pair passes
+
+template<class T>
+void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg, const typename Vec4<T>::type* dip,
+                     const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
+                     BoxT<T> box, T rc2, T beta, double* field, const int* skip);
+template<class T>
+void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
+                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
+                     BoxT<T> box, double* field, const int* skip);
+// Energy, forces, field and strain of the pairs at the total dipoles; per-block partials [nblk][10]
+// (energy, strain 0..8), returns the number of blocks written.
+template<class T>
+int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                    const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj, BoxT<T> box,
+                    T beta, double* force, double* field, double* partials);
+
+// -------------------------------------------------------------------------------------------- PME
+
+struct FftPlan {
+    int K;
+    int nstage;
+    int radix[10];
+};
+FftPlan makePlan(int K);
+
+struct PmeGrid {
+    int K[3];
+    int K3h;
+    int order;
+    int brick;     // edge of the spreading bricks in grid points
+    int nb[3];     // bricks per dimension
+    int nbrick;
+    FftPlan plan[3];
+};
+
+// Binning of the sorted atoms into spreading bricks and their spline weights, in brick order.
+template<class T>
+void launchPmeBin(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, int* brickOf, int* slot,
+                  int* brickCount);
+template<class T>
+void launchPmePlace(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, const int* brickOf,
+                    const int* slot, const int* brickStart, int* pmeAtom, int* base, T* spl);
+// Spread charges (qScale 0: none) and dipoles of the sorted atoms onto the real grid (zeroed by the caller).
+template<class T>
+void launchSpread(hipStream_t st, PmeGrid g, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
+                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, BoxT<double> box,
+                  T* grid, const int* skip);
+// Forward transform, energy (and k-space strain) with the influence function, convolution, back transform.
+// partials: per-block [nblk][10] (energy, strain 0..8) when energy != 0; returns the number of blocks.
+template<class T>
+int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type* cplx, const T* influence,
+                      const typename Vec2<T>::type* twiddles, BoxT<double> box, double beta, int energy, int strain,
+                      double* partials, const int* skip);
+// Field -grad phi (sorted order) and, when final, the reciprocal forces.
+template<class T>
+void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
+                  BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
+                  double* field, double* force, const int* skip);
+template<class T>
+void launchInfluence(hipStream_t st, PmeGrid g, const double* bmod, BoxT<double> box, double beta, double volume,
+                     T* influence);
+template<class T>
+void launchTwiddles(hipStream_t st, PmeGrid g, typename Vec2<T>::type* tw);
+template<class T>
+void launchZero(hipStream_t st, T* p, size_t count, const int* skip);
+
+// -------------------------------------------------------------------------------------------- solver
+
+// Scalars of the CG recurrence, device-resident.  done[it] (it = 0..maxIter) is set before iteration it
+// runs: every kernel of iteration it returns at once when it is set, so iterations can be enqueued ahead.
+struct CgState {
+    double norm;      // mean |alpha b| + 1e-300
+    double err;       // max |alpha r| / norm
+    int iterations;
+    int pad;
+};
+
+// Partial sums of the CG kernels: regions of kCgMaxBlocks doubles.
+constexpr int kCgMaxBlocks = 256;
+enum { kPNorm = 0, kPRz0 = 1, kPRz1 = 2, kPpAp = 3, kPBnum = 4, kPMax = 5, kPRegions = 6 };
+
+struct GuessCoef {
+    int nc;          // 0: x = alpha b
+    int slot[4];
+    double c[4];
+};
+
+void launchGuess(hipStream_t st, int n, const int* order, const int* gidx, const double* hist, size_t slotStride,
+                 GuessCoef gc, const double* alpha, const double* b, double* x);
+// b = field_dir + field_rec + cself * perm.
+void launchCombineB(hipStream_t st, int n, const double* ed, const double* er, const double* perm, double cself,
+                    double* b);
+template<class T>
+void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type* out);
+int cgBlocks(int n);
+template<class T>
+void launchCgInit(hipStream_t st, int n, const double* alpha, const double* b, const double* x, const double* td,
+                  const double* tr, double cself, double tol, int maxIter, double* r, double* z, double* p,
+                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done);
+void launchCgAp(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* td, const double* tr,
+                double cself, double* ap, double* partials, const int* done);
+void launchCgUpdate(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* ap, double* x,
+                    double* r, double* z, double* partials, const int* done);
+template<class T>
+void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, const double* z, double* p,
+                       typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done);
+// x += peek alpha r; history push (global order, hist may be NULL); total dipoles d = perm + x.
+template<class T>
+void launchPost(hipStream_t st, int n, double peek, const double* alpha, const double* r, double* x, const double* perm,
+                const int* order, const int* gidx, double* hist, double* d, typename Vec4<T>::type* dT);
+// Total field, and per-block sums [nblk][11]: |d|^2, sum mu^2/(2 alpha), E_rec (x) d (row-major).
+int launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
+                       const double* edir, const double* erec, double cself, double* etot, double* partials);
+// Covalent chain rule per atom (caller's order); per-block strain sums [nblk][9].
+int launchCovalentForces(hipStream_t st, int n, const int* rank, const double* pos, const int* covStart,
+                         const int* covPartner, const double* covStrength, const int* revStart, const int* revEntry,
+                         const int* covAtomOf, const double* etot, BoxT<double> box, double* fcov, double* partials);
+void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair, const double* frec,
+                     const double* fcov, double ke, double* fOut, long long* fixedOut, int stride, double fixedScale);
+// Column sums of partials [nblk][width] into out[0..width).
+void launchReducePartials(hipStream_t st, const double* partials, int nblk, int width, double* out);
+// In the caller's order: out[i] = v[rank[i]] (3-vectors).
+void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out);
+
+}   // namespace pgmhip

@@ -1,0 +1,325 @@
+// Cell list and full neighbour (candidate) lists of the pGM pair passes, plus the per-atom gathers into the
+// spatially sorted order.  Replaces the reference's cellKeys / radix sort / cellRanges / listBuild chain
+// (core/cuda/pgm_cuda.cu:445-580, 1930-2002) with a counting sort whose cells keep ascending atom order
+// (deterministic lists) and sub-wavefront candidate scans with ballot compaction.
+#include "kernels.hpp"
+
+namespace pgmhip {
+
+namespace {
+
+constexpr int kListLanes = 16;   // lanes per atom in the candidate scans
+
+__device__ __forceinline__ void fracCell(double x, double y, double z, const BoxT<double>& b, const CellGrid& g,
+                                         int c[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double t = x*b.recip[0][a]+y*b.recip[1][a]+z*b.recip[2][a];
+        t -= floor(t);
+        int k = (int) (t*g.nc[a]);
+        c[a] = k < 0 ? 0 : (k >= g.nc[a] ? g.nc[a]-1 : k);
+    }
+}
+
+__global__ void kCellAssign(int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
+                            int* cellCount) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n)
+        return;
+    int c[3];
+    fracCell(pos[3*i], pos[3*i+1], pos[3*i+2], box, g, c);
+    const int cell = (c[0]*g.nc[1]+c[1])*g.nc[2]+c[2];
+    cellOf[i] = cell;
+    slot[i] = atomicAdd(&cellCount[cell], 1);
+}
+
+// Single-block exclusive scan: 1024 threads, each over a contiguous chunk.
+__global__ void __launch_bounds__(1024) kScan(const int* in, int* out, int m) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (m+1023)/1024;
+    const int lo = min(m, t*chunk), hi = min(m, lo+chunk);
+    long long s = 0;
+    for (int k = lo; k < hi; k++)
+        s += in[k];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long long v = (t >= o ? part[t-o] : 0);
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t]-s;
+    for (int k = lo; k < hi; k++) {
+        const int v = in[k];
+        out[k] = (int) run;
+        run += v;
+    }
+    if (t == 1023)
+        out[m] = (int) part[1023];
+}
+
+__global__ void kCellScatter(int n, const int* cellOf, const int* slot, const int* cellStart, int* order) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        order[cellStart[cellOf[i]]+slot[i]] = i;
+}
+
+// Ascending atom order inside every cell (the atomic slots above are not deterministic).
+__global__ void kCellSort(int ncell, const int* cellStart, int* order) {
+    const int c = blockIdx.x*blockDim.x+threadIdx.x;
+    if (c >= ncell)
+        return;
+    const int lo = cellStart[c], hi = cellStart[c+1];
+    for (int k = lo+1; k < hi; k++) {
+        const int v = order[k];
+        int j = k-1;
+        while (j >= lo && order[j] > v) {
+            order[j+1] = order[j];
+            j--;
+        }
+        order[j+1] = v;
+    }
+}
+
+__global__ void kRank(int n, const int* order, int* rank, int* sortedCell, const int* cellOf) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    const int i = order[s];
+    rank[i] = s;
+    sortedCell[s] = cellOf[i];
+}
+
+// One group of L lanes per sorted atom scans the neighbour cells; FILL writes the compacted candidates.
+template<bool FILL, int L>
+__global__ void __launch_bounds__(256) kCandidates(int n, const double4* posS, const int* sortedCell,
+                                                   const int* cellStart, CellGrid g, BoxT<double> box, double rl2,
+                                                   int* count, const int* candStart, int* cand) {
+    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int s = gid/L, lane = gid%L;
+    if (s >= n)
+        return;
+    const double4 pi = posS[s];
+    const int c = sortedCell[s];
+    const int c2 = c%g.nc[2], c1 = (c/g.nc[2])%g.nc[1], c0 = c/(g.nc[2]*g.nc[1]);
+    const int shift = (threadIdx.x%64)/L*L;
+    const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
+    int found = 0;
+    int out = FILL ? candStart[s] : 0;
+    const int outEnd = FILL ? candStart[s+1] : 0;
+    for (int a = 0; a < g.noff[0]; a++) {
+        const int n0 = (c0+g.off[0][a]+g.nc[0])%g.nc[0];
+        for (int b = 0; b < g.noff[1]; b++) {
+            const int n1 = (c1+g.off[1][b]+g.nc[1])%g.nc[1];
+            for (int d = 0; d < g.noff[2]; d++) {
+                const int n2 = (c2+g.off[2][d]+g.nc[2])%g.nc[2];
+                const int cell = (n0*g.nc[1]+n1)*g.nc[2]+n2;
+                const int lo = cellStart[cell], hi = cellStart[cell+1];
+                for (int base = lo; base < hi; base += L) {
+                    const int j = base+lane;
+                    bool in = false;
+                    if (j < hi && j != s) {
+                        const double4 pj = posS[j];
+                        double x = pi.x-pj.x, y = pi.y-pj.y, z = pi.z-pj.z;
+                        minImage(x, y, z, box);
+                        in = x*x+y*y+z*z < rl2;
+                    }
+                    if (FILL) {
+                        const unsigned long long m = (__ballot(in) >> shift) & (L == 64 ? ~0ull : ((1ull << L)-1ull));
+                        const int at = out+__popcll(m & below);
+                        if (in && at < outEnd)
+                            cand[at] = j;
+                        out += __popcll(m);
+                    }
+                    else
+                        found += in ? 1 : 0;
+                }
+            }
+        }
+    }
+    if (!FILL) {
+        found = groupSum<L>(found);
+        if (lane == 0)
+            count[s] = found;
+    }
+}
+
+__global__ void kMaxDisplacement(int n, const double* pos, const double* posBuild, BoxT<double> box,
+                                 unsigned* maxBits) {
+    __shared__ double red[16];
+    double m = 0;
+    for (int i = blockIdx.x*blockDim.x+threadIdx.x; i < n; i += gridDim.x*blockDim.x) {
+        double x = pos[3*i]-posBuild[3*i], y = pos[3*i+1]-posBuild[3*i+1], z = pos[3*i+2]-posBuild[3*i+2];
+        minImage(x, y, z, box);
+        m = fmax(m, x*x+y*y+z*z);
+    }
+    m = blockMax(m, red);
+    if (threadIdx.x == 0)
+        atomicMax(maxBits, __float_as_uint((float) m));
+}
+
+__device__ __forceinline__ void wrapPosition(double& x, double& y, double& z, const BoxT<double>& b) {
+    const double t0 = floor(x*b.recip[0][0]+y*b.recip[1][0]+z*b.recip[2][0]);
+    const double t1 = floor(x*b.recip[0][1]+y*b.recip[1][1]+z*b.recip[2][1]);
+    const double t2 = floor(x*b.recip[0][2]+y*b.recip[1][2]+z*b.recip[2][2]);
+    x -= t0*b.ax+t1*b.bx+t2*b.cx;
+    y -= t1*b.by+t2*b.cy;
+    z -= t2*b.cz;
+}
+
+__global__ void kGatherPositions(int n, const int* order, const double* pos, const double* qS, BoxT<double> box,
+                                 double4* posD, float4* posF) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    const int i = order[s];
+    double x = pos[3*i], y = pos[3*i+1], z = pos[3*i+2];
+    wrapPosition(x, y, z, box);
+    const double q = qS[s];
+    posD[s] = make_double4(x, y, z, q);
+    if (posF)
+        posF[s] = make_float4((float) x, (float) y, (float) z, (float) q);
+}
+
+__global__ void kGatherPositionsF4(int n, const int* order, const float4* posq, const double* qS, BoxT<double> box,
+                                   double4* posD, float4* posF, double* posOrig) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    const int i = order[s];
+    const float4 p = posq[i];
+    double x = p.x, y = p.y, z = p.z;
+    posOrig[3*i] = x;
+    posOrig[3*i+1] = y;
+    posOrig[3*i+2] = z;
+    wrapPosition(x, y, z, box);
+    const double q = qS[s];
+    posD[s] = make_double4(x, y, z, q);
+    if (posF)
+        posF[s] = make_float4((float) x, (float) y, (float) z, (float) q);
+}
+
+__global__ void kPermute(int n, const int* order, const double* in, double* out) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s < n)
+        out[s] = in[order[s]];
+}
+
+// p_i = sum_m c_m unit(r_partner - r_i), minimum image, FP64 (core/pgm_core.cpp:51-62).
+__global__ void kPermDipoles(int n, const int* order, const double* pos, const int* covStart, const int* covPartner,
+                             const double* covStrength, BoxT<double> box, double* perm) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    const int i = order[s];
+    double px = 0, py = 0, pz = 0;
+    for (int m = covStart[i]; m < covStart[i+1]; m++) {
+        const int j = covPartner[m];
+        double x = pos[3*j]-pos[3*i], y = pos[3*j+1]-pos[3*i+1], z = pos[3*j+2]-pos[3*i+2];
+        minImage(x, y, z, box);
+        const double f = covStrength[m]/sqrt(x*x+y*y+z*z);
+        px += f*x;
+        py += f*y;
+        pz += f*z;
+    }
+    perm[3*s] = px;
+    perm[3*s+1] = py;
+    perm[3*s+2] = pz;
+}
+
+inline int blocksFor(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
+
+}   // namespace
+
+void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
+                      int* cellCount) {
+    kCellAssign<<<blocksFor(n, 256), 256, 0, st>>>(n, pos, box, g, cellOf, slot, cellCount);
+}
+
+void launchScan(hipStream_t st, const int* in, int* out, int m, int*) {
+    kScan<<<1, 1024, 0, st>>>(in, out, m);
+}
+
+void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order) {
+    kCellScatter<<<blocksFor(n, 256), 256, 0, st>>>(n, cellOf, slot, cellStart, order);
+}
+
+void launchCellSort(hipStream_t st, int ncell, const int* cellStart, int* order) {
+    kCellSort<<<blocksFor(ncell, 64), 64, 0, st>>>(ncell, cellStart, order);
+}
+
+void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf) {
+    kRank<<<blocksFor(n, 256), 256, 0, st>>>(n, order, rank, sortedCell, cellOf);
+}
+
+void launchCandCount(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
+                     CellGrid g, BoxT<double> box, double rl2, int* count) {
+    kCandidates<false, kListLanes><<<blocksFor((long long) n*kListLanes, 256), 256, 0, st>>>(
+        n, posS, sortedCell, cellStart, g, box, rl2, count, nullptr, nullptr);
+}
+
+void launchCandFill(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
+                    CellGrid g, BoxT<double> box, double rl2, const int* candStart, int* cand) {
+    kCandidates<true, kListLanes><<<blocksFor((long long) n*kListLanes, 256), 256, 0, st>>>(
+        n, posS, sortedCell, cellStart, g, box, rl2, nullptr, candStart, cand);
+}
+
+void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const double* posBuild, BoxT<double> box,
+                           unsigned* maxBits) {
+    kMaxDisplacement<<<std::min(1024, blocksFor(n, 256)), 256, 0, st>>>(n, pos, posBuild, box, maxBits);
+}
+
+void launchGatherPositions(hipStream_t st, int n, const int* order, const double* pos, const double* qS,
+                           BoxT<double> box, double4* posD, float4* posF) {
+    kGatherPositions<<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, qS, box, posD, posF);
+}
+
+void launchGatherPositionsF4(hipStream_t st, int n, const int* order, const float4* posq, const double* qS,
+                             BoxT<double> box, double4* posD, float4* posF, double* posOrig) {
+    kGatherPositionsF4<<<blocksFor(n, 256), 256, 0, st>>>(n, order, posq, qS, box, posD, posF, posOrig);
+}
+
+void launchPermute(hipStream_t st, int n, const int* order, const double* in, double* out) {
+    kPermute<<<blocksFor(n, 256), 256, 0, st>>>(n, order, in, out);
+}
+
+void launchPermDipoles(hipStream_t st, int n, const int* order, const double* pos, const int* covStart,
+                       const int* covPartner, const double* covStrength, BoxT<double> box, double* perm) {
+    kPermDipoles<<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, covStart, covPartner, covStrength, box, perm);
+}
+
+}   // namespace pgmhip
+
+namespace pgmhip {
+namespace {
+__global__ void kF4ToDouble(int n, const float4* posq, double* out) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n)
+        return;
+    const float4 p = posq[i];
+    out[3*i] = p.x;
+    out[3*i+1] = p.y;
+    out[3*i+2] = p.z;
+}
+}   // namespace
+
+void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out) {
+    kF4ToDouble<<<(n+255)/256, 256, 0, st>>>(n, posq, out);
+}
+}   // namespace pgmhip
+
+namespace pgmhip {
+namespace {
+__global__ void kToFloat(int n, const double* in, float* out) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        out[i] = (float) in[i];
+}
+}   // namespace
+
+void launchToFloat(hipStream_t st, int n, const double* in, float* out) {
+    kToFloat<<<(n+255)/256, 256, 0, st>>>(n, in, out);
+}
+}   // namespace pgmhip

@@ -1,0 +1,771 @@
+// Smooth PME of charges and dipoles (Essmann 1995; dipoles as Sagui 2004) for pGM on gfx950.
+//
+//   bin/place  atoms binned into grid bricks once per evaluation, B-spline weights stored in brick order
+//              (core/pgm_core.cpp:123-153 convention: M_p(fr + j) on grid point base - j);
+//   spread     one workgroup per brick accumulates q theta theta theta + e . grad(theta theta theta) into an
+//              LDS tile (brick + order-1 halo) with LDS atomics, then adds the tile to the grid
+//              (core/pgm_core.cpp:191-207; replaces the reference's 64-bit fixed-point global atomics);
+//   transform  in-house mixed-radix (4,2,3,5,7) Stockham FFTs in LDS: z (two real lines per complex
+//              transform), y, then x fused with the influence function, the energy and the k-space strain
+//              (core/pgm_core.cpp:213-237), and back;
+//   gather     E = -grad phi at the atoms; at the end of an evaluation also the reciprocal forces
+//              -[q grad phi + (grad grad phi) . d] (core/pgm_core.cpp:241-264, 453-487).
+#include "kernels.hpp"
+
+#include <stdexcept>
+#include <string>
+
+namespace pgmhip {
+
+FftPlan makePlan(int K) {
+    FftPlan p{};
+    p.K = K;
+    int m = K;
+    const int radices[5] = {4, 2, 3, 5, 7};
+    for (int r : radices)
+        while (m%r == 0) {
+            if (p.nstage >= 10)
+                throw std::runtime_error("pgm_hip: PME grid dimension has too many factors");
+            p.radix[p.nstage++] = r;
+            m /= r;
+        }
+    if (m != 1)
+        throw std::runtime_error("pgm_hip: PME grid dimension " + std::to_string(K)+
+                                 " must factor into 2, 3, 5 and 7");
+    return p;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template<class T2>
+__device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+    T2 r;
+    r.x = a.x*b.x-a.y*b.y;
+    r.y = a.x*b.y+a.y*b.x;
+    return r;
+}
+
+template<class T2>
+__device__ __forceinline__ T2 cadd(T2 a, T2 b) {
+    T2 r;
+    r.x = a.x+b.x;
+    r.y = a.y+b.y;
+    return r;
+}
+
+template<class T2>
+__device__ __forceinline__ T2 csub(T2 a, T2 b) {
+    T2 r;
+    r.x = a.x-b.x;
+    r.y = a.y-b.y;
+    return r;
+}
+
+template<class T2>
+__device__ __forceinline__ T2 conjIf(T2 a, bool c) {
+    if (c)
+        a.y = -a.y;
+    return a;
+}
+
+// -i a (forward) or +i a (inverse)
+template<class T2>
+__device__ __forceinline__ T2 rotQuarter(T2 a, bool inverse) {
+    T2 r;
+    if (inverse) {
+        r.x = -a.y;
+        r.y = a.x;
+    }
+    else {
+        r.x = a.y;
+        r.y = -a.x;
+    }
+    return r;
+}
+
+// Mixed-radix Stockham FFT of nlines lines of length K held in LDS (line l at buf[l*K]); tw[j] =
+// e^{-2 pi i j/K} in LDS.  Returns the buffer that holds the result (natural order).
+template<class T>
+__device__ typename Vec2<T>::type* ldsFft(typename Vec2<T>::type* src, typename Vec2<T>::type* dst,
+                                          const typename Vec2<T>::type* tw, const FftPlan& p, int nlines,
+                                          bool inverse) {
+    typedef typename Vec2<T>::type T2;
+    const int K = p.K;
+    int Ns = 1;
+    for (int st = 0; st < p.nstage; st++) {
+        const int R = p.radix[st], m = K/R, span = K/(Ns*R);
+        for (int idx = threadIdx.x; idx < nlines*m; idx += blockDim.x) {
+            const int l = idx/m, j = idx-l*m, kk = j%Ns;
+            const T2* in = src+l*K;
+            T2* out = dst+l*K+(j/Ns)*Ns*R+kk;
+            T2 v[7];
+            for (int r = 0; r < R; r++) {
+                const T2 a = in[j+r*m];
+                v[r] = (r == 0 ? a : cmul(a, conjIf(tw[r*kk*span], inverse)));
+            }
+            if (R == 2) {
+                out[0] = cadd(v[0], v[1]);
+                out[Ns] = csub(v[0], v[1]);
+            }
+            else if (R == 4) {
+                const T2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+                const T2 s13 = cadd(v[1], v[3]), d13 = rotQuarter(csub(v[1], v[3]), inverse);
+                out[0] = cadd(s02, s13);
+                out[Ns] = cadd(d02, d13);
+                out[2*Ns] = csub(s02, s13);
+                out[3*Ns] = csub(d02, d13);
+            }
+            else if (R == 3) {
+                const T c = T(-0.5), sn = (inverse ? T(0.86602540378443864676) : T(-0.86602540378443864676));
+                const T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+                T2 t;
+                t.x = v[0].x+c*s.x;
+                t.y = v[0].y+c*s.y;
+                out[0] = cadd(v[0], s);
+                T2 u;
+                u.x = -sn*d.y;
+                u.y = sn*d.x;
+                out[Ns] = cadd(t, u);
+                out[2*Ns] = csub(t, u);
+            }
+            else {
+                for (int s = 0; s < R; s++) {
+                    T2 acc = v[0];
+                    for (int r = 1; r < R; r++)
+                        acc = cadd(acc, cmul(v[r], conjIf(tw[((r*s)%R)*m], inverse)));
+                    out[s*Ns] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        T2* t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    return src;
+}
+
+// ---------------------------------------------------------------------------------------- binning
+
+template<class T>
+__device__ __forceinline__ void fracGrid(const double4& p, const BoxT<double>& box, const PmeGrid& g, int base[3],
+                                         T fr[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double t = p.x*box.recip[0][a]+p.y*box.recip[1][a]+p.z*box.recip[2][a];
+        t -= floor(t);
+        const double w = g.K[a]*t;
+        int b = (int) floor(w);
+        fr[a] = (T) (w-b);
+        if (b >= g.K[a])
+            b -= g.K[a];
+        base[a] = b;
+    }
+}
+
+template<class T>
+__global__ void kPmeBin(int n, const double4* posD, BoxT<double> box, PmeGrid g, int* brickOf, int* slot,
+                        int* brickCount) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    int base[3];
+    T fr[3];
+    fracGrid<T>(posD[s], box, g, base, fr);
+    const int b = ((base[0]/g.brick)*g.nb[1]+base[1]/g.brick)*g.nb[2]+base[2]/g.brick;
+    brickOf[s] = b;
+    slot[s] = atomicAdd(&brickCount[b], 1);
+}
+
+template<class T, int P>
+__global__ void kPmePlace(int n, const double4* posD, BoxT<double> box, PmeGrid g, const int* brickOf,
+                          const int* slot, const int* brickStart, int* pmeAtom, int* baseOut, T* spl) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    int base[3];
+    T fr[3];
+    fracGrid<T>(posD[s], box, g, base, fr);
+    const int k = brickStart[brickOf[s]]+slot[s];
+    pmeAtom[k] = s;
+    T* o = spl+(size_t) k*9*P;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        baseOut[3*k+a] = base[a];
+        T th[P], dth[P], d2th[P];
+        bsplines<T, P>(fr[a], th, dth, d2th);
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            o[a*3*P+j] = th[j];
+            o[a*3*P+P+j] = dth[j];
+            o[a*3*P+2*P+j] = d2th[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------- spreading
+
+template<class T, int P>
+__global__ void __launch_bounds__(kBlock) kSpread(PmeGrid g, const int* __restrict__ brickStart,
+                                                  const int* __restrict__ pmeAtom, const int* __restrict__ baseArr,
+                                                  const T* __restrict__ spl,
+                                                  const typename Vec4<T>::type* __restrict__ dip,
+                                                  const typename Vec4<T>::type* __restrict__ posq, int withQ,
+                                                  BoxT<double> box, T* __restrict__ grid, const int* skip) {
+    typedef typename Vec4<T>::type T4;
+    extern __shared__ unsigned char smemRaw[];
+    T* tile = reinterpret_cast<T*>(smemRaw);
+    if (skip && *skip)
+        return;
+    const int bid = blockIdx.x;
+    const int b2 = bid%g.nb[2], b1 = (bid/g.nb[2])%g.nb[1], b0 = bid/(g.nb[2]*g.nb[1]);
+    const int E = g.brick+P-1;
+    const int lo0 = b0*g.brick-(P-1), lo1 = b1*g.brick-(P-1), lo2 = b2*g.brick-(P-1);
+    for (int i = threadIdx.x; i < E*E*E; i += blockDim.x)
+        tile[i] = T(0);
+    __syncthreads();
+    T rc[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            rc[k][a] = (T) (g.K[a]*box.recip[k][a]);
+    const int first = brickStart[bid], count = brickStart[bid+1]-first;
+    for (int it = threadIdx.x; it < count*P; it += blockDim.x) {
+        const int k = first+it/P, j1 = it%P;
+        const int s = pmeAtom[k];
+        const T4 d = dip[s];
+        const T q = withQ ? posq[s].w : T(0);
+        const T e0 = d.x*rc[0][0]+d.y*rc[1][0]+d.z*rc[2][0];
+        const T e1 = d.x*rc[0][1]+d.y*rc[1][1]+d.z*rc[2][1];
+        const T e2 = d.x*rc[0][2]+d.y*rc[1][2]+d.z*rc[2][2];
+        const T* o = spl+(size_t) k*9*P;
+        const T t1 = o[j1], u1 = o[P+j1];
+        const int l0 = baseArr[3*k]-j1-lo0, l1b = baseArr[3*k+1]-lo1, l2b = baseArr[3*k+2]-lo2;
+        T t3[P], u3[P];
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            t3[j] = o[6*P+j];
+            u3[j] = o[7*P+j];
+        }
+#pragma unroll
+        for (int j2 = 0; j2 < P; j2++) {
+            const T t2 = o[3*P+j2], u2 = o[4*P+j2];
+            const T A = (q*t1+e0*u1)*t2+e1*t1*u2;
+            const T C = e2*t1*t2;
+            T* row = tile+(l0*E+(l1b-j2))*E+l2b;
+#pragma unroll
+            for (int j3 = 0; j3 < P; j3++)
+                atomicAdd(row-j3, A*t3[j3]+C*u3[j3]);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < E*E*E; i += blockDim.x) {
+        const T v = tile[i];
+        if (v == T(0))
+            continue;
+        const int x2 = i%E, x1 = (i/E)%E, x0 = i/(E*E);
+        int g0 = lo0+x0, g1 = lo1+x1, g2 = lo2+x2;
+        g0 += (g0 < 0 ? g.K[0] : 0);
+        g0 -= (g0 >= g.K[0] ? g.K[0] : 0);
+        g1 += (g1 < 0 ? g.K[1] : 0);
+        g1 -= (g1 >= g.K[1] ? g.K[1] : 0);
+        g2 += (g2 < 0 ? g.K[2] : 0);
+        g2 -= (g2 >= g.K[2] ? g.K[2] : 0);
+        unsafeAtomicAdd(grid+((size_t) g0*g.K[1]+g1)*g.K[2]+g2, v);
+    }
+}
+
+// ---------------------------------------------------------------------------------------- transforms
+
+template<class T>
+__device__ __forceinline__ void loadTwiddles(typename Vec2<T>::type* dst, const typename Vec2<T>::type* src, int K) {
+    for (int i = threadIdx.x; i < K; i += blockDim.x)
+        dst[i] = src[i];
+}
+
+// z: two real lines per complex FFT, unpacked into their half spectra.
+template<class T, bool INVERSE>
+__global__ void __launch_bounds__(kBlock) kFftZ(PmeGrid g, T* __restrict__ real, typename Vec2<T>::type* __restrict__ cplx,
+                                                const typename Vec2<T>::type* __restrict__ twAll, int pairsPerBlock,
+                                                const int* skip) {
+    typedef typename Vec2<T>::type T2;
+    extern __shared__ unsigned char smemRaw[];
+    if (skip && *skip)
+        return;
+    const FftPlan& p = g.plan[2];
+    const int K = g.K[2], Kh = g.K3h, nlines = g.K[0]*g.K[1];
+    T2* tw = reinterpret_cast<T2*>(smemRaw);
+    T2* b0 = tw+K;
+    T2* b1 = b0+pairsPerBlock*K;
+    loadTwiddles<T>(tw, twAll+g.K[0]+g.K[1], K);
+    const int pair0 = blockIdx.x*pairsPerBlock;
+    const int npairs = min(pairsPerBlock, (nlines+1)/2-pair0);
+    if (!INVERSE) {
+        for (int i = threadIdx.x; i < npairs*K; i += blockDim.x) {
+            const int l = i/K, k = i-l*K;
+            const int la = 2*(pair0+l), lb = la+1;
+            T2 z;
+            z.x = real[(size_t) la*K+k];
+            z.y = (lb < nlines ? real[(size_t) lb*K+k] : T(0));
+            b0[l*K+k] = z;
+        }
+    }
+    else {
+        for (int i = threadIdx.x; i < npairs*K; i += blockDim.x) {
+            const int l = i/K, k = i-l*K;
+            const int la = 2*(pair0+l), lb = la+1;
+            const bool lower = k < Kh;
+            const int kk = lower ? k : K-k;
+            T2 X = cplx[(size_t) la*Kh+kk];
+            T2 Y;
+            Y.x = Y.y = T(0);
+            if (lb < nlines)
+                Y = cplx[(size_t) lb*Kh+kk];
+            if (kk == 0 || (K%2 == 0 && kk == K/2)) {
+                X.y = T(0);
+                Y.y = T(0);
+            }
+            T2 z;
+            if (lower) {
+                z.x = X.x-Y.y;
+                z.y = X.y+Y.x;
+            }
+            else {
+                z.x = X.x+Y.y;
+                z.y = Y.x-X.y;
+            }
+            b0[l*K+k] = z;
+        }
+    }
+    __syncthreads();
+    T2* res = ldsFft<T>(b0, b1, tw, p, npairs, INVERSE);
+    if (!INVERSE) {
+        for (int i = threadIdx.x; i < npairs*Kh; i += blockDim.x) {
+            const int l = i/Kh, k = i-l*Kh;
+            const int la = 2*(pair0+l), lb = la+1;
+            const T2 Z = res[l*K+k], Zc = res[l*K+(K-k)%K];
+            T2 X, Y;
+            X.x = T(0.5)*(Z.x+Zc.x);
+            X.y = T(0.5)*(Z.y-Zc.y);
+            Y.x = T(0.5)*(Z.y+Zc.y);
+            Y.y = T(-0.5)*(Z.x-Zc.x);
+            cplx[(size_t) la*Kh+k] = X;
+            if (lb < nlines)
+                cplx[(size_t) lb*Kh+k] = Y;
+        }
+    }
+    else {
+        for (int i = threadIdx.x; i < npairs*K; i += blockDim.x) {
+            const int l = i/K, k = i-l*K;
+            const int la = 2*(pair0+l), lb = la+1;
+            const T2 z = res[l*K+k];
+            real[(size_t) la*K+k] = z.x;
+            if (lb < nlines)
+                real[(size_t) lb*K+k] = z.y;
+        }
+    }
+}
+
+// y: lines of K2 for fixed (x, z), a tile of consecutive z per block.
+template<class T, bool INVERSE>
+__global__ void __launch_bounds__(kBlock) kFftY(PmeGrid g, typename Vec2<T>::type* __restrict__ cplx,
+                                                const typename Vec2<T>::type* __restrict__ twAll, int zt,
+                                                const int* skip) {
+    typedef typename Vec2<T>::type T2;
+    extern __shared__ unsigned char smemRaw[];
+    if (skip && *skip)
+        return;
+    const int K = g.K[1], Kh = g.K3h;
+    const int ntile = (Kh+zt-1)/zt;
+    const int x = blockIdx.x/ntile, z0 = (blockIdx.x%ntile)*zt;
+    const int nl = min(zt, Kh-z0);
+    T2* tw = reinterpret_cast<T2*>(smemRaw);
+    T2* b0 = tw+K;
+    T2* b1 = b0+zt*K;
+    loadTwiddles<T>(tw, twAll+g.K[0], K);
+    for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
+        const int y = i/nl, l = i-y*nl;
+        b0[l*K+y] = cplx[((size_t) x*K+y)*Kh+z0+l];
+    }
+    __syncthreads();
+    T2* res = ldsFft<T>(b0, b1, tw, g.plan[1], nl, INVERSE);
+    for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
+        const int y = i/nl, l = i-y*nl;
+        cplx[((size_t) x*K+y)*Kh+z0+l] = res[l*K+y];
+    }
+}
+
+// x: forward, influence function (energy, strain), back; a tile of consecutive z for fixed y per block.
+template<class T>
+__global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>::type* __restrict__ cplx,
+                                                    const T* __restrict__ infl,
+                                                    const typename Vec2<T>::type* __restrict__ twAll, int zt,
+                                                    BoxT<double> box, double beta, int energy, int strain,
+                                                    double* __restrict__ partials, const int* skip) {
+    typedef typename Vec2<T>::type T2;
+    extern __shared__ unsigned char smemRaw[];
+    __shared__ double red[kBlock/64][10];
+    if (skip && *skip)
+        return;
+    const int K = g.K[0], K2 = g.K[1], K3 = g.K[2], Kh = g.K3h;
+    const int ntile = (Kh+zt-1)/zt;
+    const int y = blockIdx.x/ntile, z0 = (blockIdx.x%ntile)*zt;
+    const int nl = min(zt, Kh-z0);
+    T2* tw = reinterpret_cast<T2*>(smemRaw);
+    T2* b0 = tw+K;
+    T2* b1 = b0+zt*K;
+    loadTwiddles<T>(tw, twAll, K);
+    for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
+        const int xx = i/nl, l = i-xx*nl;
+        b0[l*K+xx] = cplx[((size_t) xx*K2+y)*Kh+z0+l];
+    }
+    __syncthreads();
+    T2* res = ldsFft<T>(b0, b1, tw, g.plan[0], nl, false);
+    T2* other = (res == b0 ? b1 : b0);
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double fac = kPi*kPi/(beta*beta);
+    for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
+        const int l = i/K, m1 = i-l*K, m3 = z0+l;
+        T2 F = res[l*K+m1];
+        const T G = infl[((size_t) m1*K2+y)*Kh+m3];
+        if (energy) {
+            const double w = (m3 == 0 || (K3%2 == 0 && m3 == K3/2)) ? 1.0 : 2.0;
+            const double em = 0.5*w*(double) G*((double) F.x*F.x+(double) F.y*F.y);
+            acc[0] += em;
+            if (strain && G != T(0)) {
+                const int f1 = (m1 <= K/2 ? m1 : m1-K), f2 = (y <= K2/2 ? y : y-K2);
+                double mv[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    mv[k] = f1*box.recip[k][0]+f2*box.recip[k][1]+m3*box.recip[k][2];
+                const double c2 = 2.0*(fac+1.0/(mv[0]*mv[0]+mv[1]*mv[1]+mv[2]*mv[2]));
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++)
+                        acc[1+3*a+b] += em*((a == b ? -1.0 : 0.0)+c2*mv[a]*mv[b]);
+            }
+        }
+        F.x *= G;
+        F.y *= G;
+        res[l*K+m1] = F;
+    }
+    __syncthreads();
+    T2* res2 = ldsFft<T>(res, other, tw, g.plan[0], nl, true);
+    for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
+        const int xx = i/nl, l = i-xx*nl;
+        cplx[((size_t) xx*K2+y)*Kh+z0+l] = res2[l*K+xx];
+    }
+    if (energy) {
+        const int nq = strain ? 10 : 1;
+        for (int q = 0; q < nq; q++) {
+            const double v = groupSum<64>(acc[q]);
+            if ((threadIdx.x & 63) == 0)
+                red[threadIdx.x >> 6][q] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 10) {
+            double v = 0;
+            if (threadIdx.x < nq)
+                for (int w = 0; w < kBlock/64; w++)
+                    v += red[w][threadIdx.x];
+            partials[(size_t) blockIdx.x*10+threadIdx.x] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------- gathering
+
+constexpr int kGatherLanes = 8;
+
+template<class T, int P, bool FINAL>
+__global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* __restrict__ pmeAtom,
+                                                  const int* __restrict__ baseArr, const T* __restrict__ spl,
+                                                  const T* __restrict__ phi, BoxT<double> box,
+                                                  const typename Vec4<T>::type* __restrict__ dip,
+                                                  const typename Vec4<T>::type* __restrict__ posq,
+                                                  double* __restrict__ field, double* __restrict__ force,
+                                                  const int* skip) {
+    typedef typename Vec4<T>::type T4;
+    if (skip && *skip)
+        return;
+    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int k = gid/kGatherLanes, lane = gid%kGatherLanes;
+    if (k >= n)
+        return;
+    const T* o = spl+(size_t) k*9*P;
+    const int B0 = baseArr[3*k], B1 = baseArr[3*k+1], B2 = baseArr[3*k+2];
+    const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
+    T t3[P], u3[P], v3[P];
+    int g3[P];
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        t3[j] = o[6*P+j];
+        u3[j] = o[7*P+j];
+        v3[j] = o[8*P+j];
+        int z = B2-j;
+        g3[j] = z < 0 ? z+K2 : z;
+    }
+    double f[3] = {0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = lane; c < P*P; c += kGatherLanes) {
+        const int j1 = c/P, j2 = c-j1*P;
+        int x0 = B0-j1, x1 = B1-j2;
+        x0 += (x0 < 0 ? K0 : 0);
+        x1 += (x1 < 0 ? K1 : 0);
+        const T* row = phi+((size_t) x0*K1+x1)*K2;
+        T st = 0, su = 0, sv = 0;
+#pragma unroll
+        for (int j3 = 0; j3 < P; j3++) {
+            const T ph = row[g3[j3]];
+            st += t3[j3]*ph;
+            su += u3[j3]*ph;
+            if (FINAL)
+                sv += v3[j3]*ph;
+        }
+        const T t1 = o[j1], u1 = o[P+j1], t2 = o[3*P+j2], u2 = o[4*P+j2];
+        f[0] += (double) (u1*t2*st);
+        f[1] += (double) (t1*u2*st);
+        f[2] += (double) (t1*t2*su);
+        if (FINAL) {
+            const T v1 = o[2*P+j1], v2 = o[5*P+j2];
+            h[0] += (double) (v1*t2*st);   // 00
+            h[1] += (double) (t1*v2*st);   // 11
+            h[2] += (double) (t1*t2*sv);   // 22
+            h[3] += (double) (u1*u2*st);   // 01
+            h[4] += (double) (u1*t2*su);   // 02
+            h[5] += (double) (t1*u2*su);   // 12
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        f[a] = groupSum<kGatherLanes>(f[a]);
+    if (FINAL)
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+            h[a] = groupSum<kGatherLanes>(h[a]);
+    if (lane != 0)
+        return;
+    const int s = pmeAtom[k];
+    double rk[3][3];
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            rk[kk][a] = g.K[a]*box.recip[kk][a];
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+        field[3*s+kk] = -(rk[kk][0]*f[0]+rk[kk][1]*f[1]+rk[kk][2]*f[2]);
+    if (FINAL) {
+        const T4 d = dip[s];
+        const double q = (double) posq[s].w;
+        double e[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            e[a] = rk[0][a]*d.x+rk[1][a]*d.y+rk[2][a]*d.z;
+        const double H[3][3] = {{h[0], h[3], h[4]}, {h[3], h[1], h[5]}, {h[4], h[5], h[2]}};
+        double c[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            c[a] = q*f[a]+e[0]*H[a][0]+e[1]*H[a][1]+e[2]*H[a][2];
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++)
+            force[3*s+kk] = -(rk[kk][0]*c[0]+rk[kk][1]*c[1]+rk[kk][2]*c[2]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------- setup
+
+template<class T>
+__global__ void kInfluence(PmeGrid g, const double* bmod, BoxT<double> box, double beta, double volume, T* infl) {
+    const int K1 = g.K[0], K2 = g.K[1], Kh = g.K3h;
+    const size_t total = (size_t) K1*K2*Kh;
+    const size_t idx = (size_t) blockIdx.x*blockDim.x+threadIdx.x;
+    if (idx >= total)
+        return;
+    const int m3 = idx%Kh, m2 = (idx/Kh)%K2, m1 = idx/((size_t) Kh*K2);
+    const int f1 = (m1 <= K1/2 ? m1 : m1-K1), f2 = (m2 <= K2/2 ? m2 : m2-K2);
+    double val = 0;
+    if (f1 != 0 || f2 != 0 || m3 != 0) {
+        double mv[3];
+        for (int k = 0; k < 3; k++)
+            mv[k] = f1*box.recip[k][0]+f2*box.recip[k][1]+m3*box.recip[k][2];
+        const double m2s = mv[0]*mv[0]+mv[1]*mv[1]+mv[2]*mv[2];
+        const double b = bmod[m1]*bmod[K1+m2]*bmod[K1+K2+m3];
+        val = exp(-kPi*kPi/(beta*beta)*m2s)/(kPi*volume*m2s)/b;
+    }
+    infl[idx] = (T) val;
+}
+
+template<class T>
+__global__ void kTwiddles(PmeGrid g, typename Vec2<T>::type* tw) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    const int total = g.K[0]+g.K[1]+g.K[2];
+    if (i >= total)
+        return;
+    const int a = (i < g.K[0] ? 0 : (i < g.K[0]+g.K[1] ? 1 : 2));
+    const int j = i-(a == 0 ? 0 : (a == 1 ? g.K[0] : g.K[0]+g.K[1]));
+    double sn, cs;
+    sincospi(-2.0*j/g.K[a], &sn, &cs);
+    typename Vec2<T>::type v;
+    v.x = (T) cs;
+    v.y = (T) sn;
+    tw[i] = v;
+}
+
+template<class T>
+__global__ void kZero(T* p, size_t count, const int* skip) {
+    if (skip && *skip)
+        return;
+    for (size_t i = (size_t) blockIdx.x*blockDim.x+threadIdx.x; i < count; i += (size_t) gridDim.x*blockDim.x)
+        p[i] = T(0);
+}
+
+inline int nblk(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
+
+template<class T>
+int linesPerBlock(int K) {
+    const int budget = (int) (48*1024/(2*sizeof(typename Vec2<T>::type)))-K;   // elements per buffer
+    return std::max(1, std::min(16, budget/(2*K)));
+}
+
+}   // namespace
+
+template<class T>
+void launchPmeBin(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, int* brickOf, int* slot,
+                  int* brickCount) {
+    kPmeBin<T><<<nblk(n, 256), 256, 0, st>>>(n, posD, box, g, brickOf, slot, brickCount);
+}
+
+template<class T>
+void launchPmePlace(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, const int* brickOf,
+                    const int* slot, const int* brickStart, int* pmeAtom, int* base, T* spl) {
+    switch (g.order) {
+#define PGM_CASE(P)                                                                                                    \
+    case P:                                                                                                            \
+        kPmePlace<T, P><<<nblk(n, 128), 128, 0, st>>>(n, posD, box, g, brickOf, slot, brickStart, pmeAtom, base, spl); \
+        break;
+        PGM_CASE(4)
+        PGM_CASE(5)
+        PGM_CASE(6)
+        PGM_CASE(7)
+        PGM_CASE(8)
+#undef PGM_CASE
+    default:
+        throw std::runtime_error("pgm_hip: PME order must be 4..8");
+    }
+}
+
+template<class T>
+void launchSpread(hipStream_t st, PmeGrid g, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
+                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, BoxT<double> box,
+                  T* grid, const int* skip) {
+    const int E = g.brick+g.order-1;
+    const size_t lds = (size_t) E*E*E*sizeof(T);
+    switch (g.order) {
+#define PGM_CASE(P)                                                                                           \
+    case P:                                                                                                   \
+        kSpread<T, P><<<g.nbrick, kBlock, lds, st>>>(g, brickStart, pmeAtom, base, spl, dip, posq, withQ, box, \
+                                                     grid, skip);                                             \
+        break;
+        PGM_CASE(4)
+        PGM_CASE(5)
+        PGM_CASE(6)
+        PGM_CASE(7)
+        PGM_CASE(8)
+#undef PGM_CASE
+    default:
+        throw std::runtime_error("pgm_hip: PME order must be 4..8");
+    }
+}
+
+template<class T>
+int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type* cplx, const T* influence,
+                      const typename Vec2<T>::type* tw, BoxT<double> box, double beta, int energy, int strain,
+                      double* partials, const int* skip) {
+    typedef typename Vec2<T>::type T2;
+    const int nlines = g.K[0]*g.K[1];
+    // z forward
+    const int pz = linesPerBlock<T>(g.K[2]);
+    const int zBlocks = nblk((nlines+1)/2, pz);
+    const size_t ldsZ = (size_t) (g.K[2]+2*pz*g.K[2])*sizeof(T2);
+    kFftZ<T, false><<<zBlocks, kBlock, ldsZ, st>>>(g, real, cplx, tw, pz, skip);
+    const int zy = linesPerBlock<T>(g.K[1]);
+    const int yBlocks = g.K[0]*((g.K3h+zy-1)/zy);
+    const size_t ldsY = (size_t) (g.K[1]+2*zy*g.K[1])*sizeof(T2);
+    kFftY<T, false><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
+    const int zx = linesPerBlock<T>(g.K[0]);
+    const int xBlocks = g.K[1]*((g.K3h+zx-1)/zx);
+    const size_t ldsX = (size_t) (g.K[0]+2*zx*g.K[0])*sizeof(T2);
+    kFftXConv<T><<<xBlocks, kBlock, ldsX, st>>>(g, cplx, influence, tw, zx, box, beta, energy, strain, partials, skip);
+    kFftY<T, true><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
+    kFftZ<T, true><<<zBlocks, kBlock, ldsZ, st>>>(g, real, cplx, tw, pz, skip);
+    return xBlocks;
+}
+
+template<class T>
+void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
+                  BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
+                  double* field, double* force, const int* skip) {
+    const int blocks = nblk((long long) n*kGatherLanes, kBlock);
+    switch (g.order) {
+#define PGM_CASE(P)                                                                                                  \
+    case P:                                                                                                          \
+        if (final_)                                                                                                  \
+            kGather<T, P, true><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,      \
+                                                           force, skip);                                             \
+        else                                                                                                         \
+            kGather<T, P, false><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,     \
+                                                            force, skip);                                            \
+        break;
+        PGM_CASE(4)
+        PGM_CASE(5)
+        PGM_CASE(6)
+        PGM_CASE(7)
+        PGM_CASE(8)
+#undef PGM_CASE
+    default:
+        throw std::runtime_error("pgm_hip: PME order must be 4..8");
+    }
+}
+
+template<class T>
+void launchInfluence(hipStream_t st, PmeGrid g, const double* bmod, BoxT<double> box, double beta, double volume,
+                     T* influence) {
+    const size_t total = (size_t) g.K[0]*g.K[1]*g.K3h;
+    kInfluence<T><<<nblk((long long) total, 256), 256, 0, st>>>(g, bmod, box, beta, volume, influence);
+}
+
+template<class T>
+void launchTwiddles(hipStream_t st, PmeGrid g, typename Vec2<T>::type* tw) {
+    kTwiddles<T><<<nblk(g.K[0]+g.K[1]+g.K[2], 256), 256, 0, st>>>(g, tw);
+}
+
+template<class T>
+void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
+    const int blocks = (int) std::min<size_t>(2048, (count+255)/256);
+    kZero<T><<<std::max(1, blocks), 256, 0, st>>>(p, count, skip);
+}
+
+#define PGM_INSTANTIATE_PME(T)                                                                                          \
+    template void launchPmeBin<T>(hipStream_t, int, const double4*, BoxT<double>, PmeGrid, int*, int*, int*);          \
+    template void launchPmePlace<T>(hipStream_t, int, const double4*, BoxT<double>, PmeGrid, const int*, const int*,   \
+                                    const int*, int*, int*, T*);                                                       \
+    template void launchSpread<T>(hipStream_t, PmeGrid, const int*, const int*, const int*, const T*,                  \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, BoxT<double>, T*, const int*);      \
+    template int launchPmeConvolve<T>(hipStream_t, PmeGrid, T*, Vec2<T>::type*, const T*, const Vec2<T>::type*,        \
+                                      BoxT<double>, double, int, int, double*, const int*);                            \
+    template void launchGather<T>(hipStream_t, PmeGrid, int, const int*, const int*, const T*, const T*, BoxT<double>, \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
+    template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
+    template void launchTwiddles<T>(hipStream_t, PmeGrid, Vec2<T>::type*);                                             \
+    template void launchZero<T>(hipStream_t, T*, size_t, const int*);
+PGM_INSTANTIATE_PME(float)
+PGM_INSTANTIATE_PME(double)
+template void launchZero<int>(hipStream_t, int*, size_t, const int*);
+template void launchZero<unsigned>(hipStream_t, unsigned*, size_t, const int*);
+
+}   // namespace pgmhip

@@ -1,0 +1,262 @@
+"""Python binding of the HIP engine (ctypes over the C ABI of include/pgm_hip.h).
+
+`Engine` mirrors pgmcuda::Engine (core/cuda/pgm_cuda.h:47-127): set_atoms, set_covalent, compute,
+compute_f32 + forces, compute_device (float4 positions, fixed-point forces), iterations, residual,
+induced/permanent dipoles, reset_history, list_builds, num_candidates.  Errors raise `PgmError` with the
+engine's message.  The library must be built (pgm-md_amd/build.py); a missing library is an error, never a
+fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpgm_hip.so")
+
+PGM_DOUBLE, PGM_MIXED = 0, 1
+
+
+class PgmError(RuntimeError):
+    pass
+
+
+class Settings(C.Structure):
+    """pgm_settings (include/pgm_hip.h)."""
+
+    _fields_ = [
+        ("cutoff", C.c_double),
+        ("beta", C.c_double),
+        ("grid", C.c_int * 3),
+        ("order", C.c_int),
+        ("tol", C.c_double),
+        ("max_iter", C.c_int),
+        ("peek", C.c_double),
+        ("predictor", C.c_int),
+        ("ke", C.c_double),
+        ("device", C.c_int),
+        ("skin", C.c_double),
+        ("precision", C.c_int),
+        ("polarizable", C.c_int),
+    ]
+
+
+class Terms(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("direct", "reciprocal", "self", "background", "polarization", "gvdw")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @property
+    def total(self) -> float:
+        return sum(getattr(self, k) for k, _ in self._fields_)
+
+
+_lib = None
+EXPORTS = [
+    "pgm_default_settings", "pgm_create", "pgm_destroy", "pgm_last_error", "pgm_set_atoms", "pgm_set_covalent",
+    "pgm_compute", "pgm_compute_f32", "pgm_forces", "pgm_compute_device", "pgm_launch_device", "pgm_finish_device",
+    "pgm_compute_device_f64", "pgm_iterations", "pgm_residual", "pgm_induced_dipoles", "pgm_permanent_dipoles",
+    "pgm_reset_history", "pgm_list_builds", "pgm_num_candidates", "pgm_late_list_checks", "pgm_set_history",
+    "pgm_freeze_history",
+]  # fmt: skip
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libpgm_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PgmError(f"{path} is not built: run pgm-md_amd/build.py (no CPU fallback exists)")
+    lib = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    lib.pgm_default_settings.argtypes = [C.POINTER(Settings)]
+    lib.pgm_create.restype = vp
+    lib.pgm_create.argtypes = [C.POINTER(Settings)]
+    lib.pgm_destroy.argtypes = [vp]
+    lib.pgm_last_error.restype = C.c_char_p
+    lib.pgm_last_error.argtypes = [vp]
+    lib.pgm_set_atoms.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int]
+    lib.pgm_set_covalent.argtypes = [vp, C.c_int, vp, vp, vp]
+    lib.pgm_compute.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.POINTER(Terms)]
+    lib.pgm_compute_f32.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(Terms)]
+    lib.pgm_forces.argtypes = [vp, vp]
+    lib.pgm_compute_device.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_double, vp, C.POINTER(Terms)]
+    lib.pgm_launch_device.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]
+    lib.pgm_finish_device.argtypes = [vp, C.POINTER(Terms), ip]
+    lib.pgm_compute_device_f64.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, C.POINTER(Terms)]
+    lib.pgm_iterations.argtypes = [vp]
+    lib.pgm_residual.argtypes = [vp]
+    lib.pgm_residual.restype = C.c_double
+    lib.pgm_induced_dipoles.argtypes = [vp, vp]
+    lib.pgm_permanent_dipoles.argtypes = [vp, vp]
+    lib.pgm_reset_history.argtypes = [vp]
+    lib.pgm_list_builds.argtypes = [vp]
+    lib.pgm_list_builds.restype = C.c_long
+    lib.pgm_num_candidates.argtypes = [vp]
+    lib.pgm_num_candidates.restype = C.c_longlong
+    lib.pgm_late_list_checks.argtypes = [vp]
+    lib.pgm_late_list_checks.restype = C.c_long
+    lib.pgm_set_history.argtypes = [vp, C.c_int, vp]
+    lib.pgm_freeze_history.argtypes = [vp, C.c_int]
+    _ = dp
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def make_settings(**kw) -> Settings:
+    lib = load()
+    s = Settings()
+    lib.pgm_default_settings(C.byref(s))
+    for k, v in kw.items():
+        if k == "grid":
+            for a in range(3):
+                s.grid[a] = int(v[a])
+        elif k == "precision":
+            s.precision = {"double": PGM_DOUBLE, "mixed": PGM_MIXED}.get(v, v)
+        elif k in ("periodic",):
+            if not v:
+                raise PgmError("the HIP engine runs periodic systems (PME) only")
+        elif hasattr(s, k):
+            setattr(s, k, type(getattr(s, k))(v))
+        else:
+            raise PgmError(f"unknown setting {k}")
+    return s
+
+
+class Engine:
+    """pGM electrostatics on the GPU; parameters and positions in the caller's atom order."""
+
+    def __init__(self, settings: Settings | None = None, **kw):
+        self.lib = load()
+        self.settings = settings if settings is not None else make_settings(**kw)
+        self.h = self.lib.pgm_create(C.byref(self.settings))
+        if not self.h:
+            raise PgmError(self.lib.pgm_last_error(None).decode())
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pgm_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise PgmError(self.lib.pgm_last_error(self.h).decode())
+
+    @classmethod
+    def for_system(cls, system, precision="mixed", **overrides) -> "Engine":
+        st = dict(system.settings)
+        st.update(overrides)
+        keys = ("cutoff", "beta", "grid", "order", "tol", "max_iter", "peek", "predictor", "ke", "skin", "polarizable")
+        kw = {k: st[k] for k in keys if k in st}
+        if "device" in st:
+            kw["device"] = st["device"]
+        e = cls(precision=precision, **kw)
+        e.set_atoms(system.q, system.radius, system.alpha)
+        e.set_covalent(system.cov_atom, system.cov_partner, system.cov_strength)
+        return e
+
+    def set_atoms(self, q, radius, alpha, global_index=None, n_global=None):
+        q, radius, alpha = _f64(q), _f64(radius), _f64(alpha)
+        gi = None if global_index is None else np.ascontiguousarray(global_index, np.int32)
+        self.n = len(q)
+        ng = n_global if n_global is not None else self.n
+        self._keep = (q, radius, alpha, gi)
+        self._check(self.lib.pgm_set_atoms(self.h, self.n, _ptr(q), _ptr(radius), _ptr(alpha), _ptr(gi), ng))
+
+    def set_covalent(self, atom, partner, strength):
+        a = np.ascontiguousarray(atom, np.int32)
+        p = np.ascontiguousarray(partner, np.int32)
+        s = _f64(strength)
+        self._check(self.lib.pgm_set_covalent(self.h, len(a), _ptr(a), _ptr(p), _ptr(s)))
+
+    def compute(self, pos, box, new_step=True, forces=True, strain=False):
+        """Return (terms dict, forces (n,3) or None, strain (3,3) or None)."""
+        pos = _f64(pos, (self.n, 3))
+        box = _f64(box, (3, 3))
+        f = np.zeros((self.n, 3)) if forces else None
+        S = np.zeros(9) if strain else None
+        t = Terms()
+        self._check(self.lib.pgm_compute(self.h, _ptr(pos), _ptr(box), int(bool(new_step)), _ptr(f), _ptr(S), C.byref(t)))
+        return t.as_dict(), f, (S.reshape(3, 3) if strain else None)
+
+    def compute_f32(self, pos, box, new_step=True):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(self.n, 3)
+        box = _f64(box, (3, 3))
+        t = Terms()
+        self._check(self.lib.pgm_compute_f32(self.h, _ptr(pos), _ptr(box), int(bool(new_step)), None, C.byref(t)))
+        f = np.zeros((self.n, 3))
+        self._check(self.lib.pgm_forces(self.h, _ptr(f)))
+        return t.as_dict(), f
+
+    def compute_device_f64(self, pos_ptr: int, box, new_step=True, f_ptr: int | None = None, stream: int = 0):
+        box = _f64(box, (3, 3))
+        t = Terms()
+        self._check(self.lib.pgm_compute_device_f64(self.h, pos_ptr, _ptr(box), int(bool(new_step)), f_ptr, None,
+                                                    stream, C.byref(t)))  # fmt: skip
+        return t.as_dict()
+
+    def compute_device(self, posq_ptr: int, box, new_step, fixed_ptr: int, stride: int, scale: float, stream: int = 0):
+        box = _f64(box, (3, 3))
+        t = Terms()
+        rc = self.lib.pgm_compute_device(
+            self.h, posq_ptr, _ptr(box), int(bool(new_step)), fixed_ptr, stride, scale, stream, C.byref(t)
+        )
+        self._check(rc)
+        return t.as_dict()
+
+    def iterations(self) -> int:
+        return self.lib.pgm_iterations(self.h)
+
+    def residual(self) -> float:
+        return self.lib.pgm_residual(self.h)
+
+    def induced_dipoles(self):
+        m = np.zeros((self.n, 3))
+        self._check(self.lib.pgm_induced_dipoles(self.h, _ptr(m)))
+        return m
+
+    def permanent_dipoles(self):
+        m = np.zeros((self.n, 3))
+        self._check(self.lib.pgm_permanent_dipoles(self.h, _ptr(m)))
+        return m
+
+    def reset_history(self):
+        self._check(self.lib.pgm_reset_history(self.h))
+
+    def set_history(self, sets):
+        sets = np.ascontiguousarray(np.asarray(sets, np.float64))
+        self._check(self.lib.pgm_set_history(self.h, len(sets), _ptr(sets)))
+
+    def freeze_history(self, frozen=True):
+        self._check(self.lib.pgm_freeze_history(self.h, int(bool(frozen))))
+
+    def list_builds(self) -> int:
+        return self.lib.pgm_list_builds(self.h)
+
+    def num_candidates(self) -> int:
+        return self.lib.pgm_num_candidates(self.h)
+
+
+def total_energy(terms: dict) -> float:
+    return sum(terms.values())
+
+
+def rel_rms(f, fref) -> float:
+    """sqrt(mean(dF^2)) / sqrt(mean(F_ref^2)) over all components (tests/xplat/omm_vs_amber.py:39-41)."""
+    f, fref = np.asarray(f), np.asarray(fref)
+    return float(np.sqrt(((f - fref) ** 2).mean()) / np.sqrt((fref**2).mean()))
