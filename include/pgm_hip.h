@@ -106,6 +106,8 @@ int pgm_reset_history(pgm_engine* e);
 long pgm_list_builds(const pgm_engine* e);
 long long pgm_num_candidates(const pgm_engine* e);
 long pgm_late_list_checks(const pgm_engine* e);
+/* No counterpart in the reference: pairs within the cutoff at the last evaluation, each counted once. */
+long long pgm_num_pairs(pgm_engine* e);
 
 /* No counterpart in the reference: load nset prior solutions (each 3*n_global doubles, global order, most
  * recent first) as the predictor history, e.g. the converged dipoles of earlier steps of a trajectory. */
@@ -113,6 +115,16 @@ int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets);
 /* No counterpart in the reference: freeze (1) or unfreeze (0) the history, so that repeated evaluations
  * of one workload start from the same predicted guess (benchmarks). */
 int pgm_freeze_history(pgm_engine* e, int frozen);
+
+/* No counterpart in the reference (whose engine prints host wall clocks under PGM_CUDA_TIMING,
+ * core/cuda/pgm_cuda.cu:1814-1817): HIP-event timing of the kernel groups of every evaluation, for
+ * profiling.  Phases, in order: list check/rebuild, per-evaluation preparation, pair setup, pair field
+ * (matrix-vector products), PME spread, PME transforms + convolution, PME gather, CG vector kernels,
+ * final pair pass, final reductions and force output.  Iterations enqueued ahead that found the solve
+ * finished are not counted.  pgm_set_timing resets the sums. */
+enum { PGM_TIMING_PHASES = 10 };
+int pgm_set_timing(pgm_engine* e, int on);
+int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,7 @@ def load(kind: str = "port") -> C.CDLL:
     lib.orc_strain.argtypes = [C.c_void_p, _DP]
     lib.orc_set_history_key.argtypes = [C.c_void_p, C.c_longlong]
     lib.orc_reset_history.argtypes = [C.c_void_p]
+    lib.orc_push_history.argtypes = [C.c_void_p, _DP]
     lib.orc_set_topology.argtypes = [C.c_void_p, _DP, _DP, _DP, C.c_int, _IP, _IP, _DP]
     lib.orc_set_geometry.argtypes = [C.c_void_p, _DP, C.c_void_p]
     lib.orc_field.argtypes = [C.c_void_p, _DP, _DP, _DP]
@@ -169,6 +170,10 @@ class Oracle:
 
     def reset_history(self):
         _check(self.lib, self.lib.orc_reset_history(self.h))
+
+    def push_history(self, mu):
+        """Push one prior solution (oldest first when pushing several)."""
+        _check(self.lib, self.lib.orc_push_history(self.h, _f64(mu, (self.n, 3))))
 
     def set_topology(self, q, radius, alpha, cov_atom, cov_partner, cov_strength):
         rc = self.lib.orc_set_topology(

@@ -16,6 +16,8 @@
 #include <complex>
 #include <cstring>
 #include <deque>
+#include <memory>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -148,68 +150,117 @@ inline void minImage(V3& v, const double box[3][3]) {
 
 // ------------------------------------------------------------------------------------------------ FFT
 
-/* Mixed-radix complex FFT (recursive decimation in time over the prime factors of n), unnormalized,
- * sign -1 forward, +1 backward; the twiddles come from one table of e^{sign 2 pi i j / n}. */
+/* Mixed-radix complex FFT (Stockham autosort over the prime factors of n, radix 4 first), unnormalized,
+ * sign -1 forward, +1 backward.  Plans are cached per (n, sign). */
 class Fft1d {
 public:
-    Fft1d(int n, int sign) : n_(n), table_(n), scratch_(n) {
-        for (int j = 0; j < n; j++) {
-            const double ang = sign*2.0*kPi*j/n;
-            table_[j] = cd(std::cos(ang), std::sin(ang));
-        }
+    Fft1d(int n, int sign) : n_(n), work_(n) {
         int m = n;
+        while (m%4 == 0) {
+            radix_.push_back(4);
+            m /= 4;
+        }
         for (int p = 2; p <= m; p++)
             while (m%p == 0) {
-                factors_.push_back(p);
+                radix_.push_back(p);
                 m /= p;
             }
+        int ns = 1;
+        for (int R : radix_) {
+            stageTw_.emplace_back(R*ns);
+            for (int r = 0; r < R; r++)
+                for (int k = 0; k < ns; k++) {
+                    const double ang = sign*2.0*kPi*r*k/(double) (ns*R);
+                    stageTw_.back()[r*ns+k] = cd(std::cos(ang), std::sin(ang));
+                }
+            roots_.emplace_back(R);
+            for (int r = 0; r < R; r++) {
+                const double ang = sign*2.0*kPi*r/(double) R;
+                roots_.back()[r] = cd(std::cos(ang), std::sin(ang));
+            }
+            ns *= R;
+        }
     }
     void run(cd* data) {
-        std::vector<cd> out(n_);
-        rec(data, 1, out.data(), n_, 0);
-        std::copy(out.begin(), out.end(), data);
+        cd* src = data;
+        cd* dst = work_.data();
+        int ns = 1;
+        cd v[64];
+        for (size_t st = 0; st < radix_.size(); st++) {
+            const int R = radix_[st], m = n_/R;
+            const std::vector<cd>& tw = stageTw_[st];
+            const std::vector<cd>& root = roots_[st];
+            for (int j = 0; j < m; j++) {
+                const int k = j%ns;
+                for (int r = 0; r < R; r++)
+                    v[r] = src[j+r*m]*tw[r*ns+k];
+                cd* out = dst+(j/ns)*ns*R+k;
+                if (R == 2) {
+                    out[0] = v[0]+v[1];
+                    out[ns] = v[0]-v[1];
+                }
+                else if (R == 4) {
+                    const cd a = v[0]+v[2], b = v[0]-v[2], c = v[1]+v[3], d = (v[1]-v[3])*root[1];
+                    out[0] = a+c;
+                    out[ns] = b+d;
+                    out[2*ns] = a-c;
+                    out[3*ns] = b-d;
+                }
+                else
+                    for (int q = 0; q < R; q++) {
+                        cd acc = 0;
+                        for (int r = 0; r < R; r++)
+                            acc += v[r]*root[(r*q)%R];
+                        out[q*ns] = acc;
+                    }
+            }
+            std::swap(src, dst);
+            ns *= R;
+        }
+        if (src != data)
+            std::copy(src, src+n_, data);
     }
 
 private:
     int n_;
-    std::vector<cd> table_, scratch_;
-    std::vector<int> factors_;
-
-    void rec(const cd* in, int stride, cd* out, int len, int level) {
-        if (len == 1) {
-            out[0] = in[0];
-            return;
-        }
-        const int p = factors_[level], m = len/p, tw = n_/len;
-        for (int q = 0; q < p; q++)
-            rec(in+(size_t) q*stride, stride*p, out+(size_t) q*m, m, level+1);
-        std::vector<cd> t(p), r(p);
-        for (int k = 0; k < m; k++) {
-            for (int q = 0; q < p; q++)
-                t[q] = out[q*m+k]*table_[((size_t) q*k*tw)%n_];
-            for (int s = 0; s < p; s++) {
-                cd acc = 0;
-                for (int q = 0; q < p; q++)
-                    acc += t[q]*table_[((size_t) ((q*s)%p)*(n_/p))%n_];
-                r[s] = acc;
-            }
-            for (int s = 0; s < p; s++)
-                out[s*m+k] = r[s];
-        }
-    }
+    std::vector<cd> work_;
+    std::vector<int> radix_;
+    std::vector<std::vector<cd> > stageTw_, roots_;
 };
+
+Fft1d& fftPlan(int n, int sign) {
+    thread_local std::vector<std::pair<std::pair<int, int>, std::unique_ptr<Fft1d> > > cache;
+    for (auto& e : cache)
+        if (e.first.first == n && e.first.second == sign)
+            return *e.second;
+    if (n > 0) {
+        int m = n;
+        for (int p = 2; p <= m; p++)
+            while (m%p == 0) {
+                if (p > 64)
+                    throw std::invalid_argument("pgm_oracle: FFT length has a prime factor above 64");
+                m /= p;
+            }
+    }
+    cache.emplace_back(std::make_pair(n, sign), std::unique_ptr<Fft1d>(new Fft1d(n, sign)));
+    return *cache.back().second;
+}
 
 /* In-place 3D complex FFT of a row-major K1 x K2 x K3 array along all three axes. */
 void fft3d(std::vector<cd>& a, int K1, int K2, int K3, int sign) {
     const int K[3] = {K1, K2, K3};
     const size_t strides[3] = {(size_t) K2*K3, (size_t) K3, 1};
     for (int ax = 2; ax >= 0; ax--) {
-        Fft1d f(K[ax], sign);
+        Fft1d& f = fftPlan(K[ax], sign);
         std::vector<cd> line(K[ax]);
         const int o1 = (ax == 0 ? 1 : 0), o2 = (ax == 2 ? 1 : 2);
         for (int i = 0; i < K[o1]; i++)
             for (int j = 0; j < K[o2]; j++) {
                 const size_t base = i*strides[o1]+j*strides[o2];
+                if (ax == 2) {
+                    f.run(&a[base]);
+                    continue;
+                }
                 for (int k = 0; k < K[ax]; k++)
                     line[k] = a[base+k*strides[ax]];
                 f.run(line.data());
@@ -934,6 +985,18 @@ int orc_reset_history(void* h) {
     return guarded([&] {
         E(h)->history.clear();
         E(h)->havePos = false;
+    });
+}
+
+int orc_push_history(void* h, const double* mu) {
+    return guarded([&] {
+        Engine* e = E(h);
+        std::vector<V3> m(e->n);
+        for (int i = 0; i < e->n; i++)
+            m[i] = V3{mu[3*i], mu[3*i+1], mu[3*i+2]};
+        e->history.push_front(m);
+        while (e->history.size() > 4)
+            e->history.pop_back();
     });
 }
 

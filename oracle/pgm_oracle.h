@@ -50,6 +50,8 @@ double orc_residual(void* h);
 int orc_strain(void* h, double* out9);
 int orc_set_history_key(void* h, long long key);
 int orc_reset_history(void* h);
+/* Push one solution (3n, most recent last pushed) into the predictor history, as a finished step would. */
+int orc_push_history(void* h, const double* mu);
 int orc_set_topology(void* h, const double* q, const double* radius, const double* alpha, int ncov,
                      const int* covAtom, const int* covPartner, const double* covStrength);
 int orc_set_geometry(void* h, const double* pos, const double* box9);

@@ -168,6 +168,11 @@ int orc_reset_history(void* h) {
     return guarded([&] { H(h)->eng->resetHistory(); });
 }
 
+int orc_push_history(void*, const double*) {
+    g_err = "ref_driver: pgmcore::Engine has no history import";
+    return -1;
+}
+
 int orc_set_topology(void* h, const double* q, const double* radius, const double* alpha, int ncov,
                      const int* covAtom, const int* covPartner, const double* covStrength) {
     return guarded([&] {

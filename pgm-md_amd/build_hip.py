@@ -1,6 +1,6 @@
 """Builds the HIP engine for gfx950 into pgm-md_amd/lib/libpgm_hip.so (C ABI: include/pgm_hip.h).
 
-    python pgm-md_amd/build.py [--jobs N]
+    python pgm-md_amd/build_hip.py [--jobs N]
 
 hipcc cross-compiles without a GPU.  Packed FP32 instructions are disabled for all device code
 (-Xclang -target-feature -Xclang -packed-fp32-ops).

@@ -144,6 +144,21 @@ enum {
 
 enum class PosSource { HostF64, HostF32, DeviceF64, DeviceF4 };
 
+// Kernel groups timed with HIP events when timing is on (pgm_set_timing).
+enum Phase {
+    kPhList = 0,      // list check and rebuild
+    kPhPrep,          // sorted positions, permanent dipoles, PME bricks and splines
+    kPhPairSetup,     // compaction + stored kernels + field of q and p
+    kPhPairField,     // stored-kernel dipole field (one per matrix-vector product)
+    kPhSpread,
+    kPhFft,           // z, y, x+convolution, y, z transforms
+    kPhGather,
+    kPhCg,            // guess, CG vector kernels, post
+    kPhPairFinal,
+    kPhFinal,         // total field, covalent chain rule, reductions, forces out
+    kNumPhases
+};
+
 class Engine {
 public:
     explicit Engine(const pgm_settings& s) : s_(s) {
@@ -161,6 +176,8 @@ public:
             (void) hipSetDevice(s_.device);
         if (st_)
             (void) hipStreamSynchronize(st_);
+        for (hipEvent_t ev : evPool_)
+            (void) hipEventDestroy(ev);
         if (evIn_)
             (void) hipEventDestroy(evIn_);
         if (evOut_)
@@ -269,12 +286,15 @@ public:
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
             setBox(boxIn);
             stagePositions(src, pos);
+            tb(kPhList);
             checkList();
+            te();
             if (s_.precision == PGM_MIXED)
                 run<float>(newStep, fHost || fDev || fixed || keepForces, strainOut != nullptr);
             else
                 run<double>(newStep, fHost || fDev || fixed || keepForces, strainOut != nullptr);
             // forces out, caller's order
+            tb(kPhFinal);
             if (fDev || fixed)
                 launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
                                 fcovD_.as<double>(), s_.ke, fDev, fixed, stride, fixedScale);
@@ -286,8 +306,10 @@ public:
                 stageH_.reserve(3*sizeof(double)*n_);
                 HIPCHK(hipMemcpyAsync(stageH_.p, forcesOutD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost, st_));
             }
+            te();
             HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, kResCount*sizeof(double), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
+            collectTiming();
             assemble(strainOut != nullptr);
             if (fHost) {
                 const double* f = stageH_.as<double>();
@@ -310,6 +332,20 @@ public:
     double residual() const { return residual_; }
     long listBuilds() const { return listBuilds_; }
     long long numCandidates() const { return nCand_; }
+    // Pairs within the cutoff at the last evaluation (each counted once).
+    long long numPairs() {
+        if (!haveResults_)
+            return 0;
+        long long total = 0;
+        withDevice([&] {
+            std::vector<int> nb(n_);
+            HIPCHK(hipMemcpyAsync(nb.data(), nbD_.p, n_*sizeof(int), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            for (int v : nb)
+                total += v;
+        });
+        return total/2;
+    }
     long lateListChecks() const { return 0; }
     const std::vector<double>& forces() const { return forcesH_; }
 
@@ -350,6 +386,23 @@ public:
     }
 
     void freezeHistory(bool f) { frozen_ = f; }
+
+    void setTiming(bool on) {
+        timing_ = on;
+        resetTiming();
+    }
+    void resetTiming() {
+        for (int k = 0; k < kNumPhases; k++) {
+            phaseMs_[k] = 0;
+            phaseCount_[k] = 0;
+        }
+    }
+    void timing(double* ms, long* counts) const {
+        for (int k = 0; k < kNumPhases; k++) {
+            ms[k] = phaseMs_[k];
+            counts[k] = phaseCount_[k];
+        }
+    }
     const pgm_terms& lastTerms() const { return terms_; }
 
 private:
@@ -380,6 +433,60 @@ private:
     bool haveResults_ = false;
     pgm_terms terms_{};
     double strain_[9] = {0};
+    // kernel timing
+    struct Timed {
+        int phase, iteration;
+        hipEvent_t a, b;
+    };
+    bool timing_ = false;
+    std::vector<hipEvent_t> evPool_;
+    size_t evUsed_ = 0;
+    std::vector<Timed> timed_;
+    int openPhase_ = -1, openIter_ = -1;
+    hipEvent_t openEv_ = nullptr;
+    double phaseMs_[kNumPhases] = {0};
+    long phaseCount_[kNumPhases] = {0};
+
+    hipEvent_t nextEvent() {
+        if (evUsed_ == evPool_.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            evPool_.push_back(e);
+        }
+        return evPool_[evUsed_++];
+    }
+    // Brackets the kernels of one phase (iteration: CG iteration index, -1 outside the CG loop).
+    void tb(int phase, int iteration = -1) {
+        if (!timing_)
+            return;
+        openPhase_ = phase;
+        openIter_ = iteration;
+        openEv_ = nextEvent();
+        HIPCHK(hipEventRecord(openEv_, st_));
+    }
+    void te() {
+        if (!timing_ || openPhase_ < 0)
+            return;
+        hipEvent_t b = nextEvent();
+        HIPCHK(hipEventRecord(b, st_));
+        timed_.push_back(Timed{openPhase_, openIter_, openEv_, b});
+        openPhase_ = -1;
+    }
+    // After the evaluation's final synchronization: accumulate the launches that did work.
+    void collectTiming() {
+        if (!timing_)
+            return;
+        for (const Timed& t : timed_) {
+            if (t.iteration >= iterations_)
+                continue;   // enqueued ahead, returned at once
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+            phaseMs_[t.phase] += ms;
+            phaseCount_[t.phase]++;
+        }
+        timed_.clear();
+        evUsed_ = 0;
+    }
     // device buffers
     DevBuf qD_, rgD_, alphaD_, gidxD_, posInD_, posBuildD_;
     const double* posDev_ = nullptr;   // the FP64 positions (caller's order) of this evaluation
@@ -698,27 +805,35 @@ private:
 
     template<class T>
     void pmeSolve(const typename Vec4<T>::type* dip, bool withQ, const typename Vec4<T>::type* posq, int energyMode,
-                  bool strain, double* field, double* force, const int* skip, double* res) {
+                  bool strain, double* field, double* force, const int* skip, double* res, int it = -1) {
         typedef typename Vec2<T>::type T2;
         const BoxT<double> bd = makeBox<double>(H_, recip_);
         const size_t nreal = (size_t) pme_.K[0]*pme_.K[1]*pme_.K[2];
+        tb(kPhSpread, it);
         launchZero<T>(st_, gridD_.as<T>(), nreal, skip);
         launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
                         posq, withQ ? 1 : 0, bd, gridD_.as<T>(), skip);
+        te();
+        tb(kPhFft, it);
         const int nb = launchPmeConvolve<T>(st_, pme_, gridD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(),
                                             bd, s_.beta, energyMode, strain ? 1 : 0, partD_.as<double>(), skip);
         if (energyMode)
             launchReducePartials(st_, partD_.as<double>(), nb, 10, res);
+        te();
+        tb(kPhGather, it);
         launchGather<T>(st_, pme_, n_, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), gridD_.as<T>(), bd, dip,
                         posq, force ? 1 : 0, field, force, skip);
+        te();
     }
 
     template<class T>
     void matvec(const typename Vec4<T>::type* posT, const typename Vec4<T>::type* dipT, const BoxT<T>& bt,
-                const int* skip) {
+                const int* skip, int it = -1) {
+        tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                            g12D_.as<typename Vec2<T>::type>(), bt, tdD_.as<double>(), skip);
-        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr);
+        te();
+        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
     }
 
     template<class T>
@@ -738,6 +853,7 @@ private:
         for (DevBuf* b : {&permTD_, &pTD_, &dTD_})
             b->reserve(n*sizeof(T4));
         // sorted positions (and, for FP32, the FP32 copy and radii)
+        tb(kPhPrep);
         T4* posT;
         if (sizeof(T) == sizeof(float)) {
             posFD_.reserve(n*sizeof(float4));
@@ -777,6 +893,7 @@ private:
         launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), pme_.nbrick, nullptr);
         launchPmePlace<T>(st_, n, posDD_.as<double4>(), bd, pme_, brickOfD_.as<int>(), pslotD_.as<int>(),
                           brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
+        te();
         // partial-sum storage
         const int pairBlocks = (int) (((long long) n*16+255)/256);
         const int xBlocks = pme_.K[1]*pme_.K3h;   // upper bound of the convolution blocks
@@ -790,9 +907,11 @@ private:
         HIPCHK(hipMemsetAsync(resD_.p, 0, kResCount*sizeof(double), st_));
         // field of the charges and permanent dipoles: pair setup + PME
         const T rc2 = (T) (s_.cutoff*s_.cutoff);
+        tb(kPhPairSetup);
         launchPairSetup<T>(st_, n, posT, rgT, permTD_.as<T4>(), candStartD_.as<int>(), candD_.as<int>(),
                            nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), bt, rc2, (T) s_.beta,
                            edD_.as<double>(), nullptr);
+        te();
         pmeSolve<T>(permTD_.as<T4>(), true, posT, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
         // induced dipoles
@@ -801,14 +920,18 @@ private:
             const GuessCoef gc = guessCoefficients(newStep);
             const size_t stride = 3*(size_t) nGlobal_;
             histD_.reserve(kNHist*stride*sizeof(double));
+            tb(kPhCg);
             launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
                         alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
             launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
+            te();
             matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
+            tb(kPhCg);
             launchCgInit<T>(st_, n, alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>(), tdD_.as<double>(),
                             trD_.as<double>(), cself, s_.tol, s_.max_iter, rD_.as<double>(), zD_.as<double>(),
                             pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(),
                             doneD_.as<int>());
+            te();
             int it = 0;
             int chunk = std::max(1, lastIterations_);
             scalH_.reserve(64);
@@ -816,7 +939,8 @@ private:
                 const int stop = std::min(s_.max_iter, it+chunk);
                 for (; it < stop; it++) {
                     const int* skip = doneD_.as<int>()+it;
-                    matvec<T>(posT, pTD_.as<T4>(), bt, skip);
+                    matvec<T>(posT, pTD_.as<T4>(), bt, skip, it);
+                    tb(kPhCg, it);
                     launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
                                trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
                     launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(),
@@ -824,6 +948,7 @@ private:
                                    doneD_.as<int>());
                     launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(),
                                          pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+                    te();
                 }
                 HIPCHK(hipMemcpyAsync(scalH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
                 HIPCHK(hipMemcpyAsync(scalH_.as<char>()+sizeof(CgState), doneD_.as<int>()+it, sizeof(int),
@@ -844,9 +969,11 @@ private:
             const bool push = !frozen_ && (newStep || h == 0);
             if (push)
                 pushSlot = freeSlot();
+            tb(kPhCg);
             launchPost<T>(st_, n, s_.peek, alphaSD_.as<double>(), rD_.as<double>(), xD_.as<double>(),
                           permD_.as<double>(), orderD_.as<int>(), gidxD_.as<int>(),
                           push ? histD_.as<double>()+pushSlot*stride : nullptr, dD_.as<double>(), dTD_.as<T4>());
+            te();
         }
         else {
             HIPCHK(hipMemsetAsync(xD_.p, 0, v3, st_));
@@ -858,12 +985,15 @@ private:
         }
         // final stage
         double* res = resD_.as<double>();
+        tb(kPhPairFinal);
         const int nbPair = launchPairFinal<T>(st_, n, posT, rgT, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(),
                                               pjD_.as<int>(), bt, (T) s_.beta, fpairD_.as<double>(), edD_.as<double>(),
                                               partD_.as<double>());
         launchReducePartials(st_, partD_.as<double>(), nbPair, 10, res+kResPair);
+        te();
         pmeSolve<T>(dTD_.as<T4>(), true, posT, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr,
                     res+kResRec);
+        tb(kPhFinal);
         const int nbComb = launchFinalCombine(st_, n, dD_.as<double>(), xD_.as<double>(), alphaSD_.as<double>(),
                                               edD_.as<double>(), erD_.as<double>(), cself, etotD_.as<double>(),
                                               partD_.as<double>());
@@ -873,6 +1003,7 @@ private:
                                                revStartD_.as<int>(), revEntryD_.as<int>(), covAtomOfD_.as<int>(),
                                                etotD_.as<double>(), bd, fcovD_.as<double>(), partD_.as<double>());
         launchReducePartials(st_, partD_.as<double>(), nbCov, 9, res+kResCov);
+        te();
         if (pushSlot >= 0) {
             histSlots_.push_front(pushSlot);
             while (histSlots_.size() > 4)
@@ -1062,10 +1193,29 @@ int pgm_reset_history(pgm_engine* e) {
 
 long pgm_list_builds(const pgm_engine* e) { return (e && e->eng) ? e->eng->listBuilds() : -1; }
 long long pgm_num_candidates(const pgm_engine* e) { return (e && e->eng) ? e->eng->numCandidates() : -1; }
+long long pgm_num_pairs(pgm_engine* e) {
+    long long v = -1;
+    if (guarded(e, [&](Engine& g) { v = g.numPairs(); }) != 0)
+        return -1;
+    return v;
+}
+
 long pgm_late_list_checks(const pgm_engine* e) { return (e && e->eng) ? e->eng->lateListChecks() : -1; }
 
 int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets) {
     return guarded(e, [&](Engine& g) { g.setHistory(nset, mu_sets); });
+}
+
+int pgm_set_timing(pgm_engine* e, int on) {
+    return guarded(e, [&](Engine& g) { g.setTiming(on != 0); });
+}
+
+int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases) {
+    return guarded(e, [&](Engine& g) {
+        if (nphases < pgmhip::kNumPhases)
+            throw std::invalid_argument("pgm_hip: pgm_get_timing needs room for every phase");
+        g.timing(ms, counts);
+    });
 }
 
 int pgm_freeze_history(pgm_engine* e, int frozen) {

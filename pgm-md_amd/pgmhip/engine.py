@@ -3,7 +3,7 @@
 `Engine` mirrors pgmcuda::Engine (core/cuda/pgm_cuda.h:47-127): set_atoms, set_covalent, compute,
 compute_f32 + forces, compute_device (float4 positions, fixed-point forces), iterations, residual,
 induced/permanent dipoles, reset_history, list_builds, num_candidates.  Errors raise `PgmError` with the
-engine's message.  The library must be built (pgm-md_amd/build.py); a missing library is an error, never a
+engine's message.  The library must be built (pgm-md_amd/build_hip.py); a missing library is an error, never a
 fallback.
 """
 
@@ -61,8 +61,10 @@ EXPORTS = [
     "pgm_compute", "pgm_compute_f32", "pgm_forces", "pgm_compute_device", "pgm_launch_device", "pgm_finish_device",
     "pgm_compute_device_f64", "pgm_iterations", "pgm_residual", "pgm_induced_dipoles", "pgm_permanent_dipoles",
     "pgm_reset_history", "pgm_list_builds", "pgm_num_candidates", "pgm_late_list_checks", "pgm_set_history",
-    "pgm_freeze_history",
+    "pgm_freeze_history", "pgm_set_timing", "pgm_get_timing", "pgm_num_pairs",
 ]  # fmt: skip
+PHASES = ["list", "prep", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg", "pair_final",
+          "final"]  # fmt: skip
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:
@@ -71,7 +73,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise PgmError(f"{path} is not built: run pgm-md_amd/build.py (no CPU fallback exists)")
+        raise PgmError(f"{path} is not built: run pgm-md_amd/build_hip.py (no CPU fallback exists)")
     lib = C.CDLL(path)
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     lib.pgm_default_settings.argtypes = [C.POINTER(Settings)]
@@ -103,6 +105,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_late_list_checks.restype = C.c_long
     lib.pgm_set_history.argtypes = [vp, C.c_int, vp]
     lib.pgm_freeze_history.argtypes = [vp, C.c_int]
+    lib.pgm_set_timing.argtypes = [vp, C.c_int]
+    lib.pgm_num_pairs.argtypes = [vp]
+    lib.pgm_num_pairs.restype = C.c_longlong
+    lib.pgm_get_timing.argtypes = [vp, vp, vp, C.c_int]
     _ = dp
     _lib = lib
     return lib
@@ -244,6 +250,19 @@ class Engine:
 
     def freeze_history(self, frozen=True):
         self._check(self.lib.pgm_freeze_history(self.h, int(bool(frozen))))
+
+    def set_timing(self, on=True):
+        self._check(self.lib.pgm_set_timing(self.h, int(bool(on))))
+
+    def timing(self) -> dict:
+        """{phase: (total ms, launches counted)} since set_timing."""
+        ms = np.zeros(len(PHASES))
+        cnt = np.zeros(len(PHASES), dtype=np.int64)
+        self._check(self.lib.pgm_get_timing(self.h, _ptr(ms), _ptr(cnt), len(PHASES)))
+        return {p: (float(ms[k]), int(cnt[k])) for k, p in enumerate(PHASES)}
+
+    def num_pairs(self) -> int:
+        return self.lib.pgm_num_pairs(self.h)
 
     def list_builds(self) -> int:
         return self.lib.pgm_list_builds(self.h)
