@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libpgm_hip.so")
+EXPORTS_MAP = os.path.join(CSRC, "exports.map")
 SOURCES = ["nblist.hip", "pairs.hip", "pme.hip", "solver.hip", "engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -46,7 +47,8 @@ def build(jobs: int = 5, verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, obj_dir), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    # export the C ABI only: no C++ symbol of the library can interpose on (or be interposed by) a caller's
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, f"-Wl,--version-script={EXPORTS_MAP}", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")

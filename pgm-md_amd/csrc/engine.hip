@@ -142,6 +142,8 @@ enum {
 
 }   // namespace
 
+namespace {
+
 enum class PosSource { HostF64, HostF32, DeviceF64, DeviceF4 };
 
 // Kernel groups timed with HIP events when timing is on (pgm_set_timing).
@@ -159,9 +161,9 @@ enum Phase {
     kNumPhases
 };
 
-class Engine {
+class DeviceEngine {
 public:
-    explicit Engine(const pgm_settings& s) : s_(s) {
+    explicit DeviceEngine(const pgm_settings& s) : s_(s) {
         validate();
         withDevice([&] {
             HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -169,7 +171,7 @@ public:
             HIPCHK(hipEventCreateWithFlags(&evOut_, hipEventDisableTiming));
         });
     }
-    ~Engine() {
+    ~DeviceEngine() {
         int prev = -1;
         (void) hipGetDevice(&prev);
         if (s_.device >= 0)
@@ -1034,15 +1036,17 @@ private:
     }
 };
 
+}   // namespace
+
 }   // namespace pgmhip
 
 // ------------------------------------------------------------------------------------------------ C ABI
 
-using pgmhip::Engine;
+using pgmhip::DeviceEngine;
 using pgmhip::PosSource;
 
 struct pgm_engine {
-    std::unique_ptr<Engine> eng;
+    std::unique_ptr<DeviceEngine> eng;
 };
 
 namespace {
@@ -1091,7 +1095,7 @@ pgm_engine* pgm_create(const pgm_settings* s) {
     }
     try {
         pgm_engine* e = new pgm_engine;
-        e->eng.reset(new Engine(*s));
+        e->eng.reset(new DeviceEngine(*s));
         return e;
     }
     catch (const std::exception& ex) {
@@ -1110,16 +1114,16 @@ const char* pgm_last_error(const pgm_engine* e) {
 
 int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, const double* alpha,
                   const int* global_index, int n_global) {
-    return guarded(e, [&](Engine& g) { g.setAtoms(n, q, radius, alpha, global_index, n_global); });
+    return guarded(e, [&](DeviceEngine& g) { g.setAtoms(n, q, radius, alpha, global_index, n_global); });
 }
 
 int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength) {
-    return guarded(e, [&](Engine& g) { g.setCovalent(ncov, atom, partner, strength); });
+    return guarded(e, [&](DeviceEngine& g) { g.setCovalent(ncov, atom, partner, strength); });
 }
 
 int pgm_compute(pgm_engine* e, const double* pos, const double* box, int new_step, double* f, double* strain,
                 pgm_terms* terms) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         g.evaluate(PosSource::HostF64, pos, box, new_step != 0, f, nullptr, nullptr, 0, 0.0, strain, terms, nullptr,
                    false);
     });
@@ -1127,14 +1131,14 @@ int pgm_compute(pgm_engine* e, const double* pos, const double* box, int new_ste
 
 int pgm_compute_f32(pgm_engine* e, const float* pos, const double* box, int new_step, double* strain,
                     pgm_terms* terms) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         g.evaluate(PosSource::HostF32, pos, box, new_step != 0, nullptr, nullptr, nullptr, 0, 0.0, strain, terms,
                    nullptr, true);
     });
 }
 
 int pgm_forces(pgm_engine* e, double* out) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         const std::vector<double>& f = g.forces();
         if (f.empty())
             throw std::logic_error("pgm_hip: no forces: pgm_compute_f32 has not run");
@@ -1144,7 +1148,7 @@ int pgm_forces(pgm_engine* e, double* out) {
 
 int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                        int stride, double fixed_scale, void* stream, pgm_terms* terms) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
                    nullptr, terms, static_cast<hipStream_t>(stream), false);
     });
@@ -1152,14 +1156,14 @@ int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int 
 
 int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                       int stride, double fixed_scale, void* stream) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
                    nullptr, nullptr, static_cast<hipStream_t>(stream), false);
     });
 }
 
 int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         (void) g;
         if (terms)
             *terms = g.lastTerms();
@@ -1170,7 +1174,7 @@ int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
 
 int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,
                            double* strain, void* stream, pgm_terms* terms) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         g.evaluate(PosSource::DeviceF64, pos_dev, box, new_step != 0, nullptr, f_dev, nullptr, 0, 0.0, strain, terms,
                    static_cast<hipStream_t>(stream), false);
     });
@@ -1180,22 +1184,22 @@ int pgm_iterations(const pgm_engine* e) { return (e && e->eng) ? e->eng->iterati
 double pgm_residual(const pgm_engine* e) { return (e && e->eng) ? e->eng->residual() : -1.0; }
 
 int pgm_induced_dipoles(pgm_engine* e, double* mu) {
-    return guarded(e, [&](Engine& g) { g.dipoles(true, mu); });
+    return guarded(e, [&](DeviceEngine& g) { g.dipoles(true, mu); });
 }
 
 int pgm_permanent_dipoles(pgm_engine* e, double* p) {
-    return guarded(e, [&](Engine& g) { g.dipoles(false, p); });
+    return guarded(e, [&](DeviceEngine& g) { g.dipoles(false, p); });
 }
 
 int pgm_reset_history(pgm_engine* e) {
-    return guarded(e, [&](Engine& g) { g.resetHistory(); });
+    return guarded(e, [&](DeviceEngine& g) { g.resetHistory(); });
 }
 
 long pgm_list_builds(const pgm_engine* e) { return (e && e->eng) ? e->eng->listBuilds() : -1; }
 long long pgm_num_candidates(const pgm_engine* e) { return (e && e->eng) ? e->eng->numCandidates() : -1; }
 long long pgm_num_pairs(pgm_engine* e) {
     long long v = -1;
-    if (guarded(e, [&](Engine& g) { v = g.numPairs(); }) != 0)
+    if (guarded(e, [&](DeviceEngine& g) { v = g.numPairs(); }) != 0)
         return -1;
     return v;
 }
@@ -1203,15 +1207,15 @@ long long pgm_num_pairs(pgm_engine* e) {
 long pgm_late_list_checks(const pgm_engine* e) { return (e && e->eng) ? e->eng->lateListChecks() : -1; }
 
 int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets) {
-    return guarded(e, [&](Engine& g) { g.setHistory(nset, mu_sets); });
+    return guarded(e, [&](DeviceEngine& g) { g.setHistory(nset, mu_sets); });
 }
 
 int pgm_set_timing(pgm_engine* e, int on) {
-    return guarded(e, [&](Engine& g) { g.setTiming(on != 0); });
+    return guarded(e, [&](DeviceEngine& g) { g.setTiming(on != 0); });
 }
 
 int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases) {
-    return guarded(e, [&](Engine& g) {
+    return guarded(e, [&](DeviceEngine& g) {
         if (nphases < pgmhip::kNumPhases)
             throw std::invalid_argument("pgm_hip: pgm_get_timing needs room for every phase");
         g.timing(ms, counts);
@@ -1219,7 +1223,7 @@ int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases) {
 }
 
 int pgm_freeze_history(pgm_engine* e, int frozen) {
-    return guarded(e, [&](Engine& g) { g.freezeHistory(frozen != 0); });
+    return guarded(e, [&](DeviceEngine& g) { g.freezeHistory(frozen != 0); });
 }
 
 }   // extern "C"

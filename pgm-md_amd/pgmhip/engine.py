@@ -74,6 +74,13 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         return _lib
     if not os.path.exists(path):
         raise PgmError(f"{path} is not built: run pgm-md_amd/build_hip.py (no CPU fallback exists)")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7.  Loaded first, it serves this
+    # library too (same soname); loaded after ROCm's copy, torch's HIP layer would bind to the wrong runtime
+    # and find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     lib.pgm_default_settings.argtypes = [C.POINTER(Settings)]
@@ -192,7 +199,9 @@ class Engine:
 
     def compute(self, pos, box, new_step=True, forces=True, strain=False):
         """Return (terms dict, forces (n,3) or None, strain (3,3) or None)."""
-        pos = _f64(pos, (self.n, 3))
+        pos = _f64(pos).reshape(-1, 3)
+        if self.n and len(pos) != self.n:
+            raise PgmError(f"positions for {len(pos)} atoms, engine has {self.n}")
         box = _f64(box, (3, 3))
         f = np.zeros((self.n, 3)) if forces else None
         S = np.zeros(9) if strain else None

@@ -1,0 +1,119 @@
+// The reference's C++ APIs on the GPU through include/pgm_hip.hpp, against the FP64 oracle (oracle/liborc.so):
+// random water-like molecules in the triclinic box of core/tests/test_core.cpp, pgmcore-style (key rule,
+// strain) and pgmcuda-style (computeF + forces()) calls.  Prints one line per check; exit status 0 = pass.
+// Built by __graft_entry__.build() into tests/cpp/test_facade (g++, linked to libpgm_hip.so and liborc.so).
+#include "pgm_hip.hpp"
+#include "pgm_oracle.h"
+
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+using pgmhip::core::V3;
+
+int main() {
+    const V3 box[3] = {{2.0, 0, 0}, {-0.5, 1.9, 0}, {-0.4, -0.7, 1.8}};
+    std::mt19937 rng(11);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<V3> pos;
+    pgmhip::core::Topology top;
+    for (int m = 0; m < 24; m++) {
+        const double f[3] = {U(rng), U(rng), U(rng)};
+        V3 o;
+        for (int k = 0; k < 3; k++)
+            o[k] = f[0]*box[0][k]+f[1]*box[1][k]+f[2]*box[2][k];
+        const int base = (int) pos.size();
+        pos.push_back(o);
+        for (int h = 0; h < 2; h++) {
+            const double th = 2*M_PI*U(rng), ph = std::acos(2*U(rng)-1);
+            pos.push_back(V3{o[0]+0.096*std::sin(ph)*std::cos(th), o[1]+0.096*std::sin(ph)*std::sin(th),
+                             o[2]+0.096*std::cos(ph)});
+        }
+        top.q.insert(top.q.end(), {-0.8, 0.4, 0.4});
+        top.radius.insert(top.radius.end(), {0.0605, 0.0536, 0.0536});
+        top.alpha.insert(top.alpha.end(), {1.1e-3, 0.33e-3, 0.33e-3});
+        for (int h = 1; h <= 2; h++) {
+            top.covAtom.insert(top.covAtom.end(), {base, base+h});
+            top.covPartner.insert(top.covPartner.end(), {base+h, base});
+            top.covStrength.insert(top.covStrength.end(), {-0.019, 0.0086});
+        }
+    }
+    const int n = (int) pos.size();
+    pgmhip::core::Settings s;
+    s.cutoff = 0.85;
+    s.beta = 5.0;
+    s.grid[0] = s.grid[1] = s.grid[2] = 48;
+    s.tol = 1e-12;
+    s.maxIter = 1000;
+    s.peek = 0;
+    s.predictor = 0;
+    pgmhip::core::Engine gpu(top, s);
+    std::vector<V3> F(n, V3{0, 0, 0});
+    const double e = gpu.compute(pos, box, &F);
+
+    orc_settings os{1, s.cutoff, s.beta, {48, 48, 48}, 6, 1, s.tol, s.maxIter, 0.0, 0, s.ke};
+    void* o = orc_create(&os, n, top.q.data(), top.radius.data(), top.alpha.data(), (int) top.covAtom.size(),
+                         top.covAtom.data(), top.covPartner.data(), top.covStrength.data());
+    std::vector<double> fo(3*n, 0.0);
+    double eo = 0;
+    double b9[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+            b9[3*r+c] = box[r][c];
+    orc_compute(o, &pos[0][0], b9, fo.data(), &eo);
+    double so[9];
+    orc_strain(o, so);
+    double df = 0, fmax = 0, ds = 0, smax = 0;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            df = std::max(df, std::fabs(F[i][k]-fo[3*i+k]));
+            fmax = std::max(fmax, std::fabs(fo[3*i+k]));
+        }
+    for (int k = 0; k < 9; k++) {
+        ds = std::max(ds, std::fabs(gpu.strainDerivative()[k]-so[k]));
+        smax = std::max(smax, std::fabs(so[k]));
+    }
+    int fails = 0;
+    const bool okE = std::fabs(e-eo) <= 1e-11*std::fabs(eo), okF = df <= 1e-9*fmax, okS = ds <= 1e-9*smax;
+    printf("core::Engine FP64: E %.10f oracle %.10f (%s); max|dF| %.2e (%s); max|dS| %.2e (%s); %d iterations\n", e,
+           eo, okE ? "ok" : "FAIL", df, okF ? "ok" : "FAIL", ds, okS ? "ok" : "FAIL", gpu.iterations());
+    fails += !okE+!okF+!okS;
+    // same key: starts from the solution, no iterations
+    gpu.setHistoryKey(7);
+    gpu.compute(pos, box, nullptr);
+    gpu.compute(pos, box, nullptr);
+    const bool okKey = gpu.iterations() <= 1;   // starts from the converged solution of the same key
+    printf("core::Engine same history key: %d iterations (%s)\n", gpu.iterations(), okKey ? "ok" : "FAIL");
+    fails += !okKey;
+    // pgmcuda-style engine, mixed precision, float positions
+    pgmhip::Settings g;
+    g.cutoff = 0.85;
+    g.beta = 5.0;
+    g.grid[0] = g.grid[1] = g.grid[2] = 48;
+    g.tol = 1e-8;
+    g.skin = 0.0;
+    pgmhip::Engine mix(g);
+    mix.setAtoms(n, top.q.data(), top.radius.data(), top.alpha.data(), nullptr, n);
+    mix.setCovalent((int) top.covAtom.size(), top.covAtom.data(), top.covPartner.data(), top.covStrength.data());
+    std::vector<float> pf(3*n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++)
+            pf[3*i+k] = (float) pos[i][k];
+    double bb[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+            bb[r][c] = box[r][c];
+    const pgmhip::Terms t = mix.computeF(pf.data(), bb, true, nullptr);
+    double num = 0, den = 0;
+    for (int k = 0; k < 3*n; k++) {
+        num += (mix.forces()[k]-fo[k])*(mix.forces()[k]-fo[k]);
+        den += fo[k]*fo[k];
+    }
+    const double rel = std::sqrt(num/den);
+    const bool okM = rel < 5e-5 && std::fabs(t.total()/eo-1) < 1e-6;
+    printf("Engine mixed computeF: force rel-RMS %.2e, energy rel %.2e (%s)\n", rel, t.total()/eo-1, okM ? "ok" : "FAIL");
+    fails += !okM;
+    orc_destroy(o);
+    printf(fails ? "FAILED\n" : "PASSED\n");
+    return fails;
+}

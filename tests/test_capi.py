@@ -52,3 +52,14 @@ def test_null_handle_calls_fail_cleanly():
     assert lib.pgm_set_atoms(None, 1, None, None, None, None, 1) != 0
     assert lib.pgm_iterations(None) == -1
     assert lib.pgm_num_pairs(None) == -1
+
+
+def test_only_the_c_abi_is_exported():
+    """No C++ symbol leaves the library: a caller's own classes (e.g. the C++ facade's pgmhip::Engine) cannot
+    interpose on the engine's internals."""
+    import subprocess
+
+    out = subprocess.run(["nm", "-D", "--defined-only", E.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    names = [ln.split()[-1] for ln in out.splitlines() if ln.strip()]
+    exported = [n for n in names if not n.startswith("__hip") and n not in ("_init", "_fini")]
+    assert exported and all(n.startswith("pgm_") for n in exported), [n for n in exported if not n.startswith("pgm_")]
