@@ -494,7 +494,7 @@ private:
     const double* posDev_ = nullptr;   // the FP64 positions (caller's order) of this evaluation
     DevBuf covStartD_, covPartnerD_, covStrengthD_, covAtomOfD_, revStartD_, revEntryD_;
     DevBuf cellOfD_, slotD_, cellCountD_, cellStartD_, orderD_, rankD_, sortedCellD_, scratchD_;
-    DevBuf candCountD_, candStartD_, candD_, nbD_, pjD_, g12D_;
+    DevBuf candCountD_, candStartD_, candD_, nbD_, pjD_, g12D_, g03D_, subOfD_;
     DevBuf qSD_, rgSD_, alphaSD_, posDD_, posFD_;
     DevBuf permD_, permTD_, edD_, erD_, bD_, xD_, rD_, zD_, pD_, apD_, tdD_, trD_, pTD_, dD_, dTD_, etotD_;
     DevBuf fpairD_, frecD_, fcovD_, forcesOutD_, tmpD_;
@@ -733,10 +733,12 @@ private:
         rankD_.reserve(n*sizeof(int));
         sortedCellD_.reserve(n*sizeof(int));
         HIPCHK(hipMemsetAsync(cellCountD_.p, 0, (g.ncell+1)*sizeof(int), st_));
-        launchCellAssign(st_, n, posDev_, bd, g, cellOfD_.as<int>(), slotD_.as<int>(), cellCountD_.as<int>());
+        subOfD_.reserve(n*sizeof(int));
+        launchCellAssign(st_, n, posDev_, bd, g, cellOfD_.as<int>(), slotD_.as<int>(), cellCountD_.as<int>(),
+                         subOfD_.as<int>());
         launchScan(st_, cellCountD_.as<int>(), cellStartD_.as<int>(), g.ncell, nullptr);
         launchCellScatter(st_, n, cellOfD_.as<int>(), slotD_.as<int>(), cellStartD_.as<int>(), orderD_.as<int>());
-        launchCellSort(st_, g.ncell, cellStartD_.as<int>(), orderD_.as<int>());
+        launchCellSort(st_, g.ncell, cellStartD_.as<int>(), subOfD_.as<int>(), orderD_.as<int>());
         launchRank(st_, n, orderD_.as<int>(), rankD_.as<int>(), sortedCellD_.as<int>(), cellOfD_.as<int>());
         // sorted parameters
         qSD_.reserve(n*sizeof(double));
@@ -748,7 +750,7 @@ private:
         launchPermute(st_, n, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
         posDD_.reserve(n*sizeof(double4));
         launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(), nullptr);
-        // candidates
+        // candidates within rl, full lists in per-atom segments
         candCountD_.reserve(n*sizeof(int));
         candStartD_.reserve((n+1)*sizeof(int));
         launchCandCount(st_, n, posDD_.as<double4>(), sortedCellD_.as<int>(), cellStartD_.as<int>(), g, bd, rl_*rl_,
@@ -763,9 +765,11 @@ private:
             throw std::runtime_error("pgm_hip: pair list larger than 2^31 entries");
         nCand_ = total;
         const size_t cap = (size_t) total+64;
+        const size_t g2 = (s_.precision == PGM_MIXED ? sizeof(float2) : sizeof(double2));
         candD_.reserve(cap*sizeof(int));
         pjD_.reserve(cap*sizeof(int));
-        g12D_.reserve(cap*(s_.precision == PGM_MIXED ? sizeof(float2) : sizeof(double2)));
+        g12D_.reserve(cap*g2);
+        g03D_.reserve(cap*g2);
         launchCandFill(st_, n, posDD_.as<double4>(), sortedCellD_.as<int>(), cellStartD_.as<int>(), g, bd, rl_*rl_,
                        candStartD_.as<int>(), candD_.as<int>());
         posBuildD_.reserve(3*n*sizeof(double));
@@ -907,27 +911,42 @@ private:
         doneD_.reserve((s_.max_iter+2)*sizeof(int));
         nbD_.reserve(n*sizeof(int));
         HIPCHK(hipMemsetAsync(resD_.p, 0, kResCount*sizeof(double), st_));
-        // field of the charges and permanent dipoles: pair setup + PME
-        const T rc2 = (T) (s_.cutoff*s_.cutoff);
-        tb(kPhPairSetup);
-        launchPairSetup<T>(st_, n, posT, rgT, permTD_.as<T4>(), candStartD_.as<int>(), candD_.as<int>(),
-                           nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), bt, rc2, (T) s_.beta,
-                           edD_.as<double>(), nullptr);
-        te();
-        pmeSolve<T>(permTD_.as<T4>(), true, posT, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
-        launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
-        // induced dipoles
-        int pushSlot = -1;
-        if (polarizable_ && s_.polarizable) {
-            const GuessCoef gc = guessCoefficients(newStep);
-            const size_t stride = 3*(size_t) nGlobal_;
+        // the initial guess: from the history when there is one, then its first product rides on the pair setup
+        const bool induce = polarizable_ && s_.polarizable;
+        const GuessCoef gc = guessCoefficients(newStep);
+        const bool dual = induce && gc.nc > 0;
+        const size_t stride = 3*(size_t) nGlobal_;
+        if (induce)
             histD_.reserve(kNHist*stride*sizeof(double));
+        if (dual) {
             tb(kPhCg);
             launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
                         alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
             launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
             te();
-            matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
+        }
+        // field of the charges and permanent dipoles (and T x0): pair setup + PME
+        const T rc2 = (T) (s_.cutoff*s_.cutoff);
+        tb(kPhPairSetup);
+        launchPairSetup<T>(st_, n, posT, rgT, permTD_.as<T4>(), dual ? pTD_.as<T4>() : nullptr,
+                           candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
+                           g03D_.as<T2>(), bt, rc2, (T) s_.beta, edD_.as<double>(), tdD_.as<double>());
+        te();
+        pmeSolve<T>(permTD_.as<T4>(), true, posT, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
+        launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
+        // induced dipoles
+        int pushSlot = -1;
+        if (induce) {
+            if (dual)
+                pmeSolve<T>(pTD_.as<T4>(), false, posT, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
+            else {
+                tb(kPhCg);
+                launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
+                            alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
+                launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
+                te();
+                matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
+            }
             tb(kPhCg);
             launchCgInit<T>(st_, n, alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>(), tdD_.as<double>(),
                             trD_.as<double>(), cself, s_.tol, s_.max_iter, rD_.as<double>(), zD_.as<double>(),
@@ -988,8 +1007,9 @@ private:
         // final stage
         double* res = resD_.as<double>();
         tb(kPhPairFinal);
-        const int nbPair = launchPairFinal<T>(st_, n, posT, rgT, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(),
-                                              pjD_.as<int>(), bt, (T) s_.beta, fpairD_.as<double>(), edD_.as<double>(),
+        const int nbPair = launchPairFinal<T>(st_, n, posT, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(),
+                                              pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
+                                              fpairD_.as<double>(), edD_.as<double>(),
                                               partD_.as<double>());
         launchReducePartials(st_, partD_.as<double>(), nbPair, 10, res+kResPair);
         te();

@@ -19,12 +19,12 @@ struct CellGrid {
 
 // Cell of every atom (fractional cells of the wrapped position), the per-cell counts and each atom's slot.
 void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
-                      int* cellCount);
+                      int* cellCount, int* subOf);
 // Exclusive scan of m ints into out[0..m] (out[m] = total); scratch: >= 1024 ints.
 void launchScan(hipStream_t st, const int* in, int* out, int m, int* scratch);
-// Sorted order: atoms grouped by cell, ascending original index inside a cell.
+// Sorted order: atoms grouped by cell, inside a cell by (2x2x2 sub-cell, original index).
 void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order);
-void launchCellSort(hipStream_t st, int ncell, const int* cellStart, int* order);
+void launchCellSort(hipStream_t st, int ncell, const int* cellStart, const int* subOf, int* order);
 void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf);
 // Full candidate lists (both directions) of the sorted atoms within rl: counts, then the lists.
 void launchCandCount(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
@@ -52,20 +52,25 @@ void launchPermDipoles(hipStream_t st, int n, const int* order, const double* po
 
 // -------------------------------------------------------------------------------------------- pair passes
 
+// Per evaluation: every atom's candidates compacted in place to the pairs within rc (pj: partner, g12: G1, G2,
+// g03: G0, G3, in the atom's candidate segment), and the direct field of the charges and dipoles dip and, when
+// dual is given, of the dipoles dual alone (the first CG product).  nb: pairs per atom.
 template<class T>
-void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg, const typename Vec4<T>::type* dip,
-                     const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
-                     BoxT<T> box, T rc2, T beta, double* field, const int* skip);
+void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                     const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
+                     const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
+                     BoxT<T> box, T rc2, T beta, double* field, double* fieldDual);
+// The dipole part of a CG matrix-vector product from the stored G1, G2.
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxT<T> box, double* field, const int* skip);
-// Energy, forces, field and strain of the pairs at the total dipoles; per-block partials [nblk][10]
-// (energy, strain 0..8), returns the number of blocks written.
+// Energy, forces, field and strain of the pairs at the total dipoles from the stored G0..G3; per-block
+// partials [nblk][10] (energy, strain 0..8, halved), returns the number of blocks written.
 template<class T>
-int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
-                    const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj, BoxT<T> box,
-                    T beta, double* force, double* field, double* partials);
+int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
+                    const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
+                    const typename Vec2<T>::type* g03, BoxT<T> box, double* force, double* field, double* partials);
 
 // -------------------------------------------------------------------------------------------- PME
 

@@ -1,7 +1,8 @@
 // Cell list and full neighbour (candidate) lists of the pGM pair passes, plus the per-atom gathers into the
 // spatially sorted order.  Replaces the reference's cellKeys / radix sort / cellRanges / listBuild chain
-// (core/cuda/pgm_cuda.cu:445-580, 1930-2002) with a counting sort whose cells keep ascending atom order
-// (deterministic lists) and sub-wavefront candidate scans with ballot compaction.
+// (core/cuda/pgm_cuda.cu:445-580, 1930-2002) with a counting sort (atoms inside a cell ordered by 2x2x2
+// sub-cell, then atom index: deterministic, and neighbouring list entries spatially close) and sub-wavefront
+// candidate scans with ballot compaction into full per-atom lists.
 #include "kernels.hpp"
 
 namespace pgmhip {
@@ -11,25 +12,30 @@ namespace {
 constexpr int kListLanes = 16;   // lanes per atom in the candidate scans
 
 __device__ __forceinline__ void fracCell(double x, double y, double z, const BoxT<double>& b, const CellGrid& g,
-                                         int c[3]) {
+                                         int c[3], int& sub) {
+    sub = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         double t = x*b.recip[0][a]+y*b.recip[1][a]+z*b.recip[2][a];
         t -= floor(t);
-        int k = (int) (t*g.nc[a]);
-        c[a] = k < 0 ? 0 : (k >= g.nc[a] ? g.nc[a]-1 : k);
+        const double w = t*g.nc[a];
+        int k = (int) w;
+        k = k < 0 ? 0 : (k >= g.nc[a] ? g.nc[a]-1 : k);
+        c[a] = k;
+        sub |= (w-k >= 0.5 ? 1 : 0) << a;
     }
 }
 
 __global__ void kCellAssign(int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
-                            int* cellCount) {
+                            int* cellCount, int* subOf) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n)
         return;
-    int c[3];
-    fracCell(pos[3*i], pos[3*i+1], pos[3*i+2], box, g, c);
+    int c[3], sub;
+    fracCell(pos[3*i], pos[3*i+1], pos[3*i+2], box, g, c, sub);
     const int cell = (c[0]*g.nc[1]+c[1])*g.nc[2]+c[2];
     cellOf[i] = cell;
+    subOf[i] = sub;
     slot[i] = atomicAdd(&cellCount[cell], 1);
 }
 
@@ -66,16 +72,17 @@ __global__ void kCellScatter(int n, const int* cellOf, const int* slot, const in
         order[cellStart[cellOf[i]]+slot[i]] = i;
 }
 
-// Ascending atom order inside every cell (the atomic slots above are not deterministic).
-__global__ void kCellSort(int ncell, const int* cellStart, int* order) {
+// Inside every cell: ascending (sub-cell, atom index) (the atomic slots above are not deterministic).
+__global__ void kCellSort(int ncell, const int* cellStart, const int* subOf, int* order) {
     const int c = blockIdx.x*blockDim.x+threadIdx.x;
     if (c >= ncell)
         return;
     const int lo = cellStart[c], hi = cellStart[c+1];
     for (int k = lo+1; k < hi; k++) {
         const int v = order[k];
+        const long long kv = ((long long) subOf[v] << 32) | (unsigned) v;
         int j = k-1;
-        while (j >= lo && order[j] > v) {
+        while (j >= lo && (((long long) subOf[order[j]] << 32) | (unsigned) order[j]) > kv) {
             order[j+1] = order[j];
             j--;
         }
@@ -234,8 +241,8 @@ inline int blocksFor(long long threads, int bs) { return (int) ((threads+bs-1)/b
 }   // namespace
 
 void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
-                      int* cellCount) {
-    kCellAssign<<<blocksFor(n, 256), 256, 0, st>>>(n, pos, box, g, cellOf, slot, cellCount);
+                      int* cellCount, int* subOf) {
+    kCellAssign<<<blocksFor(n, 256), 256, 0, st>>>(n, pos, box, g, cellOf, slot, cellCount, subOf);
 }
 
 void launchScan(hipStream_t st, const int* in, int* out, int m, int*) {
@@ -246,8 +253,8 @@ void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot
     kCellScatter<<<blocksFor(n, 256), 256, 0, st>>>(n, cellOf, slot, cellStart, order);
 }
 
-void launchCellSort(hipStream_t st, int ncell, const int* cellStart, int* order) {
-    kCellSort<<<blocksFor(ncell, 64), 64, 0, st>>>(ncell, cellStart, order);
+void launchCellSort(hipStream_t st, int ncell, const int* cellStart, const int* subOf, int* order) {
+    kCellSort<<<blocksFor(ncell, 64), 64, 0, st>>>(ncell, cellStart, subOf, order);
 }
 
 void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf) {

@@ -1,12 +1,14 @@
-// Real-space pGM pair passes over the full neighbour lists (one group of L lanes per atom, no atomics):
-//   setup  once per evaluation: compacts the candidates within the cutoff, stores (j, G1, G2) per entry and
-//          returns the field of the charges and permanent dipoles (reference pairSetup / fieldPassDual,
-//          core/cuda/pgm_cuda.cu:168-219, 1110-1185);
-//   field  the dipole part of the CG matrix-vector product from the stored kernels (fieldPass,
+// Real-space pGM pair passes over the full neighbour lists, one group of L = 16 lanes per atom, no atomics:
+//   setup  once per evaluation: compacts the candidates within the cutoff in place, stores (j, G1, G2) and
+//          (G0, G3) per pair, returns the field of the charges and permanent dipoles and, dual, the first CG
+//          product T x0 (reference pairSetup / pairSetupDual, core/cuda/pgm_cuda.cu:168-219, 1110-1185);
+//   field  the dipole part of a CG matrix-vector product from the stored G1, G2 (fieldPass,
 //          core/cuda/pgm_cuda.cu:322-352);
-//   final  energy, forces, field and strain at the total dipoles, the kernels recomputed (finalPass,
+//   final  energy, forces, field and strain at the total dipoles from the stored G0..G3 (finalPass,
 //          core/cuda/pgm_cuda.cu:354-438; core/pgm_core.cpp:422-440).
-// T = float: FP32 pair arithmetic, FP64 accumulation per lane; T = double: FP64 throughout.
+// The field and final passes stream the list: every lane first issues the loads of U entries (index and
+// kernels, then the partners' positions and dipoles), then computes, so U independent gathers are in flight.
+// T = float: FP32 pair arithmetic with FP64 accumulation per lane; T = double: FP64 throughout.
 #include "kernels.hpp"
 
 namespace pgmhip {
@@ -16,14 +18,17 @@ namespace {
 constexpr int kPairLanes = 16;
 constexpr int kPairBlock = 256;
 
-template<class T, int L>
+template<class T, int L, bool DUAL>
 __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const T* __restrict__ rg,
                                                          const typename Vec4<T>::type* __restrict__ dip,
+                                                         const typename Vec4<T>::type* __restrict__ dual,
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ cand, int* __restrict__ nb,
                                                          int* __restrict__ pj, typename Vec2<T>::type* __restrict__ g12,
-                                                         BoxT<T> box, T rc2, T beta, double* __restrict__ field) {
+                                                         typename Vec2<T>::type* __restrict__ g03, BoxT<T> box, T rc2,
+                                                         T beta, double* __restrict__ field,
+                                                         double* __restrict__ fieldDual) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
@@ -36,13 +41,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
     const T4 pi = pos[s];
     const T rgi = rg[s];
     const int start = candStart[s], cnt = candStart[s+1]-start;
-    double ex = 0, ey = 0, ez = 0;
+    double ex = 0, ey = 0, ez = 0, fx = 0, fy = 0, fz = 0;
     int kept = 0;
     for (int base = 0; base < cnt; base += L) {
         const int k = base+lane;
         bool in = false;
         int j = 0;
-        T G[3] = {T(0), T(0), T(0)};
+        T G[4] = {T(0), T(0), T(0), T(0)};
         if (k < cnt) {
             j = cand[start+k];
             const T4 pjv = pos[j];
@@ -53,38 +58,69 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
             if (in) {
                 const T r = sqrt(r2);
                 const T a = rsqrtT(rgi+rg[j]);
-                pairKernels<T, 3>(a, beta, r, G);
+                pairKernels<T, 4>(a, beta, r, G);
                 const T4 d = dip[j];
-                const T djx = d.x*x+d.y*y+d.z*z;
-                const T c1 = pjv.w*G[1], c2 = djx*G[2];
-                ex += (double) ((c1+c2)*x-d.x*G[1]);
-                ey += (double) ((c1+c2)*y-d.y*G[1]);
-                ez += (double) ((c1+c2)*z-d.z*G[1]);
+                const T c = pjv.w*G[1]+(d.x*x+d.y*y+d.z*z)*G[2];
+                ex += (double) (c*x-d.x*G[1]);
+                ey += (double) (c*y-d.y*G[1]);
+                ez += (double) (c*z-d.z*G[1]);
+                if (DUAL) {
+                    const T4 u = dual[j];
+                    const T c2 = (u.x*x+u.y*y+u.z*z)*G[2];
+                    fx += (double) (c2*x-u.x*G[1]);
+                    fy += (double) (c2*y-u.y*G[1]);
+                    fz += (double) (c2*z-u.z*G[1]);
+                }
             }
         }
         const unsigned long long m = (__ballot(in) >> shift) & gmask;
         if (in) {
             const int idx = start+kept+__popcll(m & below);
             pj[idx] = j;
-            T2 g;
-            g.x = G[1];
-            g.y = G[2];
-            g12[idx] = g;
+            T2 v;
+            v.x = G[1];
+            v.y = G[2];
+            g12[idx] = v;
+            v.x = G[0];
+            v.y = G[3];
+            g03[idx] = v;
         }
         kept += __popcll(m);
     }
     ex = groupSum<L>(ex);
     ey = groupSum<L>(ey);
     ez = groupSum<L>(ez);
+    if (DUAL) {
+        fx = groupSum<L>(fx);
+        fy = groupSum<L>(fy);
+        fz = groupSum<L>(fz);
+    }
     if (lane == 0) {
         field[3*s] = ex;
         field[3*s+1] = ey;
         field[3*s+2] = ez;
+        if (DUAL) {
+            fieldDual[3*s] = fx;
+            fieldDual[3*s+1] = fy;
+            fieldDual[3*s+2] = fz;
+        }
         nb[s] = kept;
     }
 }
 
-template<class T, int L>
+template<class T>
+__device__ __forceinline__ void fieldTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& pj,
+                                          const typename Vec4<T>::type& d, const typename Vec2<T>::type& g,
+                                          const BoxT<T>& box, double& ex, double& ey, double& ez) {
+    T x = pi.x-pj.x, y = pi.y-pj.y, z = pi.z-pj.z;
+    minImage(x, y, z, box);
+    const T c2 = (d.x*x+d.y*y+d.z*z)*g.y;
+    ex += (double) (c2*x-d.x*g.x);
+    ey += (double) (c2*y-d.y*g.x);
+    ez += (double) (c2*z-d.z*g.x);
+}
+
+template<class T, int L, int U>
 __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const typename Vec4<T>::type* __restrict__ dip,
                                                          const int* __restrict__ candStart,
@@ -102,17 +138,28 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     const T4 pi = pos[s];
     const int start = candStart[s], cnt = nb[s];
     double ex = 0, ey = 0, ez = 0;
-    for (int k = lane; k < cnt; k += L) {
+    int k = lane;
+    for (; k+(U-1)*L < cnt; k += U*L) {
+        int j[U];
+        T2 g[U];
+        T4 pjv[U], d[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            j[u] = pj[start+k+u*L];
+            g[u] = g12[start+k+u*L];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            pjv[u] = pos[j[u]];
+            d[u] = dip[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            fieldTerm<T>(pi, pjv[u], d[u], g[u], box, ex, ey, ez);
+    }
+    for (; k < cnt; k += L) {
         const int j = pj[start+k];
-        const T2 g = g12[start+k];
-        const T4 pjv = pos[j];
-        T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-        minImage(x, y, z, box);
-        const T4 d = dip[j];
-        const T c2 = (d.x*x+d.y*y+d.z*z)*g.y;
-        ex += (double) (c2*x-d.x*g.x);
-        ey += (double) (c2*y-d.y*g.x);
-        ez += (double) (c2*z-d.z*g.x);
+        fieldTerm<T>(pi, pos[j], dip[j], g12[start+k], box, ex, ey, ez);
     }
     ex = groupSum<L>(ex);
     ey = groupSum<L>(ey);
@@ -124,15 +171,52 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     }
 }
 
-template<class T, int L>
+// One pair of the final pass at fixed total dipoles: energy, dE/dx, field at i, strain (core/pgm_core.cpp:422-440).
+template<class T>
+__device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& di,
+                                          const typename Vec4<T>::type& pjv, const typename Vec4<T>::type& dj,
+                                          const typename Vec2<T>::type& ga, const typename Vec2<T>::type& gb,
+                                          const BoxT<T>& box, double* acc, double* f, double* e) {
+    const T G0 = gb.x, G1 = ga.x, G2 = ga.y, G3 = gb.y;
+    T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+    minImage(x, y, z, box);
+    const T qi = pi.w, qj = pjv.w;
+    const T dix = di.x*x+di.y*y+di.z*z, djx = dj.x*x+dj.y*y+dj.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
+    const T qd = qi*djx-qj*dix;
+    acc[0] += (double) (qi*qj*G0+(qd+didj)*G1-dix*djx*G2);
+    const T sc = -qi*qj*G1-(qd+didj)*G2+dix*djx*G3;
+    const T gx = sc*x+(qi*dj.x-qj*di.x)*G1-(djx*di.x+dix*dj.x)*G2;
+    const T gy = sc*y+(qi*dj.y-qj*di.y)*G1-(djx*di.y+dix*dj.y)*G2;
+    const T gz = sc*z+(qi*dj.z-qj*di.z)*G1-(djx*di.z+dix*dj.z)*G2;
+    f[0] -= (double) gx;
+    f[1] -= (double) gy;
+    f[2] -= (double) gz;
+    acc[1] += (double) (gx*x);
+    acc[2] += (double) (gx*y);
+    acc[3] += (double) (gx*z);
+    acc[4] += (double) (gy*x);
+    acc[5] += (double) (gy*y);
+    acc[6] += (double) (gy*z);
+    acc[7] += (double) (gz*x);
+    acc[8] += (double) (gz*y);
+    acc[9] += (double) (gz*z);
+    const T c = qj*G1+djx*G2;
+    e[0] += (double) (c*x-dj.x*G1);
+    e[1] += (double) (c*y-dj.y*G1);
+    e[2] += (double) (c*z-dj.z*G1);
+}
+
+template<class T, int L, int U>
 __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename Vec4<T>::type* __restrict__ pos,
-                                                         const T* __restrict__ rg,
                                                          const typename Vec4<T>::type* __restrict__ dip,
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ nb, const int* __restrict__ pj,
-                                                         BoxT<T> box, T beta, double* __restrict__ force,
-                                                         double* __restrict__ field, double* __restrict__ partials) {
+                                                         const typename Vec2<T>::type* __restrict__ g12,
+                                                         const typename Vec2<T>::type* __restrict__ g03, BoxT<T> box,
+                                                         double* __restrict__ force, double* __restrict__ field,
+                                                         double* __restrict__ partials) {
     typedef typename Vec4<T>::type T4;
+    typedef typename Vec2<T>::type T2;
     __shared__ double red[kPairBlock/64][10];
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
@@ -140,57 +224,44 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
     if (s < n) {
         const T4 pi = pos[s];
         const T4 di = dip[s];
-        const T qi = pi.w, rgi = rg[s];
         const int start = candStart[s], cnt = nb[s];
-        double fx = 0, fy = 0, fz = 0, ex = 0, ey = 0, ez = 0;
-        for (int k = lane; k < cnt; k += L) {
-            const int j = pj[start+k];
-            const T4 pjv = pos[j];
-            T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-            minImage(x, y, z, box);
-            const T r = sqrt(x*x+y*y+z*z);
-            const T a = rsqrtT(rgi+rg[j]);
-            T G[4];
-            pairKernels<T, 4>(a, beta, r, G);
-            const T4 dj = dip[j];
-            const T qj = pjv.w;
-            const T dix = di.x*x+di.y*y+di.z*z, djx = dj.x*x+dj.y*y+dj.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
-            const T qd = qi*djx-qj*dix;
-            acc[0] += (double) (qi*qj*G[0]+(qd+didj)*G[1]-dix*djx*G[2]);
-            const T sc = -qi*qj*G[1]-(qd+didj)*G[2]+dix*djx*G[3];
-            const T gx = sc*x+(qi*dj.x-qj*di.x)*G[1]-(djx*di.x+dix*dj.x)*G[2];
-            const T gy = sc*y+(qi*dj.y-qj*di.y)*G[1]-(djx*di.y+dix*dj.y)*G[2];
-            const T gz = sc*z+(qi*dj.z-qj*di.z)*G[1]-(djx*di.z+dix*dj.z)*G[2];
-            fx -= (double) gx;
-            fy -= (double) gy;
-            fz -= (double) gz;
-            acc[1] += (double) (gx*x);
-            acc[2] += (double) (gx*y);
-            acc[3] += (double) (gx*z);
-            acc[4] += (double) (gy*x);
-            acc[5] += (double) (gy*y);
-            acc[6] += (double) (gy*z);
-            acc[7] += (double) (gz*x);
-            acc[8] += (double) (gz*y);
-            acc[9] += (double) (gz*z);
-            const T c = qj*G[1]+djx*G[2];
-            ex += (double) (c*x-dj.x*G[1]);
-            ey += (double) (c*y-dj.y*G[1]);
-            ez += (double) (c*z-dj.z*G[1]);
+        double f[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+        int k = lane;
+        for (; k+(U-1)*L < cnt; k += U*L) {
+            int j[U];
+            T2 ga[U], gb[U];
+            T4 pjv[U], dj[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                j[u] = pj[start+k+u*L];
+                ga[u] = g12[start+k+u*L];
+                gb[u] = g03[start+k+u*L];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                pjv[u] = pos[j[u]];
+                dj[u] = dip[j[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                finalTerm<T>(pi, di, pjv[u], dj[u], ga[u], gb[u], box, acc, f, e);
         }
-        fx = groupSum<L>(fx);
-        fy = groupSum<L>(fy);
-        fz = groupSum<L>(fz);
-        ex = groupSum<L>(ex);
-        ey = groupSum<L>(ey);
-        ez = groupSum<L>(ez);
+        for (; k < cnt; k += L) {
+            const int j = pj[start+k];
+            finalTerm<T>(pi, di, pos[j], dip[j], g12[start+k], g03[start+k], box, acc, f, e);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            f[c] = groupSum<L>(f[c]);
+            e[c] = groupSum<L>(e[c]);
+        }
         if (lane == 0) {
-            force[3*s] = fx;
-            force[3*s+1] = fy;
-            force[3*s+2] = fz;
-            field[3*s] = ex;
-            field[3*s+1] = ey;
-            field[3*s+2] = ez;
+            force[3*s] = f[0];
+            force[3*s+1] = f[1];
+            force[3*s+2] = f[2];
+            field[3*s] = e[0];
+            field[3*s+1] = e[1];
+            field[3*s+2] = e[2];
         }
     }
     // each pair is visited from both atoms: half of the sums
@@ -215,38 +286,45 @@ inline int groupsBlocks(int n, int L) { return (int) (((long long) n*L+kPairBloc
 
 template<class T>
 void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
-                     const typename Vec4<T>::type* dip, const int* candStart, const int* cand, int* nb, int* pj,
-                     typename Vec2<T>::type* g12, BoxT<T> box, T rc2, T beta, double* field, const int*) {
-    kPairSetup<T, kPairLanes><<<groupsBlocks(n, kPairLanes), kPairBlock, 0, st>>>(n, pos, rg, dip, candStart, cand, nb,
-                                                                                 pj, g12, box, rc2, beta, field);
+                     const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
+                     const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
+                     BoxT<T> box, T rc2, T beta, double* field, double* fieldDual) {
+    const int blocks = groupsBlocks(n, kPairLanes);
+    if (dual)
+        kPairSetup<T, kPairLanes, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, cand, nb, pj,
+                                                                       g12, g03, box, rc2, beta, field, fieldDual);
+    else
+        kPairSetup<T, kPairLanes, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, cand, nb, pj,
+                                                                        g12, g03, box, rc2, beta, field, nullptr);
 }
 
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxT<T> box, double* field, const int* skip) {
-    kPairField<T, kPairLanes><<<groupsBlocks(n, kPairLanes), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12,
-                                                                                 box, field, skip);
+    kPairField<T, kPairLanes, 4><<<groupsBlocks(n, kPairLanes), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj,
+                                                                                    g12, box, field, skip);
 }
 
 template<class T>
-int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
-                    const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj, BoxT<T> box,
-                    T beta, double* force, double* field, double* partials) {
+int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
+                    const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
+                    const typename Vec2<T>::type* g03, BoxT<T> box, double* force, double* field, double* partials) {
     const int blocks = groupsBlocks(n, kPairLanes);
-    kPairFinal<T, kPairLanes><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, candStart, nb, pj, box, beta, force,
-                                                            field, partials);
+    kPairFinal<T, kPairLanes, 2><<<blocks, kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12, g03, box, force,
+                                                               field, partials);
     return blocks;
 }
 
 #define PGM_INSTANTIATE_PAIRS(T)                                                                                      \
     template void launchPairSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \
-                                     const int*, const int*, int*, int*, Vec2<T>::type*, BoxT<T>, T, T, double*,      \
-                                     const int*);                                                                     \
+                                     const Vec4<T>::type*, const int*, const int*, int*, int*, Vec2<T>::type*,        \
+                                     Vec2<T>::type*, BoxT<T>, T, T, double*, double*);                                \
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
                                      const int*, const int*, const Vec2<T>::type*, BoxT<T>, double*, const int*);     \
-    template int launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,          \
-                                    const int*, const int*, const int*, BoxT<T>, T, double*, double*, double*);
+    template int launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,        \
+                                    const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxT<T>,      \
+                                    double*, double*, double*);
 PGM_INSTANTIATE_PAIRS(float)
 PGM_INSTANTIATE_PAIRS(double)
 
