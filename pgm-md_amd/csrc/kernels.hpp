@@ -72,6 +72,14 @@ int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, co
                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                     const typename Vec2<T>::type* g03, BoxT<T> box, double* force, double* field, double* partials);
 
+// Parameter derivatives of the pair energy at fixed dipoles: potential phi_i = dU/dq_i and dU/dR_i, sorted
+// order, each from the atom's own visits of its pairs.
+template<class T>
+void launchPairParam(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                     const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj,
+                     const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box, double* phi,
+                     double* dR);
+
 // -------------------------------------------------------------------------------------------- PME
 
 struct FftPlan {
@@ -113,7 +121,7 @@ int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type
 template<class T>
 void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
                   BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
-                  double* field, double* force, const int* skip);
+                  double* field, double* force, const int* skip, double* potential = nullptr);
 template<class T>
 void launchInfluence(hipStream_t st, PmeGrid g, const double* bmod, BoxT<double> box, double beta, double volume,
                      T* influence);
@@ -179,5 +187,16 @@ void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair
 void launchReducePartials(hipStream_t st, const double* partials, int nblk, int width, double* out);
 // In the caller's order: out[i] = v[rank[i]] (3-vectors).
 void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out);
+// In the sorted order: out[s] = v[order[s]] (3-vectors).
+void launchSort3(hipStream_t st, int n, const int* order, const double* v, double* out);
+// Energy derivatives at fixed dipoles per parameter type [type][4] = (charge, covalent strength, polarizability,
+// radius), kJ/mol per unit: per-block partials of the atom terms [nblk][4 ntypes]; returns the block count.
+int launchParamAtoms(hipStream_t st, int n, int ntypes, const int* order, const int* atomType, const double* q,
+                     const double* mu, const double* alpha, const double* phiDir, const double* phiRec,
+                     const double* dRdir, double beta, double qTotal, double volume, double ke, double* partials);
+// The covalent-strength terms -E_atom . u_m, per entry in the CSR order (typeOfEntry), per-block partials.
+int launchParamCov(hipStream_t st, int ncov, int ntypes, const int* covAtomOf, const int* covPartner,
+                   const int* typeOfEntry, const int* rank, const double* pos, const double* etot, BoxT<double> box,
+                   double ke, double* partials);
 
 }   // namespace pgmhip

@@ -1,7 +1,8 @@
 // Real-space pGM pair passes over the full neighbour lists, one group of L = 16 lanes per atom, no atomics:
-//   setup  once per evaluation: compacts the candidates within the cutoff in place, stores (j, G1, G2) and
-//          (G0, G3) per pair, returns the field of the charges and permanent dipoles and, dual, the first CG
-//          product T x0 (reference pairSetup / pairSetupDual, core/cuda/pgm_cuda.cu:168-219, 1110-1185);
+//   setup  once per evaluation: compacts the candidates within the cutoff (a cheap pass), then evaluates G0..G3
+//          over the compacted pairs with every lane busy, stores (j, G1, G2) and (G0, G3) per pair, and returns
+//          the field of the charges and permanent dipoles and, dual, the first CG product T x0 (reference
+//          pairSetup / pairSetupDual, core/cuda/pgm_cuda.cu:168-219, 1110-1185);
 //   field  the dipole part of a CG matrix-vector product from the stored G1, G2 (fieldPass,
 //          core/cuda/pgm_cuda.cu:322-352);
 //   final  energy, forces, field and strain at the total dipoles from the stored G0..G3 (finalPass,
@@ -18,19 +19,13 @@ namespace {
 constexpr int kPairLanes = 16;
 constexpr int kPairBlock = 256;
 
-template<class T, int L, bool DUAL>
-__global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
-                                                         const T* __restrict__ rg,
-                                                         const typename Vec4<T>::type* __restrict__ dip,
-                                                         const typename Vec4<T>::type* __restrict__ dual,
-                                                         const int* __restrict__ candStart,
-                                                         const int* __restrict__ cand, int* __restrict__ nb,
-                                                         int* __restrict__ pj, typename Vec2<T>::type* __restrict__ g12,
-                                                         typename Vec2<T>::type* __restrict__ g03, BoxT<T> box, T rc2,
-                                                         T beta, double* __restrict__ field,
-                                                         double* __restrict__ fieldDual) {
+// Compaction: every atom's candidates within rc, in place in its segment (pj), and the count (nb).
+template<class T, int L>
+__global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec4<T>::type* __restrict__ pos,
+                                                       const int* __restrict__ candStart,
+                                                       const int* __restrict__ cand, int* __restrict__ nb,
+                                                       int* __restrict__ pj, BoxT<T> box, T rc2) {
     typedef typename Vec4<T>::type T4;
-    typedef typename Vec2<T>::type T2;
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
@@ -39,72 +34,123 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
     const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
     const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << L)-1ull));
     const T4 pi = pos[s];
-    const T rgi = rg[s];
     const int start = candStart[s], cnt = candStart[s+1]-start;
-    double ex = 0, ey = 0, ez = 0, fx = 0, fy = 0, fz = 0;
     int kept = 0;
     for (int base = 0; base < cnt; base += L) {
         const int k = base+lane;
         bool in = false;
         int j = 0;
-        T G[4] = {T(0), T(0), T(0), T(0)};
         if (k < cnt) {
             j = cand[start+k];
             const T4 pjv = pos[j];
             T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
             minImage(x, y, z, box);
-            const T r2 = x*x+y*y+z*z;
-            in = r2 < rc2;
-            if (in) {
-                const T r = sqrt(r2);
-                const T a = rsqrtT(rgi+rg[j]);
-                pairKernels<T, 4>(a, beta, r, G);
-                const T4 d = dip[j];
-                const T c = pjv.w*G[1]+(d.x*x+d.y*y+d.z*z)*G[2];
-                ex += (double) (c*x-d.x*G[1]);
-                ey += (double) (c*y-d.y*G[1]);
-                ez += (double) (c*z-d.z*G[1]);
-                if (DUAL) {
-                    const T4 u = dual[j];
-                    const T c2 = (u.x*x+u.y*y+u.z*z)*G[2];
-                    fx += (double) (c2*x-u.x*G[1]);
-                    fy += (double) (c2*y-u.y*G[1]);
-                    fz += (double) (c2*z-u.z*G[1]);
-                }
-            }
+            in = x*x+y*y+z*z < rc2;
         }
         const unsigned long long m = (__ballot(in) >> shift) & gmask;
-        if (in) {
-            const int idx = start+kept+__popcll(m & below);
-            pj[idx] = j;
-            T2 v;
-            v.x = G[1];
-            v.y = G[2];
-            g12[idx] = v;
-            v.x = G[0];
-            v.y = G[3];
-            g03[idx] = v;
-        }
+        if (in)
+            pj[start+kept+__popcll(m & below)] = j;
         kept += __popcll(m);
     }
-    ex = groupSum<L>(ex);
-    ey = groupSum<L>(ey);
-    ez = groupSum<L>(ez);
+    if (lane == 0)
+        nb[s] = kept;
+}
+
+// The kernels G0..G3 of one compacted pair, the field of (q, dip) at i and, DUAL, of dual at i.
+template<class T, bool DUAL>
+__device__ __forceinline__ void setupTerm(const typename Vec4<T>::type& pi, T rgi, const typename Vec4<T>::type& pjv,
+                                          T rgj, const typename Vec4<T>::type& d, const typename Vec4<T>::type& u,
+                                          const BoxT<T>& box, T beta, typename Vec2<T>::type& ga,
+                                          typename Vec2<T>::type& gb, double* e, double* f) {
+    T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+    minImage(x, y, z, box);
+    const T r = sqrt(x*x+y*y+z*z);
+    T G[4];
+    pairKernels<T, 4>(rsqrtT(rgi+rgj), beta, r, G);
+    const T c = pjv.w*G[1]+(d.x*x+d.y*y+d.z*z)*G[2];
+    e[0] += (double) (c*x-d.x*G[1]);
+    e[1] += (double) (c*y-d.y*G[1]);
+    e[2] += (double) (c*z-d.z*G[1]);
     if (DUAL) {
-        fx = groupSum<L>(fx);
-        fy = groupSum<L>(fy);
-        fz = groupSum<L>(fz);
+        const T c2 = (u.x*x+u.y*y+u.z*z)*G[2];
+        f[0] += (double) (c2*x-u.x*G[1]);
+        f[1] += (double) (c2*y-u.y*G[1]);
+        f[2] += (double) (c2*z-u.z*G[1]);
+    }
+    ga.x = G[1];
+    ga.y = G[2];
+    gb.x = G[0];
+    gb.y = G[3];
+}
+
+// Kernels and fields over the compacted pairs: every lane busy, U entries in flight per lane.
+template<class T, int L, int U, bool DUAL>
+__global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
+                                                         const T* __restrict__ rg,
+                                                         const typename Vec4<T>::type* __restrict__ dip,
+                                                         const typename Vec4<T>::type* __restrict__ dual,
+                                                         const int* __restrict__ candStart,
+                                                         const int* __restrict__ nb, const int* __restrict__ pj,
+                                                         typename Vec2<T>::type* __restrict__ g12,
+                                                         typename Vec2<T>::type* __restrict__ g03, BoxT<T> box, T beta,
+                                                         double* __restrict__ field, double* __restrict__ fieldDual) {
+    typedef typename Vec4<T>::type T4;
+    typedef typename Vec2<T>::type T2;
+    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int s = gid/L, lane = gid%L;
+    if (s >= n)
+        return;
+    const T4 pi = pos[s];
+    const T rgi = rg[s];
+    const int start = candStart[s], cnt = nb[s];
+    double e[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    int k = lane;
+    for (; k+(U-1)*L < cnt; k += U*L) {
+        int j[U];
+        T4 pjv[U], d[U], u[U];
+        T rgj[U];
+#pragma unroll
+        for (int q = 0; q < U; q++)
+            j[q] = pj[start+k+q*L];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            pjv[q] = pos[j[q]];
+            d[q] = dip[j[q]];
+            rgj[q] = rg[j[q]];
+            if (DUAL)
+                u[q] = dual[j[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            T2 ga, gb;
+            setupTerm<T, DUAL>(pi, rgi, pjv[q], rgj[q], d[q], u[q], box, beta, ga, gb, e, f);
+            g12[start+k+q*L] = ga;
+            g03[start+k+q*L] = gb;
+        }
+    }
+    for (; k < cnt; k += L) {
+        const int j = pj[start+k];
+        T2 ga, gb;
+        const T4 u = DUAL ? dual[j] : pos[j];
+        setupTerm<T, DUAL>(pi, rgi, pos[j], rg[j], dip[j], u, box, beta, ga, gb, e, f);
+        g12[start+k] = ga;
+        g03[start+k] = gb;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        e[c] = groupSum<L>(e[c]);
+        if (DUAL)
+            f[c] = groupSum<L>(f[c]);
     }
     if (lane == 0) {
-        field[3*s] = ex;
-        field[3*s+1] = ey;
-        field[3*s+2] = ez;
+        field[3*s] = e[0];
+        field[3*s+1] = e[1];
+        field[3*s+2] = e[2];
         if (DUAL) {
-            fieldDual[3*s] = fx;
-            fieldDual[3*s+1] = fy;
-            fieldDual[3*s+2] = fz;
+            fieldDual[3*s] = f[0];
+            fieldDual[3*s+1] = f[1];
+            fieldDual[3*s+2] = f[2];
         }
-        nb[s] = kept;
     }
 }
 
@@ -280,6 +326,60 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
     }
 }
 
+// Parameter derivatives of the pair energy at fixed dipoles, per atom i over its pairs (each pair is visited from
+// both atoms; each visit takes the part that belongs to its own atom):
+//   phi_i  = sum_j q_j G0 + (d_j . x) G1                   (dU/dq_i, x = r_i - r_j)
+//   dR_i   = sum_j dU_ij/da . da/dR_i,  da/dR_i = -2 R_i a^3, a = (2 (R_i^2 + R_j^2))^-1/2
+// with dG_n/da = (2/sqrt(pi)) (2 a^2)^n exp(-a^2 r^2), the derivative of B_n[erf(a r)/r] in a (the beta part of
+// G_n does not depend on a; beyond the damping cut the a part is dropped, and so is its derivative).
+template<class T, int L>
+__global__ void __launch_bounds__(kPairBlock) kPairParam(int n, const typename Vec4<T>::type* __restrict__ pos,
+                                                         const T* __restrict__ rg,
+                                                         const typename Vec4<T>::type* __restrict__ dip,
+                                                         const int* __restrict__ candStart,
+                                                         const int* __restrict__ nb, const int* __restrict__ pj,
+                                                         const typename Vec2<T>::type* __restrict__ g12,
+                                                         const typename Vec2<T>::type* __restrict__ g03, BoxT<T> box,
+                                                         double* __restrict__ phi, double* __restrict__ dR) {
+    typedef typename Vec4<T>::type T4;
+    typedef typename Vec2<T>::type T2;
+    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int s = gid/L, lane = gid%L;
+    if (s >= n)
+        return;
+    const T4 pi = pos[s];
+    const T4 di = dip[s];
+    const T rgi = rg[s];
+    const T Ri = sqrt(T(0.5)*rgi);
+    const int start = candStart[s], cnt = nb[s];
+    double ph = 0, dr = 0;
+    for (int k = lane; k < cnt; k += L) {
+        const int j = pj[start+k];
+        const T2 ga = g12[start+k], gb = g03[start+k];
+        const T4 pjv = pos[j];
+        const T4 dj = dip[j];
+        T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+        minImage(x, y, z, box);
+        const T djx = dj.x*x+dj.y*y+dj.z*z, dix = di.x*x+di.y*y+di.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
+        ph += (double) (pjv.w*gb.x+djx*ga.x);
+        const T r2 = x*x+y*y+z*z;
+        const T a = rsqrtT(rgi+rg[j]);
+        if (a*sqrt(r2) < PairPrec<T>::kDampCut) {
+            const T a2 = a*a;
+            const T e0 = T(kTwoOverSqrtPi)*expT(-a2*r2);
+            const T e1 = T(2)*a2*e0, e2 = T(2)*a2*e1;
+            const T dUda = pi.w*pjv.w*e0+(pi.w*djx-pjv.w*dix+didj)*e1-dix*djx*e2;
+            dr += (double) (dUda*(T(-2)*Ri*a*a2));
+        }
+    }
+    ph = groupSum<L>(ph);
+    dr = groupSum<L>(dr);
+    if (lane == 0) {
+        phi[s] = ph;
+        dR[s] = dr;
+    }
+}
+
 inline int groupsBlocks(int n, int L) { return (int) (((long long) n*L+kPairBlock-1)/kPairBlock); }
 
 }   // namespace
@@ -290,12 +390,13 @@ void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                      const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
                      BoxT<T> box, T rc2, T beta, double* field, double* fieldDual) {
     const int blocks = groupsBlocks(n, kPairLanes);
+    kCompact<T, kPairLanes><<<blocks, kPairBlock, 0, st>>>(n, pos, candStart, cand, nb, pj, box, rc2);
     if (dual)
-        kPairSetup<T, kPairLanes, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, cand, nb, pj,
-                                                                       g12, g03, box, rc2, beta, field, fieldDual);
+        kPairSetup<T, kPairLanes, 2, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, nb, pj,
+                                                                          g12, g03, box, beta, field, fieldDual);
     else
-        kPairSetup<T, kPairLanes, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, cand, nb, pj,
-                                                                        g12, g03, box, rc2, beta, field, nullptr);
+        kPairSetup<T, kPairLanes, 2, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, nb, pj,
+                                                                           g12, g03, box, beta, field, nullptr);
 }
 
 template<class T>
@@ -316,6 +417,15 @@ int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, co
     return blocks;
 }
 
+template<class T>
+void launchPairParam(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                     const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj,
+                     const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box, double* phi,
+                     double* dR) {
+    kPairParam<T, kPairLanes><<<groupsBlocks(n, kPairLanes), kPairBlock, 0, st>>>(n, pos, rg, dip, candStart, nb, pj,
+                                                                                 g12, g03, box, phi, dR);
+}
+
 #define PGM_INSTANTIATE_PAIRS(T)                                                                                      \
     template void launchPairSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \
                                      const Vec4<T>::type*, const int*, const int*, int*, int*, Vec2<T>::type*,        \
@@ -324,7 +434,10 @@ int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, co
                                      const int*, const int*, const Vec2<T>::type*, BoxT<T>, double*, const int*);     \
     template int launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,        \
                                     const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxT<T>,      \
-                                    double*, double*, double*);
+                                    double*, double*, double*);                                                       \
+    template void launchPairParam<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \
+                                     const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
+                                     BoxT<T>, double*, double*);
 PGM_INSTANTIATE_PAIRS(float)
 PGM_INSTANTIATE_PAIRS(double)
 

@@ -491,7 +491,7 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
                                                   const typename Vec4<T>::type* __restrict__ dip,
                                                   const typename Vec4<T>::type* __restrict__ posq,
                                                   double* __restrict__ field, double* __restrict__ force,
-                                                  const int* skip) {
+                                                  const int* skip, double* __restrict__ potential) {
     typedef typename Vec4<T>::type T4;
     if (skip && *skip)
         return;
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
         int z = B2-j;
         g3[j] = z < 0 ? z+K2 : z;
     }
-    double f[3] = {0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
+    double f[3] = {0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0}, pot = 0;
     for (int c = lane; c < P*P; c += kGatherLanes) {
         const int j1 = c/P, j2 = c-j1*P;
         int x0 = B0-j1, x1 = B1-j2;
@@ -532,6 +532,8 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
         f[0] += (double) (u1*t2*st);
         f[1] += (double) (t1*u2*st);
         f[2] += (double) (t1*t2*su);
+        if (potential)
+            pot += (double) (t1*t2*st);
         if (FINAL) {
             const T v1 = o[2*P+j1], v2 = o[5*P+j2];
             h[0] += (double) (v1*t2*st);   // 00
@@ -545,6 +547,8 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
 #pragma unroll
     for (int a = 0; a < 3; a++)
         f[a] = groupSum<kGatherLanes>(f[a]);
+    if (potential)
+        pot = groupSum<kGatherLanes>(pot);
     if (FINAL)
 #pragma unroll
         for (int a = 0; a < 6; a++)
@@ -552,6 +556,8 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
     if (lane != 0)
         return;
     const int s = pmeAtom[k];
+    if (potential)
+        potential[s] = pot;
     double rk[3][3];
 #pragma unroll
     for (int kk = 0; kk < 3; kk++)
@@ -711,17 +717,17 @@ int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type
 template<class T>
 void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
                   BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
-                  double* field, double* force, const int* skip) {
+                  double* field, double* force, const int* skip, double* potential) {
     const int blocks = nblk((long long) n*kGatherLanes, kBlock);
     switch (g.order) {
 #define PGM_CASE(P)                                                                                                  \
     case P:                                                                                                          \
         if (final_)                                                                                                  \
             kGather<T, P, true><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,      \
-                                                           force, skip);                                             \
+                                                           force, skip, potential);                                  \
         else                                                                                                         \
             kGather<T, P, false><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,     \
-                                                            force, skip);                                            \
+                                                            force, skip, potential);                                 \
         break;
         PGM_CASE(4)
         PGM_CASE(5)
@@ -761,7 +767,8 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
     template int launchPmeConvolve<T>(hipStream_t, PmeGrid, T*, Vec2<T>::type*, const T*, const Vec2<T>::type*,        \
                                       BoxT<double>, double, int, int, double*, const int*);                            \
     template void launchGather<T>(hipStream_t, PmeGrid, int, const int*, const int*, const T*, const T*, BoxT<double>, \
-                                  const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*,       \
+                                  double*);                                                                            \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
     template void launchTwiddles<T>(hipStream_t, PmeGrid, Vec2<T>::type*);                                             \
     template void launchZero<T>(hipStream_t, T*, size_t, const int*);
