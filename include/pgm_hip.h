@@ -116,6 +116,26 @@ int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets);
  * of one workload start from the same predicted guess (benchmarks). */
 int pgm_freeze_history(pgm_engine* e, int frozen);
 
+/* No counterpart in the reference (it has no parameter derivatives; its parameters come from the prmtop
+ * sections read by openmm/python/pgm_prmtop.py:54-115): gradients of a loss L(E, F) with respect to the
+ * per-atom charges, radii and polarizabilities and the per-entry covalent strengths, for force-matching fits.
+ * Evaluates at pos (as a same-step re-evaluation: the history is not changed), then solves the adjoint
+ * induction system with the same symmetric matrix and CG as the forward solve (tolerance tol) and recomputes
+ * the PME grids.  dl_dforces: dL/dF (3n doubles, caller's order; NULL for an energy-only loss); dl_denergy:
+ * dL/dE.  Outputs are assigned (not added), caller's order: grad_q, grad_radius, grad_alpha (n doubles each),
+ * grad_cov (ncov doubles, the order of pgm_set_covalent); any may be NULL.  Units: kJ/mol per e, nm, nm^3,
+ * e nm times the units of dL/dE (per kJ/mol) and dL/dF (per kJ/mol/nm). */
+int pgm_backward(pgm_engine* e, const double* pos, const double* box, const double* dl_dforces, double dl_denergy,
+                 double* grad_q, double* grad_radius, double* grad_alpha, double* grad_cov);
+/* The same with device positions, cotangent and outputs, ordered after the work on stream (a hipStream_t;
+ * NULL: the null stream). */
+int pgm_backward_device(pgm_engine* e, const double* pos_dev, const double* box, const double* dl_dforces_dev,
+                        double dl_denergy, double* grad_q_dev, double* grad_radius_dev, double* grad_alpha_dev,
+                        double* grad_cov_dev, void* stream);
+/* CG iterations and final residual of the last adjoint solve. */
+int pgm_adjoint_iterations(const pgm_engine* e);
+double pgm_adjoint_residual(const pgm_engine* e);
+
 /* No counterpart in the reference (whose engine prints host wall clocks under PGM_CUDA_TIMING,
  * core/cuda/pgm_cuda.cu:1814-1817): HIP-event timing of the kernel groups of every evaluation, for
  * profiling.  Phases, in order: list check/rebuild, per-evaluation preparation, pair setup, pair field

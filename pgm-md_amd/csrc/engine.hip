@@ -330,6 +330,74 @@ public:
         });
     }
 
+    // Parameter gradients of a loss L(E, F) at the given positions: evaluates there (same step: the last solution
+    // as the guess), then the adjoint.  g: dL/dF (caller's order, host or device, may be null), wE: dL/dE.
+    void backward(PosSource src, const void* pos, const double* boxIn, const double* g, bool gDevice, double wE,
+                  double* gq, double* gR, double* galpha, double* gc, bool outDevice, hipStream_t caller) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: backward before setAtoms");
+        evaluate(src, pos, boxIn, false, nullptr, nullptr, nullptr, 0, 0.0, nullptr, nullptr, caller, false);
+        withDevice([&] {
+            HIPCHK(hipEventRecord(evIn_, caller));
+            HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
+            const size_t v3 = 3*sizeof(double)*n_;
+            const int ncov = std::max(covN_, 0);
+            const double* gDev = nullptr;
+            if (g && gDevice)
+                gDev = g;
+            else if (g) {
+                gInD_.reserve(v3);
+                stageH_.reserve(v3);
+                std::memcpy(stageH_.p, g, v3);
+                HIPCHK(hipMemcpyAsync(gInD_.p, stageH_.p, v3, hipMemcpyHostToDevice, st_));
+                gDev = gInD_.as<double>();
+            }
+            double *oq = gq, *oR = gR, *oa = galpha, *oc = gc;
+            if (!outDevice) {
+                gradD_.reserve((3*(size_t) n_+ncov+1)*sizeof(double));
+                double* base = gradD_.as<double>();
+                oq = gq ? base : nullptr;
+                oR = gR ? base+n_ : nullptr;
+                oa = galpha ? base+2*n_ : nullptr;
+                oc = gc ? base+3*n_ : nullptr;
+            }
+            // the adjoint's kernels stay out of the phase timing of the evaluations
+            const bool wasTiming = timing_;
+            timing_ = false;
+            try {
+                if (s_.precision == PGM_MIXED)
+                    adjoint<float>(gDev, wE, oq, oR, oa, oc);
+                else
+                    adjoint<double>(gDev, wE, oq, oR, oa, oc);
+            }
+            catch (...) {
+                timing_ = wasTiming;
+                throw;
+            }
+            timing_ = wasTiming;
+            HIPCHK(hipStreamSynchronize(st_));
+            if (!outDevice) {
+                const size_t bytes = (3*(size_t) n_+ncov)*sizeof(double);
+                stageH_.reserve(bytes);
+                HIPCHK(hipMemcpyAsync(stageH_.p, gradD_.p, bytes, hipMemcpyDeviceToHost, st_));
+                HIPCHK(hipStreamSynchronize(st_));
+                const double* h = stageH_.as<double>();
+                if (gq)
+                    std::copy(h, h+n_, gq);
+                if (gR)
+                    std::copy(h+n_, h+2*n_, gR);
+                if (galpha)
+                    std::copy(h+2*n_, h+3*n_, galpha);
+                if (gc)
+                    std::copy(h+3*n_, h+3*n_+ncov, gc);
+            }
+            HIPCHK(hipEventRecord(evOut_, st_));
+            HIPCHK(hipStreamWaitEvent(caller, evOut_, 0));
+        });
+    }
+    int adjointIterations() const { return adjIterations_; }
+    double adjointResidual() const { return adjResidual_; }
+
     int iterations() const { return iterations_; }
     double residual() const { return residual_; }
     long listBuilds() const { return listBuilds_; }
@@ -492,7 +560,11 @@ private:
     // device buffers
     DevBuf qD_, rgD_, alphaD_, gidxD_, posInD_, posBuildD_;
     const double* posDev_ = nullptr;   // the FP64 positions (caller's order) of this evaluation
-    DevBuf covStartD_, covPartnerD_, covStrengthD_, covAtomOfD_, revStartD_, revEntryD_;
+    DevBuf covStartD_, covPartnerD_, covStrengthD_, covAtomOfD_, revStartD_, revEntryD_, covInputD_;
+    // parameter gradients
+    DevBuf pairOutD_, pmeOutD_, gridTD_, gSD_, gTD_, pdotD_, lamD_, adjBD_, ddotD_, ddotTD_, etotDotD_, gInD_, gradD_;
+    int adjIterations_ = 0, lastAdjIterations_ = 0;
+    double adjResidual_ = 0;
     DevBuf cellOfD_, slotD_, cellCountD_, cellStartD_, orderD_, rankD_, sortedCellD_, scratchD_;
     DevBuf candCountD_, candStartD_, candD_, nbD_, pjD_, g12D_, g03D_, subOfD_;
     DevBuf qSD_, rgSD_, alphaSD_, posDD_, posFD_;
@@ -549,13 +621,14 @@ private:
         std::vector<int> fill(start.begin(), start.end()-1), rfill(rstart.begin(), rstart.end()-1);
         std::vector<int> part(ncov), atomOf(ncov), rev(ncov);
         std::vector<double> str(ncov);
-        std::vector<int> entryOf(ncov);
+        std::vector<int> entryOf(ncov), inputOf(ncov);
         for (int m = 0; m < ncov; m++) {
             const int k = fill[atom[m]]++;
             part[k] = partner[m];
             str[k] = strength[m];
             atomOf[k] = atom[m];
             entryOf[m] = k;
+            inputOf[k] = m;
         }
         for (int m = 0; m < ncov; m++)
             rev[rfill[partner[m]]++] = entryOf[m];
@@ -565,6 +638,7 @@ private:
         upload(covAtomOfD_, atomOf.data(), ncov*sizeof(int));
         upload(revStartD_, rstart.data(), (n_+1)*sizeof(int));
         upload(revEntryD_, rev.data(), ncov*sizeof(int));
+        upload(covInputD_, inputOf.data(), ncov*sizeof(int));
         covN_ = ncov;
         haveResults_ = false;
     }
@@ -842,6 +916,54 @@ private:
         pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
     }
 
+    // Jacobi-preconditioned CG on the polarizable rows of (alpha^-1 - T) x = b from the guess x, whose product
+    // T x is in tdD_ / trD_.  Iterations are enqueued ahead in chunks of the last solve's count, each behind a
+    // device flag, and the host reads the state once per chunk.  Returns the iterations; err: the final residual.
+    template<class T>
+    int cgSolve(const typename Vec4<T>::type* posT, const BoxT<T>& bt, const double* b, double* x, int& lastIts,
+                double& err) {
+        typedef typename Vec4<T>::type T4;
+        const int n = n_;
+        const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        tb(kPhCg);
+        launchCgInit<T>(st_, n, alphaSD_.as<double>(), b, x, tdD_.as<double>(), trD_.as<double>(), cself, s_.tol,
+                        s_.max_iter, rD_.as<double>(), zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(),
+                        cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+        te();
+        int it = 0;
+        int chunk = std::max(1, lastIts);
+        scalH_.reserve(64);
+        while (true) {
+            const int stop = std::min(s_.max_iter, it+chunk);
+            for (; it < stop; it++) {
+                const int* skip = doneD_.as<int>()+it;
+                matvec<T>(posT, pTD_.as<T4>(), bt, skip, it);
+                tb(kPhCg, it);
+                launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
+                           trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
+                launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(), x,
+                               rD_.as<double>(), zD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
+                launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(),
+                                     pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+                te();
+            }
+            HIPCHK(hipMemcpyAsync(scalH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipMemcpyAsync(scalH_.as<char>()+sizeof(CgState), doneD_.as<int>()+it, sizeof(int),
+                                  hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            int doneNow;
+            std::memcpy(&doneNow, scalH_.as<char>()+sizeof(CgState), sizeof(int));
+            if (doneNow || it >= s_.max_iter)
+                break;
+            chunk = 1;
+        }
+        CgState cs;
+        std::memcpy(&cs, scalH_.p, sizeof(CgState));
+        lastIts = cs.iterations;
+        err = cs.err;
+        return cs.iterations;
+    }
+
     template<class T>
     void run(bool newStep, bool wantForces, bool wantStrain) {
         typedef typename Vec4<T>::type T4;
@@ -947,45 +1069,7 @@ private:
                 te();
                 matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
             }
-            tb(kPhCg);
-            launchCgInit<T>(st_, n, alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>(), tdD_.as<double>(),
-                            trD_.as<double>(), cself, s_.tol, s_.max_iter, rD_.as<double>(), zD_.as<double>(),
-                            pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(),
-                            doneD_.as<int>());
-            te();
-            int it = 0;
-            int chunk = std::max(1, lastIterations_);
-            scalH_.reserve(64);
-            while (true) {
-                const int stop = std::min(s_.max_iter, it+chunk);
-                for (; it < stop; it++) {
-                    const int* skip = doneD_.as<int>()+it;
-                    matvec<T>(posT, pTD_.as<T4>(), bt, skip, it);
-                    tb(kPhCg, it);
-                    launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
-                               trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
-                    launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(),
-                                   xD_.as<double>(), rD_.as<double>(), zD_.as<double>(), cgPartD_.as<double>(),
-                                   doneD_.as<int>());
-                    launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(),
-                                         pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
-                    te();
-                }
-                HIPCHK(hipMemcpyAsync(scalH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
-                HIPCHK(hipMemcpyAsync(scalH_.as<char>()+sizeof(CgState), doneD_.as<int>()+it, sizeof(int),
-                                      hipMemcpyDeviceToHost, st_));
-                HIPCHK(hipStreamSynchronize(st_));
-                int doneNow;
-                std::memcpy(&doneNow, scalH_.as<char>()+sizeof(CgState), sizeof(int));
-                if (doneNow || it >= s_.max_iter)
-                    break;
-                chunk = 1;
-            }
-            CgState cs;
-            std::memcpy(&cs, scalH_.p, sizeof(CgState));
-            iterations_ = cs.iterations;
-            residual_ = cs.err;
-            lastIterations_ = iterations_;
+            iterations_ = cgSolve<T>(posT, bt, bD_.as<double>(), xD_.as<double>(), lastIterations_, residual_);
             const int h = (int) histSlots_.size();
             const bool push = !frozen_ && (newStep || h == 0);
             if (push)
@@ -1032,6 +1116,110 @@ private:
                 histSlots_.pop_back();
         }
         haveResults_ = true;
+    }
+
+    // Parameter gradients after an evaluation at the same positions (the sorted state, lists and stored kernels
+    // of that evaluation are reused; the PME grids are recomputed).  gCaller: dL/dF (caller's order, device), or
+    // null for the energy gradient alone; wE: dL/dE.  Outputs in the caller's order (device, each may be null):
+    // gq, gR, galpha per atom, gc per covalent entry in input order.
+    template<class T>
+    void adjoint(const double* gCaller, double wE, double* gq, double* gR, double* galpha, double* gc) {
+        typedef typename Vec4<T>::type T4;
+        typedef typename Vec2<T>::type T2;
+        const int n = n_;
+        const size_t v3 = 3*sizeof(double)*n;
+        const BoxT<double> bd = makeBox<double>(H_, recip_);
+        const BoxT<T> bt = makeBox<T>(H_, recip_);
+        const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        const bool tan = gCaller != nullptr;
+        const bool induce = polarizable_ && s_.polarizable;
+        T4* posT = reinterpret_cast<T4*>(sizeof(T) == sizeof(float) ? posFD_.p : posDD_.p);
+        const T* rgT = sizeof(T) == sizeof(float) ? scratchD_.as<T>() : rgSD_.as<T>();
+        const size_t nreal = (size_t) pme_.K[0]*pme_.K[1]*pme_.K[2];
+        pairOutD_.reserve(10*sizeof(double)*n);
+        pmeOutD_.reserve(10*sizeof(double)*n);
+        for (DevBuf* b : {&gSD_, &pdotD_, &lamD_, &adjBD_, &ddotD_, &etotDotD_})
+            b->reserve(v3);
+        for (DevBuf* b : {&gTD_, &ddotTD_})
+            b->reserve(n*sizeof(T4));
+        if (tan)
+            gridTD_.reserve(nreal*sizeof(T));
+        // the potential / field of the charges and total dipoles (and their tangents for the displacement g and
+        // the dipole change ddT): pair pass over the stored kernels, the PME grid and its tangent grid
+        auto tangentPass = [&](const T4* ddT) {
+            launchPairTangent<T>(st_, n, posT, rgT, dTD_.as<T4>(), ddT, ddT ? gTD_.as<T4>() : nullptr,
+                                 candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
+                                 g03D_.as<T2>(), bt, pairOutD_.as<double>());
+            launchZero<T>(st_, gridD_.as<T>(), nreal, nullptr);
+            launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
+                            dTD_.as<T4>(), posT, 1, bd, gridD_.as<T>(), nullptr);
+            launchPmeConvolve<T>(st_, pme_, gridD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), bd,
+                                 s_.beta, 0, 0, partD_.as<double>(), nullptr);
+            if (ddT) {
+                launchZero<T>(st_, gridTD_.as<T>(), nreal, nullptr);
+                launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(),
+                                splD_.as<T>(), dTD_.as<T4>(), posT, 1, bd, gridTD_.as<T>(), nullptr, ddT,
+                                gTD_.as<T4>());
+                launchPmeConvolve<T>(st_, pme_, gridTD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), bd,
+                                     s_.beta, 0, 0, partD_.as<double>(), nullptr);
+            }
+            launchGatherTangent<T>(st_, pme_, n, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
+                                   gridD_.as<T>(), ddT ? gridTD_.as<T>() : nullptr, bd,
+                                   ddT ? gTD_.as<T4>() : nullptr, pmeOutD_.as<double>());
+        };
+        adjIterations_ = 0;
+        adjResidual_ = 0;
+        const double* lambda = nullptr;
+        if (tan) {
+            launchSort3(st_, n, orderD_.as<int>(), gCaller, gSD_.as<double>());
+            launchToVec4<T>(st_, n, gSD_.as<double>(), gTD_.as<T4>());
+            launchPermTangent(st_, n, orderD_.as<int>(), posDev_, gCaller, covStartD_.as<int>(),
+                              covPartnerD_.as<int>(), covStrengthD_.as<double>(), bd, pdotD_.as<double>());
+            launchDipoleTangent<T>(st_, n, pdotD_.as<double>(), nullptr, ddotD_.as<double>(), ddotTD_.as<T4>());
+            if (induce) {
+                // lambda = (alpha^-1 - T)^-1 (-E'(g)) at fixed induced dipoles
+                tangentPass(ddotTD_.as<T4>());
+                launchAdjointRhs(st_, n, alphaSD_.as<double>(), pairOutD_.as<double>(), pmeOutD_.as<double>(),
+                                 pdotD_.as<double>(), cself, adjBD_.as<double>());
+                launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), nullptr, 0, GuessCoef{},
+                            alphaSD_.as<double>(), adjBD_.as<double>(), lamD_.as<double>());
+                launchToVec4<T>(st_, n, lamD_.as<double>(), pTD_.as<T4>());
+                matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
+                adjIterations_ = cgSolve<T>(posT, bt, adjBD_.as<double>(), lamD_.as<double>(), lastAdjIterations_,
+                                            adjResidual_);
+                lambda = lamD_.as<double>();
+                launchDipoleTangent<T>(st_, n, pdotD_.as<double>(), lambda, ddotD_.as<double>(), ddotTD_.as<T4>());
+            }
+            tangentPass(ddotTD_.as<T4>());
+        }
+        else
+            tangentPass(nullptr);
+        ParamAtomArgs a{};
+        a.n = n;
+        a.order = orderD_.as<int>();
+        a.q = qSD_.as<double>();
+        a.alpha = alphaSD_.as<double>();
+        a.mu = xD_.as<double>();
+        a.lambda = lambda;
+        a.d = dD_.as<double>();
+        a.ddot = tan ? ddotD_.as<double>() : nullptr;
+        a.pairOut = pairOutD_.as<double>();
+        a.pmeOut = pmeOutD_.as<double>();
+        a.ke = s_.ke;
+        a.wE = wE;
+        a.cself = cself;
+        a.qSelf = 2.0*s_.beta/kSqrtPi;
+        a.qBackground = kPi*Qtot_/(volume_*s_.beta*s_.beta);
+        a.gq = gq;
+        a.galpha = galpha;
+        a.gR = gR;
+        a.etot = etotD_.as<double>();
+        a.etotDot = etotDotD_.as<double>();
+        launchParamAtoms(st_, a);
+        if (gc)
+            launchParamCov(st_, covN_, covAtomOfD_.as<int>(), covPartnerD_.as<int>(), covInputD_.as<int>(),
+                           rankD_.as<int>(), posDev_, gCaller, etotD_.as<double>(), etotDotD_.as<double>(), bd, s_.ke,
+                           wE, gc);
     }
 
     void assemble(bool wantStrain) {
@@ -1229,6 +1417,26 @@ long pgm_late_list_checks(const pgm_engine* e) { return (e && e->eng) ? e->eng->
 int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets) {
     return guarded(e, [&](DeviceEngine& g) { g.setHistory(nset, mu_sets); });
 }
+
+int pgm_backward(pgm_engine* e, const double* pos, const double* box, const double* dl_dforces, double dl_denergy,
+                 double* grad_q, double* grad_radius, double* grad_alpha, double* grad_cov) {
+    return guarded(e, [&](DeviceEngine& g) {
+        g.backward(PosSource::HostF64, pos, box, dl_dforces, false, dl_denergy, grad_q, grad_radius, grad_alpha,
+                   grad_cov, false, nullptr);
+    });
+}
+
+int pgm_backward_device(pgm_engine* e, const double* pos_dev, const double* box, const double* dl_dforces_dev,
+                        double dl_denergy, double* grad_q_dev, double* grad_radius_dev, double* grad_alpha_dev,
+                        double* grad_cov_dev, void* stream) {
+    return guarded(e, [&](DeviceEngine& g) {
+        g.backward(PosSource::DeviceF64, pos_dev, box, dl_dforces_dev, true, dl_denergy, grad_q_dev, grad_radius_dev,
+                   grad_alpha_dev, grad_cov_dev, true, static_cast<hipStream_t>(stream));
+    });
+}
+
+int pgm_adjoint_iterations(const pgm_engine* e) { return (e && e->eng) ? e->eng->adjointIterations() : -1; }
+double pgm_adjoint_residual(const pgm_engine* e) { return (e && e->eng) ? e->eng->adjointResidual() : -1.0; }
 
 int pgm_set_timing(pgm_engine* e, int on) {
     return guarded(e, [&](DeviceEngine& g) { g.setTiming(on != 0); });

@@ -72,13 +72,15 @@ int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, co
                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                     const typename Vec2<T>::type* g03, BoxT<T> box, double* force, double* field, double* partials);
 
-// Parameter derivatives of the pair energy at fixed dipoles: potential phi_i = dU/dq_i and dU/dR_i, sorted
-// order, each from the atom's own visits of its pairs.
+// Parameter derivatives of the pair energy at fixed dipoles and, with dipDot (and disp) given, their tangents
+// along the displacement disp and the dipole change dipDot, sorted order, each atom from its own visits of its
+// pairs.  out (FP64): phi[n], E[3n], dU/dR[n], then with tangents phi'[n], E'[3n], (dU/dR)'[n].
 template<class T>
-void launchPairParam(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
-                     const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj,
-                     const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box, double* phi,
-                     double* dR);
+void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                       const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dipDot,
+                       const typename Vec4<T>::type* disp, const int* candStart, const int* nb, const int* pj,
+                       const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box,
+                       double* out);
 
 // -------------------------------------------------------------------------------------------- PME
 
@@ -110,7 +112,8 @@ void launchPmePlace(hipStream_t st, int n, const double4* posD, BoxT<double> box
 template<class T>
 void launchSpread(hipStream_t st, PmeGrid g, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
                   const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, BoxT<double> box,
-                  T* grid, const int* skip);
+                  T* grid, const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
+                  const typename Vec4<T>::type* disp = nullptr);
 // Forward transform, energy (and k-space strain) with the influence function, convolution, back transform.
 // partials: per-block [nblk][10] (energy, strain 0..8) when energy != 0; returns the number of blocks.
 template<class T>
@@ -121,7 +124,13 @@ int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type
 template<class T>
 void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
                   BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
-                  double* field, double* force, const int* skip, double* potential = nullptr);
+                  double* field, double* force, const int* skip);
+// Potential and field of the convolved grid phi at the atoms and, with phiDot, their tangents for atoms displaced
+// by disp: out (sorted order) phi[n], E[3n] at offsets 0 and n, phi'[n], E'[3n] at offsets 5n and 6n.
+template<class T>
+void launchGatherTangent(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl,
+                         const T* phi, const T* phiDot, BoxT<double> box, const typename Vec4<T>::type* disp,
+                         double* out);
 template<class T>
 void launchInfluence(hipStream_t st, PmeGrid g, const double* bmod, BoxT<double> box, double beta, double volume,
                      T* influence);
@@ -187,16 +196,39 @@ void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair
 void launchReducePartials(hipStream_t st, const double* partials, int nblk, int width, double* out);
 // In the caller's order: out[i] = v[rank[i]] (3-vectors).
 void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out);
+
+// -------------------------------------------------------------------------------------------- parameter gradients
+
 // In the sorted order: out[s] = v[order[s]] (3-vectors).
 void launchSort3(hipStream_t st, int n, const int* order, const double* v, double* out);
-// Energy derivatives at fixed dipoles per parameter type [type][4] = (charge, covalent strength, polarizability,
-// radius), kJ/mol per unit: per-block partials of the atom terms [nblk][4 ntypes]; returns the block count.
-int launchParamAtoms(hipStream_t st, int n, int ntypes, const int* order, const int* atomType, const double* q,
-                     const double* mu, const double* alpha, const double* phiDir, const double* phiRec,
-                     const double* dRdir, double beta, double qTotal, double volume, double ke, double* partials);
-// The covalent-strength terms -E_atom . u_m, per entry in the CSR order (typeOfEntry), per-block partials.
-int launchParamCov(hipStream_t st, int ncov, int ntypes, const int* covAtomOf, const int* covPartner,
-                   const int* typeOfEntry, const int* rank, const double* pos, const double* etot, BoxT<double> box,
-                   double ke, double* partials);
+// Change of the sorted permanent dipoles when the atoms move by disp (caller's order, FP64 xyz).
+void launchPermTangent(hipStream_t st, int n, const int* order, const double* pos, const double* disp,
+                       const int* covStart, const int* covPartner, const double* covStrength, BoxT<double> box,
+                       double* pdot);
+// b = -(E'_pair + E'_rec + cself p') on polarizable rows (pair / PME tangent outputs, E' at offset 6n).
+void launchAdjointRhs(hipStream_t st, int n, const double* alpha, const double* pairOut, const double* pmeOut,
+                      const double* pdot, double cself, double* b);
+// d' = p' - lambda (lambda may be null), FP64 and the pass vector type.
+template<class T>
+void launchDipoleTangent(hipStream_t st, int n, const double* pdot, const double* lambda, double* ddot,
+                         typename Vec4<T>::type* ddotT);
+
+struct ParamAtomArgs {
+    int n;
+    const int* order;
+    const double *q, *alpha, *mu, *lambda, *d, *ddot;   // sorted; lambda, ddot null without a force cotangent
+    const double *pairOut, *pmeOut;                     // pair / PME tangent-pass outputs
+    double ke, wE, cself;
+    double qSelf;                                       // 2 beta / sqrt(pi)
+    double qBackground;                                 // pi Q / (V beta^2)
+    double *gq, *galpha, *gR;                           // caller's order (each may be null)
+    double *etot, *etotDot;                             // sorted total field and its tangent
+};
+// Per-atom charge, polarizability and radius gradients; total field (and tangent) for the covalent terms.
+void launchParamAtoms(hipStream_t st, const ParamAtomArgs& a);
+// Covalent-strength gradients per entry, written at the entry's input index.
+void launchParamCov(hipStream_t st, int ncov, const int* covAtomOf, const int* covPartner, const int* inputOf,
+                    const int* rank, const double* pos, const double* disp, const double* etot,
+                    const double* etotDot, BoxT<double> box, double ke, double wE, double* gc);
 
 }   // namespace pgmhip

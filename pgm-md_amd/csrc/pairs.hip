@@ -326,21 +326,27 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
     }
 }
 
-// Parameter derivatives of the pair energy at fixed dipoles, per atom i over its pairs (each pair is visited from
-// both atoms; each visit takes the part that belongs to its own atom):
-//   phi_i  = sum_j q_j G0 + (d_j . x) G1                   (dU/dq_i, x = r_i - r_j)
-//   dR_i   = sum_j dU_ij/da . da/dR_i,  da/dR_i = -2 R_i a^3, a = (2 (R_i^2 + R_j^2))^-1/2
-// with dG_n/da = (2/sqrt(pi)) (2 a^2)^n exp(-a^2 r^2), the derivative of B_n[erf(a r)/r] in a (the beta part of
-// G_n does not depend on a; beyond the damping cut the a part is dropped, and so is its derivative).
-template<class T, int L>
-__global__ void __launch_bounds__(kPairBlock) kPairParam(int n, const typename Vec4<T>::type* __restrict__ pos,
-                                                         const T* __restrict__ rg,
-                                                         const typename Vec4<T>::type* __restrict__ dip,
-                                                         const int* __restrict__ candStart,
-                                                         const int* __restrict__ nb, const int* __restrict__ pj,
-                                                         const typename Vec2<T>::type* __restrict__ g12,
-                                                         const typename Vec2<T>::type* __restrict__ g03, BoxT<T> box,
-                                                         double* __restrict__ phi, double* __restrict__ dR) {
+// Parameter derivatives of the pair energy at fixed dipoles d and their tangents along a displacement (x' = g_i - g_j
+// for the pair separation x = r_i - r_j) and a dipole change d', per atom i over its compacted pairs (each pair is
+// visited from both atoms; each visit takes the part that belongs to its own atom):
+//   phi_i = sum_j q_j G0 + (d_j . x) G1                                  (dU/dq_i)
+//   E_i   = sum_j q_j x G1 - d_j G1 + (d_j . x) x G2                     (field, -dU/dd_i)
+//   R_i   = sum_j dU_ij/da (-2 R_i a^3),  a = (2 (R_i^2 + R_j^2))^-1/2     (dU/dR_i)
+// with U_ij = q_i q_j G0 + (q_i d_j.x - q_j d_i.x + d_i.d_j) G1 - (d_i.x)(d_j.x) G2 and
+// dG_n/da = (2/sqrt(pi)) (2 a^2)^n exp(-a^2 r^2), the a-derivative of B_n[erf(a r)/r] (the beta part of G_n does
+// not depend on a; beyond the damping cut the a part is dropped, and its derivative with it).  The tangents use
+// dG_n = -(x . x') G_{n+1}, so G0..G3 stored by the setup pass cover phi' and E'.
+template<class T, int L, bool TAN>
+__global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename Vec4<T>::type* __restrict__ pos,
+                                                           const T* __restrict__ rg,
+                                                           const typename Vec4<T>::type* __restrict__ dip,
+                                                           const typename Vec4<T>::type* __restrict__ dipDot,
+                                                           const typename Vec4<T>::type* __restrict__ disp,
+                                                           const int* __restrict__ candStart,
+                                                           const int* __restrict__ nb, const int* __restrict__ pj,
+                                                           const typename Vec2<T>::type* __restrict__ g12,
+                                                           const typename Vec2<T>::type* __restrict__ g03,
+                                                           BoxT<T> box, double* __restrict__ out) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
@@ -349,34 +355,84 @@ __global__ void __launch_bounds__(kPairBlock) kPairParam(int n, const typename V
         return;
     const T4 pi = pos[s];
     const T4 di = dip[s];
+    T4 ddi, gi;
+    if (TAN) {
+        ddi = dipDot[s];
+        gi = disp[s];
+    }
     const T rgi = rg[s];
     const T Ri = sqrt(T(0.5)*rgi);
     const int start = candStart[s], cnt = nb[s];
-    double ph = 0, dr = 0;
+    // phi, E[3], R, phi', E'[3], R'
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = lane; k < cnt; k += L) {
         const int j = pj[start+k];
         const T2 ga = g12[start+k], gb = g03[start+k];
+        const T G0 = gb.x, G1 = ga.x, G2 = ga.y, G3 = gb.y;
         const T4 pjv = pos[j];
         const T4 dj = dip[j];
         T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
         minImage(x, y, z, box);
+        const T qj = pjv.w, qi = pi.w;
         const T djx = dj.x*x+dj.y*y+dj.z*z, dix = di.x*x+di.y*y+di.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
-        ph += (double) (pjv.w*gb.x+djx*ga.x);
+        acc[0] += (double) (qj*G0+djx*G1);
+        const T c = qj*G1+djx*G2;
+        acc[1] += (double) (c*x-dj.x*G1);
+        acc[2] += (double) (c*y-dj.y*G1);
+        acc[3] += (double) (c*z-dj.z*G1);
         const T r2 = x*x+y*y+z*z;
         const T a = rsqrtT(rgi+rg[j]);
-        if (a*sqrt(r2) < PairPrec<T>::kDampCut) {
-            const T a2 = a*a;
-            const T e0 = T(kTwoOverSqrtPi)*expT(-a2*r2);
-            const T e1 = T(2)*a2*e0, e2 = T(2)*a2*e1;
-            const T dUda = pi.w*pjv.w*e0+(pi.w*djx-pjv.w*dix+didj)*e1-dix*djx*e2;
-            dr += (double) (dUda*(T(-2)*Ri*a*a2));
+        const T a2 = a*a;
+        const bool damp = a*sqrt(r2) < PairPrec<T>::kDampCut;
+        T e0 = 0, e1 = 0, e2 = 0;
+        const T w = T(-2)*Ri*a*a2;
+        const T mixed = qi*djx-qj*dix+didj;
+        if (damp) {
+            e0 = T(kTwoOverSqrtPi)*expT(-a2*r2);
+            e1 = T(2)*a2*e0;
+            e2 = T(2)*a2*e1;
+            acc[4] += (double) ((qi*qj*e0+mixed*e1-dix*djx*e2)*w);
+        }
+        if (TAN) {
+            const T4 ddj = dipDot[j];
+            const T4 gj = disp[j];
+            const T tx = gi.x-gj.x, ty = gi.y-gj.y, tz = gi.z-gj.z;
+            const T xx = x*tx+y*ty+z*tz;
+            const T djxDot = ddj.x*x+ddj.y*y+ddj.z*z+dj.x*tx+dj.y*ty+dj.z*tz;
+            acc[5] += (double) (-qj*xx*G1+djxDot*G1-djx*xx*G2);
+            // E' = q_j (x' G1 - x xx G2) - d'_j G1 + d_j xx G2 + djx' x G2 + djx x' G2 - djx x xx G3
+            const T cx = qj*G1+djx*G2, cxx = -(qj*G2+djx*G3)*xx+djxDot*G2;
+            acc[6] += (double) (cx*tx+cxx*x-ddj.x*G1+dj.x*xx*G2);
+            acc[7] += (double) (cx*ty+cxx*y-ddj.y*G1+dj.y*xx*G2);
+            acc[8] += (double) (cx*tz+cxx*z-ddj.z*G1+dj.z*xx*G2);
+            if (damp) {
+                const T dixDot = ddi.x*x+ddi.y*y+ddi.z*z+di.x*tx+di.y*ty+di.z*tz;
+                const T didjDot = ddi.x*dj.x+ddi.y*dj.y+ddi.z*dj.z+di.x*ddj.x+di.y*ddj.y+di.z*ddj.z;
+                const T sh = T(-2)*a2*xx;   // de_n = sh e_n
+                const T mixedDot = qi*djxDot-qj*dixDot+didjDot;
+                const T v = (qi*qj*e0+mixed*e1-dix*djx*e2)*sh+mixedDot*e1-(dixDot*djx+dix*djxDot)*e2;
+                acc[9] += (double) (v*w);
+            }
         }
     }
-    ph = groupSum<L>(ph);
-    dr = groupSum<L>(dr);
+#pragma unroll
+    for (int q = 0; q < 10; q++)
+        if (TAN || q < 5)
+            acc[q] = groupSum<L>(acc[q]);
     if (lane == 0) {
-        phi[s] = ph;
-        dR[s] = dr;
+        // out: phi[n], E[3n], R[n], phi'[n], E'[3n], R'[n]
+        out[s] = acc[0];
+        out[n+3*s] = acc[1];
+        out[n+3*s+1] = acc[2];
+        out[n+3*s+2] = acc[3];
+        out[4*n+s] = acc[4];
+        if (TAN) {
+            out[5*n+s] = acc[5];
+            out[6*n+3*s] = acc[6];
+            out[6*n+3*s+1] = acc[7];
+            out[6*n+3*s+2] = acc[8];
+            out[9*n+s] = acc[9];
+        }
     }
 }
 
@@ -418,12 +474,18 @@ int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, co
 }
 
 template<class T>
-void launchPairParam(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
-                     const typename Vec4<T>::type* dip, const int* candStart, const int* nb, const int* pj,
-                     const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box, double* phi,
-                     double* dR) {
-    kPairParam<T, kPairLanes><<<groupsBlocks(n, kPairLanes), kPairBlock, 0, st>>>(n, pos, rg, dip, candStart, nb, pj,
-                                                                                 g12, g03, box, phi, dR);
+void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                       const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dipDot,
+                       const typename Vec4<T>::type* disp, const int* candStart, const int* nb, const int* pj,
+                       const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box,
+                       double* out) {
+    const int blocks = groupsBlocks(n, kPairLanes);
+    if (dipDot)
+        kPairTangent<T, kPairLanes, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dipDot, disp, candStart,
+                                                                         nb, pj, g12, g03, box, out);
+    else
+        kPairTangent<T, kPairLanes, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, pos, candStart, nb,
+                                                                          pj, g12, g03, box, out);
 }
 
 #define PGM_INSTANTIATE_PAIRS(T)                                                                                      \
@@ -435,9 +497,9 @@ void launchPairParam(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
     template int launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,        \
                                     const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxT<T>,      \
                                     double*, double*, double*);                                                       \
-    template void launchPairParam<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \
-                                     const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
-                                     BoxT<T>, double*, double*);
+    template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,       \
+                                       const Vec4<T>::type*, const Vec4<T>::type*, const int*, const int*,            \
+                                       const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxT<T>, double*);
 PGM_INSTANTIATE_PAIRS(float)
 PGM_INSTANTIATE_PAIRS(double)
 

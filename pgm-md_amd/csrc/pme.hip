@@ -208,13 +208,19 @@ __global__ void kPmePlace(int n, const double4* posD, BoxT<double> box, PmeGrid 
 
 // ---------------------------------------------------------------------------------------- spreading
 
-template<class T, int P>
+// TAN: instead of the charges and dipoles, the change of their grid density when every atom moves by g_s and its
+// dipole changes by d'_s: the density q theta(r - r_s) + d . grad_s theta moves to first order by
+// (q g + d') . grad_s theta + g^T (grad_s grad_s theta) d, i.e. a dipole q g + d' and a symmetric second moment
+// (g d^T + d g^T)/2 on the second spline derivatives.
+template<class T, int P, bool TAN>
 __global__ void __launch_bounds__(kBlock) kSpread(PmeGrid g, const int* __restrict__ brickStart,
                                                   const int* __restrict__ pmeAtom, const int* __restrict__ baseArr,
                                                   const T* __restrict__ spl,
                                                   const typename Vec4<T>::type* __restrict__ dip,
                                                   const typename Vec4<T>::type* __restrict__ posq, int withQ,
-                                                  BoxT<double> box, T* __restrict__ grid, const int* skip) {
+                                                  BoxT<double> box, T* __restrict__ grid, const int* skip,
+                                                  const typename Vec4<T>::type* __restrict__ dipDot,
+                                                  const typename Vec4<T>::type* __restrict__ disp) {
     typedef typename Vec4<T>::type T4;
     // The tile accumulates in FP64 for both precisions: ds_add_f64 is several times faster than ds_add_f32 on
     // gfx950, and the FP64 sums lose nothing.
@@ -240,28 +246,54 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeGrid g, const int* __restri
         const int k = first+it/P, j1 = it%P;
         const int s = pmeAtom[k];
         const T4 d = dip[s];
-        const T q = withQ ? posq[s].w : T(0);
-        const T e0 = d.x*rc[0][0]+d.y*rc[1][0]+d.z*rc[2][0];
-        const T e1 = d.x*rc[0][1]+d.y*rc[1][1]+d.z*rc[2][1];
-        const T e2 = d.x*rc[0][2]+d.y*rc[1][2]+d.z*rc[2][2];
+        const T q = (withQ || TAN) ? posq[s].w : T(0);
+        T e[3], D[3], S[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            e[a] = d.x*rc[0][a]+d.y*rc[1][a]+d.z*rc[2][a];
+        if (TAN) {
+            const T4 dd = dipDot[s], gs = disp[s];
+            T ga[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                ga[a] = gs.x*rc[0][a]+gs.y*rc[1][a]+gs.z*rc[2][a];
+                D[a] = q*ga[a]+dd.x*rc[0][a]+dd.y*rc[1][a]+dd.z*rc[2][a];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    S[a][b] = T(0.5)*(ga[a]*e[b]+ga[b]*e[a]);
+        }
         const T* o = spl+(size_t) k*9*P;
-        const T t1 = o[j1], u1 = o[P+j1];
+        const T t1 = o[j1], u1 = o[P+j1], v1 = o[2*P+j1];
         const int l0 = baseArr[3*k]-j1-lo0, l1b = baseArr[3*k+1]-lo1, l2b = baseArr[3*k+2]-lo2;
-        T t3[P], u3[P];
+        T t3[P], u3[P], v3[P];
 #pragma unroll
         for (int j = 0; j < P; j++) {
             t3[j] = o[6*P+j];
             u3[j] = o[7*P+j];
+            if (TAN)
+                v3[j] = o[8*P+j];
         }
 #pragma unroll
         for (int j2 = 0; j2 < P; j2++) {
             const T t2 = o[3*P+j2], u2 = o[4*P+j2];
-            const T A = (q*t1+e0*u1)*t2+e1*t1*u2;
-            const T C = e2*t1*t2;
+            T A, C, V = 0;
+            if (TAN) {
+                const T v2 = o[5*P+j2];
+                A = D[0]*u1*t2+D[1]*t1*u2+S[0][0]*v1*t2+S[1][1]*t1*v2+T(2)*S[0][1]*u1*u2;
+                C = D[2]*t1*t2+T(2)*(S[0][2]*u1*t2+S[1][2]*t1*u2);
+                V = S[2][2]*t1*t2;
+            }
+            else {
+                A = (q*t1+e[0]*u1)*t2+e[1]*t1*u2;
+                C = e[2]*t1*t2;
+            }
             double* row = tile+(l0*E+(l1b-j2))*E+l2b;
 #pragma unroll
             for (int j3 = 0; j3 < P; j3++)
-                atomicAdd(row-j3, (double) (A*t3[j3]+C*u3[j3]));
+                atomicAdd(row-j3, (double) (TAN ? A*t3[j3]+C*u3[j3]+V*v3[j3] : A*t3[j3]+C*u3[j3]));
         }
     }
     __syncthreads();
@@ -491,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
                                                   const typename Vec4<T>::type* __restrict__ dip,
                                                   const typename Vec4<T>::type* __restrict__ posq,
                                                   double* __restrict__ field, double* __restrict__ force,
-                                                  const int* skip, double* __restrict__ potential) {
+                                                  const int* skip) {
     typedef typename Vec4<T>::type T4;
     if (skip && *skip)
         return;
@@ -512,7 +544,7 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
         int z = B2-j;
         g3[j] = z < 0 ? z+K2 : z;
     }
-    double f[3] = {0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0}, pot = 0;
+    double f[3] = {0, 0, 0}, h[6] = {0, 0, 0, 0, 0, 0};
     for (int c = lane; c < P*P; c += kGatherLanes) {
         const int j1 = c/P, j2 = c-j1*P;
         int x0 = B0-j1, x1 = B1-j2;
@@ -532,8 +564,6 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
         f[0] += (double) (u1*t2*st);
         f[1] += (double) (t1*u2*st);
         f[2] += (double) (t1*t2*su);
-        if (potential)
-            pot += (double) (t1*t2*st);
         if (FINAL) {
             const T v1 = o[2*P+j1], v2 = o[5*P+j2];
             h[0] += (double) (v1*t2*st);   // 00
@@ -547,8 +577,6 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
 #pragma unroll
     for (int a = 0; a < 3; a++)
         f[a] = groupSum<kGatherLanes>(f[a]);
-    if (potential)
-        pot = groupSum<kGatherLanes>(pot);
     if (FINAL)
 #pragma unroll
         for (int a = 0; a < 6; a++)
@@ -556,8 +584,6 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
     if (lane != 0)
         return;
     const int s = pmeAtom[k];
-    if (potential)
-        potential[s] = pot;
     double rk[3][3];
 #pragma unroll
     for (int kk = 0; kk < 3; kk++)
@@ -582,6 +608,108 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
 #pragma unroll
         for (int kk = 0; kk < 3; kk++)
             force[3*s+kk] = -(rk[kk][0]*c[0]+rk[kk][1]*c[1]+rk[kk][2]*c[2]);
+    }
+}
+
+// Potential and field of the convolved grid phi at the atoms and, TAN, their tangents when every atom moves by g_s
+// and the grid changes by phiDot: phi' = g . grad phi + phiDot, E' = -(grad grad phi) g - grad phiDot.
+// out (FP64, sorted order): phi[n], E[3n] at offsets 0 and n, phi'[n], E'[3n] at offsets 5n and 6n.
+template<class T, int P, bool TAN>
+__global__ void __launch_bounds__(kBlock) kGatherTangent(PmeGrid g, int n, const int* __restrict__ pmeAtom,
+                                                         const int* __restrict__ baseArr, const T* __restrict__ spl,
+                                                         const T* __restrict__ phi, const T* __restrict__ phiDot,
+                                                         BoxT<double> box,
+                                                         const typename Vec4<T>::type* __restrict__ disp,
+                                                         double* __restrict__ out) {
+    typedef typename Vec4<T>::type T4;
+    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int k = gid/kGatherLanes, lane = gid%kGatherLanes;
+    if (k >= n)
+        return;
+    const T* o = spl+(size_t) k*9*P;
+    const int B0 = baseArr[3*k], B1 = baseArr[3*k+1], B2 = baseArr[3*k+2];
+    const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
+    T t3[P], u3[P], v3[P];
+    int g3[P];
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        t3[j] = o[6*P+j];
+        u3[j] = o[7*P+j];
+        v3[j] = o[8*P+j];
+        int z = B2-j;
+        g3[j] = z < 0 ? z+K2 : z;
+    }
+    // pot, f[3], h[6] of phi; pot', f'[3] of phiDot
+    double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = lane; c < P*P; c += kGatherLanes) {
+        const int j1 = c/P, j2 = c-j1*P;
+        int x0 = B0-j1, x1 = B1-j2;
+        x0 += (x0 < 0 ? K0 : 0);
+        x1 += (x1 < 0 ? K1 : 0);
+        const size_t rowOff = ((size_t) x0*K1+x1)*K2;
+        T st = 0, su = 0, sv = 0, dt = 0, du = 0;
+#pragma unroll
+        for (int j3 = 0; j3 < P; j3++) {
+            const T ph = phi[rowOff+g3[j3]];
+            st += t3[j3]*ph;
+            su += u3[j3]*ph;
+            if (TAN) {
+                sv += v3[j3]*ph;
+                const T pd = phiDot[rowOff+g3[j3]];
+                dt += t3[j3]*pd;
+                du += u3[j3]*pd;
+            }
+        }
+        const T t1 = o[j1], u1 = o[P+j1], v1 = o[2*P+j1], t2 = o[3*P+j2], u2 = o[4*P+j2], v2 = o[5*P+j2];
+        acc[0] += (double) (t1*t2*st);
+        acc[1] += (double) (u1*t2*st);
+        acc[2] += (double) (t1*u2*st);
+        acc[3] += (double) (t1*t2*su);
+        if (TAN) {
+            acc[4] += (double) (v1*t2*st);   // 00
+            acc[5] += (double) (t1*v2*st);   // 11
+            acc[6] += (double) (t1*t2*sv);   // 22
+            acc[7] += (double) (u1*u2*st);   // 01
+            acc[8] += (double) (u1*t2*su);   // 02
+            acc[9] += (double) (t1*u2*su);   // 12
+            acc[10] += (double) (t1*t2*dt);
+            acc[11] += (double) (u1*t2*dt);
+            acc[12] += (double) (t1*u2*dt);
+            acc[13] += (double) (t1*t2*du);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 14; a++)
+        if (TAN || a < 4)
+            acc[a] = groupSum<kGatherLanes>(acc[a]);
+    if (lane != 0)
+        return;
+    const int s = pmeAtom[k];
+    double rk[3][3];
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            rk[kk][a] = g.K[a]*box.recip[kk][a];
+    out[s] = acc[0];
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+        out[n+3*s+kk] = -(rk[kk][0]*acc[1]+rk[kk][1]*acc[2]+rk[kk][2]*acc[3]);
+    if (TAN) {
+        const T4 gs = disp[s];
+        double ga[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            ga[a] = rk[0][a]*gs.x+rk[1][a]*gs.y+rk[2][a]*gs.z;
+        const double H[3][3] = {{acc[4], acc[7], acc[8]}, {acc[7], acc[5], acc[9]}, {acc[8], acc[9], acc[6]}};
+        out[5*n+s] = ga[0]*acc[1]+ga[1]*acc[2]+ga[2]*acc[3]+acc[10];
+        double c[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            c[a] = H[a][0]*ga[0]+H[a][1]*ga[1]+H[a][2]*ga[2]+acc[11+a];
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++)
+            out[6*n+3*s+kk] = -(rk[kk][0]*c[0]+rk[kk][1]*c[1]+rk[kk][2]*c[2]);
     }
 }
 
@@ -670,14 +798,18 @@ void launchPmePlace(hipStream_t st, int n, const double4* posD, BoxT<double> box
 template<class T>
 void launchSpread(hipStream_t st, PmeGrid g, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
                   const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, BoxT<double> box,
-                  T* grid, const int* skip) {
+                  T* grid, const int* skip, const typename Vec4<T>::type* dipDot, const typename Vec4<T>::type* disp) {
     const int E = g.brick+g.order-1;
     const size_t lds = (size_t) E*E*E*sizeof(double);
     switch (g.order) {
-#define PGM_CASE(P)                                                                                           \
-    case P:                                                                                                   \
-        kSpread<T, P><<<g.nbrick, kBlock, lds, st>>>(g, brickStart, pmeAtom, base, spl, dip, posq, withQ, box, \
-                                                     grid, skip);                                             \
+#define PGM_CASE(P)                                                                                                 \
+    case P:                                                                                                         \
+        if (dipDot)                                                                                                 \
+            kSpread<T, P, true><<<g.nbrick, kBlock, lds, st>>>(g, brickStart, pmeAtom, base, spl, dip, posq, withQ, \
+                                                               box, grid, skip, dipDot, disp);                      \
+        else                                                                                                        \
+            kSpread<T, P, false><<<g.nbrick, kBlock, lds, st>>>(g, brickStart, pmeAtom, base, spl, dip, posq,       \
+                                                                withQ, box, grid, skip, dip, dip);                  \
         break;
         PGM_CASE(4)
         PGM_CASE(5)
@@ -715,19 +847,45 @@ int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type
 }
 
 template<class T>
+void launchGatherTangent(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl,
+                         const T* phi, const T* phiDot, BoxT<double> box, const typename Vec4<T>::type* disp,
+                         double* out) {
+    const int blocks = nblk((long long) n*kGatherLanes, kBlock);
+    switch (g.order) {
+#define PGM_CASE(P)                                                                                                \
+    case P:                                                                                                        \
+        if (phiDot)                                                                                                \
+            kGatherTangent<T, P, true><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, phiDot, box, disp, \
+                                                                  out);                                            \
+        else                                                                                                       \
+            kGatherTangent<T, P, false><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, phi, box, disp,  \
+                                                                   out);                                           \
+        break;
+        PGM_CASE(4)
+        PGM_CASE(5)
+        PGM_CASE(6)
+        PGM_CASE(7)
+        PGM_CASE(8)
+#undef PGM_CASE
+    default:
+        throw std::runtime_error("pgm_hip: PME order must be 4..8");
+    }
+}
+
+template<class T>
 void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
                   BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
-                  double* field, double* force, const int* skip, double* potential) {
+                  double* field, double* force, const int* skip) {
     const int blocks = nblk((long long) n*kGatherLanes, kBlock);
     switch (g.order) {
 #define PGM_CASE(P)                                                                                                  \
     case P:                                                                                                          \
         if (final_)                                                                                                  \
             kGather<T, P, true><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,      \
-                                                           force, skip, potential);                                  \
+                                                           force, skip);                                             \
         else                                                                                                         \
             kGather<T, P, false><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,     \
-                                                            force, skip, potential);                                 \
+                                                            force, skip);                                            \
         break;
         PGM_CASE(4)
         PGM_CASE(5)
@@ -763,12 +921,14 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
     template void launchPmePlace<T>(hipStream_t, int, const double4*, BoxT<double>, PmeGrid, const int*, const int*,   \
                                     const int*, int*, int*, T*);                                                       \
     template void launchSpread<T>(hipStream_t, PmeGrid, const int*, const int*, const int*, const T*,                  \
-                                  const Vec4<T>::type*, const Vec4<T>::type*, int, BoxT<double>, T*, const int*);      \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, BoxT<double>, T*, const int*,       \
+                                  const Vec4<T>::type*, const Vec4<T>::type*);                                         \
+    template void launchGatherTangent<T>(hipStream_t, PmeGrid, int, const int*, const int*, const T*, const T*,        \
+                                         const T*, BoxT<double>, const Vec4<T>::type*, double*);                       \
     template int launchPmeConvolve<T>(hipStream_t, PmeGrid, T*, Vec2<T>::type*, const T*, const Vec2<T>::type*,        \
                                       BoxT<double>, double, int, int, double*, const int*);                            \
     template void launchGather<T>(hipStream_t, PmeGrid, int, const int*, const int*, const T*, const T*, BoxT<double>, \
-                                  const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*,       \
-                                  double*);                                                                            \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
     template void launchTwiddles<T>(hipStream_t, PmeGrid, Vec2<T>::type*);                                             \
     template void launchZero<T>(hipStream_t, T*, size_t, const int*);

@@ -61,7 +61,8 @@ EXPORTS = [
     "pgm_compute", "pgm_compute_f32", "pgm_forces", "pgm_compute_device", "pgm_launch_device", "pgm_finish_device",
     "pgm_compute_device_f64", "pgm_iterations", "pgm_residual", "pgm_induced_dipoles", "pgm_permanent_dipoles",
     "pgm_reset_history", "pgm_list_builds", "pgm_num_candidates", "pgm_late_list_checks", "pgm_set_history",
-    "pgm_freeze_history", "pgm_set_timing", "pgm_get_timing", "pgm_num_pairs",
+    "pgm_freeze_history", "pgm_set_timing", "pgm_get_timing", "pgm_num_pairs", "pgm_backward",
+    "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg", "pair_final",
           "final"]  # fmt: skip
@@ -116,6 +117,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_num_pairs.argtypes = [vp]
     lib.pgm_num_pairs.restype = C.c_longlong
     lib.pgm_get_timing.argtypes = [vp, vp, vp, C.c_int]
+    lib.pgm_backward.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp, vp, vp]
+    lib.pgm_backward_device.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp, vp, vp, vp]
+    lib.pgm_adjoint_iterations.argtypes = [vp]
+    lib.pgm_adjoint_residual.argtypes = [vp]
+    lib.pgm_adjoint_residual.restype = C.c_double
     _ = dp
     _lib = lib
     return lib
@@ -160,6 +166,7 @@ class Engine:
         if not self.h:
             raise PgmError(self.lib.pgm_last_error(None).decode())
         self.n = 0
+        self.ncov = 0
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -196,6 +203,7 @@ class Engine:
         p = np.ascontiguousarray(partner, np.int32)
         s = _f64(strength)
         self._check(self.lib.pgm_set_covalent(self.h, len(a), _ptr(a), _ptr(p), _ptr(s)))
+        self.ncov = len(a)
 
     def compute(self, pos, box, new_step=True, forces=True, strain=False):
         """Return (terms dict, forces (n,3) or None, strain (3,3) or None)."""
@@ -233,6 +241,37 @@ class Engine:
         )
         self._check(rc)
         return t.as_dict()
+
+    def backward(self, pos, box, dl_dforces=None, dl_denergy=0.0) -> dict:
+        """Gradients of a loss L(E, F) with respect to the per-atom parameters (pgm_backward).
+
+        dl_dforces: dL/dF (n,3) or None; dl_denergy: dL/dE.  Returns {"q", "radius", "alpha"} (n,) and
+        {"cov_strength"} (ncov,) in the orders of set_atoms / set_covalent.
+        """
+        pos = _f64(pos).reshape(-1, 3)
+        if len(pos) != self.n:
+            raise PgmError(f"positions for {len(pos)} atoms, engine has {self.n}")
+        box = _f64(box, (3, 3))
+        g = None if dl_dforces is None else _f64(dl_dforces).reshape(self.n, 3)
+        gq, gr, ga = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        gc = np.zeros(max(self.ncov, 1))
+        rc = self.lib.pgm_backward(self.h, _ptr(pos), _ptr(box), _ptr(g), float(dl_denergy), _ptr(gq), _ptr(gr),
+                                   _ptr(ga), _ptr(gc))  # fmt: skip
+        self._check(rc)
+        return {"q": gq, "radius": gr, "alpha": ga, "cov_strength": gc[: self.ncov]}
+
+    def backward_device(self, pos_ptr: int, box, g_ptr: int | None, dl_denergy: float, gq_ptr, gr_ptr, ga_ptr, gc_ptr,
+                        stream: int = 0):  # fmt: skip
+        box = _f64(box, (3, 3))
+        rc = self.lib.pgm_backward_device(self.h, pos_ptr, _ptr(box), g_ptr, float(dl_denergy), gq_ptr, gr_ptr, ga_ptr,
+                                          gc_ptr, stream)  # fmt: skip
+        self._check(rc)
+
+    def adjoint_iterations(self) -> int:
+        return self.lib.pgm_adjoint_iterations(self.h)
+
+    def adjoint_residual(self) -> float:
+        return self.lib.pgm_adjoint_residual(self.h)
 
     def iterations(self) -> int:
         return self.lib.pgm_iterations(self.h)

@@ -71,3 +71,41 @@ def pgm_parameters(path: str) -> dict:
         "cov_partner": partners.astype(np.int32),
         "cov_strength": np.asarray(s["POL_GAUSS_COVALENT_DIPOLES_LIST"], float) * 0.1,
     }
+
+
+def parameter_types(params: dict, atom_names=None) -> dict:
+    """Group the per-atom parameters of `pgm_parameters` into types for fitting.
+
+    Atoms that share (name, q, radius, alpha) share an atom type (the name only when `atom_names` is given,
+    e.g. the prmtop's AMBER_ATOM_TYPE); covalent entries that share (carrier type, partner type, strength)
+    share a covalent type.  Returns the type index arrays ("atom_type" (n,), "cov_type" (ncov,)) and the
+    per-type values ("q", "radius", "alpha" per atom type, "cov_strength" per covalent type); expanding the
+    per-type values by the index arrays gives back the per-atom parameters exactly.
+    """
+    q = np.asarray(params["q"], float)
+    r = np.asarray(params["radius"], float)
+    a = np.asarray(params["alpha"], float)
+    names = [""] * len(q) if atom_names is None else [str(x) for x in atom_names]
+    if len(names) != len(q):
+        raise ValueError("one atom name per atom")
+    keys: dict = {}
+    atom_type = np.empty(len(q), np.int64)
+    for i in range(len(q)):
+        atom_type[i] = keys.setdefault((names[i], q[i], r[i], a[i]), len(keys))
+    tq, tr, ta = (np.zeros(len(keys)) for _ in range(3))
+    for (_, qi, ri, ai), t in keys.items():
+        tq[t], tr[t], ta[t] = qi, ri, ai
+    ca = np.asarray(params["cov_atom"], np.int64)
+    cp = np.asarray(params["cov_partner"], np.int64)
+    cs = np.asarray(params["cov_strength"], float)
+    ckeys: dict = {}
+    cov_type = np.empty(len(ca), np.int64)
+    for m in range(len(ca)):
+        cov_type[m] = ckeys.setdefault((int(atom_type[ca[m]]), int(atom_type[cp[m]]), cs[m]), len(ckeys))
+    tc = np.zeros(len(ckeys))
+    for (_, _, sm), t in ckeys.items():
+        tc[t] = sm
+    return {
+        "atom_type": atom_type, "cov_type": cov_type, "q": tq, "radius": tr, "alpha": ta, "cov_strength": tc,
+        "cov_atom": ca.astype(np.int32), "cov_partner": cp.astype(np.int32),
+    }  # fmt: skip

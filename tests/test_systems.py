@@ -101,3 +101,25 @@ def test_golden_fixture_parameters(golden):
     assert np.allclose(golden["cov_strength"][:4], [-0.019120, -0.019120, 0.0085877, 0.0085877], **close)
     np.testing.assert_array_equal(golden["cov_atom"][:4], [0, 0, 1, 2])
     np.testing.assert_array_equal(golden["cov_partner"][:4], [1, 2, 0, 0])
+
+
+def test_parameter_types_round_trip(golden):
+    """Types group atoms with equal parameters; expanding the per-type values gives the per-atom ones back."""
+    from pgmhip.prmtop import parameter_types
+
+    params = {k: golden[k] for k in ("q", "radius", "alpha", "cov_atom", "cov_partner", "cov_strength")}
+    ty = parameter_types(params)
+    at, ct = ty["atom_type"], ty["cov_type"]
+    assert len(ty["q"]) == 2  # water: O and H
+    assert np.array_equal(ty["q"][at], params["q"])
+    assert np.array_equal(ty["radius"][at], params["radius"])
+    assert np.array_equal(ty["alpha"][at], params["alpha"])
+    assert np.array_equal(ty["cov_strength"][ct], params["cov_strength"])
+    assert np.array_equal(ty["cov_atom"], params["cov_atom"]) and np.array_equal(ty["cov_partner"], params["cov_partner"])
+    # names split otherwise equal atoms
+    names = np.where(np.arange(len(at)) % 2 == 0, "A", "B")
+    ty2 = parameter_types(params, names)
+    assert len(ty2["q"]) > len(ty["q"])
+    assert np.array_equal(ty2["q"][ty2["atom_type"]], params["q"])
+    with pytest.raises(ValueError):
+        parameter_types(params, names[:-1])
