@@ -132,6 +132,11 @@ public:
     long listBuilds() const { return pgm_list_builds(e_); }
     int numCandidates() const { return (int) pgm_num_candidates(e_); }
     long lateListChecks() const { return pgm_late_list_checks(e_); }
+    /* No counterpart in the reference: parameter gradients of a loss L(E, F) (pgm_backward). */
+    void backward(const double* pos, const double box[3][3], const double* dLdF, double dLdE, double* gradQ,
+                  double* gradRadius, double* gradAlpha, double* gradCov) {
+        check(pgm_backward(e_, pos, &box[0][0], dLdF, dLdE, gradQ, gradRadius, gradAlpha, gradCov));
+    }
     pgm_engine* handle() const { return e_; }
 
 private:
