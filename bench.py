@@ -35,6 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "pgm-md_amd"))
 METRIC = "pGM E+F evals/s @96k atoms, mixed vs FP64 GPU at ≤5e-5 rel-RMS force error"
 PEAK_TFLOPS = {"mixed": 157.3, "double": 78.6}  # MI355X published FP32 / FP64 rates
 HBM_PEAK_GBS = 8000.0
+CAL_COPY_BYTES = 512 << 20  # the calibration copy of the profiler pass
 
 # Algorithmic flops (add/mul/sub 1, FMA 2; exp/erfc/sqrt/rsqrt/rint not counted) per half pair within the
 # cutoff (P), per atom (N) and per grid: geometry 26, G1+G2 32, G0..G3 44, dual field body 76, dipole
@@ -110,6 +111,77 @@ def roofline(phases, P, N, K, n, precision):
     }  # fmt: skip
 
 
+def snapshot_batch(S, nsnap=64):
+    """The uneven snapshot batch as one engine's arrays: offsets, grids, boxes, per-atom parameters, covalent."""
+    systems = [S.config3_snapshot(k) for k in range(nsnap)]
+    off = np.concatenate([[0], np.cumsum([s.n for s in systems])]).astype(np.int32)
+    cat = lambda key: np.concatenate([getattr(s, key) for s in systems])  # noqa: E731
+    return {
+        "systems": systems, "off": off, "pos": cat("pos"), "boxes": np.stack([s.box for s in systems]),
+        "grids": np.array([s.settings["grid"] for s in systems], np.int32), "q": cat("q"), "radius": cat("radius"),
+        "alpha": cat("alpha"), "cov_strength": cat("cov_strength"),
+        "cov_atom": np.concatenate([s.cov_atom + off[i] for i, s in enumerate(systems)]),
+        "cov_partner": np.concatenate([s.cov_partner + off[i] for i, s in enumerate(systems)]),
+    }  # fmt: skip
+
+
+def batch_engine(E, b, precision, tol):
+    st = b["systems"][0].settings
+    keys = ("cutoff", "beta", "order", "max_iter", "ke", "skin", "polarizable")
+    eng = E.Engine(precision=precision, grid=tuple(st["grid"]), tol=tol, predictor=0, peek=0.0,
+                   **{k: st[k] for k in keys})  # fmt: skip
+    eng.set_atoms(b["q"], b["radius"], b["alpha"])
+    eng.set_covalent(b["cov_atom"], b["cov_partner"], b["cov_strength"])
+    eng.set_batch(b["off"], b["grids"])
+    return eng
+
+
+def adjoint_batch(torch, E, S, reps, log):
+    """Force-matching gradients over the 64-snapshot batch in one engine: forward + adjoint per batch, FP64 and
+    mixed, tau 1e-8, the pair-pass lanes picked by the autotuner.  Unit: snapshot-gradients/s."""
+    b = snapshot_batch(S)
+    n = int(b["off"][-1])
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    pos_d = torch.from_numpy(b["pos"].copy()).to(dev)
+    f_d = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    g_d = torch.from_numpy(np.random.default_rng(9).normal(0.0, 1e-3, (n, 3))).to(dev)
+    grads = [torch.empty(m, dtype=torch.float64, device=dev) for m in (n, n, n, len(b["cov_atom"]))]
+    out, ref = {}, None
+    for prec in ("double", "mixed"):
+        eng = batch_engine(E, b, prec, 1e-8)
+        _, f = eng.compute_batch(b["pos"], b["boxes"])
+        lanes, tune = eng.autotune(3)
+        g = eng.backward_batch(b["pos"], b["boxes"], g_d.cpu().numpy(), None)
+        if ref is None:
+            ref = (f, g)
+        fwd_it, adj_it = eng.iterations(), eng.adjoint_iterations()
+
+        def step():
+            eng.compute_batch_device(pos_d.data_ptr(), b["boxes"], False, f_d.data_ptr(), stream)
+            eng.backward_batch_device(pos_d.data_ptr(), b["boxes"], g_d.data_ptr(), None,
+                                      *[t.data_ptr() for t in grads], stream)  # fmt: skip
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        gq = np.concatenate([g[k] for k in ("q", "radius", "alpha")])
+        gq0 = np.concatenate([ref[1][k] for k in ("q", "radius", "alpha")])
+        out[prec] = {
+            "snapshot_gradients_per_s": round(64 / dt, 2), "ms_per_batch": round(1000 * dt, 3),
+            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes,
+            "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()},
+            "force_relrms_vs_fp64": E.rel_rms(f, ref[0]),
+            "gradient_maxrel_vs_fp64": float(np.abs(gq - gq0).max() / np.abs(gq0).max()),
+        }  # fmt: skip
+        log(f"adjoint batch {prec}: {64 / dt:.1f} snapshot-gradients/s, lanes {lanes}")
+    return {"snapshots": 64, "atoms": n, "tol": 1e-8, **out}
+
+
 def cpu_baseline(sy, sets, budget_s=60.0):
     """The FP64 CPU oracle (single thread) on the same workload and guess: one full evaluation."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -135,6 +207,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-reps", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-snapshot adjoint batch")
+    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--profile", type=int, default=0, metavar="N",
+                    help="profiler pass: N mixed evaluations of the workload and one calibration copy of "
+                         f"{CAL_COPY_BYTES >> 20} MiB, no timing, no JSON line")  # fmt: skip
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs one system on one GPU; --gpus must be 1 (multi-GPU sharding "
@@ -158,6 +235,20 @@ def main():
     pos_d = torch.from_numpy(sy.pos.copy()).to(dev)
     f_d = torch.zeros((sy.n, 3), dtype=torch.float64, device=dev)
     box = sy.box
+    if a.profile:
+        eng = E.Engine.for_system(sy, precision="mixed")
+        eng.set_history(sets)
+        eng.freeze_history(True)
+        for _ in range(a.profile):
+            eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
+        # a plain device copy of known size: the scale of the HBM byte counters
+        src = torch.ones(CAL_COPY_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile pass: {a.profile} mixed evaluations, {eng.iterations()} CG iterations, "
+            f"calibration copy {CAL_COPY_BYTES} bytes read + {CAL_COPY_BYTES} bytes written")  # fmt: skip
+        return
 
     engines, results = {}, {}
     for prec in ("mixed", "double"):
@@ -220,6 +311,8 @@ def main():
         out["accuracy"]["oracle_cg_iterations"] = it_o
     else:
         out["cpu_baseline"] = None
+    if not a.no_batch:
+        out["adjoint_batch"] = adjoint_batch(torch, E, S, a.batch_reps, log)
     print(json.dumps(out))
 
 

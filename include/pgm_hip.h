@@ -136,13 +136,45 @@ int pgm_backward_device(pgm_engine* e, const double* pos_dev, const double* box,
 int pgm_adjoint_iterations(const pgm_engine* e);
 double pgm_adjoint_residual(const pgm_engine* e);
 
+/* No counterpart in the reference (it evaluates one system per engine): a batch of independent periodic
+ * systems packed into one engine, for parameter fitting over many snapshots of uneven size.  Atoms
+ * [atom_offsets[s], atom_offsets[s+1]) of those set by pgm_set_atoms form system s (atom_offsets[0] = 0,
+ * atom_offsets[nsys] = n), with PME grid grids[3s..3s+2] (NULL: the settings' grid for all).  Covalent
+ * dipoles may not join two systems.  Every pass of an evaluation is one launch over all systems (the FFTs one
+ * per grid shape over the stacked grids of that shape), and one CG solves the block-diagonal induction system.
+ * The single-system entry points refuse a batch of more than one system; pgm_set_atoms with another atom count
+ * resets the engine to one system. */
+int pgm_set_batch(pgm_engine* e, int nsys, const int* atom_offsets, const int* grids);
+int pgm_num_systems(const pgm_engine* e);
+/* One evaluation of every system of the batch: boxes 9 doubles per system (reduced triclinic rows); forces
+ * added to f (3n doubles, caller's order, may be NULL); terms one per system (may be NULL). */
+int pgm_compute_batch(pgm_engine* e, const double* pos, const double* boxes, int new_step, double* f,
+                      pgm_terms* terms);
+int pgm_compute_batch_device(pgm_engine* e, const double* pos_dev, const double* boxes, int new_step, double* f_dev,
+                             void* stream, pgm_terms* terms);
+/* pgm_backward over a batch: dl_denergy holds dL/dE per system (NULL: zero). */
+int pgm_backward_batch(pgm_engine* e, const double* pos, const double* boxes, const double* dl_dforces,
+                       const double* dl_denergy, double* grad_q, double* grad_radius, double* grad_alpha,
+                       double* grad_cov);
+int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double* boxes,
+                              const double* dl_dforces_dev, const double* dl_denergy, double* grad_q_dev,
+                              double* grad_radius_dev, double* grad_alpha_dev, double* grad_cov_dev, void* stream);
+
+/* No counterpart in the reference: the autotuner of the pair passes.  After an evaluation, times the
+ * compaction, setup, (iterations + 1) matrix-vector and final pair passes on the engine's current lists with
+ * 8, 16 and 32 lanes per atom (reps repetitions each), keeps the fastest for later evaluations and returns it
+ * in *lanes (may be NULL) and the three timings in ms[3] (ms per evaluation; may be NULL).  pgm_set_lanes sets
+ * the lanes directly (8, 16 or 32; 16 by default). */
+int pgm_autotune(pgm_engine* e, int reps, int* lanes, double* ms);
+int pgm_set_lanes(pgm_engine* e, int lanes);
+
 /* No counterpart in the reference (whose engine prints host wall clocks under PGM_CUDA_TIMING,
  * core/cuda/pgm_cuda.cu:1814-1817): HIP-event timing of the kernel groups of every evaluation, for
- * profiling.  Phases, in order: list check/rebuild, per-evaluation preparation, pair setup, pair field
- * (matrix-vector products), PME spread, PME transforms + convolution, PME gather, CG vector kernels,
- * final pair pass, final reductions and force output.  Iterations enqueued ahead that found the solve
+ * profiling.  Phases, in order: list check/rebuild, per-evaluation preparation, pair compaction, pair
+ * setup, pair field (matrix-vector products), PME spread, PME transforms + convolution, PME gather, CG
+ * vector kernels, final pair pass, final reductions and force output.  Iterations enqueued ahead that found the solve
  * finished are not counted.  pgm_set_timing resets the sums. */
-enum { PGM_TIMING_PHASES = 10 };
+enum { PGM_TIMING_PHASES = 11 };
 int pgm_set_timing(pgm_engine* e, int on);
 int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases);
 

@@ -22,7 +22,7 @@ EXPORTS_MAP = os.path.join(CSRC, "exports.map")
 SOURCES = ["nblist.hip", "pairs.hip", "pme.hip", "solver.hip", "adjoint.hip", "engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Werror=shadow", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]  # fmt: skip
 

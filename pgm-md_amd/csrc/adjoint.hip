@@ -21,11 +21,13 @@ inline int nblk(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
 
 // Change of the permanent dipoles of the sorted atoms when the atoms move by disp (caller's order):
 // p_i = sum_m c_m v_m/|v_m|, v_m = r_partner - r_i, so p'_i = sum_m c_m (v'_m - (u_m . v'_m) u_m)/|v_m|.
+template<bool BATCH>
 __global__ void kPermTangent(int n, const int* order, const double* pos, const double* disp, const int* covStart,
-                             const int* covPartner, const double* covStrength, BoxT<double> box, double* pdot) {
+                             const int* covPartner, const double* covStrength, BoxSel<double> bs, double* pdot) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
+    const BoxT<double> box = selectBox<BATCH>(bs, s);
     const int i = order[s];
     double p[3] = {0, 0, 0};
     for (int m = covStart[i]; m < covStart[i+1]; m++) {
@@ -95,8 +97,10 @@ __global__ void kParamAtoms(ParamAtomArgs a) {
     const bool tan = a.ddot != nullptr;
     // charges
     const double phi = a.pairOut[s]+a.pmeOut[s];
-    const double dq = ke*(phi-a.qSelf*a.q[s]-a.qBackground);
-    double gq = a.wE*dq;
+    const double qBack = a.sysOf ? a.qBackSys[a.sysOf[s]] : a.qBackground;
+    const double dq = ke*(phi-a.qSelf*a.q[s]-qBack);
+    const double wE = a.sysOf && a.wESys ? a.wESys[a.sysOf[s]] : a.wE;
+    double gq = wE*dq;
     if (tan)
         gq -= ke*(a.pairOut[5*n+s]+a.pmeOut[5*n+s]);
     // polarizabilities
@@ -110,12 +114,12 @@ __global__ void kParamAtoms(ParamAtomArgs a) {
             if (a.lambda)
                 ml += a.mu[3*s+c]*a.lambda[3*s+c];
         }
-        galpha = a.wE*(-ke*mm/(2*al*al));
+        galpha = wE*(-ke*mm/(2*al*al));
         if (tan)
             galpha -= ke*ml/(al*al);
     }
     // radii
-    double gR = a.wE*ke*a.pairOut[4*n+s];
+    double gR = wE*ke*a.pairOut[4*n+s];
     if (tan)
         gR -= ke*a.pairOut[9*n+s];
     if (a.gq)
@@ -134,13 +138,16 @@ __global__ void kParamAtoms(ParamAtomArgs a) {
 }
 
 // One covalent entry (CSR order k, written at its input index): dL/dc = -wE ke E.u + ke (E'.u + E.u').
+template<bool BATCH>
 __global__ void kParamCov(int ncov, const int* covAtomOf, const int* covPartner, const int* inputOf, const int* rank,
                           const double* pos, const double* disp, const double* etot, const double* etotDot,
-                          BoxT<double> box, double ke, double wE, double* gc) {
+                          BoxSel<double> bs, double ke, double wE0, const double* wESys, double* gc) {
     const int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= ncov)
         return;
     const int i = covAtomOf[k], j = covPartner[k];
+    const BoxT<double> box = selectBox<BATCH>(bs, i);
+    const double wE = BATCH && wESys ? wESys[bs.sysOf[i]] : wE0;
     double x = pos[3*j]-pos[3*i], y = pos[3*j+1]-pos[3*i+1], z = pos[3*j+2]-pos[3*i+2];
     minImage(x, y, z, box);
     const double len = sqrt(x*x+y*y+z*z);
@@ -164,9 +171,14 @@ __global__ void kParamCov(int ncov, const int* covAtomOf, const int* covPartner,
 }   // namespace
 
 void launchPermTangent(hipStream_t st, int n, const int* order, const double* pos, const double* disp,
-                       const int* covStart, const int* covPartner, const double* covStrength, BoxT<double> box,
+                       const int* covStart, const int* covPartner, const double* covStrength, BoxSel<double> box,
                        double* pdot) {
-    kPermTangent<<<nblk(n, 256), 256, 0, st>>>(n, order, pos, disp, covStart, covPartner, covStrength, box, pdot);
+    if (box.table)
+        kPermTangent<true><<<nblk(n, 256), 256, 0, st>>>(n, order, pos, disp, covStart, covPartner, covStrength, box,
+                                                         pdot);
+    else
+        kPermTangent<false><<<nblk(n, 256), 256, 0, st>>>(n, order, pos, disp, covStart, covPartner, covStrength, box,
+                                                          pdot);
 }
 
 void launchSort3(hipStream_t st, int n, const int* order, const double* v, double* out) {
@@ -190,10 +202,16 @@ void launchParamAtoms(hipStream_t st, const ParamAtomArgs& a) {
 
 void launchParamCov(hipStream_t st, int ncov, const int* covAtomOf, const int* covPartner, const int* inputOf,
                     const int* rank, const double* pos, const double* disp, const double* etot,
-                    const double* etotDot, BoxT<double> box, double ke, double wE, double* gc) {
-    if (ncov > 0)
-        kParamCov<<<nblk(ncov, 256), 256, 0, st>>>(ncov, covAtomOf, covPartner, inputOf, rank, pos, disp, etot,
-                                                    etotDot, box, ke, wE, gc);
+                    const double* etotDot, BoxSel<double> box, double ke, double wE, const double* wESys,
+                    double* gc) {
+    if (ncov <= 0)
+        return;
+    if (box.table)
+        kParamCov<true><<<nblk(ncov, 256), 256, 0, st>>>(ncov, covAtomOf, covPartner, inputOf, rank, pos, disp, etot,
+                                                         etotDot, box, ke, wE, wESys, gc);
+    else
+        kParamCov<false><<<nblk(ncov, 256), 256, 0, st>>>(ncov, covAtomOf, covPartner, inputOf, rank, pos, disp,
+                                                          etot, etotDot, box, ke, wE, wESys, gc);
 }
 
 template void launchDipoleTangent<float>(hipStream_t, int, const double*, const double*, double*, float4*);

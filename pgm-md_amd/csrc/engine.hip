@@ -10,6 +10,11 @@
 // The CG iterations are enqueued ahead (as many as the last evaluation needed) with per-iteration done
 // flags on the device, so a typical evaluation synchronizes with the host only for the list check and the
 // result read-back.
+//
+// A batch (pgm_set_batch) packs several independent periodic systems, each with its own box and PME grid,
+// into one engine: every pass is one launch over the atoms (or bricks) of all systems, the transforms run once
+// per grid shape over the stacked grids of that shape, and the CG solves the block-diagonal system of all
+// of them at once.
 #include "kernels.hpp"
 
 #include "../../include/pgm_hip.h"
@@ -30,6 +35,7 @@ namespace {
 constexpr double kMinPolarizability = 1e-9;   // core/pgm_constants.h:9
 constexpr int kNHist = 5;                     // 4 predictor entries + the slot this step writes
 constexpr int kBrick = 8;
+constexpr int kChunkAtoms = 2048;             // rows per first-stage block of the per-system sums
 
 void hipCheck(hipError_t e, const char* what) {
     if (e != hipSuccess)
@@ -131,7 +137,8 @@ std::vector<double> bsplineModuli(int K, int p) {
     return mod;
 }
 
-// The results slots of one evaluation (device, read back once).
+
+// The result slots of one system's evaluation (device, read back once).
 enum {
     kResPair = 0,      // energy, strain[9] (pair, already halved)
     kResRec = 10,      // energy, strain[9] (k-space)
@@ -150,7 +157,8 @@ enum class PosSource { HostF64, HostF32, DeviceF64, DeviceF4 };
 enum Phase {
     kPhList = 0,      // list check and rebuild
     kPhPrep,          // sorted positions, permanent dipoles, PME bricks and splines
-    kPhPairSetup,     // compaction + stored kernels + field of q and p
+    kPhPairCompact,   // candidates within the cutoff, compacted in place
+    kPhPairSetup,     // stored kernels + field of q and p (and the first CG product)
     kPhPairField,     // stored-kernel dipole field (one per matrix-vector product)
     kPhSpread,
     kPhFft,           // z, y, x+convolution, y, z transforms
@@ -233,19 +241,27 @@ public:
                 haveList_ = false;
                 lastIterations_ = 0;
             }
+            if (n != n_ || sys_.empty()) {
+                // a new atom count: one system (a batch partition is set again with pgm_set_batch)
+                sys_.assign(1, Sys{});
+                sys_[0].a0 = 0;
+                sys_[0].a1 = n;
+                for (int a = 0; a < 3; a++)
+                    sys_[0].grid[a] = s_.grid[a];
+                batch_ = false;
+                layoutDirty_ = true;
+            }
             n_ = n;
             nGlobal_ = ng;
             qH_.assign(q, q+n);
             radH_.assign(radius, radius+n);
             alphaH_.resize(n);
             polarizable_ = false;
-            q2_ = Qtot_ = 0;
             for (int i = 0; i < n; i++) {
                 alphaH_[i] = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
                 polarizable_ = polarizable_ || alphaH_[i] > 0;
-                q2_ += q[i]*q[i];
-                Qtot_ += q[i];
             }
+            refreshCharges();
             gidxH_ = gi;
             std::vector<double> rg(n);
             for (int i = 0; i < n; i++)
@@ -266,35 +282,84 @@ public:
             throw std::logic_error("pgm_hip: setCovalent before setAtoms");
         if (ncov < 0 || (ncov > 0 && (!atom || !partner || !strength)))
             throw std::invalid_argument("pgm_hip: setCovalent: bad arrays");
-        for (int m = 0; m < ncov; m++)
+        for (int m = 0; m < ncov; m++) {
             if (atom[m] < 0 || atom[m] >= n_ || partner[m] < 0 || partner[m] >= n_ || atom[m] == partner[m])
                 throw std::invalid_argument("pgm_hip: setCovalent: atom index out of range");
+            if (systemOf(atom[m]) != systemOf(partner[m]))
+                throw std::invalid_argument("pgm_hip: setCovalent: a covalent dipole joins two systems of the batch");
+        }
         withDevice([&] {
             HIPCHK(hipStreamSynchronize(st_));
             setCovalentImpl(ncov, atom, partner, strength);
         });
     }
 
-    // One evaluation.  fHost / fDev / fixed: force outputs (added); strainOut added to.
-    void evaluate(PosSource src, const void* pos, const double* boxIn, bool newStep, double* fHost, double* fDev,
+    // Partition the atoms into k independent periodic systems: atoms [offsets[s], offsets[s+1]) form system s
+    // with PME grid grids[3s..3s+2] (nullptr: the settings' grid for every system).
+    void setSystems(int k, const int* offsets, const int* grids) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: set_batch before setAtoms");
+        if (k < 1 || !offsets)
+            throw std::invalid_argument("pgm_hip: set_batch needs k >= 1 and k+1 offsets");
+        if (offsets[0] != 0 || offsets[k] != n_)
+            throw std::invalid_argument("pgm_hip: set_batch: offsets must run from 0 to the atom count");
+        for (int s = 0; s < k; s++) {
+            if (offsets[s+1] <= offsets[s])
+                throw std::invalid_argument("pgm_hip: set_batch: every system needs at least one atom");
+            for (int a = 0; a < 3; a++) {
+                const int K = grids ? grids[3*s+a] : s_.grid[a];
+                if (K < s_.order)
+                    throw std::invalid_argument("pgm_hip: set_batch: a PME grid is smaller than the order");
+                (void) makePlan(K);
+            }
+        }
+        for (size_t m = 0; m < covAtomH_.size(); m++) {
+            const int sa = (int) (std::upper_bound(offsets, offsets+k+1, covAtomH_[m])-offsets)-1;
+            const int sp = (int) (std::upper_bound(offsets, offsets+k+1, covPartnerH_[m])-offsets)-1;
+            if (sa != sp)
+                throw std::invalid_argument("pgm_hip: set_batch: a covalent dipole joins two systems");
+        }
+        withDevice([&] {
+            HIPCHK(hipStreamSynchronize(st_));
+            sys_.assign(k, Sys{});
+            for (int s = 0; s < k; s++) {
+                sys_[s].a0 = offsets[s];
+                sys_[s].a1 = offsets[s+1];
+                for (int a = 0; a < 3; a++)
+                    sys_[s].grid[a] = grids ? grids[3*s+a] : s_.grid[a];
+            }
+            batch_ = k > 1;
+            layoutDirty_ = true;
+            haveList_ = false;
+            haveResults_ = false;
+            refreshCharges();
+        });
+    }
+    int numSystems() const { return (int) sys_.size(); }
+
+    // One evaluation of every system.  boxes: 9 doubles per system.  fHost / fDev / fixed: force outputs
+    // (added); strainOut (single system only) added to; terms: one per system.
+    void evaluate(PosSource src, const void* pos, const double* boxes, bool newStep, double* fHost, double* fDev,
                   long long* fixed, int stride, double fixedScale, double* strainOut, pgm_terms* terms,
                   hipStream_t caller, bool keepForces) {
         if (n_ <= 0)
             throw std::logic_error("pgm_hip: compute before setAtoms");
-        if (!pos || !boxIn)
+        if (!pos || !boxes)
             throw std::invalid_argument("pgm_hip: compute needs positions and a box");
+        if (strainOut && sys_.size() > 1)
+            throw std::invalid_argument("pgm_hip: the strain derivative is per system: not available for a batch");
         withDevice([&] {
             HIPCHK(hipEventRecord(evIn_, caller));
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
-            setBox(boxIn);
+            setBoxes(boxes);
             stagePositions(src, pos);
             tb(kPhList);
             checkList();
             te();
             if (s_.precision == PGM_MIXED)
-                run<float>(newStep, fHost || fDev || fixed || keepForces, strainOut != nullptr);
+                run<float>(newStep, strainOut != nullptr);
             else
-                run<double>(newStep, fHost || fDev || fixed || keepForces, strainOut != nullptr);
+                run<double>(newStep, strainOut != nullptr);
             // forces out, caller's order
             tb(kPhFinal);
             if (fDev || fixed)
@@ -309,7 +374,9 @@ public:
                 HIPCHK(hipMemcpyAsync(stageH_.p, forcesOutD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost, st_));
             }
             te();
-            HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, kResCount*sizeof(double), hipMemcpyDeviceToHost, st_));
+            const size_t resBytes = sys_.size()*kResCount*sizeof(double);
+            resH_.reserve(resBytes);
+            HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, resBytes, hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
             collectTiming();
             assemble(strainOut != nullptr);
@@ -322,21 +389,24 @@ public:
                 forcesH_.assign(stageH_.as<double>(), stageH_.as<double>()+3*n_);
             if (strainOut)
                 for (int k = 0; k < 9; k++)
-                    strainOut[k] += strain_[k];
+                    strainOut[k] += sys_[0].strain[k];
             if (terms)
-                *terms = terms_;
+                for (size_t s = 0; s < sys_.size(); s++)
+                    terms[s] = sys_[s].terms;
             HIPCHK(hipEventRecord(evOut_, st_));
             HIPCHK(hipStreamWaitEvent(caller, evOut_, 0));
         });
     }
 
     // Parameter gradients of a loss L(E, F) at the given positions: evaluates there (same step: the last solution
-    // as the guess), then the adjoint.  g: dL/dF (caller's order, host or device, may be null), wE: dL/dE.
-    void backward(PosSource src, const void* pos, const double* boxIn, const double* g, bool gDevice, double wE,
-                  double* gq, double* gR, double* galpha, double* gc, bool outDevice, hipStream_t caller) {
+    // as the guess), then the adjoint.  g: dL/dF (caller's order, host or device, may be null), wE: dL/dE per
+    // system.
+    void backward(PosSource src, const void* pos, const double* boxes, const double* g, bool gDevice,
+                  const double* wE, double* gq, double* gR, double* galpha, double* gc, bool outDevice,
+                  hipStream_t caller) {
         if (n_ <= 0)
             throw std::logic_error("pgm_hip: backward before setAtoms");
-        evaluate(src, pos, boxIn, false, nullptr, nullptr, nullptr, 0, 0.0, nullptr, nullptr, caller, false);
+        evaluate(src, pos, boxes, false, nullptr, nullptr, nullptr, 0, 0.0, nullptr, nullptr, caller, false);
         withDevice([&] {
             HIPCHK(hipEventRecord(evIn_, caller));
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
@@ -397,6 +467,37 @@ public:
     }
     int adjointIterations() const { return adjIterations_; }
     double adjointResidual() const { return adjResidual_; }
+
+    // Times the pair passes at every lane count on the current lists and keeps the fastest (per-evaluation cost:
+    // compaction + setup + (iterations + 1) field passes + final pass).  Returns the lanes chosen.
+    int autotune(int reps) {
+        if (!haveResults_)
+            throw std::logic_error("pgm_hip: autotune needs an evaluation first (the lists of the workload)");
+        if (reps < 1)
+            throw std::invalid_argument("pgm_hip: autotune needs reps >= 1");
+        int best = lanes_;
+        withDevice([&] {
+            double bestMs = 1e300;
+            for (int lanes : kPairLaneChoices) {
+                const double ms = s_.precision == PGM_MIXED ? timePairPasses<float>(lanes, reps)
+                                                            : timePairPasses<double>(lanes, reps);
+                tuneMs_[lanes == 8 ? 0 : (lanes == 16 ? 1 : 2)] = ms;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    best = lanes;
+                }
+            }
+            lanes_ = best;
+        });
+        return best;
+    }
+    int lanes() const { return lanes_; }
+    void setLanes(int lanes) {
+        if (lanes != 8 && lanes != 16 && lanes != 32)
+            throw std::invalid_argument("pgm_hip: pair-pass lanes must be 8, 16 or 32");
+        lanes_ = lanes;
+    }
+    const double* tuneTimes() const { return tuneMs_; }
 
     int iterations() const { return iterations_; }
     double residual() const { return residual_; }
@@ -473,25 +574,51 @@ public:
             counts[k] = phaseCount_[k];
         }
     }
-    const pgm_terms& lastTerms() const { return terms_; }
+    const pgm_terms& lastTerms() const { return sys_[0].terms; }
 
 private:
+    // One periodic system of the engine; a batch holds several, contiguous in atom index in both orders.
+    struct Sys {
+        int a0 = 0, a1 = 0;
+        int grid[3] = {0, 0, 0};
+        double H[3][3] = {{0}}, recip[3][3] = {{0}}, volume = 0;
+        double skinNow = 0, rl = 0;
+        double buildBox[3][3] = {{0}};
+        CellGrid cells{};
+        int cellOff = 0;            // first slot of its cell counts / starts (ncell + 1 slots)
+        PmeGrid pme{};
+        int group = 0, slot = 0;    // transform group (grid shape) and place in it
+        long long realOff = 0;      // its real grid in the stacked grids (elements)
+        long long cplxOff = 0;      // its half spectrum and influence function (elements)
+        int brickOff = 0;
+        bool haveInfluence = false;
+        double influenceBox[3][3] = {{0}};
+        double q2 = 0, Qtot = 0;
+        pgm_terms terms{};
+        double strain[9] = {0};
+    };
+    // The systems that share one grid shape: their grids are stacked and transformed together.
+    struct Group {
+        PmeGrid g{};
+        std::vector<int> members;
+        long long realOff = 0, cplxOff = 0;
+        long long twOff = 0, bmodOff = 0;
+        int partOff = 0, blocksPerGrid = 0;
+    };
+
     pgm_settings s_;
     hipStream_t st_ = nullptr;
     hipEvent_t evIn_ = nullptr, evOut_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1;
     bool polarizable_ = false;
-    double q2_ = 0, Qtot_ = 0;
     std::vector<double> qH_, radH_, alphaH_, forcesH_;
-    std::vector<int> gidxH_;
-    // box
-    double H_[3][3] = {{0}}, recip_[3][3] = {{0}}, volume_ = 0;
-    double influenceBox_[3][3] = {{0}};
-    bool haveInfluence_ = false;
-    double buildBox_[3][3] = {{0}};
-    double skinNow_ = 0, rl_ = 0;
-    CellGrid cells_{};
-    PmeGrid pme_{};
+    std::vector<int> gidxH_, covAtomH_, covPartnerH_;
+    // systems and the PME layout
+    std::vector<Sys> sys_;
+    std::vector<Group> groups_;
+    bool batch_ = false, layoutDirty_ = true, tablesDirty_ = true;
+    long long realTotal_ = 0, cplxTotal_ = 0, twTotal_ = 0, bmodTotal_ = 0;
+    int brickTotal_ = 0, cellTotal_ = 0, partTotal_ = 0;
     // list state
     bool haveList_ = false;
     long listBuilds_ = 0;
@@ -501,8 +628,8 @@ private:
     std::deque<int> histSlots_;
     bool frozen_ = false;
     bool haveResults_ = false;
-    pgm_terms terms_{};
-    double strain_[9] = {0};
+    int lanes_ = 16;
+    double tuneMs_[3] = {0, 0, 0};
     // kernel timing
     struct Timed {
         int phase, iteration;
@@ -561,10 +688,6 @@ private:
     DevBuf qD_, rgD_, alphaD_, gidxD_, posInD_, posBuildD_;
     const double* posDev_ = nullptr;   // the FP64 positions (caller's order) of this evaluation
     DevBuf covStartD_, covPartnerD_, covStrengthD_, covAtomOfD_, revStartD_, revEntryD_, covInputD_;
-    // parameter gradients
-    DevBuf pairOutD_, pmeOutD_, gridTD_, gSD_, gTD_, pdotD_, lamD_, adjBD_, ddotD_, ddotTD_, etotDotD_, gInD_, gradD_;
-    int adjIterations_ = 0, lastAdjIterations_ = 0;
-    double adjResidual_ = 0;
     DevBuf cellOfD_, slotD_, cellCountD_, cellStartD_, orderD_, rankD_, sortedCellD_, scratchD_;
     DevBuf candCountD_, candStartD_, candD_, nbD_, pjD_, g12D_, g03D_, subOfD_;
     DevBuf qSD_, rgSD_, alphaSD_, posDD_, posFD_;
@@ -573,6 +696,15 @@ private:
     DevBuf brickOfD_, pslotD_, brickCountD_, brickStartD_, pmeAtomD_, baseD_, splD_;
     DevBuf gridD_, cplxD_, inflD_, twD_, bmodD_;
     DevBuf cgPartD_, partD_, resD_, cgD_, doneD_, histD_, maxBitsD_;
+    DevBuf perAtomD_, chunkSumD_;
+    // systems tables (batch)
+    DevBuf sysOfD_, boxTabD_, boxTabFD_, pmeTabD_, brickSysD_, groupBoxD_, chunkStartD_, sysChunkD_;
+    DevBuf recStartD_, recSysD_, qBackD_, wED_;
+    int nChunks_ = 0, nRecRanges_ = 0;
+    // parameter gradients
+    DevBuf pairOutD_, pmeOutD_, gridTD_, gSD_, gTD_, pdotD_, lamD_, adjBD_, ddotD_, ddotTD_, etotDotD_, gInD_, gradD_;
+    int adjIterations_ = 0, lastAdjIterations_ = 0;
+    double adjResidual_ = 0;
     HostBuf resH_, stageH_, scalH_;
 
     void validate() {
@@ -607,6 +739,24 @@ private:
         HIPCHK(hipStreamSynchronize(st_));
     }
 
+    int systemOf(int atom) const {
+        for (size_t s = 0; s < sys_.size(); s++)
+            if (atom >= sys_[s].a0 && atom < sys_[s].a1)
+                return (int) s;
+        return -1;
+    }
+
+    void refreshCharges() {
+        tablesDirty_ = true;
+        for (Sys& y : sys_) {
+            y.q2 = y.Qtot = 0;
+            for (int i = y.a0; i < y.a1 && i < (int) qH_.size(); i++) {
+                y.q2 += qH_[i]*qH_[i];
+                y.Qtot += qH_[i];
+            }
+        }
+    }
+
     void setCovalentImpl(int ncov, const int* atom, const int* partner, const double* strength) {
         // CSR by atom (entries in input order per atom) and the reverse CSR by partner
         std::vector<int> start(n_+1, 0), rstart(n_+1, 0);
@@ -639,82 +789,232 @@ private:
         upload(revStartD_, rstart.data(), (n_+1)*sizeof(int));
         upload(revEntryD_, rev.data(), ncov*sizeof(int));
         upload(covInputD_, inputOf.data(), ncov*sizeof(int));
+        covAtomH_.assign(atom, atom+ncov);
+        covPartnerH_.assign(partner, partner+ncov);
         covN_ = ncov;
         haveResults_ = false;
     }
 
-    void setBox(const double* b) {
-        double H[3][3];
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++)
-                H[r][c] = b[3*r+c];
-        if (H[0][1] != 0 || H[0][2] != 0 || H[1][2] != 0)
-            throw std::invalid_argument("pgm_hip: the box must be in reduced triclinic form (a = (ax,0,0), b = (bx,by,0))");
-        if (!(H[0][0] > 0 && H[1][1] > 0 && H[2][2] > 0))
-            throw std::invalid_argument("pgm_hip: box diagonal must be positive");
-        std::memcpy(H_, H, sizeof(H));
-        const double det = H[0][0]*H[1][1]*H[2][2];
-        volume_ = det;
-        // recip = H^-1 of the lower-triangular H (rows = box vectors)
-        double R[3][3] = {{0}};
-        R[0][0] = 1.0/H[0][0];
-        R[1][1] = 1.0/H[1][1];
-        R[2][2] = 1.0/H[2][2];
-        R[1][0] = -H[1][0]/(H[0][0]*H[1][1]);
-        R[2][1] = -H[2][1]/(H[1][1]*H[2][2]);
-        R[2][0] = (H[1][0]*H[2][1]-H[1][1]*H[2][0])/(H[0][0]*H[1][1]*H[2][2]);
-        std::memcpy(recip_, R, sizeof(R));
-        // plane spacings and the list radius (skin shrunk in small boxes)
-        double wmin = 1e300;
-        double w[3];
-        for (int a = 0; a < 3; a++) {
-            w[a] = 1.0/std::sqrt(R[0][a]*R[0][a]+R[1][a]*R[1][a]+R[2][a]*R[2][a]);
-            wmin = std::min(wmin, w[a]);
+    // ------------------------------------------------------------------------------------ boxes and layout
+
+    void setBoxes(const double* boxes) {
+        if (layoutDirty_)
+            buildLayout();
+        bool changed = false;
+        for (size_t s = 0; s < sys_.size(); s++) {
+            Sys& y = sys_[s];
+            const double* b = boxes+9*s;
+            double H[3][3];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    H[r][c] = b[3*r+c];
+            if (H[0][1] != 0 || H[0][2] != 0 || H[1][2] != 0)
+                throw std::invalid_argument("pgm_hip: the box must be in reduced triclinic form (a = (ax,0,0), b = (bx,by,0))");
+            if (!(H[0][0] > 0 && H[1][1] > 0 && H[2][2] > 0))
+                throw std::invalid_argument("pgm_hip: box diagonal must be positive");
+            changed = changed || std::memcmp(H, y.H, sizeof(H)) != 0;
+            std::memcpy(y.H, H, sizeof(H));
+            y.volume = H[0][0]*H[1][1]*H[2][2];
+            // recip = H^-1 of the lower-triangular H (rows = box vectors)
+            double R[3][3] = {{0}};
+            R[0][0] = 1.0/H[0][0];
+            R[1][1] = 1.0/H[1][1];
+            R[2][2] = 1.0/H[2][2];
+            R[1][0] = -H[1][0]/(H[0][0]*H[1][1]);
+            R[2][1] = -H[2][1]/(H[1][1]*H[2][2]);
+            R[2][0] = (H[1][0]*H[2][1]-H[1][1]*H[2][0])/(H[0][0]*H[1][1]*H[2][2]);
+            std::memcpy(y.recip, R, sizeof(R));
+            // plane spacings and the list radius (skin shrunk in small boxes)
+            double wmin = 1e300;
+            for (int a = 0; a < 3; a++)
+                wmin = std::min(wmin, 1.0/std::sqrt(R[0][a]*R[0][a]+R[1][a]*R[1][a]+R[2][a]*R[2][a]));
+            if (s_.cutoff >= 0.5*wmin)
+                throw std::invalid_argument("pgm_hip: the cutoff must be below half the smallest box plane spacing");
+            y.skinNow = std::min(s_.skin, 0.5*wmin-s_.cutoff);
+            y.rl = s_.cutoff+y.skinNow;
+            if (!y.haveInfluence || std::memcmp(H, y.influenceBox, sizeof(H)) != 0)
+                setupInfluence(y);
         }
-        if (s_.cutoff >= 0.5*wmin)
-            throw std::invalid_argument("pgm_hip: the cutoff must be below half the smallest box plane spacing");
-        skinNow_ = std::min(s_.skin, 0.5*wmin-s_.cutoff);
-        rl_ = s_.cutoff+skinNow_;
-        // PME geometry
-        const bool boxChanged = !haveInfluence_ || std::memcmp(H, influenceBox_, sizeof(H)) != 0;
-        if (boxChanged)
-            setupPme(w);
+        if (batch_ && (changed || tablesDirty_))
+            uploadBoxTables();
     }
 
-    void setupPme(const double*) {
-        PmeGrid g{};
-        for (int a = 0; a < 3; a++) {
-            g.K[a] = s_.grid[a];
-            g.plan[a] = makePlan(g.K[a]);
-            g.nb[a] = (g.K[a]+kBrick-1)/kBrick;
+    // Grid shapes -> transform groups; offsets of every system's grids, bricks and cells; the batch tables.
+    void buildLayout() {
+        groups_.clear();
+        for (size_t s = 0; s < sys_.size(); s++) {
+            Sys& y = sys_[s];
+            PmeGrid g{};
+            for (int a = 0; a < 3; a++) {
+                g.K[a] = y.grid[a];
+                g.plan[a] = makePlan(g.K[a]);
+                g.nb[a] = (g.K[a]+kBrick-1)/kBrick;
+            }
+            g.K3h = g.K[2]/2+1;
+            g.order = s_.order;
+            g.brick = kBrick;
+            g.nbrick = g.nb[0]*g.nb[1]*g.nb[2];
+            y.pme = g;
+            int gi = -1;
+            for (size_t k = 0; k < groups_.size(); k++)
+                if (groups_[k].g.K[0] == g.K[0] && groups_[k].g.K[1] == g.K[1] && groups_[k].g.K[2] == g.K[2])
+                    gi = (int) k;
+            if (gi < 0) {
+                groups_.push_back(Group{});
+                gi = (int) groups_.size()-1;
+                groups_[gi].g = g;
+            }
+            y.group = gi;
+            y.slot = (int) groups_[gi].members.size();
+            groups_[gi].members.push_back((int) s);
+            y.haveInfluence = false;
         }
-        g.K3h = g.K[2]/2+1;
-        g.order = s_.order;
-        g.brick = kBrick;
-        g.nbrick = g.nb[0]*g.nb[1]*g.nb[2];
-        pme_ = g;
+        realTotal_ = cplxTotal_ = twTotal_ = bmodTotal_ = 0;
+        partTotal_ = 0;
+        const bool mixed = s_.precision == PGM_MIXED;
+        std::vector<int> recStart(1, 0), recSys;
+        for (Group& G : groups_) {
+            const PmeGrid& g = G.g;
+            const long long nreal = (long long) g.K[0]*g.K[1]*g.K[2], nc = (long long) g.K[0]*g.K[1]*g.K3h;
+            G.realOff = realTotal_;
+            G.cplxOff = cplxTotal_;
+            G.twOff = twTotal_;
+            G.bmodOff = bmodTotal_;
+            realTotal_ += nreal*(long long) G.members.size();
+            cplxTotal_ += nc*(long long) G.members.size();
+            twTotal_ += g.K[0]+g.K[1]+g.K[2];
+            bmodTotal_ += g.K[0]+g.K[1]+g.K[2];
+            G.blocksPerGrid = mixed ? convolveBlocksPerGrid<float>(g) : convolveBlocksPerGrid<double>(g);
+            G.partOff = partTotal_;
+            partTotal_ += G.blocksPerGrid*(int) G.members.size();
+            for (size_t k = 0; k < G.members.size(); k++) {
+                Sys& y = sys_[G.members[k]];
+                y.realOff = G.realOff+(long long) k*nreal;
+                y.cplxOff = G.cplxOff+(long long) k*nc;
+                recStart.push_back(G.partOff+(int) (k+1)*G.blocksPerGrid);
+                recSys.push_back(G.members[k]);
+            }
+        }
+        brickTotal_ = 0;
+        for (Sys& y : sys_) {
+            y.brickOff = brickTotal_;
+            brickTotal_ += y.pme.nbrick;
+        }
+        // device arrays of the layout
+        const size_t tsz = mixed ? sizeof(float) : sizeof(double), t2 = 2*tsz;
+        gridD_.reserve((size_t) realTotal_*tsz);
+        cplxD_.reserve((size_t) cplxTotal_*t2);
+        inflD_.reserve((size_t) cplxTotal_*tsz);
+        twD_.reserve((size_t) twTotal_*t2);
         std::vector<double> bm;
-        for (int a = 0; a < 3; a++) {
-            const std::vector<double> m = bsplineModuli(g.K[a], g.order);
-            bm.insert(bm.end(), m.begin(), m.end());
+        for (Group& G : groups_) {
+            for (int a = 0; a < 3; a++) {
+                const std::vector<double> m = bsplineModuli(G.g.K[a], G.g.order);
+                bm.insert(bm.end(), m.begin(), m.end());
+            }
+            if (mixed)
+                launchTwiddles<float>(st_, G.g, twD_.as<float2>()+G.twOff);
+            else
+                launchTwiddles<double>(st_, G.g, twD_.as<double2>()+G.twOff);
         }
         upload(bmodD_, bm.data(), bm.size()*sizeof(double));
-        const size_t nc = (size_t) g.K[0]*g.K[1]*g.K3h;
-        const BoxT<double> bd = makeBox<double>(H_, recip_);
-        if (s_.precision == PGM_MIXED) {
-            inflD_.reserve(nc*sizeof(float));
-            launchInfluence<float>(st_, g, bmodD_.as<double>(), bd, s_.beta, volume_, inflD_.as<float>());
-            twD_.reserve((g.K[0]+g.K[1]+g.K[2])*sizeof(float2));
-            launchTwiddles<float>(st_, g, twD_.as<float2>());
+        nRecRanges_ = (int) recSys.size();
+        upload(recStartD_, recStart.data(), recStart.size()*sizeof(int));
+        upload(recSysD_, recSys.data(), recSys.size()*sizeof(int));
+        // per-system sums of per-atom rows: chunks of at most kChunkAtoms atoms inside one system
+        std::vector<int> chunkStart(1, 0), sysChunk(1, 0);
+        for (Sys& y : sys_) {
+            for (int a = y.a0; a < y.a1; a += kChunkAtoms)
+                chunkStart.push_back(std::min(y.a1, a+kChunkAtoms));
+            sysChunk.push_back((int) chunkStart.size()-1);
+        }
+        nChunks_ = (int) chunkStart.size()-1;
+        upload(chunkStartD_, chunkStart.data(), chunkStart.size()*sizeof(int));
+        upload(sysChunkD_, sysChunk.data(), sysChunk.size()*sizeof(int));
+        if (batch_) {
+            std::vector<int> sysOf(n_), brickSys(brickTotal_);
+            for (size_t s = 0; s < sys_.size(); s++) {
+                for (int i = sys_[s].a0; i < sys_[s].a1; i++)
+                    sysOf[i] = (int) s;
+                for (int b = 0; b < sys_[s].pme.nbrick; b++)
+                    brickSys[sys_[s].brickOff+b] = (int) s;
+            }
+            upload(sysOfD_, sysOf.data(), n_*sizeof(int));
+            upload(brickSysD_, brickSys.data(), brickTotal_*sizeof(int));
+        }
+        layoutDirty_ = false;
+        tablesDirty_ = true;
+        haveList_ = false;
+    }
+
+    void setupInfluence(Sys& y) {
+        const Group& G = groups_[y.group];
+        const BoxT<double> bd = makeBox<double>(y.H, y.recip);
+        if (s_.precision == PGM_MIXED)
+            launchInfluence<float>(st_, y.pme, bmodD_.as<double>()+G.bmodOff, bd, s_.beta, y.volume,
+                                   inflD_.as<float>()+y.cplxOff);
+        else
+            launchInfluence<double>(st_, y.pme, bmodD_.as<double>()+G.bmodOff, bd, s_.beta, y.volume,
+                                    inflD_.as<double>()+y.cplxOff);
+        std::memcpy(y.influenceBox, y.H, sizeof(y.H));
+        y.haveInfluence = true;
+    }
+
+    void uploadBoxTables() {
+        const size_t S = sys_.size();
+        std::vector<BoxT<double>> bd(S), gb;
+        std::vector<BoxT<float>> bf(S);
+        std::vector<PmeSys> ps(S);
+        std::vector<double> qb(S);
+        for (size_t s = 0; s < S; s++) {
+            const Sys& y = sys_[s];
+            bd[s] = makeBox<double>(y.H, y.recip);
+            bf[s] = makeBox<float>(y.H, y.recip);
+            ps[s].g = y.pme;
+            ps[s].box = bd[s];
+            ps[s].gridOff = y.realOff;
+            ps[s].brickOff = y.brickOff;
+            ps[s].pad = 0;
+            qb[s] = kPi*y.Qtot/(y.volume*s_.beta*s_.beta);
+        }
+        for (const Group& G : groups_)
+            for (int m : G.members)
+                gb.push_back(bd[m]);
+        upload(boxTabD_, bd.data(), S*sizeof(BoxT<double>));
+        upload(boxTabFD_, bf.data(), S*sizeof(BoxT<float>));
+        upload(pmeTabD_, ps.data(), S*sizeof(PmeSys));
+        upload(groupBoxD_, gb.data(), gb.size()*sizeof(BoxT<double>));
+        upload(qBackD_, qb.data(), S*sizeof(double));
+        tablesDirty_ = false;
+    }
+
+    template<class T>
+    BoxSel<T> boxSel() const {
+        BoxSel<T> b{};
+        if (batch_) {
+            b.table = sizeof(T) == sizeof(float) ? reinterpret_cast<const BoxT<T>*>(boxTabFD_.p)
+                                                 : reinterpret_cast<const BoxT<T>*>(boxTabD_.p);
+            b.sysOf = sysOfD_.as<int>();
         }
         else {
-            inflD_.reserve(nc*sizeof(double));
-            launchInfluence<double>(st_, g, bmodD_.as<double>(), bd, s_.beta, volume_, inflD_.as<double>());
-            twD_.reserve((g.K[0]+g.K[1]+g.K[2])*sizeof(double2));
-            launchTwiddles<double>(st_, g, twD_.as<double2>());
+            b.box = makeBox<T>(sys_[0].H, sys_[0].recip);
+            b.table = nullptr;
+            b.sysOf = nullptr;
         }
-        std::memcpy(influenceBox_, H_, sizeof(H_));
-        haveInfluence_ = true;
+        return b;
+    }
+
+    PmeSel pmeSel() const {
+        PmeSel p{};
+        p.g = sys_[0].pme;
+        p.box = makeBox<double>(sys_[0].H, sys_[0].recip);
+        p.table = batch_ ? pmeTabD_.as<PmeSys>() : nullptr;
+        p.sysOf = batch_ ? sysOfD_.as<int>() : nullptr;
+        p.brickSys = batch_ ? brickSysD_.as<int>() : nullptr;
+        p.nbrick = brickTotal_;
+        p.order = s_.order;
+        p.brick = kBrick;
+        return p;
     }
 
     void stagePositions(PosSource src, const void* pos) {
@@ -749,86 +1049,106 @@ private:
         }
     }
 
-    double boxChange() const {
-        double dH = 0;
-        for (int a = 0; a < 3; a++) {
-            double r2 = 0;
-            for (int c = 0; c < 3; c++)
-                r2 += (H_[a][c]-buildBox_[a][c])*(H_[a][c]-buildBox_[a][c]);
-            dH += std::sqrt(r2);
-        }
-        return dH;
-    }
+    // ------------------------------------------------------------------------------------ lists
 
     void checkList() {
         bool stale = !haveList_;
         if (!stale) {
             maxBitsD_.reserve(sizeof(unsigned));
             HIPCHK(hipMemsetAsync(maxBitsD_.p, 0, sizeof(unsigned), st_));
-            launchMaxDisplacement(st_, n_, posDev_, posBuildD_.as<double>(), makeBox<double>(H_, recip_),
+            launchMaxDisplacement(st_, n_, posDev_, posBuildD_.as<double>(), boxSel<double>(),
                                   maxBitsD_.as<unsigned>());
             scalH_.reserve(64);
             HIPCHK(hipMemcpyAsync(scalH_.p, maxBitsD_.p, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
             float d2;
             std::memcpy(&d2, scalH_.p, sizeof(float));
-            stale = 2.0*std::sqrt((double) d2)+boxChange() > skinNow_;
+            double boxMove = 0, skin = 1e300;
+            for (const Sys& y : sys_) {
+                double dH = 0;
+                for (int a = 0; a < 3; a++) {
+                    double r2 = 0;
+                    for (int c = 0; c < 3; c++)
+                        r2 += (y.H[a][c]-y.buildBox[a][c])*(y.H[a][c]-y.buildBox[a][c]);
+                    dH += std::sqrt(r2);
+                }
+                boxMove = std::max(boxMove, dH);
+                skin = std::min(skin, y.skinNow);
+            }
+            stale = 2.0*std::sqrt((double) d2)+boxMove > skin;
         }
         if (stale)
             buildList();
     }
 
     void buildList() {
-        const BoxT<double> bd = makeBox<double>(H_, recip_);
-        // cells at least rl/2 wide, +-2 stencil (all distinct cells when fewer than 5 per dimension)
-        CellGrid g{};
-        for (int a = 0; a < 3; a++) {
-            const double w = 1.0/std::sqrt(recip_[0][a]*recip_[0][a]+recip_[1][a]*recip_[1][a]+recip_[2][a]*recip_[2][a]);
-            g.nc[a] = std::max(1, (int) std::floor(w/(0.5*rl_)));
-            if (g.nc[a] >= 5) {
-                g.noff[a] = 5;
-                for (int o = 0; o < 5; o++)
-                    g.off[a][o] = o-2;
-            }
-            else {
-                g.noff[a] = g.nc[a];
-                for (int o = 0; o < g.nc[a]; o++)
-                    g.off[a][o] = o;
-            }
-        }
-        g.ncell = g.nc[0]*g.nc[1]*g.nc[2];
-        cells_ = g;
         const int n = n_;
+        // cells at least rl/2 wide, +-2 stencil (all distinct cells when fewer than 5 per dimension), per system
+        cellTotal_ = 0;
+        for (Sys& y : sys_) {
+            CellGrid g{};
+            for (int a = 0; a < 3; a++) {
+                const double w =
+                    1.0/std::sqrt(y.recip[0][a]*y.recip[0][a]+y.recip[1][a]*y.recip[1][a]+y.recip[2][a]*y.recip[2][a]);
+                g.nc[a] = std::max(1, (int) std::floor(w/(0.5*y.rl)));
+                if (g.nc[a] >= 5) {
+                    g.noff[a] = 5;
+                    for (int o = 0; o < 5; o++)
+                        g.off[a][o] = o-2;
+                }
+                else {
+                    g.noff[a] = g.nc[a];
+                    for (int o = 0; o < g.nc[a]; o++)
+                        g.off[a][o] = o;
+                }
+            }
+            g.ncell = g.nc[0]*g.nc[1]*g.nc[2];
+            y.cells = g;
+            y.cellOff = cellTotal_;
+            cellTotal_ += g.ncell+1;
+        }
         cellOfD_.reserve(n*sizeof(int));
         slotD_.reserve(n*sizeof(int));
-        cellCountD_.reserve((g.ncell+1)*sizeof(int));
-        cellStartD_.reserve((g.ncell+1)*sizeof(int));
+        cellCountD_.reserve(cellTotal_*sizeof(int));
+        cellStartD_.reserve(cellTotal_*sizeof(int));
         orderD_.reserve(n*sizeof(int));
         rankD_.reserve(n*sizeof(int));
         sortedCellD_.reserve(n*sizeof(int));
-        HIPCHK(hipMemsetAsync(cellCountD_.p, 0, (g.ncell+1)*sizeof(int), st_));
         subOfD_.reserve(n*sizeof(int));
-        launchCellAssign(st_, n, posDev_, bd, g, cellOfD_.as<int>(), slotD_.as<int>(), cellCountD_.as<int>(),
-                         subOfD_.as<int>());
-        launchScan(st_, cellCountD_.as<int>(), cellStartD_.as<int>(), g.ncell, nullptr);
-        launchCellScatter(st_, n, cellOfD_.as<int>(), slotD_.as<int>(), cellStartD_.as<int>(), orderD_.as<int>());
-        launchCellSort(st_, g.ncell, cellStartD_.as<int>(), subOfD_.as<int>(), orderD_.as<int>());
-        launchRank(st_, n, orderD_.as<int>(), rankD_.as<int>(), sortedCellD_.as<int>(), cellOfD_.as<int>());
+        HIPCHK(hipMemsetAsync(cellCountD_.p, 0, cellTotal_*sizeof(int), st_));
+        for (const Sys& y : sys_) {
+            const int ns = y.a1-y.a0;
+            const BoxT<double> bd = makeBox<double>(y.H, y.recip);
+            int* cellCount = cellCountD_.as<int>()+y.cellOff;
+            int* cellStart = cellStartD_.as<int>()+y.cellOff;
+            launchCellAssign(st_, ns, posDev_+3*(size_t) y.a0, bd, y.cells, cellOfD_.as<int>()+y.a0,
+                             slotD_.as<int>()+y.a0, cellCount, subOfD_.as<int>()+y.a0);
+            launchScan(st_, cellCount, cellStart, y.cells.ncell, nullptr);
+            launchCellScatter(st_, ns, cellOfD_.as<int>()+y.a0, slotD_.as<int>()+y.a0, cellStart,
+                              orderD_.as<int>()+y.a0, y.a0);
+            launchCellSort(st_, y.cells.ncell, cellStart, subOfD_.as<int>(), orderD_.as<int>()+y.a0);
+            launchRank(st_, ns, orderD_.as<int>()+y.a0, rankD_.as<int>(), sortedCellD_.as<int>()+y.a0,
+                       cellOfD_.as<int>(), y.a0);
+        }
         // sorted parameters
         qSD_.reserve(n*sizeof(double));
         alphaSD_.reserve(n*sizeof(double));
+        rgSD_.reserve(n*sizeof(double));
         tmpD_.reserve(3*n*sizeof(double));
         launchPermute(st_, n, orderD_.as<int>(), qD_.as<double>(), qSD_.as<double>());
         launchPermute(st_, n, orderD_.as<int>(), alphaD_.as<double>(), alphaSD_.as<double>());
-        rgSD_.reserve(n*sizeof(double));
         launchPermute(st_, n, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
         posDD_.reserve(n*sizeof(double4));
-        launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(), nullptr);
+        launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), boxSel<double>(),
+                              posDD_.as<double4>(), nullptr);
         // candidates within rl, full lists in per-atom segments
         candCountD_.reserve(n*sizeof(int));
         candStartD_.reserve((n+1)*sizeof(int));
-        launchCandCount(st_, n, posDD_.as<double4>(), sortedCellD_.as<int>(), cellStartD_.as<int>(), g, bd, rl_*rl_,
-                        candCountD_.as<int>());
+        for (const Sys& y : sys_) {
+            const BoxT<double> bd = makeBox<double>(y.H, y.recip);
+            launchCandCount(st_, y.a1-y.a0, posDD_.as<double4>()+y.a0, sortedCellD_.as<int>()+y.a0,
+                            cellStartD_.as<int>()+y.cellOff, y.cells, bd, y.rl*y.rl, candCountD_.as<int>()+y.a0);
+        }
         launchScan(st_, candCountD_.as<int>(), candStartD_.as<int>(), n, nullptr);
         scalH_.reserve(64);
         HIPCHK(hipMemcpyAsync(scalH_.p, candStartD_.as<int>()+n, sizeof(int), hipMemcpyDeviceToHost, st_));
@@ -844,11 +1164,16 @@ private:
         pjD_.reserve(cap*sizeof(int));
         g12D_.reserve(cap*g2);
         g03D_.reserve(cap*g2);
-        launchCandFill(st_, n, posDD_.as<double4>(), sortedCellD_.as<int>(), cellStartD_.as<int>(), g, bd, rl_*rl_,
-                       candStartD_.as<int>(), candD_.as<int>());
+        for (const Sys& y : sys_) {
+            const BoxT<double> bd = makeBox<double>(y.H, y.recip);
+            launchCandFill(st_, y.a1-y.a0, posDD_.as<double4>()+y.a0, sortedCellD_.as<int>()+y.a0,
+                           cellStartD_.as<int>()+y.cellOff, y.cells, bd, y.rl*y.rl, candStartD_.as<int>()+y.a0,
+                           candD_.as<int>(), y.a0);
+        }
         posBuildD_.reserve(3*n*sizeof(double));
         HIPCHK(hipMemcpyAsync(posBuildD_.p, posDev_, 3*n*sizeof(double), hipMemcpyDeviceToDevice, st_));
-        std::memcpy(buildBox_, H_, sizeof(H_));
+        for (Sys& y : sys_)
+            std::memcpy(y.buildBox, y.H, sizeof(y.H));
         haveList_ = true;
         listBuilds_++;
     }
@@ -883,35 +1208,53 @@ private:
         return histSlots_.back();
     }
 
+    // ------------------------------------------------------------------------------------ passes
+
+    // Spread + transforms (per grid shape) into grid (the stacked real grids): the convolved potential grids.
+    // energyMode: per-system k-space energy (and strain) into res[s][kResRec..].
+    template<class T>
+    void pmeGrids(const typename Vec4<T>::type* dip, bool withQ, const typename Vec4<T>::type* posq, int energyMode,
+                  bool strain, T* grid, const int* skip, double* res, int it,
+                  const typename Vec4<T>::type* dipDot = nullptr, const typename Vec4<T>::type* disp = nullptr) {
+        typedef typename Vec2<T>::type T2;
+        const PmeSel ps = pmeSel();
+        tb(kPhSpread, it);
+        launchZero<T>(st_, grid, (size_t) realTotal_, skip);
+        launchSpread<T>(st_, ps, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
+                        posq, withQ ? 1 : 0, grid, skip, dipDot, disp);
+        te();
+        tb(kPhFft, it);
+        const BoxT<double> one = makeBox<double>(sys_[0].H, sys_[0].recip);
+        int boxOff = 0;
+        for (const Group& G : groups_) {
+            const BoxT<double>* boxes = batch_ ? groupBoxD_.as<BoxT<double>>()+boxOff : nullptr;
+            launchPmeConvolve<T>(st_, G.g, (int) G.members.size(), grid+G.realOff, cplxD_.as<T2>()+G.cplxOff,
+                                 inflD_.as<T>()+G.cplxOff, twD_.as<T2>()+G.twOff, one, boxes, s_.beta, energyMode,
+                                 strain ? 1 : 0, partD_.as<double>()+10*(size_t) G.partOff, skip);
+            boxOff += (int) G.members.size();
+        }
+        if (energyMode)
+            launchRangeSums(st_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
+                            kResCount, res+kResRec);
+        te();
+    }
+
     template<class T>
     void pmeSolve(const typename Vec4<T>::type* dip, bool withQ, const typename Vec4<T>::type* posq, int energyMode,
                   bool strain, double* field, double* force, const int* skip, double* res, int it = -1) {
-        typedef typename Vec2<T>::type T2;
-        const BoxT<double> bd = makeBox<double>(H_, recip_);
-        const size_t nreal = (size_t) pme_.K[0]*pme_.K[1]*pme_.K[2];
-        tb(kPhSpread, it);
-        launchZero<T>(st_, gridD_.as<T>(), nreal, skip);
-        launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
-                        posq, withQ ? 1 : 0, bd, gridD_.as<T>(), skip);
-        te();
-        tb(kPhFft, it);
-        const int nb = launchPmeConvolve<T>(st_, pme_, gridD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(),
-                                            bd, s_.beta, energyMode, strain ? 1 : 0, partD_.as<double>(), skip);
-        if (energyMode)
-            launchReducePartials(st_, partD_.as<double>(), nb, 10, res);
-        te();
+        pmeGrids<T>(dip, withQ, posq, energyMode, strain, gridD_.as<T>(), skip, res, it);
         tb(kPhGather, it);
-        launchGather<T>(st_, pme_, n_, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), gridD_.as<T>(), bd, dip,
+        launchGather<T>(st_, pmeSel(), n_, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), gridD_.as<T>(), dip,
                         posq, force ? 1 : 0, field, force, skip);
         te();
     }
 
     template<class T>
-    void matvec(const typename Vec4<T>::type* posT, const typename Vec4<T>::type* dipT, const BoxT<T>& bt,
-                const int* skip, int it = -1) {
+    void matvec(const typename Vec4<T>::type* posT, const typename Vec4<T>::type* dipT, const int* skip,
+                int it = -1) {
         tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                           g12D_.as<typename Vec2<T>::type>(), bt, tdD_.as<double>(), skip);
+                           g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_);
         te();
         pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
     }
@@ -920,8 +1263,7 @@ private:
     // T x is in tdD_ / trD_.  Iterations are enqueued ahead in chunks of the last solve's count, each behind a
     // device flag, and the host reads the state once per chunk.  Returns the iterations; err: the final residual.
     template<class T>
-    int cgSolve(const typename Vec4<T>::type* posT, const BoxT<T>& bt, const double* b, double* x, int& lastIts,
-                double& err) {
+    int cgSolve(const typename Vec4<T>::type* posT, const double* b, double* x, int& lastIts, double& err) {
         typedef typename Vec4<T>::type T4;
         const int n = n_;
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
@@ -937,7 +1279,7 @@ private:
             const int stop = std::min(s_.max_iter, it+chunk);
             for (; it < stop; it++) {
                 const int* skip = doneD_.as<int>()+it;
-                matvec<T>(posT, pTD_.as<T4>(), bt, skip, it);
+                matvec<T>(posT, pTD_.as<T4>(), skip, it);
                 tb(kPhCg, it);
                 launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
                            trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
@@ -964,15 +1306,31 @@ private:
         return cs.iterations;
     }
 
+    // Per-system sums of per-atom rows [n][width] into res[s][slot..slot+width).
+    void systemSums(const double* rows, int width, double* res, int slot) {
+        chunkSumD_.reserve((size_t) std::max(nChunks_, 1)*11*sizeof(double));
+        launchRangeSums(st_, rows, width, chunkStartD_.as<int>(), nChunks_, nullptr, width, chunkSumD_.as<double>());
+        launchRangeSums(st_, chunkSumD_.as<double>(), width, sysChunkD_.as<int>(), (int) sys_.size(), nullptr,
+                        kResCount, res+slot);
+    }
+
     template<class T>
-    void run(bool newStep, bool wantForces, bool wantStrain) {
+    typename Vec4<T>::type* posT() {
+        return reinterpret_cast<typename Vec4<T>::type*>(sizeof(T) == sizeof(float) ? posFD_.p : posDD_.p);
+    }
+    template<class T>
+    const T* rgT() {
+        return sizeof(T) == sizeof(float) ? scratchD_.as<T>() : rgSD_.as<T>();
+    }
+
+    template<class T>
+    void run(bool newStep, bool wantStrain) {
         typedef typename Vec4<T>::type T4;
         typedef typename Vec2<T>::type T2;
-        (void) wantForces;
         const int n = n_;
         const size_t v3 = 3*sizeof(double)*n;
-        const BoxT<double> bd = makeBox<double>(H_, recip_);
-        const BoxT<T> bt = makeBox<T>(H_, recip_);
+        const BoxSel<double> bd = boxSel<double>();
+        const BoxSel<T> bt = boxSel<T>();
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
         // per-atom buffers
         for (DevBuf* b : {&permD_, &edD_, &erD_, &bD_, &xD_, &rD_, &zD_, &pD_, &apD_, &tdD_, &trD_, &dD_, &etotD_,
@@ -980,59 +1338,47 @@ private:
             b->reserve(v3);
         for (DevBuf* b : {&permTD_, &pTD_, &dTD_})
             b->reserve(n*sizeof(T4));
+        perAtomD_.reserve((size_t) n*11*sizeof(double));
         // sorted positions (and, for FP32, the FP32 copy and radii)
         tb(kPhPrep);
-        T4* posT;
         if (sizeof(T) == sizeof(float)) {
             posFD_.reserve(n*sizeof(float4));
             launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
                                   posFD_.as<float4>());
-            posT = reinterpret_cast<T4*>(posFD_.p);
-        }
-        else {
-            launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
-                                  nullptr);
-            posT = reinterpret_cast<T4*>(posDD_.p);
-        }
-        T* rgT;
-        if (sizeof(T) == sizeof(float)) {
             scratchD_.reserve(n*sizeof(float));
             launchToFloat(st_, n, rgSD_.as<double>(), scratchD_.as<float>());
-            rgT = scratchD_.as<T>();
         }
         else
-            rgT = rgSD_.as<T>();
+            launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
+                                  nullptr);
+        T4* pos = posT<T>();
         launchPermDipoles(st_, n, orderD_.as<int>(), posDev_, covStartD_.as<int>(), covPartnerD_.as<int>(),
                           covStrengthD_.as<double>(), bd, permD_.as<double>());
         launchToVec4<T>(st_, n, permD_.as<double>(), permTD_.as<T4>());
         // PME bricks and splines
+        const PmeSel ps = pmeSel();
         brickOfD_.reserve(n*sizeof(int));
         pslotD_.reserve(n*sizeof(int));
-        brickCountD_.reserve((pme_.nbrick+1)*sizeof(int));
-        brickStartD_.reserve((pme_.nbrick+1)*sizeof(int));
+        brickCountD_.reserve((brickTotal_+1)*sizeof(int));
+        brickStartD_.reserve((brickTotal_+1)*sizeof(int));
         pmeAtomD_.reserve(n*sizeof(int));
         baseD_.reserve(3*n*sizeof(int));
-        splD_.reserve((size_t) n*9*pme_.order*sizeof(T));
-        gridD_.reserve((size_t) pme_.K[0]*pme_.K[1]*pme_.K[2]*sizeof(T));
-        cplxD_.reserve((size_t) pme_.K[0]*pme_.K[1]*pme_.K3h*sizeof(T2));
-        HIPCHK(hipMemsetAsync(brickCountD_.p, 0, (pme_.nbrick+1)*sizeof(int), st_));
-        launchPmeBin<T>(st_, n, posDD_.as<double4>(), bd, pme_, brickOfD_.as<int>(), pslotD_.as<int>(),
+        splD_.reserve((size_t) n*9*s_.order*sizeof(T));
+        HIPCHK(hipMemsetAsync(brickCountD_.p, 0, (brickTotal_+1)*sizeof(int), st_));
+        launchPmeBin<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                         brickCountD_.as<int>());
-        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), pme_.nbrick, nullptr);
-        launchPmePlace<T>(st_, n, posDD_.as<double4>(), bd, pme_, brickOfD_.as<int>(), pslotD_.as<int>(),
+        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_, nullptr);
+        launchPmePlace<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                           brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
         te();
         // partial-sum storage
-        const int pairBlocks = (int) (((long long) n*16+255)/256);
-        const int xBlocks = pme_.K[1]*pme_.K3h;   // upper bound of the convolution blocks
-        partD_.reserve((size_t) std::max({pairBlocks, xBlocks, (n+255)/256, 1})*11*sizeof(double));
+        partD_.reserve((size_t) std::max(partTotal_, 1)*10*sizeof(double));
         cgPartD_.reserve(kPRegions*kCgMaxBlocks*sizeof(double));
-        resD_.reserve(kResCount*sizeof(double));
-        resH_.reserve(kResCount*sizeof(double));
+        resD_.reserve(sys_.size()*kResCount*sizeof(double));
         cgD_.reserve(sizeof(CgState));
         doneD_.reserve((s_.max_iter+2)*sizeof(int));
         nbD_.reserve(n*sizeof(int));
-        HIPCHK(hipMemsetAsync(resD_.p, 0, kResCount*sizeof(double), st_));
+        HIPCHK(hipMemsetAsync(resD_.p, 0, sys_.size()*kResCount*sizeof(double), st_));
         // the initial guess: from the history when there is one, then its first product rides on the pair setup
         const bool induce = polarizable_ && s_.polarizable;
         const GuessCoef gc = guessCoefficients(newStep);
@@ -1047,29 +1393,32 @@ private:
             launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
             te();
         }
-        // field of the charges and permanent dipoles (and T x0): pair setup + PME
-        const T rc2 = (T) (s_.cutoff*s_.cutoff);
-        tb(kPhPairSetup);
-        launchPairSetup<T>(st_, n, posT, rgT, permTD_.as<T4>(), dual ? pTD_.as<T4>() : nullptr,
-                           candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
-                           g03D_.as<T2>(), bt, rc2, (T) s_.beta, edD_.as<double>(), tdD_.as<double>());
+        // field of the charges and permanent dipoles (and T x0): compaction, pair setup + PME
+        tb(kPhPairCompact);
+        launchPairCompact<T>(st_, n, pos, candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), bt,
+                             (T) (s_.cutoff*s_.cutoff), lanes_);
         te();
-        pmeSolve<T>(permTD_.as<T4>(), true, posT, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
+        tb(kPhPairSetup);
+        launchPairSetup<T>(st_, n, pos, rgT<T>(), permTD_.as<T4>(), dual ? pTD_.as<T4>() : nullptr,
+                           candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
+                           (T) s_.beta, edD_.as<double>(), tdD_.as<double>(), lanes_);
+        te();
+        pmeSolve<T>(permTD_.as<T4>(), true, pos, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
         // induced dipoles
         int pushSlot = -1;
         if (induce) {
             if (dual)
-                pmeSolve<T>(pTD_.as<T4>(), false, posT, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
+                pmeSolve<T>(pTD_.as<T4>(), false, pos, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
             else {
                 tb(kPhCg);
                 launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
                             alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
                 launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
                 te();
-                matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
+                matvec<T>(pos, pTD_.as<T4>(), nullptr);
             }
-            iterations_ = cgSolve<T>(posT, bt, bD_.as<double>(), xD_.as<double>(), lastIterations_, residual_);
+            iterations_ = cgSolve<T>(pos, bD_.as<double>(), xD_.as<double>(), lastIterations_, residual_);
             const int h = (int) histSlots_.size();
             const bool push = !frozen_ && (newStep || h == 0);
             if (push)
@@ -1091,24 +1440,21 @@ private:
         // final stage
         double* res = resD_.as<double>();
         tb(kPhPairFinal);
-        const int nbPair = launchPairFinal<T>(st_, n, posT, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(),
-                                              pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
-                                              fpairD_.as<double>(), edD_.as<double>(),
-                                              partD_.as<double>());
-        launchReducePartials(st_, partD_.as<double>(), nbPair, 10, res+kResPair);
+        launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
+                           g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(), edD_.as<double>(),
+                           perAtomD_.as<double>(), lanes_);
+        systemSums(perAtomD_.as<double>(), 10, res, kResPair);
         te();
-        pmeSolve<T>(dTD_.as<T4>(), true, posT, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr,
-                    res+kResRec);
+        pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
         tb(kPhFinal);
-        const int nbComb = launchFinalCombine(st_, n, dD_.as<double>(), xD_.as<double>(), alphaSD_.as<double>(),
-                                              edD_.as<double>(), erD_.as<double>(), cself, etotD_.as<double>(),
-                                              partD_.as<double>());
-        launchReducePartials(st_, partD_.as<double>(), nbComb, 11, res+kResComb);
-        const int nbCov = launchCovalentForces(st_, n, rankD_.as<int>(), posDev_, covStartD_.as<int>(),
-                                               covPartnerD_.as<int>(), covStrengthD_.as<double>(),
-                                               revStartD_.as<int>(), revEntryD_.as<int>(), covAtomOfD_.as<int>(),
-                                               etotD_.as<double>(), bd, fcovD_.as<double>(), partD_.as<double>());
-        launchReducePartials(st_, partD_.as<double>(), nbCov, 9, res+kResCov);
+        launchFinalCombine(st_, n, dD_.as<double>(), xD_.as<double>(), alphaSD_.as<double>(), edD_.as<double>(),
+                           erD_.as<double>(), cself, etotD_.as<double>(), perAtomD_.as<double>());
+        systemSums(perAtomD_.as<double>(), 11, res, kResComb);
+        launchCovalentForces(st_, n, rankD_.as<int>(), posDev_, covStartD_.as<int>(), covPartnerD_.as<int>(),
+                             covStrengthD_.as<double>(), revStartD_.as<int>(), revEntryD_.as<int>(),
+                             covAtomOfD_.as<int>(), etotD_.as<double>(), bd, fcovD_.as<double>(),
+                             perAtomD_.as<double>());
+        systemSums(perAtomD_.as<double>(), 9, res, kResCov);
         te();
         if (pushSlot >= 0) {
             histSlots_.push_front(pushSlot);
@@ -1118,24 +1464,55 @@ private:
         haveResults_ = true;
     }
 
-    // Parameter gradients after an evaluation at the same positions (the sorted state, lists and stored kernels
-    // of that evaluation are reused; the PME grids are recomputed).  gCaller: dL/dF (caller's order, device), or
-    // null for the energy gradient alone; wE: dL/dE.  Outputs in the caller's order (device, each may be null):
-    // gq, gR, galpha per atom, gc per covalent entry in input order.
+    // Milliseconds per evaluation spent in the pair passes at the given lane count (current lists and dipoles).
     template<class T>
-    void adjoint(const double* gCaller, double wE, double* gq, double* gR, double* galpha, double* gc) {
+    double timePairPasses(int lanes, int reps) {
         typedef typename Vec4<T>::type T4;
         typedef typename Vec2<T>::type T2;
         const int n = n_;
+        const BoxSel<T> bt = boxSel<T>();
+        T4* pos = posT<T>();
+        hipEvent_t a = nextEvent(), b = nextEvent();
+        auto once = [&] {
+            launchPairCompact<T>(st_, n, pos, candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
+                                 bt, (T) (s_.cutoff*s_.cutoff), lanes);
+            launchPairSetup<T>(st_, n, pos, rgT<T>(), permTD_.as<T4>(), pTD_.as<T4>(), candStartD_.as<int>(),
+                               nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, (T) s_.beta,
+                               edD_.as<double>(), tdD_.as<double>(), lanes);
+            for (int k = 0; k <= iterations_; k++)
+                launchPairField<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
+                                   g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes);
+            launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
+                               g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(), edD_.as<double>(),
+                               perAtomD_.as<double>(), lanes);
+        };
+        once();   // warm-up
+        HIPCHK(hipEventRecord(a, st_));
+        for (int r = 0; r < reps; r++)
+            once();
+        HIPCHK(hipEventRecord(b, st_));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        evUsed_ -= 2;
+        return ms/reps;
+    }
+
+    // Parameter gradients after an evaluation at the same positions (the sorted state, lists and stored kernels
+    // of that evaluation are reused; the PME grids are recomputed).  gCaller: dL/dF (caller's order, device), or
+    // null for the energy gradient alone; wE: dL/dE per system.  Outputs in the caller's order (device, each may
+    // be null): gq, gR, galpha per atom, gc per covalent entry in input order.
+    template<class T>
+    void adjoint(const double* gCaller, const double* wE, double* gq, double* gR, double* galpha, double* gc) {
+        typedef typename Vec4<T>::type T4;
+        const int n = n_;
         const size_t v3 = 3*sizeof(double)*n;
-        const BoxT<double> bd = makeBox<double>(H_, recip_);
-        const BoxT<T> bt = makeBox<T>(H_, recip_);
+        const BoxSel<double> bd = boxSel<double>();
+        const BoxSel<T> bt = boxSel<T>();
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
         const bool tan = gCaller != nullptr;
         const bool induce = polarizable_ && s_.polarizable;
-        T4* posT = reinterpret_cast<T4*>(sizeof(T) == sizeof(float) ? posFD_.p : posDD_.p);
-        const T* rgT = sizeof(T) == sizeof(float) ? scratchD_.as<T>() : rgSD_.as<T>();
-        const size_t nreal = (size_t) pme_.K[0]*pme_.K[1]*pme_.K[2];
+        T4* pos = posT<T>();
         pairOutD_.reserve(10*sizeof(double)*n);
         pmeOutD_.reserve(10*sizeof(double)*n);
         for (DevBuf* b : {&gSD_, &pdotD_, &lamD_, &adjBD_, &ddotD_, &etotDotD_})
@@ -1143,29 +1520,23 @@ private:
         for (DevBuf* b : {&gTD_, &ddotTD_})
             b->reserve(n*sizeof(T4));
         if (tan)
-            gridTD_.reserve(nreal*sizeof(T));
+            gridTD_.reserve((size_t) realTotal_*sizeof(T));
+        if (batch_)
+            upload(wED_, wE, sys_.size()*sizeof(double));
         // the potential / field of the charges and total dipoles (and their tangents for the displacement g and
-        // the dipole change ddT): pair pass over the stored kernels, the PME grid and its tangent grid
+        // the dipole change ddT): pair pass over the stored kernels, the PME grids and their tangent grids
         auto tangentPass = [&](const T4* ddT) {
-            launchPairTangent<T>(st_, n, posT, rgT, dTD_.as<T4>(), ddT, ddT ? gTD_.as<T4>() : nullptr,
-                                 candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
-                                 g03D_.as<T2>(), bt, pairOutD_.as<double>());
-            launchZero<T>(st_, gridD_.as<T>(), nreal, nullptr);
-            launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
-                            dTD_.as<T4>(), posT, 1, bd, gridD_.as<T>(), nullptr);
-            launchPmeConvolve<T>(st_, pme_, gridD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), bd,
-                                 s_.beta, 0, 0, partD_.as<double>(), nullptr);
-            if (ddT) {
-                launchZero<T>(st_, gridTD_.as<T>(), nreal, nullptr);
-                launchSpread<T>(st_, pme_, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(),
-                                splD_.as<T>(), dTD_.as<T4>(), posT, 1, bd, gridTD_.as<T>(), nullptr, ddT,
-                                gTD_.as<T4>());
-                launchPmeConvolve<T>(st_, pme_, gridTD_.as<T>(), cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), bd,
-                                     s_.beta, 0, 0, partD_.as<double>(), nullptr);
-            }
-            launchGatherTangent<T>(st_, pme_, n, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
-                                   gridD_.as<T>(), ddT ? gridTD_.as<T>() : nullptr, bd,
-                                   ddT ? gTD_.as<T4>() : nullptr, pmeOutD_.as<double>());
+            launchPairTangent<T>(st_, n, pos, rgT<T>(), dTD_.as<T4>(), ddT, ddT ? gTD_.as<T4>() : nullptr,
+                                 candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
+                                 g12D_.as<typename Vec2<T>::type>(), g03D_.as<typename Vec2<T>::type>(), bt,
+                                 pairOutD_.as<double>());
+            pmeGrids<T>(dTD_.as<T4>(), true, pos, 0, false, gridD_.as<T>(), nullptr, nullptr, -1);
+            if (ddT)
+                pmeGrids<T>(dTD_.as<T4>(), true, pos, 0, false, gridTD_.as<T>(), nullptr, nullptr, -1, ddT,
+                            gTD_.as<T4>());
+            launchGatherTangent<T>(st_, pmeSel(), n, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
+                                   gridD_.as<T>(), ddT ? gridTD_.as<T>() : nullptr, ddT ? gTD_.as<T4>() : nullptr,
+                                   pmeOutD_.as<double>());
         };
         adjIterations_ = 0;
         adjResidual_ = 0;
@@ -1184,8 +1555,8 @@ private:
                 launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), nullptr, 0, GuessCoef{},
                             alphaSD_.as<double>(), adjBD_.as<double>(), lamD_.as<double>());
                 launchToVec4<T>(st_, n, lamD_.as<double>(), pTD_.as<T4>());
-                matvec<T>(posT, pTD_.as<T4>(), bt, nullptr);
-                adjIterations_ = cgSolve<T>(posT, bt, adjBD_.as<double>(), lamD_.as<double>(), lastAdjIterations_,
+                matvec<T>(pos, pTD_.as<T4>(), nullptr);
+                adjIterations_ = cgSolve<T>(pos, adjBD_.as<double>(), lamD_.as<double>(), lastAdjIterations_,
                                             adjResidual_);
                 lambda = lamD_.as<double>();
                 launchDipoleTangent<T>(st_, n, pdotD_.as<double>(), lambda, ddotD_.as<double>(), ddotTD_.as<T4>());
@@ -1206,10 +1577,13 @@ private:
         a.pairOut = pairOutD_.as<double>();
         a.pmeOut = pmeOutD_.as<double>();
         a.ke = s_.ke;
-        a.wE = wE;
+        a.wE = wE[0];
         a.cself = cself;
         a.qSelf = 2.0*s_.beta/kSqrtPi;
-        a.qBackground = kPi*Qtot_/(volume_*s_.beta*s_.beta);
+        a.qBackground = kPi*sys_[0].Qtot/(sys_[0].volume*s_.beta*s_.beta);
+        a.qBackSys = batch_ ? qBackD_.as<double>() : nullptr;
+        a.wESys = batch_ ? wED_.as<double>() : nullptr;
+        a.sysOf = batch_ ? sysOfD_.as<int>() : nullptr;
         a.gq = gq;
         a.galpha = galpha;
         a.gR = gR;
@@ -1219,27 +1593,31 @@ private:
         if (gc)
             launchParamCov(st_, covN_, covAtomOfD_.as<int>(), covPartnerD_.as<int>(), covInputD_.as<int>(),
                            rankD_.as<int>(), posDev_, gCaller, etotD_.as<double>(), etotDotD_.as<double>(), bd, s_.ke,
-                           wE, gc);
+                           wE[0], batch_ ? wED_.as<double>() : nullptr, gc);
     }
 
     void assemble(bool wantStrain) {
-        const double* r = resH_.as<double>();
+        const double* rAll = resH_.as<double>();
         const double ke = s_.ke;
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
-        terms_.direct = ke*r[kResPair];
-        terms_.reciprocal = ke*r[kResRec];
-        terms_.self = ke*(-(s_.beta/kSqrtPi)*q2_-0.5*cself*r[kResComb]);
-        const double bg = -kPi*Qtot_*Qtot_/(2.0*volume_*s_.beta*s_.beta);
-        terms_.background = ke*bg;
-        terms_.polarization = ke*r[kResComb+1];
-        terms_.gvdw = 0;
-        if (wantStrain) {
-            for (int k = 0; k < 9; k++)
-                strain_[k] = r[kResPair+1+k]+r[kResRec+1+k]+r[kResComb+2+k]+r[kResCov+k];
-            for (int a = 0; a < 3; a++)
-                strain_[4*a] -= bg;
-            for (int k = 0; k < 9; k++)
-                strain_[k] *= ke;
+        for (size_t s = 0; s < sys_.size(); s++) {
+            Sys& y = sys_[s];
+            const double* r = rAll+s*kResCount;
+            y.terms.direct = ke*r[kResPair];
+            y.terms.reciprocal = ke*r[kResRec];
+            y.terms.self = ke*(-(s_.beta/kSqrtPi)*y.q2-0.5*cself*r[kResComb]);
+            const double bg = -kPi*y.Qtot*y.Qtot/(2.0*y.volume*s_.beta*s_.beta);
+            y.terms.background = ke*bg;
+            y.terms.polarization = ke*r[kResComb+1];
+            y.terms.gvdw = 0;
+            if (wantStrain) {
+                for (int k = 0; k < 9; k++)
+                    y.strain[k] = r[kResPair+1+k]+r[kResRec+1+k]+r[kResComb+2+k]+r[kResCov+k];
+                for (int a = 0; a < 3; a++)
+                    y.strain[4*a] -= bg;
+                for (int k = 0; k < 9; k++)
+                    y.strain[k] *= ke;
+            }
         }
     }
 };
@@ -1273,6 +1651,11 @@ int guarded(pgm_engine* e, Fn fn) {
         e->eng->lastError = ex.what();
         return -1;
     }
+}
+
+void requireSingle(const DeviceEngine& g) {
+    if (g.numSystems() > 1)
+        throw std::invalid_argument("pgm_hip: the engine holds a batch of systems: use the batch entry points");
 }
 }   // namespace
 
@@ -1332,6 +1715,7 @@ int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partne
 int pgm_compute(pgm_engine* e, const double* pos, const double* box, int new_step, double* f, double* strain,
                 pgm_terms* terms) {
     return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
         g.evaluate(PosSource::HostF64, pos, box, new_step != 0, f, nullptr, nullptr, 0, 0.0, strain, terms, nullptr,
                    false);
     });
@@ -1340,6 +1724,7 @@ int pgm_compute(pgm_engine* e, const double* pos, const double* box, int new_ste
 int pgm_compute_f32(pgm_engine* e, const float* pos, const double* box, int new_step, double* strain,
                     pgm_terms* terms) {
     return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
         g.evaluate(PosSource::HostF32, pos, box, new_step != 0, nullptr, nullptr, nullptr, 0, 0.0, strain, terms,
                    nullptr, true);
     });
@@ -1357,6 +1742,7 @@ int pgm_forces(pgm_engine* e, double* out) {
 int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                        int stride, double fixed_scale, void* stream, pgm_terms* terms) {
     return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
         g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
                    nullptr, terms, static_cast<hipStream_t>(stream), false);
     });
@@ -1365,6 +1751,7 @@ int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int 
 int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                       int stride, double fixed_scale, void* stream) {
     return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
         g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
                    nullptr, nullptr, static_cast<hipStream_t>(stream), false);
     });
@@ -1372,7 +1759,6 @@ int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int n
 
 int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
     return guarded(e, [&](DeviceEngine& g) {
-        (void) g;
         if (terms)
             *terms = g.lastTerms();
         if (valid)
@@ -1383,6 +1769,7 @@ int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
 int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,
                            double* strain, void* stream, pgm_terms* terms) {
     return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
         g.evaluate(PosSource::DeviceF64, pos_dev, box, new_step != 0, nullptr, f_dev, nullptr, 0, 0.0, strain, terms,
                    static_cast<hipStream_t>(stream), false);
     });
@@ -1421,7 +1808,8 @@ int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets) {
 int pgm_backward(pgm_engine* e, const double* pos, const double* box, const double* dl_dforces, double dl_denergy,
                  double* grad_q, double* grad_radius, double* grad_alpha, double* grad_cov) {
     return guarded(e, [&](DeviceEngine& g) {
-        g.backward(PosSource::HostF64, pos, box, dl_dforces, false, dl_denergy, grad_q, grad_radius, grad_alpha,
+        requireSingle(g);
+        g.backward(PosSource::HostF64, pos, box, dl_dforces, false, &dl_denergy, grad_q, grad_radius, grad_alpha,
                    grad_cov, false, nullptr);
     });
 }
@@ -1430,13 +1818,74 @@ int pgm_backward_device(pgm_engine* e, const double* pos_dev, const double* box,
                         double dl_denergy, double* grad_q_dev, double* grad_radius_dev, double* grad_alpha_dev,
                         double* grad_cov_dev, void* stream) {
     return guarded(e, [&](DeviceEngine& g) {
-        g.backward(PosSource::DeviceF64, pos_dev, box, dl_dforces_dev, true, dl_denergy, grad_q_dev, grad_radius_dev,
+        requireSingle(g);
+        g.backward(PosSource::DeviceF64, pos_dev, box, dl_dforces_dev, true, &dl_denergy, grad_q_dev, grad_radius_dev,
                    grad_alpha_dev, grad_cov_dev, true, static_cast<hipStream_t>(stream));
     });
 }
 
 int pgm_adjoint_iterations(const pgm_engine* e) { return (e && e->eng) ? e->eng->adjointIterations() : -1; }
 double pgm_adjoint_residual(const pgm_engine* e) { return (e && e->eng) ? e->eng->adjointResidual() : -1.0; }
+
+int pgm_set_batch(pgm_engine* e, int nsys, const int* atom_offsets, const int* grids) {
+    return guarded(e, [&](DeviceEngine& g) { g.setSystems(nsys, atom_offsets, grids); });
+}
+
+int pgm_num_systems(const pgm_engine* e) { return (e && e->eng) ? e->eng->numSystems() : -1; }
+
+int pgm_compute_batch(pgm_engine* e, const double* pos, const double* boxes, int new_step, double* f,
+                      pgm_terms* terms) {
+    return guarded(e, [&](DeviceEngine& g) {
+        g.evaluate(PosSource::HostF64, pos, boxes, new_step != 0, f, nullptr, nullptr, 0, 0.0, nullptr, terms,
+                   nullptr, false);
+    });
+}
+
+int pgm_compute_batch_device(pgm_engine* e, const double* pos_dev, const double* boxes, int new_step, double* f_dev,
+                             void* stream, pgm_terms* terms) {
+    return guarded(e, [&](DeviceEngine& g) {
+        g.evaluate(PosSource::DeviceF64, pos_dev, boxes, new_step != 0, nullptr, f_dev, nullptr, 0, 0.0, nullptr,
+                   terms, static_cast<hipStream_t>(stream), false);
+    });
+}
+
+int pgm_backward_batch(pgm_engine* e, const double* pos, const double* boxes, const double* dl_dforces,
+                       const double* dl_denergy, double* grad_q, double* grad_radius, double* grad_alpha,
+                       double* grad_cov) {
+    return guarded(e, [&](DeviceEngine& g) {
+        std::vector<double> wE(g.numSystems(), 0.0);
+        if (dl_denergy)
+            std::copy(dl_denergy, dl_denergy+g.numSystems(), wE.begin());
+        g.backward(PosSource::HostF64, pos, boxes, dl_dforces, false, wE.data(), grad_q, grad_radius, grad_alpha,
+                   grad_cov, false, nullptr);
+    });
+}
+
+int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double* boxes,
+                              const double* dl_dforces_dev, const double* dl_denergy, double* grad_q_dev,
+                              double* grad_radius_dev, double* grad_alpha_dev, double* grad_cov_dev, void* stream) {
+    return guarded(e, [&](DeviceEngine& g) {
+        std::vector<double> wE(g.numSystems(), 0.0);
+        if (dl_denergy)
+            std::copy(dl_denergy, dl_denergy+g.numSystems(), wE.begin());
+        g.backward(PosSource::DeviceF64, pos_dev, boxes, dl_dforces_dev, true, wE.data(), grad_q_dev,
+                   grad_radius_dev, grad_alpha_dev, grad_cov_dev, true, static_cast<hipStream_t>(stream));
+    });
+}
+
+int pgm_autotune(pgm_engine* e, int reps, int* lanes, double* ms) {
+    return guarded(e, [&](DeviceEngine& g) {
+        const int best = g.autotune(reps);
+        if (lanes)
+            *lanes = best;
+        if (ms)
+            std::copy(g.tuneTimes(), g.tuneTimes()+3, ms);
+    });
+}
+
+int pgm_set_lanes(pgm_engine* e, int lanes) {
+    return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes); });
+}
 
 int pgm_set_timing(pgm_engine* e, int on) {
     return guarded(e, [&](DeviceEngine& g) { g.setTiming(on != 0); });

@@ -8,6 +8,25 @@
 
 namespace pgmhip {
 
+// -------------------------------------------------------------------------------------------- systems
+
+// The box of an atom.  A batch packs several independent periodic systems into one set of arrays: the atoms
+// of a system are contiguous, with the same index range in the caller's and in the sorted order, so one
+// system index per atom index serves both orders.  table == nullptr: a single system with box.
+template<class T>
+struct BoxSel {
+    BoxT<T> box;
+    const BoxT<T>* table;
+    const int* sysOf;
+};
+
+template<bool BATCH, class T>
+__device__ __forceinline__ BoxT<T> selectBox(const BoxSel<T>& b, int idx) {
+    if (BATCH)
+        return b.table[b.sysOf[idx]];
+    return b.box;
+}
+
 // -------------------------------------------------------------------------------------------- pair list
 
 struct CellGrid {
@@ -23,54 +42,62 @@ void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box
 // Exclusive scan of m ints into out[0..m] (out[m] = total); scratch: >= 1024 ints.
 void launchScan(hipStream_t st, const int* in, int* out, int m, int* scratch);
 // Sorted order: atoms grouped by cell, inside a cell by (2x2x2 sub-cell, original index).
-void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order);
+// base: added to the atom indices written (the system's first atom in a batch).
+void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order,
+                       int base);
 void launchCellSort(hipStream_t st, int ncell, const int* cellStart, const int* subOf, int* order);
-void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf);
+void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf, int base);
 // Full candidate lists (both directions) of the sorted atoms within rl: counts, then the lists.
 void launchCandCount(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
                      CellGrid g, BoxT<double> box, double rl2, int* count);
 void launchCandFill(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
-                    CellGrid g, BoxT<double> box, double rl2, const int* candStart, int* cand);
+                    CellGrid g, BoxT<double> box, double rl2, const int* candStart, int* cand, int base);
 // Largest squared displacement (minimum image) since the list build, as float bits in *maxBits.
-void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const double* posBuild, BoxT<double> box,
+void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const double* posBuild, BoxSel<double> box,
                            unsigned* maxBits);
 
 // -------------------------------------------------------------------------------------------- per atom
 
 // Sorted positions wrapped into the box (xyz, w = charge) in FP64 and, for the mixed path, FP32.
 void launchGatherPositions(hipStream_t st, int n, const int* order, const double* pos, const double* qS,
-                           BoxT<double> box, double4* posD, float4* posF);
-void launchGatherPositionsF4(hipStream_t st, int n, const int* order, const float4* posq, const double* qS,
-                             BoxT<double> box, double4* posD, float4* posF, double* posOrig);
+                           BoxSel<double> box, double4* posD, float4* posF);
 void launchPermute(hipStream_t st, int n, const int* order, const double* in, double* out);
 void launchToFloat(hipStream_t st, int n, const double* in, float* out);
 // Caller-order float4 (xyz, w ignored) or float xyz positions to FP64 xyz.
 void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out);
 // Permanent dipoles of the sorted atoms from the covalent CSR (caller's order, positions in FP64).
 void launchPermDipoles(hipStream_t st, int n, const int* order, const double* pos, const int* covStart,
-                       const int* covPartner, const double* covStrength, BoxT<double> box, double* perm);
+                       const int* covPartner, const double* covStrength, BoxSel<double> box, double* perm);
 
 // -------------------------------------------------------------------------------------------- pair passes
 
-// Per evaluation: every atom's candidates compacted in place to the pairs within rc (pj: partner, g12: G1, G2,
-// g03: G0, G3, in the atom's candidate segment), and the direct field of the charges and dipoles dip and, when
-// dual is given, of the dipoles dual alone (the first CG product).  nb: pairs per atom.
+// Lanes per atom of the pair passes: 8, 16 or 32 (picked per engine by the autotuner, 16 by default).
+constexpr int kPairLaneChoices[3] = {8, 16, 32};
+
+// Per evaluation: every atom's candidates compacted in place to the pairs within rc (pj: partner; nb: pairs
+// per atom), in the atom's candidate segment.
+template<class T>
+void launchPairCompact(hipStream_t st, int n, const typename Vec4<T>::type* pos, const int* candStart,
+                       const int* cand, int* nb, int* pj, BoxSel<T> box, T rc2, int lanes);
+// Per evaluation, over the compacted pairs: the kernels (g12: G1, G2; g03: G0, G3), the direct field of the
+// charges and dipoles dip and, when dual is given, of the dipoles dual alone (the first CG product).
 template<class T>
 void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                      const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
-                     const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
-                     BoxT<T> box, T rc2, T beta, double* field, double* fieldDual);
+                     const int* nb, const int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
+                     BoxSel<T> box, T beta, double* field, double* fieldDual, int lanes);
 // The dipole part of a CG matrix-vector product from the stored G1, G2.
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     BoxT<T> box, double* field, const int* skip);
-// Energy, forces, field and strain of the pairs at the total dipoles from the stored G0..G3; per-block
-// partials [nblk][10] (energy, strain 0..8, halved), returns the number of blocks written.
+                     BoxSel<T> box, double* field, const int* skip, int lanes);
+// Energy, forces, field and strain of the pairs at the total dipoles from the stored G0..G3; per atom
+// (sorted order) perAtom[n][10] = energy, strain 0..8 (each pair's half).
 template<class T>
-int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
-                    const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                    const typename Vec2<T>::type* g03, BoxT<T> box, double* force, double* field, double* partials);
+void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
+                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
+                     const typename Vec2<T>::type* g03, BoxSel<T> box, double* force, double* field,
+                     double* perAtom, int lanes);
 
 // Parameter derivatives of the pair energy at fixed dipoles and, with dipDot (and disp) given, their tangents
 // along the displacement disp and the dipole change dipDot, sorted order, each atom from its own visits of its
@@ -79,7 +106,7 @@ template<class T>
 void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                        const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dipDot,
                        const typename Vec4<T>::type* disp, const int* candStart, const int* nb, const int* pj,
-                       const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box,
+                       const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
                        double* out);
 
 // -------------------------------------------------------------------------------------------- PME
@@ -101,36 +128,60 @@ struct PmeGrid {
     FftPlan plan[3];
 };
 
+// One system's PME grid in a batch: its grid, box, the offset of its real grid and its first global brick.
+struct PmeSys {
+    PmeGrid g;
+    BoxT<double> box;
+    long long gridOff;
+    int brickOff;
+    int pad;
+};
+// The grid of an atom or a brick: one system (g, box), or per system from a table (batch).
+struct PmeSel {
+    PmeGrid g;
+    BoxT<double> box;
+    const PmeSys* table;    // batch: per system; nullptr: a single system
+    const int* sysOf;       // batch: system of every (sorted) atom index
+    const int* brickSys;    // batch: system of every global brick
+    int nbrick;             // bricks of all systems
+    int order;
+    int brick;
+};
+
 // Binning of the sorted atoms into spreading bricks and their spline weights, in brick order.
 template<class T>
-void launchPmeBin(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, int* brickOf, int* slot,
-                  int* brickCount);
+void launchPmeBin(hipStream_t st, int n, const double4* posD, PmeSel ps, int* brickOf, int* slot, int* brickCount);
 template<class T>
-void launchPmePlace(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, const int* brickOf,
-                    const int* slot, const int* brickStart, int* pmeAtom, int* base, T* spl);
-// Spread charges (qScale 0: none) and dipoles of the sorted atoms onto the real grid (zeroed by the caller).
+void launchPmePlace(hipStream_t st, int n, const double4* posD, PmeSel ps, const int* brickOf, const int* slot,
+                    const int* brickStart, int* pmeAtom, int* base, T* spl);
+// Spread charges (withQ) and dipoles of the sorted atoms onto the real grids (zeroed by the caller) or, with
+// dipDot, the change of that density for displacements disp and dipole changes dipDot.
 template<class T>
-void launchSpread(hipStream_t st, PmeGrid g, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
-                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, BoxT<double> box,
-                  T* grid, const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
+void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
+                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, T* grid,
+                  const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
                   const typename Vec4<T>::type* disp = nullptr);
-// Forward transform, energy (and k-space strain) with the influence function, convolution, back transform.
-// partials: per-block [nblk][10] (energy, strain 0..8) when energy != 0; returns the number of blocks.
+// ngrid stacked grids of one shape g (contiguous real and half-spectrum grids and influence functions): forward
+// transform, energy (and k-space strain) with the influence function, convolution, back transform.  boxes:
+// per stacked grid (nullptr: box for all).  partials: per block [nblk][10] (energy, strain 0..8) when energy !=
+// 0, the blocks of grid k being [k nblk/ngrid, (k+1) nblk/ngrid); returns nblk.
 template<class T>
-int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type* cplx, const T* influence,
-                      const typename Vec2<T>::type* twiddles, BoxT<double> box, double beta, int energy, int strain,
-                      double* partials, const int* skip);
+int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Vec2<T>::type* cplx, const T* influence,
+                      const typename Vec2<T>::type* twiddles, BoxT<double> box, const BoxT<double>* boxes, double beta,
+                      int energy, int strain, double* partials, const int* skip);
+// Blocks of the x pass per grid (the rows of its energy partials).
+template<class T>
+int convolveBlocksPerGrid(PmeGrid g);
 // Field -grad phi (sorted order) and, when final, the reciprocal forces.
 template<class T>
-void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
-                  BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
-                  double* field, double* force, const int* skip);
+void launchGather(hipStream_t st, PmeSel ps, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
+                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_, double* field,
+                  double* force, const int* skip);
 // Potential and field of the convolved grid phi at the atoms and, with phiDot, their tangents for atoms displaced
 // by disp: out (sorted order) phi[n], E[3n] at offsets 0 and n, phi'[n], E'[3n] at offsets 5n and 6n.
 template<class T>
-void launchGatherTangent(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl,
-                         const T* phi, const T* phiDot, BoxT<double> box, const typename Vec4<T>::type* disp,
-                         double* out);
+void launchGatherTangent(hipStream_t st, PmeSel ps, int n, const int* pmeAtom, const int* base, const T* spl,
+                         const T* phi, const T* phiDot, const typename Vec4<T>::type* disp, double* out);
 template<class T>
 void launchInfluence(hipStream_t st, PmeGrid g, const double* bmod, BoxT<double> box, double beta, double volume,
                      T* influence);
@@ -183,17 +234,22 @@ void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, c
 template<class T>
 void launchPost(hipStream_t st, int n, double peek, const double* alpha, const double* r, double* x, const double* perm,
                 const int* order, const int* gidx, double* hist, double* d, typename Vec4<T>::type* dT);
-// Total field, and per-block sums [nblk][11]: |d|^2, sum mu^2/(2 alpha), E_rec (x) d (row-major).
-int launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
-                       const double* edir, const double* erec, double cself, double* etot, double* partials);
-// Covalent chain rule per atom (caller's order); per-block strain sums [nblk][9].
-int launchCovalentForces(hipStream_t st, int n, const int* rank, const double* pos, const int* covStart,
-                         const int* covPartner, const double* covStrength, const int* revStart, const int* revEntry,
-                         const int* covAtomOf, const double* etot, BoxT<double> box, double* fcov, double* partials);
+// Total field, and per atom perAtom[n][11]: |d|^2, mu^2/(2 alpha), E_rec (x) d (row-major).
+void launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
+                        const double* edir, const double* erec, double cself, double* etot, double* perAtom);
+// Covalent chain rule per atom (caller's order); per atom strain perAtom[n][9].
+void launchCovalentForces(hipStream_t st, int n, const int* rank, const double* pos, const int* covStart,
+                          const int* covPartner, const double* covStrength, const int* revStart, const int* revEntry,
+                          const int* covAtomOf, const double* etot, BoxSel<double> box, double* fcov,
+                          double* perAtom);
 void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair, const double* frec,
                      const double* fcov, double ke, double* fOut, long long* fixedOut, int stride, double fixedScale);
 // Column sums of partials [nblk][width] into out[0..width).
 void launchReducePartials(hipStream_t st, const double* partials, int nblk, int width, double* out);
+// Column sums of row ranges: out[outRow(r)*outStride + c] = sum of rows[k][c], k in [starts[r], starts[r+1]),
+// for r < nr, c < width; outRow(r) = outIndex[r], or r when outIndex is null.  One block per range.
+void launchRangeSums(hipStream_t st, const double* rows, int width, const int* starts, int nr, const int* outIndex,
+                     int outStride, double* out);
 // In the caller's order: out[i] = v[rank[i]] (3-vectors).
 void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out);
 
@@ -203,7 +259,7 @@ void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, doub
 void launchSort3(hipStream_t st, int n, const int* order, const double* v, double* out);
 // Change of the sorted permanent dipoles when the atoms move by disp (caller's order, FP64 xyz).
 void launchPermTangent(hipStream_t st, int n, const int* order, const double* pos, const double* disp,
-                       const int* covStart, const int* covPartner, const double* covStrength, BoxT<double> box,
+                       const int* covStart, const int* covPartner, const double* covStrength, BoxSel<double> box,
                        double* pdot);
 // b = -(E'_pair + E'_rec + cself p') on polarizable rows (pair / PME tangent outputs, E' at offset 6n).
 void launchAdjointRhs(hipStream_t st, int n, const double* alpha, const double* pairOut, const double* pmeOut,
@@ -220,7 +276,10 @@ struct ParamAtomArgs {
     const double *pairOut, *pmeOut;                     // pair / PME tangent-pass outputs
     double ke, wE, cself;
     double qSelf;                                       // 2 beta / sqrt(pi)
-    double qBackground;                                 // pi Q / (V beta^2)
+    double qBackground;                                 // pi Q / (V beta^2) (single system)
+    const double* qBackSys;                             // batch: pi Q / (V beta^2) per system
+    const double* wESys;                                // batch: dL/dE per system
+    const int* sysOf;                                   // batch: system of every atom index
     double *gq, *galpha, *gR;                           // caller's order (each may be null)
     double *etot, *etotDot;                             // sorted total field and its tangent
 };
@@ -229,6 +288,7 @@ void launchParamAtoms(hipStream_t st, const ParamAtomArgs& a);
 // Covalent-strength gradients per entry, written at the entry's input index.
 void launchParamCov(hipStream_t st, int ncov, const int* covAtomOf, const int* covPartner, const int* inputOf,
                     const int* rank, const double* pos, const double* disp, const double* etot,
-                    const double* etotDot, BoxT<double> box, double ke, double wE, double* gc);
+                    const double* etotDot, BoxSel<double> box, double ke, double wE, const double* wESys,
+                    double* gc);
 
 }   // namespace pgmhip

@@ -66,10 +66,10 @@ __global__ void __launch_bounds__(1024) kScan(const int* in, int* out, int m) {
         out[m] = (int) part[1023];
 }
 
-__global__ void kCellScatter(int n, const int* cellOf, const int* slot, const int* cellStart, int* order) {
+__global__ void kCellScatter(int n, const int* cellOf, const int* slot, const int* cellStart, int* order, int base) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i < n)
-        order[cellStart[cellOf[i]]+slot[i]] = i;
+        order[cellStart[cellOf[i]]+slot[i]] = i+base;
 }
 
 // Inside every cell: ascending (sub-cell, atom index) (the atomic slots above are not deterministic).
@@ -90,12 +90,12 @@ __global__ void kCellSort(int ncell, const int* cellStart, const int* subOf, int
     }
 }
 
-__global__ void kRank(int n, const int* order, int* rank, int* sortedCell, const int* cellOf) {
+__global__ void kRank(int n, const int* order, int* rank, int* sortedCell, const int* cellOf, int base) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
     const int i = order[s];
-    rank[i] = s;
+    rank[i] = s+base;
     sortedCell[s] = cellOf[i];
 }
 
@@ -103,7 +103,7 @@ __global__ void kRank(int n, const int* order, int* rank, int* sortedCell, const
 template<bool FILL, int L>
 __global__ void __launch_bounds__(256) kCandidates(int n, const double4* posS, const int* sortedCell,
                                                    const int* cellStart, CellGrid g, BoxT<double> box, double rl2,
-                                                   int* count, const int* candStart, int* cand) {
+                                                   int* count, const int* candStart, int* cand, int jOffset) {
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) kCandidates(int n, const double4* posS, c
                         const unsigned long long m = (__ballot(in) >> shift) & (L == 64 ? ~0ull : ((1ull << L)-1ull));
                         const int at = out+__popcll(m & below);
                         if (in && at < outEnd)
-                            cand[at] = j;
+                            cand[at] = j+jOffset;
                         out += __popcll(m);
                     }
                     else
@@ -153,11 +153,13 @@ __global__ void __launch_bounds__(256) kCandidates(int n, const double4* posS, c
     }
 }
 
-__global__ void kMaxDisplacement(int n, const double* pos, const double* posBuild, BoxT<double> box,
+template<bool BATCH>
+__global__ void kMaxDisplacement(int n, const double* pos, const double* posBuild, BoxSel<double> bs,
                                  unsigned* maxBits) {
     __shared__ double red[16];
     double m = 0;
     for (int i = blockIdx.x*blockDim.x+threadIdx.x; i < n; i += gridDim.x*blockDim.x) {
+        const BoxT<double> box = selectBox<BATCH>(bs, i);
         double x = pos[3*i]-posBuild[3*i], y = pos[3*i+1]-posBuild[3*i+1], z = pos[3*i+2]-posBuild[3*i+2];
         minImage(x, y, z, box);
         m = fmax(m, x*x+y*y+z*z);
@@ -176,31 +178,15 @@ __device__ __forceinline__ void wrapPosition(double& x, double& y, double& z, co
     z -= t2*b.cz;
 }
 
-__global__ void kGatherPositions(int n, const int* order, const double* pos, const double* qS, BoxT<double> box,
+template<bool BATCH>
+__global__ void kGatherPositions(int n, const int* order, const double* pos, const double* qS, BoxSel<double> bs,
                                  double4* posD, float4* posF) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
+    const BoxT<double> box = selectBox<BATCH>(bs, s);
     const int i = order[s];
     double x = pos[3*i], y = pos[3*i+1], z = pos[3*i+2];
-    wrapPosition(x, y, z, box);
-    const double q = qS[s];
-    posD[s] = make_double4(x, y, z, q);
-    if (posF)
-        posF[s] = make_float4((float) x, (float) y, (float) z, (float) q);
-}
-
-__global__ void kGatherPositionsF4(int n, const int* order, const float4* posq, const double* qS, BoxT<double> box,
-                                   double4* posD, float4* posF, double* posOrig) {
-    const int s = blockIdx.x*blockDim.x+threadIdx.x;
-    if (s >= n)
-        return;
-    const int i = order[s];
-    const float4 p = posq[i];
-    double x = p.x, y = p.y, z = p.z;
-    posOrig[3*i] = x;
-    posOrig[3*i+1] = y;
-    posOrig[3*i+2] = z;
     wrapPosition(x, y, z, box);
     const double q = qS[s];
     posD[s] = make_double4(x, y, z, q);
@@ -215,11 +201,13 @@ __global__ void kPermute(int n, const int* order, const double* in, double* out)
 }
 
 // p_i = sum_m c_m unit(r_partner - r_i), minimum image, FP64 (core/pgm_core.cpp:51-62).
+template<bool BATCH>
 __global__ void kPermDipoles(int n, const int* order, const double* pos, const int* covStart, const int* covPartner,
-                             const double* covStrength, BoxT<double> box, double* perm) {
+                             const double* covStrength, BoxSel<double> bs, double* perm) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
+    const BoxT<double> box = selectBox<BATCH>(bs, s);
     const int i = order[s];
     double px = 0, py = 0, pz = 0;
     for (int m = covStart[i]; m < covStart[i+1]; m++) {
@@ -249,43 +237,46 @@ void launchScan(hipStream_t st, const int* in, int* out, int m, int*) {
     kScan<<<1, 1024, 0, st>>>(in, out, m);
 }
 
-void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order) {
-    kCellScatter<<<blocksFor(n, 256), 256, 0, st>>>(n, cellOf, slot, cellStart, order);
+void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order,
+                       int base) {
+    kCellScatter<<<blocksFor(n, 256), 256, 0, st>>>(n, cellOf, slot, cellStart, order, base);
 }
 
 void launchCellSort(hipStream_t st, int ncell, const int* cellStart, const int* subOf, int* order) {
     kCellSort<<<blocksFor(ncell, 64), 64, 0, st>>>(ncell, cellStart, subOf, order);
 }
 
-void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf) {
-    kRank<<<blocksFor(n, 256), 256, 0, st>>>(n, order, rank, sortedCell, cellOf);
+void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf, int base) {
+    kRank<<<blocksFor(n, 256), 256, 0, st>>>(n, order, rank, sortedCell, cellOf, base);
 }
 
 void launchCandCount(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
                      CellGrid g, BoxT<double> box, double rl2, int* count) {
     kCandidates<false, kListLanes><<<blocksFor((long long) n*kListLanes, 256), 256, 0, st>>>(
-        n, posS, sortedCell, cellStart, g, box, rl2, count, nullptr, nullptr);
+        n, posS, sortedCell, cellStart, g, box, rl2, count, nullptr, nullptr, 0);
 }
 
 void launchCandFill(hipStream_t st, int n, const double4* posS, const int* sortedCell, const int* cellStart,
-                    CellGrid g, BoxT<double> box, double rl2, const int* candStart, int* cand) {
+                    CellGrid g, BoxT<double> box, double rl2, const int* candStart, int* cand, int base) {
     kCandidates<true, kListLanes><<<blocksFor((long long) n*kListLanes, 256), 256, 0, st>>>(
-        n, posS, sortedCell, cellStart, g, box, rl2, nullptr, candStart, cand);
+        n, posS, sortedCell, cellStart, g, box, rl2, nullptr, candStart, cand, base);
 }
 
-void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const double* posBuild, BoxT<double> box,
+void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const double* posBuild, BoxSel<double> box,
                            unsigned* maxBits) {
-    kMaxDisplacement<<<std::min(1024, blocksFor(n, 256)), 256, 0, st>>>(n, pos, posBuild, box, maxBits);
+    const int nb = std::min(1024, blocksFor(n, 256));
+    if (box.table)
+        kMaxDisplacement<true><<<nb, 256, 0, st>>>(n, pos, posBuild, box, maxBits);
+    else
+        kMaxDisplacement<false><<<nb, 256, 0, st>>>(n, pos, posBuild, box, maxBits);
 }
 
 void launchGatherPositions(hipStream_t st, int n, const int* order, const double* pos, const double* qS,
-                           BoxT<double> box, double4* posD, float4* posF) {
-    kGatherPositions<<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, qS, box, posD, posF);
-}
-
-void launchGatherPositionsF4(hipStream_t st, int n, const int* order, const float4* posq, const double* qS,
-                             BoxT<double> box, double4* posD, float4* posF, double* posOrig) {
-    kGatherPositionsF4<<<blocksFor(n, 256), 256, 0, st>>>(n, order, posq, qS, box, posD, posF, posOrig);
+                           BoxSel<double> box, double4* posD, float4* posF) {
+    if (box.table)
+        kGatherPositions<true><<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, qS, box, posD, posF);
+    else
+        kGatherPositions<false><<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, qS, box, posD, posF);
 }
 
 void launchPermute(hipStream_t st, int n, const int* order, const double* in, double* out) {
@@ -293,8 +284,13 @@ void launchPermute(hipStream_t st, int n, const int* order, const double* in, do
 }
 
 void launchPermDipoles(hipStream_t st, int n, const int* order, const double* pos, const int* covStart,
-                       const int* covPartner, const double* covStrength, BoxT<double> box, double* perm) {
-    kPermDipoles<<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, covStart, covPartner, covStrength, box, perm);
+                       const int* covPartner, const double* covStrength, BoxSel<double> box, double* perm) {
+    if (box.table)
+        kPermDipoles<true><<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, covStart, covPartner, covStrength, box,
+                                                              perm);
+    else
+        kPermDipoles<false><<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, covStart, covPartner, covStrength, box,
+                                                               perm);
 }
 
 }   // namespace pgmhip

@@ -12,6 +12,9 @@
 // T = float: FP32 pair arithmetic with FP64 accumulation per lane; T = double: FP64 throughout.
 #include "kernels.hpp"
 
+#include <stdexcept>
+#include <type_traits>
+
 namespace pgmhip {
 
 namespace {
@@ -20,16 +23,17 @@ constexpr int kPairLanes = 16;
 constexpr int kPairBlock = 256;
 
 // Compaction: every atom's candidates within rc, in place in its segment (pj), and the count (nb).
-template<class T, int L>
+template<class T, int L, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                        const int* __restrict__ candStart,
                                                        const int* __restrict__ cand, int* __restrict__ nb,
-                                                       int* __restrict__ pj, BoxT<T> box, T rc2) {
+                                                       int* __restrict__ pj, BoxSel<T> bs, T rc2) {
     typedef typename Vec4<T>::type T4;
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
     const int shift = (threadIdx.x%64)/L*L;
     const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
     const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << L)-1ull));
@@ -84,7 +88,7 @@ __device__ __forceinline__ void setupTerm(const typename Vec4<T>::type& pi, T rg
 }
 
 // Kernels and fields over the compacted pairs: every lane busy, U entries in flight per lane.
-template<class T, int L, int U, bool DUAL>
+template<class T, int L, int U, bool DUAL, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const T* __restrict__ rg,
                                                          const typename Vec4<T>::type* __restrict__ dip,
@@ -92,7 +96,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ nb, const int* __restrict__ pj,
                                                          typename Vec2<T>::type* __restrict__ g12,
-                                                         typename Vec2<T>::type* __restrict__ g03, BoxT<T> box, T beta,
+                                                         typename Vec2<T>::type* __restrict__ g03, BoxSel<T> bs, T beta,
                                                          double* __restrict__ field, double* __restrict__ fieldDual) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
@@ -100,6 +104,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
     const T4 pi = pos[s];
     const T rgi = rg[s];
     const int start = candStart[s], cnt = nb[s];
@@ -166,12 +171,12 @@ __device__ __forceinline__ void fieldTerm(const typename Vec4<T>::type& pi, cons
     ez += (double) (c2*z-d.z*g.x);
 }
 
-template<class T, int L, int U>
+template<class T, int L, int U, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const typename Vec4<T>::type* __restrict__ dip,
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ nb, const int* __restrict__ pj,
-                                                         const typename Vec2<T>::type* __restrict__ g12, BoxT<T> box,
+                                                         const typename Vec2<T>::type* __restrict__ g12, BoxSel<T> bs,
                                                          double* __restrict__ field, const int* skip) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
@@ -181,6 +186,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
     const T4 pi = pos[s];
     const int start = candStart[s], cnt = nb[s];
     double ex = 0, ey = 0, ez = 0;
@@ -252,22 +258,22 @@ __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, cons
     e[2] += (double) (c*z-dj.z*G1);
 }
 
-template<class T, int L, int U>
+template<class T, int L, int U, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const typename Vec4<T>::type* __restrict__ dip,
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ nb, const int* __restrict__ pj,
                                                          const typename Vec2<T>::type* __restrict__ g12,
-                                                         const typename Vec2<T>::type* __restrict__ g03, BoxT<T> box,
+                                                         const typename Vec2<T>::type* __restrict__ g03, BoxSel<T> bs,
                                                          double* __restrict__ force, double* __restrict__ field,
-                                                         double* __restrict__ partials) {
+                                                         double* __restrict__ perAtom) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
-    __shared__ double red[kPairBlock/64][10];
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // energy, strain (row-major)
     if (s < n) {
+        const BoxT<T> box = selectBox<BATCH>(bs, s);
         const T4 pi = pos[s];
         const T4 di = dip[s];
         const int start = candStart[s], cnt = nb[s];
@@ -301,6 +307,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             f[c] = groupSum<L>(f[c]);
             e[c] = groupSum<L>(e[c]);
         }
+#pragma unroll
+        for (int q = 0; q < 10; q++)
+            acc[q] = groupSum<L>(acc[q]);
         if (lane == 0) {
             force[3*s] = f[0];
             force[3*s+1] = f[1];
@@ -308,21 +317,11 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             field[3*s] = e[0];
             field[3*s+1] = e[1];
             field[3*s+2] = e[2];
-        }
-    }
-    // each pair is visited from both atoms: half of the sums
+            // each pair is visited from both atoms: half of the sums
 #pragma unroll
-    for (int q = 0; q < 10; q++) {
-        const double v = groupSum<64>(acc[q]);
-        if ((threadIdx.x & 63) == 0)
-            red[threadIdx.x >> 6][q] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 10) {
-        double v = 0;
-        for (int w = 0; w < kPairBlock/64; w++)
-            v += red[w][threadIdx.x];
-        partials[(size_t) blockIdx.x*10+threadIdx.x] = 0.5*v;
+            for (int q = 0; q < 10; q++)
+                perAtom[10*(size_t) s+q] = 0.5*acc[q];
+        }
     }
 }
 
@@ -336,7 +335,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
 // dG_n/da = (2/sqrt(pi)) (2 a^2)^n exp(-a^2 r^2), the a-derivative of B_n[erf(a r)/r] (the beta part of G_n does
 // not depend on a; beyond the damping cut the a part is dropped, and its derivative with it).  The tangents use
 // dG_n = -(x . x') G_{n+1}, so G0..G3 stored by the setup pass cover phi' and E'.
-template<class T, int L, bool TAN>
+template<class T, int L, bool TAN, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                            const T* __restrict__ rg,
                                                            const typename Vec4<T>::type* __restrict__ dip,
@@ -346,13 +345,14 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
                                                            const int* __restrict__ nb, const int* __restrict__ pj,
                                                            const typename Vec2<T>::type* __restrict__ g12,
                                                            const typename Vec2<T>::type* __restrict__ g03,
-                                                           BoxT<T> box, double* __restrict__ out) {
+                                                           BoxSel<T> bs, double* __restrict__ out) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     const int gid = blockIdx.x*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
     const T4 pi = pos[s];
     const T4 di = dip[s];
     T4 ddi, gi;
@@ -438,68 +438,114 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
 
 inline int groupsBlocks(int n, int L) { return (int) (((long long) n*L+kPairBlock-1)/kPairBlock); }
 
+// Runs fn with the lane count (8, 16 or 32) and the batch flag as compile-time constants.
+template<class Fn>
+void dispatchLanes(int lanes, bool batch, Fn fn) {
+    auto withBatch = [&](auto L) {
+        if (batch)
+            fn(L, std::true_type{});
+        else
+            fn(L, std::false_type{});
+    };
+    switch (lanes) {
+    case 8:
+        withBatch(std::integral_constant<int, 8>{});
+        break;
+    case 32:
+        withBatch(std::integral_constant<int, 32>{});
+        break;
+    case 16:
+        withBatch(std::integral_constant<int, 16>{});
+        break;
+    default:
+        throw std::invalid_argument("pgm_hip: pair-pass lanes must be 8, 16 or 32");
+    }
+}
+
 }   // namespace
+
+template<class T>
+void launchPairCompact(hipStream_t st, int n, const typename Vec4<T>::type* pos, const int* candStart,
+                       const int* cand, int* nb, int* pj, BoxSel<T> box, T rc2, int lanes) {
+    dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        kCompact<T, L(), B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, candStart, cand, nb, pj, box, rc2);
+    });
+}
 
 template<class T>
 void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                      const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
-                     const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
-                     BoxT<T> box, T rc2, T beta, double* field, double* fieldDual) {
-    const int blocks = groupsBlocks(n, kPairLanes);
-    kCompact<T, kPairLanes><<<blocks, kPairBlock, 0, st>>>(n, pos, candStart, cand, nb, pj, box, rc2);
-    if (dual)
-        kPairSetup<T, kPairLanes, 2, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, nb, pj,
-                                                                          g12, g03, box, beta, field, fieldDual);
-    else
-        kPairSetup<T, kPairLanes, 2, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, nb, pj,
-                                                                           g12, g03, box, beta, field, nullptr);
+                     const int* nb, const int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
+                     BoxSel<T> box, T beta, double* field, double* fieldDual, int lanes) {
+    dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        const int blocks = groupsBlocks(n, L());
+        if (dual)
+            kPairSetup<T, L(), 2, true, B()><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, nb, pj,
+                                                                            g12, g03, box, beta, field, fieldDual);
+        else
+            kPairSetup<T, L(), 2, false, B()><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, nb, pj,
+                                                                             g12, g03, box, beta, field, nullptr);
+    });
 }
 
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     BoxT<T> box, double* field, const int* skip) {
-    kPairField<T, kPairLanes, 4><<<groupsBlocks(n, kPairLanes), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj,
-                                                                                    g12, box, field, skip);
+                     BoxSel<T> box, double* field, const int* skip, int lanes) {
+    dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        kPairField<T, L(), 4, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12,
+                                                                                box, field, skip);
+    });
 }
 
 template<class T>
-int launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
-                    const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                    const typename Vec2<T>::type* g03, BoxT<T> box, double* force, double* field, double* partials) {
-    const int blocks = groupsBlocks(n, kPairLanes);
-    kPairFinal<T, kPairLanes, 2><<<blocks, kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12, g03, box, force,
-                                                               field, partials);
-    return blocks;
+void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
+                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
+                     const typename Vec2<T>::type* g03, BoxSel<T> box, double* force, double* field,
+                     double* perAtom, int lanes) {
+    dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        kPairFinal<T, L(), 2, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12,
+                                                                                g03, box, force, field, perAtom);
+    });
 }
 
 template<class T>
 void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                        const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dipDot,
                        const typename Vec4<T>::type* disp, const int* candStart, const int* nb, const int* pj,
-                       const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxT<T> box,
+                       const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
                        double* out) {
     const int blocks = groupsBlocks(n, kPairLanes);
-    if (dipDot)
-        kPairTangent<T, kPairLanes, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dipDot, disp, candStart,
-                                                                         nb, pj, g12, g03, box, out);
+    const bool batch = box.table != nullptr;
+    if (dipDot && batch)
+        kPairTangent<T, kPairLanes, true, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dipDot, disp,
+                                                                               candStart, nb, pj, g12, g03, box, out);
+    else if (dipDot)
+        kPairTangent<T, kPairLanes, true, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dipDot, disp,
+                                                                                candStart, nb, pj, g12, g03, box, out);
+    else if (batch)
+        kPairTangent<T, kPairLanes, false, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, pos, candStart,
+                                                                                nb, pj, g12, g03, box, out);
     else
-        kPairTangent<T, kPairLanes, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, pos, candStart, nb,
-                                                                          pj, g12, g03, box, out);
+        kPairTangent<T, kPairLanes, false, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, pos, candStart,
+                                                                                 nb, pj, g12, g03, box, out);
 }
 
 #define PGM_INSTANTIATE_PAIRS(T)                                                                                      \
+    template void launchPairCompact<T>(hipStream_t, int, const Vec4<T>::type*, const int*, const int*, int*, int*,   \
+                                       BoxSel<T>, T, int);                                                            \
     template void launchPairSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \
-                                     const Vec4<T>::type*, const int*, const int*, int*, int*, Vec2<T>::type*,        \
-                                     Vec2<T>::type*, BoxT<T>, T, T, double*, double*);                                \
+                                     const Vec4<T>::type*, const int*, const int*, const int*, Vec2<T>::type*,        \
+                                     Vec2<T>::type*, BoxSel<T>, T, double*, double*, int);                            \
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
-                                     const int*, const int*, const Vec2<T>::type*, BoxT<T>, double*, const int*);     \
-    template int launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,        \
-                                    const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxT<T>,      \
-                                    double*, double*, double*);                                                       \
+                                     const int*, const int*, const Vec2<T>::type*, BoxSel<T>, double*, const int*,    \
+                                     int);                                                                            \
+    template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
+                                     const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxSel<T>,   \
+                                     double*, double*, double*, int);                                                 \
     template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,       \
                                        const Vec4<T>::type*, const Vec4<T>::type*, const int*, const int*,            \
-                                       const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxT<T>, double*);
+                                       const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxSel<T>, double*);
 PGM_INSTANTIATE_PAIRS(float)
 PGM_INSTANTIATE_PAIRS(double)
 

@@ -12,6 +12,8 @@
 //              -[q grad phi + (grad grad phi) . d] (core/pgm_core.cpp:241-264, 453-487).
 #include "kernels.hpp"
 
+#include <type_traits>
+
 #include <stdexcept>
 #include <string>
 
@@ -166,26 +168,37 @@ __device__ __forceinline__ void fracGrid(const double4& p, const BoxT<double>& b
     }
 }
 
-template<class T>
-__global__ void kPmeBin(int n, const double4* posD, BoxT<double> box, PmeGrid g, int* brickOf, int* slot,
-                        int* brickCount) {
+// The grid and box of a system: the single one, or from the batch table.
+template<bool BATCH>
+__device__ __forceinline__ const PmeSys* sysEntry(const PmeSel& ps, int sys) {
+    return BATCH ? ps.table+sys : nullptr;
+}
+
+template<class T, bool BATCH>
+__global__ void kPmeBin(int n, const double4* posD, PmeSel ps, int* brickOf, int* slot, int* brickCount) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
+    const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.sysOf[s] : 0);
+    const PmeGrid& g = BATCH ? se->g : ps.g;
+    const BoxT<double>& box = BATCH ? se->box : ps.box;
     int base[3];
     T fr[3];
     fracGrid<T>(posD[s], box, g, base, fr);
-    const int b = ((base[0]/g.brick)*g.nb[1]+base[1]/g.brick)*g.nb[2]+base[2]/g.brick;
+    const int b = ((base[0]/g.brick)*g.nb[1]+base[1]/g.brick)*g.nb[2]+base[2]/g.brick+(BATCH ? se->brickOff : 0);
     brickOf[s] = b;
     slot[s] = atomicAdd(&brickCount[b], 1);
 }
 
-template<class T, int P>
-__global__ void kPmePlace(int n, const double4* posD, BoxT<double> box, PmeGrid g, const int* brickOf,
-                          const int* slot, const int* brickStart, int* pmeAtom, int* baseOut, T* spl) {
+template<class T, int P, bool BATCH>
+__global__ void kPmePlace(int n, const double4* posD, PmeSel ps, const int* brickOf, const int* slot,
+                          const int* brickStart, int* pmeAtom, int* baseOut, T* spl) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
+    const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.sysOf[s] : 0);
+    const PmeGrid& g = BATCH ? se->g : ps.g;
+    const BoxT<double>& box = BATCH ? se->box : ps.box;
     int base[3];
     T fr[3];
     fracGrid<T>(posD[s], box, g, base, fr);
@@ -212,13 +225,13 @@ __global__ void kPmePlace(int n, const double4* posD, BoxT<double> box, PmeGrid 
 // dipole changes by d'_s: the density q theta(r - r_s) + d . grad_s theta moves to first order by
 // (q g + d') . grad_s theta + g^T (grad_s grad_s theta) d, i.e. a dipole q g + d' and a symmetric second moment
 // (g d^T + d g^T)/2 on the second spline derivatives.
-template<class T, int P, bool TAN>
-__global__ void __launch_bounds__(kBlock) kSpread(PmeGrid g, const int* __restrict__ brickStart,
+template<class T, int P, bool TAN, bool BATCH>
+__global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restrict__ brickStart,
                                                   const int* __restrict__ pmeAtom, const int* __restrict__ baseArr,
                                                   const T* __restrict__ spl,
                                                   const typename Vec4<T>::type* __restrict__ dip,
                                                   const typename Vec4<T>::type* __restrict__ posq, int withQ,
-                                                  BoxT<double> box, T* __restrict__ grid, const int* skip,
+                                                  T* __restrict__ gridAll, const int* skip,
                                                   const typename Vec4<T>::type* __restrict__ dipDot,
                                                   const typename Vec4<T>::type* __restrict__ disp) {
     typedef typename Vec4<T>::type T4;
@@ -229,7 +242,12 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeGrid g, const int* __restri
     if (skip && *skip)
         return;
     const int bid = blockIdx.x;
-    const int b2 = bid%g.nb[2], b1 = (bid/g.nb[2])%g.nb[1], b0 = bid/(g.nb[2]*g.nb[1]);
+    const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.brickSys[bid] : 0);
+    const PmeGrid& g = BATCH ? se->g : ps.g;
+    const BoxT<double>& box = BATCH ? se->box : ps.box;
+    T* __restrict__ grid = gridAll+(BATCH ? se->gridOff : 0);
+    const int lb = bid-(BATCH ? se->brickOff : 0);
+    const int b2 = lb%g.nb[2], b1 = (lb/g.nb[2])%g.nb[1], b0 = lb/(g.nb[2]*g.nb[1]);
     const int E = g.brick+P-1;
     const int lo0 = b0*g.brick-(P-1), lo1 = b1*g.brick-(P-1), lo2 = b2*g.brick-(P-1);
     for (int i = threadIdx.x; i < E*E*E; i += blockDim.x)
@@ -323,7 +341,8 @@ __device__ __forceinline__ void loadTwiddles(typename Vec2<T>::type* dst, const 
 
 // z: two real lines per complex FFT, unpacked into their half spectra.
 template<class T, bool INVERSE>
-__global__ void __launch_bounds__(kBlock) kFftZ(PmeGrid g, T* __restrict__ real, typename Vec2<T>::type* __restrict__ cplx,
+__global__ void __launch_bounds__(kBlock) kFftZ(PmeGrid g, int ngrid, T* __restrict__ real,
+                                                typename Vec2<T>::type* __restrict__ cplx,
                                                 const typename Vec2<T>::type* __restrict__ twAll, int pairsPerBlock,
                                                 const int* skip) {
     typedef typename Vec2<T>::type T2;
@@ -331,7 +350,7 @@ __global__ void __launch_bounds__(kBlock) kFftZ(PmeGrid g, T* __restrict__ real,
     if (skip && *skip)
         return;
     const FftPlan& p = g.plan[2];
-    const int K = g.K[2], Kh = g.K3h, nlines = g.K[0]*g.K[1];
+    const int K = g.K[2], Kh = g.K3h, nlines = ngrid*g.K[0]*g.K[1];
     T2* tw = reinterpret_cast<T2*>(smemRaw);
     T2* b0 = tw+K;
     T2* b1 = b0+pairsPerBlock*K;
@@ -435,11 +454,12 @@ __global__ void __launch_bounds__(kBlock) kFftY(PmeGrid g, typename Vec2<T>::typ
 
 // x: forward, influence function (energy, strain), back; a tile of consecutive z for fixed y per block.
 template<class T>
-__global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>::type* __restrict__ cplx,
-                                                    const T* __restrict__ infl,
+__global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>::type* __restrict__ cplxAll,
+                                                    const T* __restrict__ inflAll,
                                                     const typename Vec2<T>::type* __restrict__ twAll, int zt,
-                                                    BoxT<double> box, double beta, int energy, int strain,
-                                                    double* __restrict__ partials, const int* skip) {
+                                                    BoxT<double> boxOne, const BoxT<double>* boxes, double beta,
+                                                    int energy, int strain, double* __restrict__ partials,
+                                                    const int* skip) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     __shared__ double red[kBlock/64][10];
@@ -447,7 +467,13 @@ __global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>:
         return;
     const int K = g.K[0], K2 = g.K[1], K3 = g.K[2], Kh = g.K3h;
     const int ntile = (Kh+zt-1)/zt;
-    const int y = blockIdx.x/ntile, z0 = (blockIdx.x%ntile)*zt;
+    // stacked grids of one shape: block -> (grid, y, z tile)
+    const int gi = blockIdx.x/(K2*ntile), lb = blockIdx.x-gi*(K2*ntile);
+    const size_t goff = (size_t) gi*K*K2*Kh;
+    typename Vec2<T>::type* __restrict__ cplx = cplxAll+goff;
+    const T* __restrict__ infl = inflAll+goff;
+    const BoxT<double>& box = boxes ? boxes[gi] : boxOne;
+    const int y = lb/ntile, z0 = (lb%ntile)*zt;
     const int nl = min(zt, Kh-z0);
     T2* tw = reinterpret_cast<T2*>(smemRaw);
     T2* b0 = tw+K;
@@ -516,10 +542,10 @@ __global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>:
 
 constexpr int kGatherLanes = 8;
 
-template<class T, int P, bool FINAL>
-__global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* __restrict__ pmeAtom,
+template<class T, int P, bool FINAL, bool BATCH>
+__global__ void __launch_bounds__(kBlock) kGather(PmeSel ps, int n, const int* __restrict__ pmeAtom,
                                                   const int* __restrict__ baseArr, const T* __restrict__ spl,
-                                                  const T* __restrict__ phi, BoxT<double> box,
+                                                  const T* __restrict__ phiAll,
                                                   const typename Vec4<T>::type* __restrict__ dip,
                                                   const typename Vec4<T>::type* __restrict__ posq,
                                                   double* __restrict__ field, double* __restrict__ force,
@@ -531,6 +557,11 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
     const int k = gid/kGatherLanes, lane = gid%kGatherLanes;
     if (k >= n)
         return;
+    const int s = pmeAtom[k];
+    const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.sysOf[s] : 0);
+    const PmeGrid& g = BATCH ? se->g : ps.g;
+    const BoxT<double>& box = BATCH ? se->box : ps.box;
+    const T* __restrict__ phi = phiAll+(BATCH ? se->gridOff : 0);
     const T* o = spl+(size_t) k*9*P;
     const int B0 = baseArr[3*k], B1 = baseArr[3*k+1], B2 = baseArr[3*k+2];
     const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
@@ -583,7 +614,6 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
             h[a] = groupSum<kGatherLanes>(h[a]);
     if (lane != 0)
         return;
-    const int s = pmeAtom[k];
     double rk[3][3];
 #pragma unroll
     for (int kk = 0; kk < 3; kk++)
@@ -614,11 +644,11 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeGrid g, int n, const int* _
 // Potential and field of the convolved grid phi at the atoms and, TAN, their tangents when every atom moves by g_s
 // and the grid changes by phiDot: phi' = g . grad phi + phiDot, E' = -(grad grad phi) g - grad phiDot.
 // out (FP64, sorted order): phi[n], E[3n] at offsets 0 and n, phi'[n], E'[3n] at offsets 5n and 6n.
-template<class T, int P, bool TAN>
-__global__ void __launch_bounds__(kBlock) kGatherTangent(PmeGrid g, int n, const int* __restrict__ pmeAtom,
+template<class T, int P, bool TAN, bool BATCH>
+__global__ void __launch_bounds__(kBlock) kGatherTangent(PmeSel ps, int n, const int* __restrict__ pmeAtom,
                                                          const int* __restrict__ baseArr, const T* __restrict__ spl,
-                                                         const T* __restrict__ phi, const T* __restrict__ phiDot,
-                                                         BoxT<double> box,
+                                                         const T* __restrict__ phiAll,
+                                                         const T* __restrict__ phiDotAll,
                                                          const typename Vec4<T>::type* __restrict__ disp,
                                                          double* __restrict__ out) {
     typedef typename Vec4<T>::type T4;
@@ -626,6 +656,12 @@ __global__ void __launch_bounds__(kBlock) kGatherTangent(PmeGrid g, int n, const
     const int k = gid/kGatherLanes, lane = gid%kGatherLanes;
     if (k >= n)
         return;
+    const int s = pmeAtom[k];
+    const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.sysOf[s] : 0);
+    const PmeGrid& g = BATCH ? se->g : ps.g;
+    const BoxT<double>& box = BATCH ? se->box : ps.box;
+    const T* __restrict__ phi = phiAll+(BATCH ? se->gridOff : 0);
+    const T* __restrict__ phiDot = phiDotAll+(BATCH ? se->gridOff : 0);
     const T* o = spl+(size_t) k*9*P;
     const int B0 = baseArr[3*k], B1 = baseArr[3*k+1], B2 = baseArr[3*k+2];
     const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
@@ -684,7 +720,6 @@ __global__ void __launch_bounds__(kBlock) kGatherTangent(PmeGrid g, int n, const
             acc[a] = groupSum<kGatherLanes>(acc[a]);
     if (lane != 0)
         return;
-    const int s = pmeAtom[k];
     double rk[3][3];
 #pragma unroll
     for (int kk = 0; kk < 3; kk++)
@@ -770,132 +805,128 @@ int linesPerBlock(int K) {
 
 }   // namespace
 
-template<class T>
-void launchPmeBin(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, int* brickOf, int* slot,
-                  int* brickCount) {
-    kPmeBin<T><<<nblk(n, 256), 256, 0, st>>>(n, posD, box, g, brickOf, slot, brickCount);
-}
-
-template<class T>
-void launchPmePlace(hipStream_t st, int n, const double4* posD, BoxT<double> box, PmeGrid g, const int* brickOf,
-                    const int* slot, const int* brickStart, int* pmeAtom, int* base, T* spl) {
-    switch (g.order) {
-#define PGM_CASE(P)                                                                                                    \
-    case P:                                                                                                            \
-        kPmePlace<T, P><<<nblk(n, 128), 128, 0, st>>>(n, posD, box, g, brickOf, slot, brickStart, pmeAtom, base, spl); \
+// Runs fn with the PME order and the batch flag as compile-time constants.
+template<class Fn>
+void dispatchOrder(int order, bool batch, Fn fn) {
+    auto withBatch = [&](auto P) {
+        if (batch)
+            fn(P, std::true_type{});
+        else
+            fn(P, std::false_type{});
+    };
+    switch (order) {
+    case 4:
+        withBatch(std::integral_constant<int, 4>{});
         break;
-        PGM_CASE(4)
-        PGM_CASE(5)
-        PGM_CASE(6)
-        PGM_CASE(7)
-        PGM_CASE(8)
-#undef PGM_CASE
+    case 5:
+        withBatch(std::integral_constant<int, 5>{});
+        break;
+    case 6:
+        withBatch(std::integral_constant<int, 6>{});
+        break;
+    case 7:
+        withBatch(std::integral_constant<int, 7>{});
+        break;
+    case 8:
+        withBatch(std::integral_constant<int, 8>{});
+        break;
     default:
         throw std::runtime_error("pgm_hip: PME order must be 4..8");
     }
 }
 
 template<class T>
-void launchSpread(hipStream_t st, PmeGrid g, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
-                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, BoxT<double> box,
-                  T* grid, const int* skip, const typename Vec4<T>::type* dipDot, const typename Vec4<T>::type* disp) {
-    const int E = g.brick+g.order-1;
+void launchPmeBin(hipStream_t st, int n, const double4* posD, PmeSel ps, int* brickOf, int* slot, int* brickCount) {
+    if (ps.table)
+        kPmeBin<T, true><<<nblk(n, 256), 256, 0, st>>>(n, posD, ps, brickOf, slot, brickCount);
+    else
+        kPmeBin<T, false><<<nblk(n, 256), 256, 0, st>>>(n, posD, ps, brickOf, slot, brickCount);
+}
+
+template<class T>
+void launchPmePlace(hipStream_t st, int n, const double4* posD, PmeSel ps, const int* brickOf, const int* slot,
+                    const int* brickStart, int* pmeAtom, int* base, T* spl) {
+    dispatchOrder(ps.order, ps.table != nullptr, [&](auto P, auto B) {
+        kPmePlace<T, P(), B()><<<nblk(n, 128), 128, 0, st>>>(n, posD, ps, brickOf, slot, brickStart, pmeAtom, base,
+                                                             spl);
+    });
+}
+
+template<class T>
+void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
+                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, T* grid,
+                  const int* skip, const typename Vec4<T>::type* dipDot, const typename Vec4<T>::type* disp) {
+    const int E = ps.brick+ps.order-1;
     const size_t lds = (size_t) E*E*E*sizeof(double);
-    switch (g.order) {
-#define PGM_CASE(P)                                                                                                 \
-    case P:                                                                                                         \
-        if (dipDot)                                                                                                 \
-            kSpread<T, P, true><<<g.nbrick, kBlock, lds, st>>>(g, brickStart, pmeAtom, base, spl, dip, posq, withQ, \
-                                                               box, grid, skip, dipDot, disp);                      \
-        else                                                                                                        \
-            kSpread<T, P, false><<<g.nbrick, kBlock, lds, st>>>(g, brickStart, pmeAtom, base, spl, dip, posq,       \
-                                                                withQ, box, grid, skip, dip, dip);                  \
-        break;
-        PGM_CASE(4)
-        PGM_CASE(5)
-        PGM_CASE(6)
-        PGM_CASE(7)
-        PGM_CASE(8)
-#undef PGM_CASE
-    default:
-        throw std::runtime_error("pgm_hip: PME order must be 4..8");
-    }
+    dispatchOrder(ps.order, ps.table != nullptr, [&](auto P, auto B) {
+        if (dipDot)
+            kSpread<T, P(), true, B()><<<ps.nbrick, kBlock, lds, st>>>(ps, brickStart, pmeAtom, base, spl, dip, posq,
+                                                                       withQ, grid, skip, dipDot, disp);
+        else
+            kSpread<T, P(), false, B()><<<ps.nbrick, kBlock, lds, st>>>(ps, brickStart, pmeAtom, base, spl, dip,
+                                                                        posq, withQ, grid, skip, dip, dip);
+    });
 }
 
 template<class T>
-int launchPmeConvolve(hipStream_t st, PmeGrid g, T* real, typename Vec2<T>::type* cplx, const T* influence,
-                      const typename Vec2<T>::type* tw, BoxT<double> box, double beta, int energy, int strain,
-                      double* partials, const int* skip) {
+int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Vec2<T>::type* cplx, const T* influence,
+                      const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
+                      int energy, int strain, double* partials, const int* skip) {
     typedef typename Vec2<T>::type T2;
-    const int nlines = g.K[0]*g.K[1];
+    const int nlines = ngrid*g.K[0]*g.K[1];
     // z forward
     const int pz = linesPerBlock<T>(g.K[2]);
     const int zBlocks = nblk((nlines+1)/2, pz);
     const size_t ldsZ = (size_t) (g.K[2]+2*pz*g.K[2])*sizeof(T2);
-    kFftZ<T, false><<<zBlocks, kBlock, ldsZ, st>>>(g, real, cplx, tw, pz, skip);
+    kFftZ<T, false><<<zBlocks, kBlock, ldsZ, st>>>(g, ngrid, real, cplx, tw, pz, skip);
+    // y: the stacked grids are one grid of ngrid K0 x-planes
     const int zy = linesPerBlock<T>(g.K[1]);
-    const int yBlocks = g.K[0]*((g.K3h+zy-1)/zy);
+    const int yBlocks = ngrid*g.K[0]*((g.K3h+zy-1)/zy);
     const size_t ldsY = (size_t) (g.K[1]+2*zy*g.K[1])*sizeof(T2);
     kFftY<T, false><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
     const int zx = linesPerBlock<T>(g.K[0]);
-    const int xBlocks = g.K[1]*((g.K3h+zx-1)/zx);
+    const int xBlocks = ngrid*g.K[1]*((g.K3h+zx-1)/zx);
     const size_t ldsX = (size_t) (g.K[0]+2*zx*g.K[0])*sizeof(T2);
-    kFftXConv<T><<<xBlocks, kBlock, ldsX, st>>>(g, cplx, influence, tw, zx, box, beta, energy, strain, partials, skip);
+    kFftXConv<T><<<xBlocks, kBlock, ldsX, st>>>(g, cplx, influence, tw, zx, box, boxes, beta, energy, strain,
+                                                partials, skip);
     kFftY<T, true><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
-    kFftZ<T, true><<<zBlocks, kBlock, ldsZ, st>>>(g, real, cplx, tw, pz, skip);
+    kFftZ<T, true><<<zBlocks, kBlock, ldsZ, st>>>(g, ngrid, real, cplx, tw, pz, skip);
     return xBlocks;
 }
 
 template<class T>
-void launchGatherTangent(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl,
-                         const T* phi, const T* phiDot, BoxT<double> box, const typename Vec4<T>::type* disp,
-                         double* out) {
-    const int blocks = nblk((long long) n*kGatherLanes, kBlock);
-    switch (g.order) {
-#define PGM_CASE(P)                                                                                                \
-    case P:                                                                                                        \
-        if (phiDot)                                                                                                \
-            kGatherTangent<T, P, true><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, phiDot, box, disp, \
-                                                                  out);                                            \
-        else                                                                                                       \
-            kGatherTangent<T, P, false><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, phi, box, disp,  \
-                                                                   out);                                           \
-        break;
-        PGM_CASE(4)
-        PGM_CASE(5)
-        PGM_CASE(6)
-        PGM_CASE(7)
-        PGM_CASE(8)
-#undef PGM_CASE
-    default:
-        throw std::runtime_error("pgm_hip: PME order must be 4..8");
-    }
+int convolveBlocksPerGrid(PmeGrid g) {
+    const int zx = linesPerBlock<T>(g.K[0]);
+    return g.K[1]*((g.K3h+zx-1)/zx);
 }
 
 template<class T>
-void launchGather(hipStream_t st, PmeGrid g, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
-                  BoxT<double> box, const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_,
-                  double* field, double* force, const int* skip) {
+void launchGatherTangent(hipStream_t st, PmeSel ps, int n, const int* pmeAtom, const int* base, const T* spl,
+                         const T* phi, const T* phiDot, const typename Vec4<T>::type* disp, double* out) {
     const int blocks = nblk((long long) n*kGatherLanes, kBlock);
-    switch (g.order) {
-#define PGM_CASE(P)                                                                                                  \
-    case P:                                                                                                          \
-        if (final_)                                                                                                  \
-            kGather<T, P, true><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,      \
-                                                           force, skip);                                             \
-        else                                                                                                         \
-            kGather<T, P, false><<<blocks, kBlock, 0, st>>>(g, n, pmeAtom, base, spl, phi, box, dip, posq, field,     \
-                                                            force, skip);                                            \
-        break;
-        PGM_CASE(4)
-        PGM_CASE(5)
-        PGM_CASE(6)
-        PGM_CASE(7)
-        PGM_CASE(8)
-#undef PGM_CASE
-    default:
-        throw std::runtime_error("pgm_hip: PME order must be 4..8");
-    }
+    dispatchOrder(ps.order, ps.table != nullptr, [&](auto P, auto B) {
+        if (phiDot)
+            kGatherTangent<T, P(), true, B()><<<blocks, kBlock, 0, st>>>(ps, n, pmeAtom, base, spl, phi, phiDot, disp,
+                                                                         out);
+        else
+            kGatherTangent<T, P(), false, B()><<<blocks, kBlock, 0, st>>>(ps, n, pmeAtom, base, spl, phi, phi, disp,
+                                                                          out);
+    });
+}
+
+template<class T>
+void launchGather(hipStream_t st, PmeSel ps, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,
+                  const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int final_, double* field,
+                  double* force, const int* skip) {
+    const int blocks = nblk((long long) n*kGatherLanes, kBlock);
+    dispatchOrder(ps.order, ps.table != nullptr, [&](auto P, auto B) {
+        if (final_)
+            kGather<T, P(), true, B()><<<blocks, kBlock, 0, st>>>(ps, n, pmeAtom, base, spl, phi, dip, posq, field,
+                                                                  force, skip);
+        else
+            kGather<T, P(), false, B()><<<blocks, kBlock, 0, st>>>(ps, n, pmeAtom, base, spl, phi, dip, posq, field,
+                                                                   force, skip);
+    });
 }
 
 template<class T>
@@ -917,21 +948,22 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
 }
 
 #define PGM_INSTANTIATE_PME(T)                                                                                          \
-    template void launchPmeBin<T>(hipStream_t, int, const double4*, BoxT<double>, PmeGrid, int*, int*, int*);          \
-    template void launchPmePlace<T>(hipStream_t, int, const double4*, BoxT<double>, PmeGrid, const int*, const int*,   \
-                                    const int*, int*, int*, T*);                                                       \
-    template void launchSpread<T>(hipStream_t, PmeGrid, const int*, const int*, const int*, const T*,                  \
-                                  const Vec4<T>::type*, const Vec4<T>::type*, int, BoxT<double>, T*, const int*,       \
+    template void launchPmeBin<T>(hipStream_t, int, const double4*, PmeSel, int*, int*, int*);                         \
+    template void launchPmePlace<T>(hipStream_t, int, const double4*, PmeSel, const int*, const int*, const int*,      \
+                                    int*, int*, T*);                                                                   \
+    template void launchSpread<T>(hipStream_t, PmeSel, const int*, const int*, const int*, const T*,                   \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, T*, const int*,                     \
                                   const Vec4<T>::type*, const Vec4<T>::type*);                                         \
-    template void launchGatherTangent<T>(hipStream_t, PmeGrid, int, const int*, const int*, const T*, const T*,        \
-                                         const T*, BoxT<double>, const Vec4<T>::type*, double*);                       \
-    template int launchPmeConvolve<T>(hipStream_t, PmeGrid, T*, Vec2<T>::type*, const T*, const Vec2<T>::type*,        \
-                                      BoxT<double>, double, int, int, double*, const int*);                            \
-    template void launchGather<T>(hipStream_t, PmeGrid, int, const int*, const int*, const T*, const T*, BoxT<double>, \
+    template void launchGatherTangent<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,         \
+                                         const T*, const Vec4<T>::type*, double*);                                     \
+    template int launchPmeConvolve<T>(hipStream_t, PmeGrid, int, T*, Vec2<T>::type*, const T*, const Vec2<T>::type*,   \
+                                      BoxT<double>, const BoxT<double>*, double, int, int, double*, const int*);       \
+    template void launchGather<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,                \
                                   const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
     template void launchTwiddles<T>(hipStream_t, PmeGrid, Vec2<T>::type*);                                             \
-    template void launchZero<T>(hipStream_t, T*, size_t, const int*);
+    template void launchZero<T>(hipStream_t, T*, size_t, const int*);                                                 \
+    template int convolveBlocksPerGrid<T>(PmeGrid);
 PGM_INSTANTIATE_PME(float)
 PGM_INSTANTIATE_PME(double)
 template void launchZero<int>(hipStream_t, int*, size_t, const int*);

@@ -250,41 +250,29 @@ __global__ void kPost(int n, double peek, const double* alpha, const double* r, 
 __global__ void __launch_bounds__(kCgBlock) kFinalCombine(int n, const double* d, const double* mu,
                                                           const double* alpha, const double* edir,
                                                           const double* erec, double cself, double* etot,
-                                                          double* partials) {
-    __shared__ double red[kCgBlock/64][11];
-    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                                                          double* perAtom) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
-    if (s < n) {
-        const double a = alpha[s];
-        double dv[3], er[3];
+    if (s >= n)
+        return;
+    const double a = alpha[s];
+    double dv[3], er[3], d2 = 0, pol = 0;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            dv[c] = d[3*s+c];
-            er[c] = erec[3*s+c];
-            etot[3*s+c] = edir[3*s+c]+er[c]+cself*dv[c];
-            acc[0] += dv[c]*dv[c];
-            if (a > 0)
-                acc[1] += mu[3*s+c]*mu[3*s+c]/(2.0*a);
-        }
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int q = 0; q < 3; q++)
-                acc[2+3*p+q] = er[p]*dv[q];
+    for (int c = 0; c < 3; c++) {
+        dv[c] = d[3*s+c];
+        er[c] = erec[3*s+c];
+        etot[3*s+c] = edir[3*s+c]+er[c]+cself*dv[c];
+        d2 += dv[c]*dv[c];
+        if (a > 0)
+            pol += mu[3*s+c]*mu[3*s+c]/(2.0*a);
     }
+    double* o = perAtom+11*(size_t) s;
+    o[0] = d2;
+    o[1] = pol;
 #pragma unroll
-    for (int q = 0; q < 11; q++) {
-        const double v = groupSum<64>(acc[q]);
-        if ((threadIdx.x & 63) == 0)
-            red[threadIdx.x >> 6][q] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 11) {
-        double v = 0;
-        for (int w = 0; w < kCgBlock/64; w++)
-            v += red[w][threadIdx.x];
-        partials[(size_t) blockIdx.x*11+threadIdx.x] = v;
-    }
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            o[2+3*p+q] = er[p]*dv[q];
 }
 
 // dE/dr_partner = c (g - (g.u) u)/|v|, g = -E_atom: +t on the atom, -t on the partner, strain t (x) v
@@ -309,49 +297,42 @@ __device__ __forceinline__ void covalentTerm(int m, const int* covAtomOf, const 
     v[2] = z;
 }
 
+// Per atom i (caller's order): the chain-rule force of the entries it carries (+t) and of those it is the
+// partner of (-t), and the strain of the entries it carries.  Atom and partner share a system.
+template<bool BATCH>
 __global__ void __launch_bounds__(kCgBlock) kCovalent(int n, const int* rank, const double* pos, const int* covStart,
                                                       const int* covPartner, const double* covStrength,
                                                       const int* revStart, const int* revEntry,
-                                                      const int* covAtomOf, const double* etot, BoxT<double> box,
-                                                      double* fcov, double* partials) {
-    __shared__ double red[kCgBlock/64][9];
-    double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                                                      const int* covAtomOf, const double* etot, BoxSel<double> bs,
+                                                      double* fcov, double* perAtom) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i < n) {
-        double f[3] = {0, 0, 0}, t[3], v[3];
-        for (int m = covStart[i]; m < covStart[i+1]; m++) {
-            covalentTerm(m, covAtomOf, covPartner, covStrength, pos, rank, etot, box, t, v);
+    if (i >= n)
+        return;
+    const BoxT<double> box = selectBox<BATCH>(bs, i);
+    double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double f[3] = {0, 0, 0}, t[3], v[3];
+    for (int m = covStart[i]; m < covStart[i+1]; m++) {
+        covalentTerm(m, covAtomOf, covPartner, covStrength, pos, rank, etot, box, t, v);
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                f[k] += t[k];
+        for (int k = 0; k < 3; k++) {
+            f[k] += t[k];
 #pragma unroll
-                for (int b = 0; b < 3; b++)
-                    S[3*k+b] += t[k]*v[b];
-            }
+            for (int b = 0; b < 3; b++)
+                S[3*k+b] += t[k]*v[b];
         }
-        for (int e = revStart[i]; e < revStart[i+1]; e++) {
-            covalentTerm(revEntry[e], covAtomOf, covPartner, covStrength, pos, rank, etot, box, t, v);
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-                f[k] -= t[k];
-        }
+    }
+    for (int e = revStart[i]; e < revStart[i+1]; e++) {
+        covalentTerm(revEntry[e], covAtomOf, covPartner, covStrength, pos, rank, etot, box, t, v);
 #pragma unroll
         for (int k = 0; k < 3; k++)
-            fcov[3*i+k] = f[k];
+            f[k] -= t[k];
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-        const double v = groupSum<64>(S[q]);
-        if ((threadIdx.x & 63) == 0)
-            red[threadIdx.x >> 6][q] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 9) {
-        double v = 0;
-        for (int w = 0; w < kCgBlock/64; w++)
-            v += red[w][threadIdx.x];
-        partials[(size_t) blockIdx.x*9+threadIdx.x] = v;
-    }
+    for (int k = 0; k < 3; k++)
+        fcov[3*i+k] = f[k];
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+        perAtom[9*(size_t) i+q] = S[q];
 }
 
 __global__ void kForcesOut(int n, const int* rank, const double* fpair, const double* frec, const double* fcov,
@@ -380,6 +361,22 @@ __global__ void __launch_bounds__(256) kReducePartials(const double* partials, i
         v = blockSum(v, smem);
         if (threadIdx.x == 0)
             out[q] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) kRangeSums(const double* rows, int width, const int* starts,
+                                                  const int* outIndex, int outStride, double* out) {
+    __shared__ double smem[40];
+    const int r = blockIdx.x;
+    const int lo = starts[r], hi = starts[r+1];
+    const int orow = outIndex ? outIndex[r] : r;
+    for (int q = 0; q < width; q++) {
+        double v = 0;
+        for (int k = lo+threadIdx.x; k < hi; k += blockDim.x)
+            v += rows[(size_t) k*width+q];
+        v = blockSum(v, smem);
+        if (threadIdx.x == 0)
+            out[(size_t) orow*outStride+q] = v;
     }
 }
 
@@ -441,20 +438,23 @@ void launchPost(hipStream_t st, int n, double peek, const double* alpha, const d
     kPost<T><<<nblk(n, 256), 256, 0, st>>>(n, peek, alpha, r, x, perm, order, gidx, hist, d, dT);
 }
 
-int launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
-                       const double* edir, const double* erec, double cself, double* etot, double* partials) {
-    const int nb = nblk(n, kCgBlock);
-    kFinalCombine<<<nb, kCgBlock, 0, st>>>(n, d, mu, alpha, edir, erec, cself, etot, partials);
-    return nb;
+void launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
+                        const double* edir, const double* erec, double cself, double* etot, double* perAtom) {
+    kFinalCombine<<<nblk(n, kCgBlock), kCgBlock, 0, st>>>(n, d, mu, alpha, edir, erec, cself, etot, perAtom);
 }
 
-int launchCovalentForces(hipStream_t st, int n, const int* rank, const double* pos, const int* covStart,
-                         const int* covPartner, const double* covStrength, const int* revStart, const int* revEntry,
-                         const int* covAtomOf, const double* etot, BoxT<double> box, double* fcov, double* partials) {
-    const int nb = nblk(n, kCgBlock);
-    kCovalent<<<nb, kCgBlock, 0, st>>>(n, rank, pos, covStart, covPartner, covStrength, revStart, revEntry, covAtomOf,
-                                       etot, box, fcov, partials);
-    return nb;
+void launchCovalentForces(hipStream_t st, int n, const int* rank, const double* pos, const int* covStart,
+                          const int* covPartner, const double* covStrength, const int* revStart, const int* revEntry,
+                          const int* covAtomOf, const double* etot, BoxSel<double> box, double* fcov,
+                          double* perAtom) {
+    if (box.table)
+        kCovalent<true><<<nblk(n, kCgBlock), kCgBlock, 0, st>>>(n, rank, pos, covStart, covPartner, covStrength,
+                                                                revStart, revEntry, covAtomOf, etot, box, fcov,
+                                                                perAtom);
+    else
+        kCovalent<false><<<nblk(n, kCgBlock), kCgBlock, 0, st>>>(n, rank, pos, covStart, covPartner, covStrength,
+                                                                 revStart, revEntry, covAtomOf, etot, box, fcov,
+                                                                 perAtom);
 }
 
 void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair, const double* frec,
@@ -464,6 +464,12 @@ void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair
 
 void launchReducePartials(hipStream_t st, const double* partials, int nblk_, int width, double* out) {
     kReducePartials<<<1, 256, 0, st>>>(partials, nblk_, width, out);
+}
+
+void launchRangeSums(hipStream_t st, const double* rows, int width, const int* starts, int nr, const int* outIndex,
+                     int outStride, double* out) {
+    if (nr > 0)
+        kRangeSums<<<nr, 256, 0, st>>>(rows, width, starts, outIndex, outStride, out);
 }
 
 void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out) {

@@ -62,10 +62,12 @@ EXPORTS = [
     "pgm_compute_device_f64", "pgm_iterations", "pgm_residual", "pgm_induced_dipoles", "pgm_permanent_dipoles",
     "pgm_reset_history", "pgm_list_builds", "pgm_num_candidates", "pgm_late_list_checks", "pgm_set_history",
     "pgm_freeze_history", "pgm_set_timing", "pgm_get_timing", "pgm_num_pairs", "pgm_backward",
-    "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual",
+    "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
+    "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
+    "pgm_autotune", "pgm_set_lanes",
 ]  # fmt: skip
-PHASES = ["list", "prep", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg", "pair_final",
-          "final"]  # fmt: skip
+PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
+          "pair_final", "final"]  # fmt: skip
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:
@@ -122,6 +124,14 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_adjoint_iterations.argtypes = [vp]
     lib.pgm_adjoint_residual.argtypes = [vp]
     lib.pgm_adjoint_residual.restype = C.c_double
+    lib.pgm_set_batch.argtypes = [vp, C.c_int, vp, vp]
+    lib.pgm_num_systems.argtypes = [vp]
+    lib.pgm_compute_batch.argtypes = [vp, vp, vp, C.c_int, vp, vp]
+    lib.pgm_compute_batch_device.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
+    lib.pgm_backward_batch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.pgm_backward_batch_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.pgm_autotune.argtypes = [vp, C.c_int, vp, vp]
+    lib.pgm_set_lanes.argtypes = [vp, C.c_int]
     _ = dp
     _lib = lib
     return lib
@@ -167,6 +177,7 @@ class Engine:
             raise PgmError(self.lib.pgm_last_error(None).decode())
         self.n = 0
         self.ncov = 0
+        self.nsys = 1
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -266,6 +277,70 @@ class Engine:
         rc = self.lib.pgm_backward_device(self.h, pos_ptr, _ptr(box), g_ptr, float(dl_denergy), gq_ptr, gr_ptr, ga_ptr,
                                           gc_ptr, stream)  # fmt: skip
         self._check(rc)
+
+    def set_batch(self, atom_offsets, grids=None):
+        """Partition the atoms into independent systems (pgm_set_batch); grids (nsys, 3) or None."""
+        off = np.ascontiguousarray(atom_offsets, np.int32)
+        g = None if grids is None else np.ascontiguousarray(np.asarray(grids).reshape(-1, 3), np.int32)
+        self._check(self.lib.pgm_set_batch(self.h, len(off) - 1, _ptr(off), _ptr(g)))
+        self.nsys = len(off) - 1
+
+    def num_systems(self) -> int:
+        return self.lib.pgm_num_systems(self.h)
+
+    def compute_batch(self, pos, boxes, new_step=True, forces=True):
+        """Return ([terms dict per system], forces (n,3) or None)."""
+        pos = _f64(pos).reshape(-1, 3)
+        if len(pos) != self.n:
+            raise PgmError(f"positions for {len(pos)} atoms, engine has {self.n}")
+        nsys = self.num_systems()
+        boxes = _f64(boxes).reshape(nsys, 3, 3)
+        f = np.zeros((self.n, 3)) if forces else None
+        terms = (Terms * nsys)()
+        rc = self.lib.pgm_compute_batch(self.h, _ptr(pos), _ptr(boxes), int(bool(new_step)), _ptr(f), terms)
+        self._check(rc)
+        return [t.as_dict() for t in terms], f
+
+    def compute_batch_device(self, pos_ptr: int, boxes, new_step=True, f_ptr: int | None = None, stream: int = 0):
+        nsys = self.num_systems()
+        boxes = _f64(boxes).reshape(nsys, 3, 3)
+        terms = (Terms * nsys)()
+        rc = self.lib.pgm_compute_batch_device(self.h, pos_ptr, _ptr(boxes), int(bool(new_step)), f_ptr, stream, terms)
+        self._check(rc)
+        return [t.as_dict() for t in terms]
+
+    def backward_batch(self, pos, boxes, dl_dforces=None, dl_denergy=None) -> dict:
+        """pgm_backward_batch: dl_denergy per system (or None)."""
+        pos = _f64(pos).reshape(-1, 3)
+        nsys = self.num_systems()
+        boxes = _f64(boxes).reshape(nsys, 3, 3)
+        g = None if dl_dforces is None else _f64(dl_dforces).reshape(self.n, 3)
+        we = None if dl_denergy is None else _f64(np.broadcast_to(np.asarray(dl_denergy, float), (nsys,)))
+        gq, gr, ga = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        gc = np.zeros(max(self.ncov, 1))
+        rc = self.lib.pgm_backward_batch(self.h, _ptr(pos), _ptr(boxes), _ptr(g), _ptr(we), _ptr(gq), _ptr(gr),
+                                         _ptr(ga), _ptr(gc))  # fmt: skip
+        self._check(rc)
+        return {"q": gq, "radius": gr, "alpha": ga, "cov_strength": gc[: self.ncov]}
+
+    def backward_batch_device(self, pos_ptr: int, boxes, g_ptr, dl_denergy, gq_ptr, gr_ptr, ga_ptr, gc_ptr,
+                              stream: int = 0):  # fmt: skip
+        nsys = self.num_systems()
+        boxes = _f64(boxes).reshape(nsys, 3, 3)
+        we = None if dl_denergy is None else _f64(np.broadcast_to(np.asarray(dl_denergy, float), (nsys,)))
+        rc = self.lib.pgm_backward_batch_device(self.h, pos_ptr, _ptr(boxes), g_ptr, _ptr(we), gq_ptr, gr_ptr, ga_ptr,
+                                                gc_ptr, stream)  # fmt: skip
+        self._check(rc)
+
+    def autotune(self, reps: int = 3):
+        """(lanes chosen, {lanes: ms per evaluation in the pair passes}) (pgm_autotune)."""
+        lanes = C.c_int(0)
+        ms = np.zeros(3)
+        self._check(self.lib.pgm_autotune(self.h, int(reps), C.byref(lanes), _ptr(ms)))
+        return lanes.value, {8: float(ms[0]), 16: float(ms[1]), 32: float(ms[2])}
+
+    def set_lanes(self, lanes: int):
+        self._check(self.lib.pgm_set_lanes(self.h, int(lanes)))
 
     def adjoint_iterations(self) -> int:
         return self.lib.pgm_adjoint_iterations(self.h)
