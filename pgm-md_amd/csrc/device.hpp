@@ -60,6 +60,25 @@ __device__ __forceinline__ double expT(double x) { return exp(x); }
 __device__ __forceinline__ float rsqrtT(float x) { return __frsqrt_rn(x); }
 __device__ __forceinline__ double rsqrtT(double x) { return rsqrt(x); }
 
+// erfc(x) for x >= 0 in FP32: the Chebyshev fit of Numerical Recipes in C, 2nd ed. (Press et al.), sec. 6.2
+// (erfcc), erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2), fractional error below 1.2e-7 for the fit and
+// below 1.3e-6 evaluated in FP32 for x <= 4.5 (the range of the pair kernels).  v_rcp / v_exp instead of the
+// library erfcf and its divisions.
+__device__ __forceinline__ float erfcFast(float x) {
+    const float t = __builtin_amdgcn_rcpf(1.0f+0.5f*x);
+    float p = 0.17087277f;
+    p = fmaf(t, p, -0.82215223f);
+    p = fmaf(t, p, 1.48851587f);
+    p = fmaf(t, p, -1.13520398f);
+    p = fmaf(t, p, 0.27886807f);
+    p = fmaf(t, p, -0.18628806f);
+    p = fmaf(t, p, 0.09678418f);
+    p = fmaf(t, p, 0.37409196f);
+    p = fmaf(t, p, 1.00002368f);
+    p = fmaf(t, p, -1.26551223f);
+    return t*__expf(fmaf(-x, x, p));
+}
+
 // B_n of erf(a r)/r, n < NMAX, by the series in s = (a r)^2 (valid for s < 1; no cancellation at short
 // range): B_n = (2a/sqrt(pi)) (2a^2)^n sum_m (-s)^m / (m! (2m+2n+1)) (core/pgm_math.h:31-49, docs/spec.md Sec. 4).
 template<class T, int NMAX>
@@ -115,6 +134,67 @@ __device__ __forceinline__ void erfcKernels(T a, T r, T ir2, T* B) {
         B[k] = (T(2*k-1)*B[k-1]+scale*g)*ir2;
         scale *= twoA2;
     }
+}
+
+// FP32 forms of erfKernels / erfcKernels: 1/r from v_rsq, erfc by erfcFast, no divisions.
+template<int NMAX>
+__device__ __forceinline__ void erfKernelsF(float a, float r, float rinv, float* B) {
+    const float x = a*r, s = x*x;
+    if (s < 1.0f) {
+        erfSeries<float, NMAX>(a, s, B);
+        return;
+    }
+    const float g = float(kTwoOverSqrtPi)*a*__expf(-s);
+    const float ir2 = rinv*rinv;
+    B[0] = (1.0f-erfcFast(x))*rinv;
+    float scale = 1.0f;
+    const float twoA2 = 2.0f*a*a;
+#pragma unroll
+    for (int k = 1; k < NMAX; k++) {
+        B[k] = (float(2*k-1)*B[k-1]-scale*g)*ir2;
+        scale *= twoA2;
+    }
+}
+
+template<int NMAX>
+__device__ __forceinline__ void erfcKernelsF(float a, float r, float rinv, float* B) {
+    const float x = a*r;
+    const float g = float(kTwoOverSqrtPi)*a*__expf(-x*x);
+    const float ir2 = rinv*rinv;
+    B[0] = erfcFast(x)*rinv;
+    float scale = 1.0f;
+    const float twoA2 = 2.0f*a*a;
+#pragma unroll
+    for (int k = 1; k < NMAX; k++) {
+        B[k] = (float(2*k-1)*B[k-1]+scale*g)*ir2;
+        scale *= twoA2;
+    }
+}
+
+// G_n for FP32 from r^2 and rg_i + rg_j (a = (rg_i + rg_j)^-1/2): the branches of pairKernels below.
+template<int NMAX>
+__device__ __forceinline__ void pairKernelsF(float rgSum, float beta, float r2, float* G) {
+    const float a = __builtin_amdgcn_rsqf(rgSum);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2*rinv;
+    float B[NMAX];
+    if (a*r < 1.0f) {
+        erfSeries<float, NMAX>(a, (a*r)*(a*r), G);
+        erfKernelsF<NMAX>(beta, r, rinv, B);
+    }
+    else {
+        erfcKernelsF<NMAX>(beta, r, rinv, G);
+        if (a*r < PairPrec<float>::kDampCut)
+            erfcKernelsF<NMAX>(a, r, rinv, B);
+        else {
+#pragma unroll
+            for (int k = 0; k < NMAX; k++)
+                B[k] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NMAX; k++)
+        G[k] -= B[k];
 }
 
 // G_n = B_n[erf(a r)/r] - B_n[erf(beta r)/r] for n in [N0, NMAX): the series difference for a r < 1, the erfc

@@ -884,7 +884,8 @@ private:
             cplxTotal_ += nc*(long long) G.members.size();
             twTotal_ += g.K[0]+g.K[1]+g.K[2];
             bmodTotal_ += g.K[0]+g.K[1]+g.K[2];
-            G.blocksPerGrid = mixed ? convolveBlocksPerGrid<float>(g) : convolveBlocksPerGrid<double>(g);
+            const int ng = (int) G.members.size();
+            G.blocksPerGrid = mixed ? convolveBlocksPerGrid<float>(g, ng) : convolveBlocksPerGrid<double>(g, ng);
             G.partOff = partTotal_;
             partTotal_ += G.blocksPerGrid*(int) G.members.size();
             for (size_t k = 0; k < G.members.size(); k++) {

@@ -169,9 +169,9 @@ template<class T>
 int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Vec2<T>::type* cplx, const T* influence,
                       const typename Vec2<T>::type* twiddles, BoxT<double> box, const BoxT<double>* boxes, double beta,
                       int energy, int strain, double* partials, const int* skip);
-// Blocks of the x pass per grid (the rows of its energy partials).
+// Blocks of the x pass per grid (the rows of its energy partials) when ngrid grids of shape g are stacked.
 template<class T>
-int convolveBlocksPerGrid(PmeGrid g);
+int convolveBlocksPerGrid(PmeGrid g, int ngrid);
 // Field -grad phi (sorted order) and, when final, the reciprocal forces.
 template<class T>
 void launchGather(hipStream_t st, PmeSel ps, int n, const int* pmeAtom, const int* base, const T* spl, const T* phi,

@@ -68,9 +68,11 @@ __device__ __forceinline__ void setupTerm(const typename Vec4<T>::type& pi, T rg
                                           typename Vec2<T>::type& gb, double* e, double* f) {
     T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
     minImage(x, y, z, box);
-    const T r = sqrt(x*x+y*y+z*z);
     T G[4];
-    pairKernels<T, 4>(rsqrtT(rgi+rgj), beta, r, G);
+    if constexpr (sizeof(T) == sizeof(float))
+        pairKernelsF<4>(rgi+rgj, beta, x*x+y*y+z*z, G);
+    else
+        pairKernels<T, 4>(rsqrtT(rgi+rgj), beta, sqrt(x*x+y*y+z*z), G);
     const T c = pjv.w*G[1]+(d.x*x+d.y*y+d.z*z)*G[2];
     e[0] += (double) (c*x-d.x*G[1]);
     e[1] += (double) (c*y-d.y*G[1]);

@@ -798,9 +798,12 @@ __global__ void kZero(T* p, size_t count, const int* skip) {
 inline int nblk(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
 
 template<class T>
-int linesPerBlock(int K) {
+int linesPerBlock(int K, long long lines) {
+    // LDS: twiddles + two line buffers within 48 KB; and enough blocks to fill 256 CUs twice over
     const int budget = (int) (48*1024/(2*sizeof(typename Vec2<T>::type)))-K;   // elements per buffer
-    return std::max(1, std::min(16, budget/(2*K)));
+    const int byLds = std::max(1, std::min(16, budget/(2*K)));
+    const int byFill = (int) std::max(1LL, lines/512);
+    return std::min(byLds, byFill);
 }
 
 }   // namespace
@@ -875,16 +878,16 @@ int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Ve
     typedef typename Vec2<T>::type T2;
     const int nlines = ngrid*g.K[0]*g.K[1];
     // z forward
-    const int pz = linesPerBlock<T>(g.K[2]);
+    const int pz = linesPerBlock<T>(g.K[2], (nlines+1)/2);
     const int zBlocks = nblk((nlines+1)/2, pz);
     const size_t ldsZ = (size_t) (g.K[2]+2*pz*g.K[2])*sizeof(T2);
     kFftZ<T, false><<<zBlocks, kBlock, ldsZ, st>>>(g, ngrid, real, cplx, tw, pz, skip);
     // y: the stacked grids are one grid of ngrid K0 x-planes
-    const int zy = linesPerBlock<T>(g.K[1]);
+    const int zy = linesPerBlock<T>(g.K[1], (long long) ngrid*g.K[0]*g.K3h);
     const int yBlocks = ngrid*g.K[0]*((g.K3h+zy-1)/zy);
     const size_t ldsY = (size_t) (g.K[1]+2*zy*g.K[1])*sizeof(T2);
     kFftY<T, false><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
-    const int zx = linesPerBlock<T>(g.K[0]);
+    const int zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
     const int xBlocks = ngrid*g.K[1]*((g.K3h+zx-1)/zx);
     const size_t ldsX = (size_t) (g.K[0]+2*zx*g.K[0])*sizeof(T2);
     kFftXConv<T><<<xBlocks, kBlock, ldsX, st>>>(g, cplx, influence, tw, zx, box, boxes, beta, energy, strain,
@@ -895,8 +898,8 @@ int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Ve
 }
 
 template<class T>
-int convolveBlocksPerGrid(PmeGrid g) {
-    const int zx = linesPerBlock<T>(g.K[0]);
+int convolveBlocksPerGrid(PmeGrid g, int ngrid) {
+    const int zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
     return g.K[1]*((g.K3h+zx-1)/zx);
 }
 
@@ -963,7 +966,7 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
     template void launchTwiddles<T>(hipStream_t, PmeGrid, Vec2<T>::type*);                                             \
     template void launchZero<T>(hipStream_t, T*, size_t, const int*);                                                 \
-    template int convolveBlocksPerGrid<T>(PmeGrid);
+    template int convolveBlocksPerGrid<T>(PmeGrid, int);
 PGM_INSTANTIATE_PME(float)
 PGM_INSTANTIATE_PME(double)
 template void launchZero<int>(hipStream_t, int*, size_t, const int*);
