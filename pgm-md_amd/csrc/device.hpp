@@ -197,6 +197,33 @@ __device__ __forceinline__ void pairKernelsF(float rgSum, float beta, float r2, 
         G[k] -= B[k];
 }
 
+// G0 and G3 of a pair from its G1, G2 (FP32), by the recurrence of the erfc forms
+// Bc_{n+1}[c] = ((2n+1) Bc_n[c] + (2c^2)^n h_c)/r^2, h_c = (2c/sqrt(pi)) exp(-c^2 r^2), applied to
+// G_n = Bc_n[beta] - Bc_n[a]: G0 = r^2 G1 + h_a - h_beta, G3 = (5 G2 + (2 beta^2)^2 h_beta - (2a^2)^2 h_a)/r^2,
+// with h_a = 0 beyond the damping cut as in pairKernelsF.  Close pairs (a r < 1, the series branch, where the
+// upward recurrence would cancel) get all four kernels again from pairKernelsF.
+__device__ __forceinline__ void g03FromG12(float G1, float G2, float r2, float rgSum, float beta, float& G0,
+                                           float& G3);
+
+__device__ __forceinline__ void g03FromG12(float G1, float G2, float r2, float rgSum, float beta, float& G0,
+                                           float& G3) {
+    const float a = __builtin_amdgcn_rsqf(rgSum);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2*rinv;
+    if (a*r < 1.0f) {
+        float G[4];
+        pairKernelsF<4>(rgSum, beta, r2, G);
+        G0 = G[0];
+        G3 = G[3];
+        return;
+    }
+    const float ha = a*r < PairPrec<float>::kDampCut ? float(kTwoOverSqrtPi)*a*__expf(-a*a*r2) : 0.0f;
+    const float hb = float(kTwoOverSqrtPi)*beta*__expf(-beta*beta*r2);
+    const float ta = 2.0f*a*a, tb = 2.0f*beta*beta;
+    G0 = r2*G1+ha-hb;
+    G3 = (5.0f*G2+tb*tb*hb-ta*ta*ha)*(rinv*rinv);
+}
+
 // G_n = B_n[erf(a r)/r] - B_n[erf(beta r)/r] for n in [N0, NMAX): the series difference for a r < 1, the erfc
 // form Bc(beta) - Bc(a) beyond (core/pgm_math.h:78-92), the Bc(a) part dropped beyond a r > kDampCut.
 template<class T, int NMAX>

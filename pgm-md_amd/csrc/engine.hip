@@ -1164,7 +1164,8 @@ private:
         candD_.reserve(cap*sizeof(int));
         pjD_.reserve(cap*sizeof(int));
         g12D_.reserve(cap*g2);
-        g03D_.reserve(cap*g2);
+        if (s_.precision == PGM_DOUBLE)   // the FP32 path rebuilds G0, G3 from G1, G2
+            g03D_.reserve(cap*g2);
         for (const Sys& y : sys_) {
             const BoxT<double> bd = makeBox<double>(y.H, y.recip);
             launchCandFill(st_, y.a1-y.a0, posDD_.as<double4>()+y.a0, sortedCellD_.as<int>()+y.a0,
@@ -1441,9 +1442,9 @@ private:
         // final stage
         double* res = resD_.as<double>();
         tb(kPhPairFinal);
-        launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                           g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(), edD_.as<double>(),
-                           perAtomD_.as<double>(), lanes_);
+        launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(), nbD_.as<int>(),
+                           pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(),
+                           edD_.as<double>(), perAtomD_.as<double>(), lanes_);
         systemSums(perAtomD_.as<double>(), 10, res, kResPair);
         te();
         pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
@@ -1483,9 +1484,9 @@ private:
             for (int k = 0; k <= iterations_; k++)
                 launchPairField<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                                    g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes);
-            launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                               g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(), edD_.as<double>(),
-                               perAtomD_.as<double>(), lanes);
+            launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(),
+                               nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
+                               fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), lanes);
         };
         once();   // warm-up
         HIPCHK(hipEventRecord(a, st_));
@@ -1527,7 +1528,7 @@ private:
         // the potential / field of the charges and total dipoles (and their tangents for the displacement g and
         // the dipole change ddT): pair pass over the stored kernels, the PME grids and their tangent grids
         auto tangentPass = [&](const T4* ddT) {
-            launchPairTangent<T>(st_, n, pos, rgT<T>(), dTD_.as<T4>(), ddT, ddT ? gTD_.as<T4>() : nullptr,
+            launchPairTangent<T>(st_, n, pos, rgT<T>(), (T) s_.beta, dTD_.as<T4>(), ddT, ddT ? gTD_.as<T4>() : nullptr,
                                  candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                                  g12D_.as<typename Vec2<T>::type>(), g03D_.as<typename Vec2<T>::type>(), bt,
                                  pairOutD_.as<double>());

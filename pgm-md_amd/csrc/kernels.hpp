@@ -79,8 +79,9 @@ constexpr int kPairLaneChoices[3] = {8, 16, 32};
 template<class T>
 void launchPairCompact(hipStream_t st, int n, const typename Vec4<T>::type* pos, const int* candStart,
                        const int* cand, int* nb, int* pj, BoxSel<T> box, T rc2, int lanes);
-// Per evaluation, over the compacted pairs: the kernels (g12: G1, G2; g03: G0, G3), the direct field of the
-// charges and dipoles dip and, when dual is given, of the dipoles dual alone (the first CG product).
+// Per evaluation, over the compacted pairs: the kernels (g12: G1, G2; g03: G0, G3, stored by the FP64 path only),
+// the direct field of the charges and dipoles dip and, when dual is given, of the dipoles dual alone (the first
+// CG product).
 template<class T>
 void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                      const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
@@ -91,19 +92,20 @@ template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxSel<T> box, double* field, const int* skip, int lanes);
-// Energy, forces, field and strain of the pairs at the total dipoles from the stored G0..G3; per atom
-// (sorted order) perAtom[n][10] = energy, strain 0..8 (each pair's half).
+// Energy, forces, field and strain of the pairs at the total dipoles from the stored kernels (FP32: G0, G3 rebuilt
+// from G1, G2 with the radii rg and beta); per atom (sorted order) perAtom[n][10] = energy, strain 0..8 (each
+// pair's half).
 template<class T>
 void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
-                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     const typename Vec2<T>::type* g03, BoxSel<T> box, double* force, double* field,
-                     double* perAtom, int lanes);
+                     const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
+                     const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
+                     double* force, double* field, double* perAtom, int lanes);
 
 // Parameter derivatives of the pair energy at fixed dipoles and, with dipDot (and disp) given, their tangents
 // along the displacement disp and the dipole change dipDot, sorted order, each atom from its own visits of its
 // pairs.  out (FP64): phi[n], E[3n], dU/dR[n], then with tangents phi'[n], E'[3n], (dU/dR)'[n].
 template<class T>
-void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg, T beta,
                        const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dipDot,
                        const typename Vec4<T>::type* disp, const int* candStart, const int* nb, const int* pj,
                        const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,

@@ -22,6 +22,11 @@ namespace {
 constexpr int kPairLanes = 16;
 constexpr int kPairBlock = 256;
 
+// FP64 stores all four kernels per pair; FP32 stores G1, G2 only and rebuilds G0, G3 where they are needed
+// (g03FromG12): the setup pass writes 8 bytes per pair instead of 16, the final pass reads 12 instead of 20.
+template<class T>
+constexpr bool kStoreG03 = sizeof(T) == sizeof(double);
+
 // Compaction: every atom's candidates within rc, in place in its segment (pj), and the count (nb).
 template<class T, int L, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec4<T>::type* __restrict__ pos,
@@ -132,7 +137,8 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
             T2 ga, gb;
             setupTerm<T, DUAL>(pi, rgi, pjv[q], rgj[q], d[q], u[q], box, beta, ga, gb, e, f);
             g12[start+k+q*L] = ga;
-            g03[start+k+q*L] = gb;
+            if (kStoreG03<T>)
+                g03[start+k+q*L] = gb;
         }
     }
     for (; k < cnt; k += L) {
@@ -141,7 +147,8 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
         const T4 u = DUAL ? dual[j] : pos[j];
         setupTerm<T, DUAL>(pi, rgi, pos[j], rg[j], dip[j], u, box, beta, ga, gb, e, f);
         g12[start+k] = ga;
-        g03[start+k] = gb;
+        if (kStoreG03<T>)
+            g03[start+k] = gb;
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -230,10 +237,19 @@ template<class T>
 __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& di,
                                           const typename Vec4<T>::type& pjv, const typename Vec4<T>::type& dj,
                                           const typename Vec2<T>::type& ga, const typename Vec2<T>::type& gb,
-                                          const BoxT<T>& box, double* acc, double* f, double* e) {
-    const T G0 = gb.x, G1 = ga.x, G2 = ga.y, G3 = gb.y;
+                                          T rgSum, T beta, const BoxT<T>& box, double* acc, double* f, double* e) {
     T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
     minImage(x, y, z, box);
+    const T G1 = ga.x, G2 = ga.y;
+    T G0, G3;
+    if constexpr (kStoreG03<T>) {
+        G0 = gb.x;
+        G3 = gb.y;
+        (void) rgSum;
+        (void) beta;
+    }
+    else
+        g03FromG12(G1, G2, x*x+y*y+z*z, rgSum, beta, G0, G3);
     const T qi = pi.w, qj = pjv.w;
     const T dix = di.x*x+di.y*y+di.z*z, djx = dj.x*x+dj.y*y+dj.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
     const T qd = qi*djx-qj*dix;
@@ -263,6 +279,7 @@ __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, cons
 template<class T, int L, int U, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const typename Vec4<T>::type* __restrict__ dip,
+                                                         const T* __restrict__ rg, T beta,
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ nb, const int* __restrict__ pj,
                                                          const typename Vec2<T>::type* __restrict__ g12,
@@ -278,6 +295,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
         const BoxT<T> box = selectBox<BATCH>(bs, s);
         const T4 pi = pos[s];
         const T4 di = dip[s];
+        const T rgi = kStoreG03<T> ? T(0) : rg[s];
         const int start = candStart[s], cnt = nb[s];
         double f[3] = {0, 0, 0}, e[3] = {0, 0, 0};
         int k = lane;
@@ -285,24 +303,31 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             int j[U];
             T2 ga[U], gb[U];
             T4 pjv[U], dj[U];
+            T rgj[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 j[u] = pj[start+k+u*L];
                 ga[u] = g12[start+k+u*L];
-                gb[u] = g03[start+k+u*L];
+                if (kStoreG03<T>)
+                    gb[u] = g03[start+k+u*L];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 pjv[u] = pos[j[u]];
                 dj[u] = dip[j[u]];
+                rgj[u] = kStoreG03<T> ? T(0) : rg[j[u]];
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                finalTerm<T>(pi, di, pjv[u], dj[u], ga[u], gb[u], box, acc, f, e);
+                finalTerm<T>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, acc, f, e);
         }
         for (; k < cnt; k += L) {
             const int j = pj[start+k];
-            finalTerm<T>(pi, di, pos[j], dip[j], g12[start+k], g03[start+k], box, acc, f, e);
+            T2 gb{};
+            if (kStoreG03<T>)
+                gb = g03[start+k];
+            finalTerm<T>(pi, di, pos[j], dip[j], g12[start+k], gb, kStoreG03<T> ? T(0) : rgi+rg[j], beta, box, acc,
+                         f, e);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -339,7 +364,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
 // dG_n = -(x . x') G_{n+1}, so G0..G3 stored by the setup pass cover phi' and E'.
 template<class T, int L, bool TAN, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename Vec4<T>::type* __restrict__ pos,
-                                                           const T* __restrict__ rg,
+                                                           const T* __restrict__ rg, T beta,
                                                            const typename Vec4<T>::type* __restrict__ dip,
                                                            const typename Vec4<T>::type* __restrict__ dipDot,
                                                            const typename Vec4<T>::type* __restrict__ disp,
@@ -369,12 +394,20 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = lane; k < cnt; k += L) {
         const int j = pj[start+k];
-        const T2 ga = g12[start+k], gb = g03[start+k];
-        const T G0 = gb.x, G1 = ga.x, G2 = ga.y, G3 = gb.y;
+        const T2 ga = g12[start+k];
+        const T G1 = ga.x, G2 = ga.y;
         const T4 pjv = pos[j];
         const T4 dj = dip[j];
         T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
         minImage(x, y, z, box);
+        T G0, G3;
+        if constexpr (kStoreG03<T>) {
+            const T2 gb = g03[start+k];
+            G0 = gb.x;
+            G3 = gb.y;
+        }
+        else
+            g03FromG12(G1, G2, x*x+y*y+z*z, rgi+rg[j], beta, G0, G3);
         const T qj = pjv.w, qi = pi.w;
         const T djx = dj.x*x+dj.y*y+dj.z*z, dix = di.x*x+di.y*y+di.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
         acc[0] += (double) (qj*G0+djx*G1);
@@ -502,17 +535,18 @@ void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
 
 template<class T>
 void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
-                     const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     const typename Vec2<T>::type* g03, BoxSel<T> box, double* force, double* field,
-                     double* perAtom, int lanes) {
+                     const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
+                     const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
+                     double* force, double* field, double* perAtom, int lanes) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
-        kPairFinal<T, L(), 2, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12,
-                                                                                g03, box, force, field, perAtom);
+        kPairFinal<T, L(), 2, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, rg, beta, candStart, nb,
+                                                                                pj, g12, g03, box, force, field,
+                                                                                perAtom);
     });
 }
 
 template<class T>
-void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg, T beta,
                        const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dipDot,
                        const typename Vec4<T>::type* disp, const int* candStart, const int* nb, const int* pj,
                        const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
@@ -520,17 +554,17 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
     const int blocks = groupsBlocks(n, kPairLanes);
     const bool batch = box.table != nullptr;
     if (dipDot && batch)
-        kPairTangent<T, kPairLanes, true, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dipDot, disp,
+        kPairTangent<T, kPairLanes, true, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, beta, dip, dipDot, disp,
                                                                                candStart, nb, pj, g12, g03, box, out);
     else if (dipDot)
-        kPairTangent<T, kPairLanes, true, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dipDot, disp,
+        kPairTangent<T, kPairLanes, true, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, beta, dip, dipDot, disp,
                                                                                 candStart, nb, pj, g12, g03, box, out);
     else if (batch)
-        kPairTangent<T, kPairLanes, false, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, pos, candStart,
-                                                                                nb, pj, g12, g03, box, out);
+        kPairTangent<T, kPairLanes, false, true><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, beta, dip, dip, pos,
+                                                                                candStart, nb, pj, g12, g03, box, out);
     else
-        kPairTangent<T, kPairLanes, false, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, pos, candStart,
-                                                                                 nb, pj, g12, g03, box, out);
+        kPairTangent<T, kPairLanes, false, false><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, beta, dip, dip, pos,
+                                                                                 candStart, nb, pj, g12, g03, box, out);
 }
 
 #define PGM_INSTANTIATE_PAIRS(T)                                                                                      \
@@ -542,10 +576,10 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
                                      const int*, const int*, const Vec2<T>::type*, BoxSel<T>, double*, const int*,    \
                                      int);                                                                            \
-    template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
-                                     const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxSel<T>,   \
-                                     double*, double*, double*, int);                                                 \
-    template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,       \
+    template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const T*, T,      \
+                                     const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
+                                     BoxSel<T>, double*, double*, double*, int);                                      \
+    template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, T, const Vec4<T>::type*,    \
                                        const Vec4<T>::type*, const Vec4<T>::type*, const int*, const int*,            \
                                        const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxSel<T>, double*);
 PGM_INSTANTIATE_PAIRS(float)
