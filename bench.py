@@ -36,13 +36,28 @@ METRIC = "pGM E+F evals/s @96k atoms, mixed vs FP64 GPU at ≤5e-5 rel-RMS force
 PEAK_TFLOPS = {"mixed": 157.3, "double": 78.6}  # MI355X published FP32 / FP64 rates
 HBM_PEAK_GBS = 8000.0
 CAL_COPY_BYTES = 512 << 20  # the calibration copy of the profiler pass
+TRAFFIC_FILE = os.path.join("profiles", "water96k_traffic.md")
+
+# Algorithmic HBM bytes per launch of the pair passes: per entry of the full list within the cutoff (E = 2P
+# entries) and per atom (N), s = 4 (mixed) or 8 (FP64) bytes per real.  Entry: the compacted index (4) and
+# the stored kernels (G1, G2: 2s; FP64 also G0, G3: 2s more) read or written once; compaction also reads the
+# candidate indices (4 per candidate).  Atom: its own and (once, as cache-resident partners) its neighbours'
+# position (4s), dipole (4s), radius parameter (s), list offsets and counts, and the FP64 outputs.
+PAIR_BYTES = {
+    # phase: (per entry, per atom, per candidate) for s = 4 / s = 8
+    "pair_compact": {"mixed": (4, 28, 4), "double": (4, 44, 4)},
+    "pair_setup": {"mixed": (12, 112, 0), "double": (36, 164, 0)},
+    "pair_field": {"mixed": (12, 68, 0), "double": (20, 100, 0)},
+    "pair_final": {"mixed": (12, 176, 0), "double": (36, 212, 0)},
+}
+# compaction fused into the setup pass: candidates in, compacted entries and kernels out, no index round trip
+PAIR_BYTES_FUSED_SETUP = {"mixed": (12, 112, 4), "double": (36, 164, 4)}
+KERNEL_OF_PHASE = {"pair_compact": "kCompact", "pair_setup": "kPairSetup", "pair_field": "kPairField",
+                   "pair_final": "kPairFinal"}  # fmt: skip
 
 # Algorithmic flops (add/mul/sub 1, FMA 2; exp/erfc/sqrt/rsqrt/rint not counted) per half pair within the
 # cutoff (P), per atom (N) and per grid: geometry 26, G1+G2 32, G0..G3 44, dual field body 76, dipole
 # matrix-vector body 36, final body 108; splines 300, spread 1750, field gather 1300, final gather 3950.
-FLOPS_PAIR_SETUP = 26 + 32 + 76
-FLOPS_PAIR_FIELD = 26 + 36
-FLOPS_PAIR_FINAL = 26 + 44 + 108
 
 
 def fft_flops(K):
@@ -70,44 +85,56 @@ def prior_dipole_sets(sy, E, nset=4, step=0.002, seed=22):
 
 
 def timed_steps(torch, eng, pos_d, f_d, box, steps, warmup, stream):
+    """Wall time of `steps` evaluations, and the per-phase table of those same evaluations: the engine brackets
+    every kernel group with HIP events on its own stream (the stream the kernels run on)."""
     for _ in range(warmup):
         eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
     torch.cuda.synchronize()
+    eng.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
     torch.cuda.synchronize()
-    return time.perf_counter() - t0
-
-
-def phase_table(eng, torch, pos_d, f_d, box, stream, reps):
-    eng.set_timing(True)
-    for _ in range(reps):
-        eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
-    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
     tab = eng.timing()
     eng.set_timing(False)
-    return {k: {"ms_per_eval": v[0] / reps, "launch_avg_us": 1000.0 * v[0] / max(v[1], 1), "launches": v[1] / reps}
-            for k, v in tab.items()}  # fmt: skip
+    phases = {k: {"ms_per_eval": v[0] / steps, "launch_avg_us": 1000.0 * v[0] / max(v[1], 1), "launches": v[1] / steps}
+              for k, v in tab.items()}  # fmt: skip
+    return dt, phases
 
 
-def roofline(phases, P, N, K, n, precision):
-    """Achieved algorithmic TFLOP/s of the kernel group that takes the most time per evaluation."""
-    per_launch = {
-        "pair_field": FLOPS_PAIR_FIELD * P,
-        "pair_setup": FLOPS_PAIR_SETUP * P,
-        "pair_final": FLOPS_PAIR_FINAL * P,
-        "pme_spread": 1750.0 * N,
-        "pme_fft": fft_flops(K),
-    }
-    dom = max(per_launch, key=lambda k: phases[k]["ms_per_eval"])
+def traffic_table(path=TRAFFIC_FILE):
+    """HBM bytes per launch by kernel from the committed PMC summary (rows `| kernel | ... | bytes |`)."""
+    out = {}
+    try:
+        with open(os.path.join(ROOT, path)) as fh:
+            for line in fh:
+                cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 2 and cells[0].startswith("k") and cells[-1].replace(",", "").isdigit():
+                    out[cells[0]] = int(cells[-1].replace(",", ""))
+    except OSError:
+        pass
+    return out
+
+
+def roofline(phases, P, C, N, precision, lanes, fused):
+    """Achieved algorithmic HBM bandwidth of the kernel that takes the most time per evaluation (pair passes)."""
+    E = 2 * P
+    table = dict(PAIR_BYTES, pair_setup=PAIR_BYTES_FUSED_SETUP) if fused else PAIR_BYTES
+    kernels = dict(KERNEL_OF_PHASE, pair_setup="kCompactSetup") if fused else KERNEL_OF_PHASE
+    per_launch = {ph: b[precision][0] * E + b[precision][1] * N + b[precision][2] * C for ph, b in table.items()}
+    live = [ph for ph in per_launch if phases.get(ph, {}).get("launches", 0) > 0]
+    dom = max(live, key=lambda k: phases[k]["ms_per_eval"])
     t = phases[dom]["launch_avg_us"] * 1e-6
-    achieved = per_launch[dom] / t / 1e12
-    peak = PEAK_TFLOPS[precision]
+    achieved = per_launch[dom] / t / 1e9
+    real = "float" if precision == "mixed" else "double"
+    prefix = f"{kernels[dom]}<{real}, {lanes},"
+    traffic = next((v for k, v in traffic_table().items() if k.startswith(prefix)), None)
     return {
-        "bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 4), "traffic": None, "launch_avg_us": round(phases[dom]["launch_avg_us"], 2),
-        "flops_per_launch": per_launch[dom],
+        "bound": "hbm", "kernel": f"{kernels[dom]}<{real}, {lanes}>", "phase": dom,
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic, "algorithmic_bytes": per_launch[dom], "launch_avg_us": round(phases[dom]["launch_avg_us"], 2),
+        "launches_per_eval": phases[dom]["launches"], "traffic_source": TRAFFIC_FILE if traffic else None,
     }  # fmt: skip
 
 
@@ -152,7 +179,7 @@ def adjoint_batch(torch, E, S, reps, log):
     for prec in ("double", "mixed"):
         eng = batch_engine(E, b, prec, 1e-8)
         _, f = eng.compute_batch(b["pos"], b["boxes"])
-        lanes, tune = eng.autotune(3)
+        (lanes, fused), tune = eng.autotune(3)
         fwd_it = eng.iterations()
         g = eng.backward_batch(b["pos"], b["boxes"], g_d.cpu().numpy(), None)
         if ref is None:
@@ -177,8 +204,8 @@ def adjoint_batch(torch, E, S, reps, log):
         gq0 = np.concatenate([ref[1][k] for k in ("q", "radius", "alpha")])
         out[prec] = {
             "snapshot_gradients_per_s": round(64 / dt, 2), "ms_per_batch": round(1000 * dt, 3),
-            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes,
-            "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()},
+            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "fused_setup": fused,
+            "autotune_ms": {f"{k[0]}{'f' if k[1] else 's'}": round(v, 4) for k, v in tune.items()},
             "force_relrms_vs_fp64": E.rel_rms(f, ref[0]),
             "gradient_maxrel_vs_fp64": float(np.abs(gq - gq0).max() / np.abs(gq0).max()),
         }  # fmt: skip
@@ -210,7 +237,6 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--timing-reps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-snapshot adjoint batch")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--profile", type=int, default=0, metavar="N",
@@ -243,6 +269,8 @@ def main():
         eng = E.Engine.for_system(sy, precision="mixed")
         eng.set_history(sets)
         eng.freeze_history(True)
+        eng.compute(sy.pos, box)
+        eng.autotune(3)
         for _ in range(a.profile):
             eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
         # a plain device copy of known size: the scale of the HBM byte counters
@@ -254,7 +282,7 @@ def main():
             f"calibration copy {CAL_COPY_BYTES} bytes read + {CAL_COPY_BYTES} bytes written")  # fmt: skip
         return
 
-    engines, results = {}, {}
+    engines, results, tuned = {}, {}, {}
     for prec in ("mixed", "double"):
         eng = E.Engine.for_system(sy, precision=prec)
         eng.set_history(sets)
@@ -262,20 +290,21 @@ def main():
         engines[prec] = eng
         t, f, _ = eng.compute(sy.pos, box)  # checked evaluation (host arrays)
         results[prec] = (t, f, eng.iterations())
-        log(f"{prec}: {eng.iterations()} CG iterations, E = {E.total_energy(t):.6f}")
+        tuned[prec] = eng.autotune(3)
+        log(f"{prec}: {eng.iterations()} CG iterations, E = {E.total_energy(t):.6f}, pair passes {tuned[prec][0]}")
     P = engines["double"].num_pairs()
+    C = engines["double"].num_candidates()
     tight = E.Engine.for_system(sy, precision="double", tol=1e-10, max_iter=500, peek=0.0)
     tight.set_history(sets)
     tt, ft, _ = tight.compute(sy.pos, box)
 
-    secs = {}
+    secs, phases = {}, {}
     for prec in ("mixed", "double"):
-        secs[prec] = timed_steps(torch, engines[prec], pos_d, f_d, box, a.steps, a.warmup, stream)
+        secs[prec], phases[prec] = timed_steps(torch, engines[prec], pos_d, f_d, box, a.steps, a.warmup, stream)
         log(f"{prec}: {a.steps / secs[prec]:.2f} evals/s")
-    phases = {p: phase_table(engines[p], torch, pos_d, f_d, box, stream, a.timing_reps) for p in ("mixed", "double")}
     n_it = results["mixed"][2]
-    roof = roofline(phases["mixed"], P, sy.n, sy.settings["grid"], n_it, "mixed")
-    roof64 = roofline(phases["double"], P, sy.n, sy.settings["grid"], results["double"][2], "double")
+    roof = roofline(phases["mixed"], P, C, sy.n, "mixed", *tuned["mixed"][0])
+    roof64 = roofline(phases["double"], P, C, sy.n, "double", *tuned["double"][0])
 
     value = a.steps / secs["mixed"]
     v64 = a.steps / secs["double"]
@@ -289,10 +318,13 @@ def main():
             "workload": "water96k: 32,000 rigid waters (96,000 atoms), 6.84 nm cubic box (3x liquid density), "
                         "rc 0.9 nm, beta 3.5/nm, PME 72^3 order 6, tau 1e-4, mu4 guess from 4 seeded prior "
                         "dipole sets, peek 0.65; FP32 pair/PME arithmetic, FP64 accumulation and CG",
-            "atoms": sy.n, "pairs_within_cutoff": P, "cg_iterations": n_it, "grid": list(sy.settings["grid"]),
+            "atoms": sy.n, "pairs_within_cutoff": P, "list_candidates": C, "cg_iterations": n_it,
+            "grid": list(sy.settings["grid"]),
+            "pair_passes": {"lanes": tuned["mixed"][0][0], "fused_setup": tuned["mixed"][0][1]},
         },
         "fp64_path": {"value": round(v64, 3), "ms_per_step": round(1000.0 * secs["double"] / a.steps, 4),
-                      "cg_iterations": results["double"][2], "roofline": roof64},
+                      "cg_iterations": results["double"][2], "roofline": roof64,
+                      "pair_passes": {"lanes": tuned["double"][0][0], "fused_setup": tuned["double"][0][1]}},
         "mixed_over_fp64": round(value / v64, 3),
         "accuracy": {
             "force_relrms_mixed_vs_fp64_gpu": E.rel_rms(fm, fd),
@@ -301,7 +333,7 @@ def main():
             "energy_rel_mixed_vs_fp64_gpu": E.total_energy(tm) / E.total_energy(td) - 1,
             "energy_rel_mixed_vs_fp64_tau1e-10": E.total_energy(tm) / E.total_energy(tt) - 1,
         },
-        "algorithmic_tflops_per_eval": round(flops_per_eval(P, sy.n, sy.settings["grid"], n_it) * value / 1e12, 3),
+        "algorithmic_tflops": round(flops_per_eval(P, sy.n, sy.settings["grid"], n_it) * value / 1e12, 3),
         "phases_ms_per_eval": {p: {k: round(v["ms_per_eval"], 4) for k, v in phases[p].items()} for p in phases},
         "roofline": roof,
     }  # fmt: skip

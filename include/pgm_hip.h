@@ -161,12 +161,13 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
                               double* grad_radius_dev, double* grad_alpha_dev, double* grad_cov_dev, void* stream);
 
 /* No counterpart in the reference: the autotuner of the pair passes.  After an evaluation, times the
- * compaction, setup, (iterations + 1) matrix-vector and final pair passes on the engine's current lists with
- * 8, 16 and 32 lanes per atom (reps repetitions each), keeps the fastest for later evaluations and returns it
- * in *lanes (may be NULL) and the three timings in ms[3] (ms per evaluation; may be NULL).  pgm_set_lanes sets
- * the lanes directly (8, 16 or 32; 16 by default). */
-int pgm_autotune(pgm_engine* e, int reps, int* lanes, double* ms);
-int pgm_set_lanes(pgm_engine* e, int lanes);
+ * compaction + setup, (iterations + 1) matrix-vector and final pair passes on the engine's current lists for
+ * two variants (compaction fused into the setup pass, or as a pass of its own) and 8, 16 and 32 lanes per
+ * atom (reps repetitions each), keeps the fastest for later evaluations and returns it in *lanes, *fused
+ * (each may be NULL) and the six timings in ms[6] (ms per evaluation, fused 8/16/32 then separate 8/16/32;
+ * may be NULL).  pgm_set_lanes sets them directly (lanes 8, 16 or 32; 16 and separate passes by default). */
+int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms);
+int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
 
 /* No counterpart in the reference (whose engine prints host wall clocks under PGM_CUDA_TIMING,
  * core/cuda/pgm_cuda.cu:1814-1817): HIP-event timing of the kernel groups of every evaluation, for

@@ -36,6 +36,17 @@ __device__ __forceinline__ void minImage(T& x, T& y, T& z, const BoxT<T>& b) {
     x -= k*b.ax;
 }
 
+// The gfx950 dispatcher deals workgroups to its 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each
+// XCD has an L2 of its own.  This renumbering gives XCD x a contiguous range of logical blocks, so the kernels
+// over spatially sorted atoms gather their partners from one slab of the system per L2 instead of from all of
+// it.  A bijection of [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcdBlock() {
+    constexpr int kXcd = 8;
+    const int nb = (int) gridDim.x, b = (int) blockIdx.x;
+    const int per = nb/kXcd, rem = nb%kXcd, x = b%kXcd, local = b/kXcd;
+    return x < rem ? x*(per+1)+local : rem*(per+1)+(x-rem)*per+local;
+}
+
 // Precision traits of the pair kernels: series length below a r = 1 and the damping cut beyond which the
 // erfc(a r) part is dropped (the reference's GPU constants, openmm/platforms/common/src/CommonPGMKernels.cpp:351-352).
 template<class T>

@@ -476,26 +476,35 @@ public:
         if (reps < 1)
             throw std::invalid_argument("pgm_hip: autotune needs reps >= 1");
         int best = lanes_;
+        bool bestFused = fused_;
         withDevice([&] {
             double bestMs = 1e300;
-            for (int lanes : kPairLaneChoices) {
-                const double ms = s_.precision == PGM_MIXED ? timePairPasses<float>(lanes, reps)
-                                                            : timePairPasses<double>(lanes, reps);
-                tuneMs_[lanes == 8 ? 0 : (lanes == 16 ? 1 : 2)] = ms;
-                if (ms < bestMs) {
-                    bestMs = ms;
-                    best = lanes;
+            for (int v = 0; v < 2; v++) {
+                const bool fused = v == 0;
+                for (int c = 0; c < 3; c++) {
+                    const int lanes = kPairLaneChoices[c];
+                    const double ms = s_.precision == PGM_MIXED ? timePairPasses<float>(lanes, fused, reps)
+                                                                : timePairPasses<double>(lanes, fused, reps);
+                    tuneMs_[3*v+c] = ms;
+                    if (ms < bestMs) {
+                        bestMs = ms;
+                        best = lanes;
+                        bestFused = fused;
+                    }
                 }
             }
             lanes_ = best;
+            fused_ = bestFused;
         });
         return best;
     }
     int lanes() const { return lanes_; }
-    void setLanes(int lanes) {
+    bool fused() const { return fused_; }
+    void setLanes(int lanes, bool fused) {
         if (lanes != 8 && lanes != 16 && lanes != 32)
             throw std::invalid_argument("pgm_hip: pair-pass lanes must be 8, 16 or 32");
         lanes_ = lanes;
+        fused_ = fused;
     }
     const double* tuneTimes() const { return tuneMs_; }
 
@@ -629,7 +638,8 @@ private:
     bool frozen_ = false;
     bool haveResults_ = false;
     int lanes_ = 16;
-    double tuneMs_[3] = {0, 0, 0};
+    bool fused_ = false;
+    double tuneMs_[6] = {0, 0, 0, 0, 0, 0};
     // kernel timing
     struct Timed {
         int phase, iteration;
@@ -1325,6 +1335,34 @@ private:
         return sizeof(T) == sizeof(float) ? scratchD_.as<T>() : rgSD_.as<T>();
     }
 
+    // The candidates within rc, their kernels and the field of (q, perm) (and of dual): one fused pass or
+    // compaction + setup.
+    template<class T>
+    void pairSetup(int lanes, bool fused, const typename Vec4<T>::type* dual) {
+        typedef typename Vec4<T>::type T4;
+        typedef typename Vec2<T>::type T2;
+        const BoxSel<T> bt = boxSel<T>();
+        const T rc2 = (T) (s_.cutoff*s_.cutoff);
+        if (fused) {
+            tb(kPhPairSetup);
+            launchPairCompactSetup<T>(st_, n_, posT<T>(), rgT<T>(), permTD_.as<T4>(), dual, candStartD_.as<int>(),
+                                      candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
+                                      g03D_.as<T2>(), bt, rc2, (T) s_.beta, edD_.as<double>(), tdD_.as<double>(),
+                                      lanes);
+            te();
+            return;
+        }
+        tb(kPhPairCompact);
+        launchPairCompact<T>(st_, n_, posT<T>(), candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(),
+                             pjD_.as<int>(), bt, rc2, lanes);
+        te();
+        tb(kPhPairSetup);
+        launchPairSetup<T>(st_, n_, posT<T>(), rgT<T>(), permTD_.as<T4>(), dual, candStartD_.as<int>(),
+                           nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, (T) s_.beta,
+                           edD_.as<double>(), tdD_.as<double>(), lanes);
+        te();
+    }
+
     template<class T>
     void run(bool newStep, bool wantStrain) {
         typedef typename Vec4<T>::type T4;
@@ -1395,16 +1433,9 @@ private:
             launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
             te();
         }
-        // field of the charges and permanent dipoles (and T x0): compaction, pair setup + PME
-        tb(kPhPairCompact);
-        launchPairCompact<T>(st_, n, pos, candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), bt,
-                             (T) (s_.cutoff*s_.cutoff), lanes_);
-        te();
-        tb(kPhPairSetup);
-        launchPairSetup<T>(st_, n, pos, rgT<T>(), permTD_.as<T4>(), dual ? pTD_.as<T4>() : nullptr,
-                           candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
-                           (T) s_.beta, edD_.as<double>(), tdD_.as<double>(), lanes_);
-        te();
+        // field of the charges and permanent dipoles (and T x0): compaction + pair setup (fused or as two
+        // passes, the autotuner's choice) + PME
+        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
         pmeSolve<T>(permTD_.as<T4>(), true, pos, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
         // induced dipoles
@@ -1468,19 +1499,17 @@ private:
 
     // Milliseconds per evaluation spent in the pair passes at the given lane count (current lists and dipoles).
     template<class T>
-    double timePairPasses(int lanes, int reps) {
+    double timePairPasses(int lanes, bool fused, int reps) {
         typedef typename Vec4<T>::type T4;
         typedef typename Vec2<T>::type T2;
         const int n = n_;
         const BoxSel<T> bt = boxSel<T>();
         T4* pos = posT<T>();
         hipEvent_t a = nextEvent(), b = nextEvent();
+        const bool wasTiming = timing_;
+        timing_ = false;
         auto once = [&] {
-            launchPairCompact<T>(st_, n, pos, candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                                 bt, (T) (s_.cutoff*s_.cutoff), lanes);
-            launchPairSetup<T>(st_, n, pos, rgT<T>(), permTD_.as<T4>(), pTD_.as<T4>(), candStartD_.as<int>(),
-                               nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, (T) s_.beta,
-                               edD_.as<double>(), tdD_.as<double>(), lanes);
+            pairSetup<T>(lanes, fused, pTD_.as<T4>());
             for (int k = 0; k <= iterations_; k++)
                 launchPairField<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                                    g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes);
@@ -1497,6 +1526,7 @@ private:
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, a, b));
         evUsed_ -= 2;
+        timing_ = wasTiming;
         return ms/reps;
     }
 
@@ -1875,18 +1905,20 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
     });
 }
 
-int pgm_autotune(pgm_engine* e, int reps, int* lanes, double* ms) {
+int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms) {
     return guarded(e, [&](DeviceEngine& g) {
         const int best = g.autotune(reps);
         if (lanes)
             *lanes = best;
+        if (fused)
+            *fused = g.fused() ? 1 : 0;
         if (ms)
-            std::copy(g.tuneTimes(), g.tuneTimes()+3, ms);
+            std::copy(g.tuneTimes(), g.tuneTimes()+6, ms);
     });
 }
 
-int pgm_set_lanes(pgm_engine* e, int lanes) {
-    return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes); });
+int pgm_set_lanes(pgm_engine* e, int lanes, int fused) {
+    return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes, fused != 0); });
 }
 
 int pgm_set_timing(pgm_engine* e, int on) {

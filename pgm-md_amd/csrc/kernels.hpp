@@ -87,6 +87,13 @@ void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                      const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
                      const int* nb, const int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
                      BoxSel<T> box, T beta, double* field, double* fieldDual, int lanes);
+// launchPairCompact + launchPairSetup in one pass (compacted pairs buffered in LDS per atom).
+template<class T>
+void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                            const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual,
+                            const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
+                            typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, double* field,
+                            double* fieldDual, int lanes);
 // The dipole part of a CG matrix-vector product from the stored G1, G2.
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,

@@ -104,7 +104,7 @@ template<bool FILL, int L>
 __global__ void __launch_bounds__(256) kCandidates(int n, const double4* posS, const int* sortedCell,
                                                    const int* cellStart, CellGrid g, BoxT<double> box, double rl2,
                                                    int* count, const int* candStart, int* cand, int jOffset) {
-    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;

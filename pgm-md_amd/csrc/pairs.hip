@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec
                                                        const int* __restrict__ cand, int* __restrict__ nb,
                                                        int* __restrict__ pj, BoxSel<T> bs, T rc2) {
     typedef typename Vec4<T>::type T4;
-    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
@@ -94,6 +94,152 @@ __device__ __forceinline__ void setupTerm(const typename Vec4<T>::type& pi, T rg
     gb.y = G[3];
 }
 
+// setupTerm from the minimum-image displacement x = r_i - r_j and the partner's charge.
+template<class T, bool DUAL>
+__device__ __forceinline__ void setupTermX(T x, T y, T z, T qj, T rgSum, const typename Vec4<T>::type& d,
+                                           const typename Vec4<T>::type& u, T beta, typename Vec2<T>::type& ga,
+                                           typename Vec2<T>::type& gb, double* e, double* f) {
+    T G[4];
+    if constexpr (sizeof(T) == sizeof(float))
+        pairKernelsF<4>(rgSum, beta, x*x+y*y+z*z, G);
+    else
+        pairKernels<T, 4>(rsqrtT(rgSum), beta, sqrt(x*x+y*y+z*z), G);
+    const T c = qj*G[1]+(d.x*x+d.y*y+d.z*z)*G[2];
+    e[0] += (double) (c*x-d.x*G[1]);
+    e[1] += (double) (c*y-d.y*G[1]);
+    e[2] += (double) (c*z-d.z*G[1]);
+    if (DUAL) {
+        const T c2 = (u.x*x+u.y*y+u.z*z)*G[2];
+        f[0] += (double) (c2*x-u.x*G[1]);
+        f[1] += (double) (c2*y-u.y*G[1]);
+        f[2] += (double) (c2*z-u.z*G[1]);
+    }
+    ga.x = G[1];
+    ga.y = G[2];
+    gb.x = G[0];
+    gb.y = G[3];
+}
+
+// LDS written by some lanes of a wavefront and read by others: no reordering across this point, and the
+// wavefront's LDS operations complete in order.
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Compaction and setup in one pass.  Each group of L lanes tests L candidates per step (one per lane) and
+// ballot-compacts those within rc, with their minimum-image displacement and charge, into its LDS buffer of 2L
+// entries; whenever L entries are buffered, all L lanes evaluate them (kernels, fields) and write (j, G1, G2)
+// as L contiguous entries of the atom's segment.  No compacted index list goes through HBM, the partner
+// positions are gathered once, and the transcendental part runs with every lane busy.
+template<class T, int L, bool DUAL, bool BATCH>
+__global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
+                                                            const T* __restrict__ rg,
+                                                            const typename Vec4<T>::type* __restrict__ dip,
+                                                            const typename Vec4<T>::type* __restrict__ dual,
+                                                            const int* __restrict__ candStart,
+                                                            const int* __restrict__ cand, int* __restrict__ nb,
+                                                            int* __restrict__ pj,
+                                                            typename Vec2<T>::type* __restrict__ g12,
+                                                            typename Vec2<T>::type* __restrict__ g03, BoxSel<T> bs,
+                                                            T rc2, T beta, double* __restrict__ field,
+                                                            double* __restrict__ fieldDual) {
+    typedef typename Vec4<T>::type T4;
+    typedef typename Vec2<T>::type T2;
+    __shared__ int bufJ[2*kPairBlock];
+    __shared__ T4 bufX[2*kPairBlock];
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
+    const int s = gid/L, lane = gid%L;
+    if (s >= n)
+        return;
+    const int g0 = 2*(threadIdx.x-lane);   // this group's 2L buffer slots
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
+    const int shift = (threadIdx.x%64)/L*L;
+    const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
+    const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << L)-1ull));
+    const T4 pi = pos[s];
+    const T rgi = rg[s];
+    const int start = candStart[s], cnt = candStart[s+1]-start;
+    double e[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    int kept = 0, buffered = 0;
+    auto evaluate = [&](int slot, int out) {
+        const int j = bufJ[slot];
+        const T4 xq = bufX[slot];
+        const T4 d = dip[j];
+        const T4 u = DUAL ? dual[j] : d;
+        T2 ga, gb;
+        setupTermX<T, DUAL>(xq.x, xq.y, xq.z, xq.w, rgi+rg[j], d, u, beta, ga, gb, e, f);
+        pj[out] = j;
+        g12[out] = ga;
+        if (kStoreG03<T>)
+            g03[out] = gb;
+    };
+    for (int first = 0; first < cnt; first += L) {
+        const int k = first+lane;
+        bool in = false;
+        int j = 0;
+        T4 xq;
+        if (k < cnt) {
+            j = cand[start+k];
+            const T4 pjv = pos[j];
+            T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+            minImage(x, y, z, box);
+            in = x*x+y*y+z*z < rc2;
+            xq.x = x;
+            xq.y = y;
+            xq.z = z;
+            xq.w = pjv.w;
+        }
+        const unsigned long long m = (__ballot(in) >> shift) & gmask;
+        if (in) {
+            const int slot = g0+buffered+__popcll(m & below);
+            bufJ[slot] = j;
+            bufX[slot] = xq;
+        }
+        buffered += __popcll(m);
+        waveSync();
+        if (buffered >= L) {
+            evaluate(g0+lane, start+kept+lane);
+            kept += L;
+            const int rest = buffered-L;
+            int tj = 0;
+            T4 tx;
+            if (lane < rest) {
+                tj = bufJ[g0+L+lane];
+                tx = bufX[g0+L+lane];
+            }
+            waveSync();
+            if (lane < rest) {
+                bufJ[g0+lane] = tj;
+                bufX[g0+lane] = tx;
+            }
+            waveSync();
+            buffered = rest;
+        }
+    }
+    if (lane < buffered)
+        evaluate(g0+lane, start+kept+lane);
+    kept += buffered;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        e[c] = groupSum<L>(e[c]);
+        if (DUAL)
+            f[c] = groupSum<L>(f[c]);
+    }
+    if (lane == 0) {
+        nb[s] = kept;
+        field[3*s] = e[0];
+        field[3*s+1] = e[1];
+        field[3*s+2] = e[2];
+        if (DUAL) {
+            fieldDual[3*s] = f[0];
+            fieldDual[3*s+1] = f[1];
+            fieldDual[3*s+2] = f[2];
+        }
+    }
+}
+
 // Kernels and fields over the compacted pairs: every lane busy, U entries in flight per lane.
 template<class T, int L, int U, bool DUAL, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
@@ -107,7 +253,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
                                                          double* __restrict__ field, double* __restrict__ fieldDual) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
-    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
@@ -191,7 +337,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     typedef typename Vec2<T>::type T2;
     if (skip && *skip)
         return;
-    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
@@ -288,7 +434,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
                                                          double* __restrict__ perAtom) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
-    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // energy, strain (row-major)
     if (s < n) {
@@ -375,7 +521,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
                                                            BoxSel<T> bs, double* __restrict__ out) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
-    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
@@ -524,6 +670,25 @@ void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
 }
 
 template<class T>
+void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
+                            const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual,
+                            const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
+                            typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, double* field,
+                            double* fieldDual, int lanes) {
+    dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        const int blocks = groupsBlocks(n, L());
+        if (dual)
+            kCompactSetup<T, L(), true, B()><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, cand, nb,
+                                                                            pj, g12, g03, box, rc2, beta, field,
+                                                                            fieldDual);
+        else
+            kCompactSetup<T, L(), false, B()><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, cand,
+                                                                             nb, pj, g12, g03, box, rc2, beta, field,
+                                                                             nullptr);
+    });
+}
+
+template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxSel<T> box, double* field, const int* skip, int lanes) {
@@ -573,6 +738,9 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
     template void launchPairSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \
                                      const Vec4<T>::type*, const int*, const int*, const int*, Vec2<T>::type*,        \
                                      Vec2<T>::type*, BoxSel<T>, T, double*, double*, int);                            \
+    template void launchPairCompactSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,  \
+                                            const Vec4<T>::type*, const int*, const int*, int*, int*, Vec2<T>::type*, \
+                                            Vec2<T>::type*, BoxSel<T>, T, T, double*, double*, int);                  \
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
                                      const int*, const int*, const Vec2<T>::type*, BoxSel<T>, double*, const int*,    \
                                      int);                                                                            \

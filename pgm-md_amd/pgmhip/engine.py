@@ -130,8 +130,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_compute_batch_device.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
     lib.pgm_backward_batch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pgm_backward_batch_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
-    lib.pgm_autotune.argtypes = [vp, C.c_int, vp, vp]
-    lib.pgm_set_lanes.argtypes = [vp, C.c_int]
+    lib.pgm_autotune.argtypes = [vp, C.c_int, vp, vp, vp]
+    lib.pgm_set_lanes.argtypes = [vp, C.c_int, C.c_int]
     _ = dp
     _lib = lib
     return lib
@@ -333,14 +333,15 @@ class Engine:
         self._check(rc)
 
     def autotune(self, reps: int = 3):
-        """(lanes chosen, {lanes: ms per evaluation in the pair passes}) (pgm_autotune)."""
-        lanes = C.c_int(0)
-        ms = np.zeros(3)
-        self._check(self.lib.pgm_autotune(self.h, int(reps), C.byref(lanes), _ptr(ms)))
-        return lanes.value, {8: float(ms[0]), 16: float(ms[1]), 32: float(ms[2])}
+        """((lanes, fused) chosen, {(lanes, fused): ms per evaluation in the pair passes}) (pgm_autotune)."""
+        lanes, fused = C.c_int(0), C.c_int(0)
+        ms = np.zeros(6)
+        self._check(self.lib.pgm_autotune(self.h, int(reps), C.byref(lanes), C.byref(fused), _ptr(ms)))
+        table = {(L, f): float(ms[3 * (0 if f else 1) + k]) for f in (True, False) for k, L in enumerate((8, 16, 32))}
+        return (lanes.value, bool(fused.value)), table
 
-    def set_lanes(self, lanes: int):
-        self._check(self.lib.pgm_set_lanes(self.h, int(lanes)))
+    def set_lanes(self, lanes: int, fused: bool = True):
+        self._check(self.lib.pgm_set_lanes(self.h, int(lanes), int(bool(fused))))
 
     def adjoint_iterations(self) -> int:
         return self.lib.pgm_adjoint_iterations(self.h)
