@@ -84,13 +84,35 @@ def prior_dipole_sets(sy, E, nset=4, step=0.002, seed=22):
     return np.stack(sets)
 
 
-def timed_steps(torch, eng, pos_d, f_d, box, steps, warmup, stream):
-    """Wall time of `steps` evaluations, and the per-phase table of those same evaluations: the engine brackets
-    every kernel group with HIP events on its own stream (the stream the kernels run on)."""
+def phase_stats(tab, steps):
+    return {k: {"ms_per_eval": v[0] / steps, "launch_avg_us": 1000.0 * v[0] / max(v[1], 1), "launches": v[1] / steps}
+            for k, v in tab.items()}  # fmt: skip
+
+
+def phase_table(eng, pos_d, f_d, box, stream, torch, reps):
+    """Every kernel group bracketed with HIP events on the engine stream (each bracket idles the GPU for a few
+    microseconds, so this pass is separate from the timed steps)."""
+    eng.set_timing(True)
+    for _ in range(reps):
+        eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
+    torch.cuda.synchronize()
+    tab = eng.timing()
+    eng.set_timing(False)
+    return phase_stats(tab, reps)
+
+
+def dominant_pair_phase(phases):
+    live = [ph for ph in PAIR_BYTES if phases.get(ph, {}).get("launches", 0) > 0]
+    return max(live, key=lambda k: phases[k]["ms_per_eval"])
+
+
+def timed_steps(torch, eng, pos_d, f_d, box, steps, warmup, stream, phase):
+    """Wall time of `steps` evaluations; during them the engine brackets the launches of `phase` (only) with
+    HIP events on its own stream, the stream those kernels run on: their live average duration."""
     for _ in range(warmup):
         eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
     torch.cuda.synchronize()
-    eng.set_timing(True)
+    eng.set_timing(True, phases=[phase])
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
@@ -98,9 +120,7 @@ def timed_steps(torch, eng, pos_d, f_d, box, steps, warmup, stream):
     dt = time.perf_counter() - t0
     tab = eng.timing()
     eng.set_timing(False)
-    phases = {k: {"ms_per_eval": v[0] / steps, "launch_avg_us": 1000.0 * v[0] / max(v[1], 1), "launches": v[1] / steps}
-              for k, v in tab.items()}  # fmt: skip
-    return dt, phases
+    return dt, phase_stats(tab, steps)[phase]
 
 
 def traffic_table(path=TRAFFIC_FILE):
@@ -117,15 +137,14 @@ def traffic_table(path=TRAFFIC_FILE):
     return out
 
 
-def roofline(phases, P, C, N, precision, lanes, fused):
-    """Achieved algorithmic HBM bandwidth of the kernel that takes the most time per evaluation (pair passes)."""
+def roofline(dom, live, P, C, N, precision, lanes, fused):
+    """Achieved algorithmic HBM bandwidth of the pair pass `dom` (the one with the most time per evaluation),
+    from its live launch average `live` over the timed steps."""
     E = 2 * P
     table = dict(PAIR_BYTES, pair_setup=PAIR_BYTES_FUSED_SETUP) if fused else PAIR_BYTES
     kernels = dict(KERNEL_OF_PHASE, pair_setup="kCompactSetup") if fused else KERNEL_OF_PHASE
     per_launch = {ph: b[precision][0] * E + b[precision][1] * N + b[precision][2] * C for ph, b in table.items()}
-    live = [ph for ph in per_launch if phases.get(ph, {}).get("launches", 0) > 0]
-    dom = max(live, key=lambda k: phases[k]["ms_per_eval"])
-    t = phases[dom]["launch_avg_us"] * 1e-6
+    t = live["launch_avg_us"] * 1e-6
     achieved = per_launch[dom] / t / 1e9
     real = "float" if precision == "mixed" else "double"
     prefix = f"{kernels[dom]}<{real}, {lanes},"
@@ -133,8 +152,8 @@ def roofline(phases, P, C, N, precision, lanes, fused):
     return {
         "bound": "hbm", "kernel": f"{kernels[dom]}<{real}, {lanes}>", "phase": dom,
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic, "algorithmic_bytes": per_launch[dom], "launch_avg_us": round(phases[dom]["launch_avg_us"], 2),
-        "launches_per_eval": phases[dom]["launches"], "traffic_source": TRAFFIC_FILE if traffic else None,
+        "traffic": traffic, "algorithmic_bytes": per_launch[dom], "launch_avg_us": round(live["launch_avg_us"], 2),
+        "launches_per_eval": live["launches"], "traffic_source": TRAFFIC_FILE if traffic else None,
     }  # fmt: skip
 
 
@@ -240,7 +259,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-snapshot adjoint batch")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--profile", type=int, default=0, metavar="N",
-                    help="profiler pass: N mixed evaluations of the workload and one calibration copy of "
+                    help="profiler pass: N evaluations of the workload per precision (each engine autotuned "
+                         "first) and one calibration copy of "
                          f"{CAL_COPY_BYTES >> 20} MiB, no timing, no JSON line")  # fmt: skip
     a = ap.parse_args()
     if a.gpus != 1:
@@ -266,19 +286,22 @@ def main():
     f_d = torch.zeros((sy.n, 3), dtype=torch.float64, device=dev)
     box = sy.box
     if a.profile:
-        eng = E.Engine.for_system(sy, precision="mixed")
-        eng.set_history(sets)
-        eng.freeze_history(True)
-        eng.compute(sy.pos, box)
-        eng.autotune(3)
-        for _ in range(a.profile):
-            eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
+        for prec in ("mixed", "double"):
+            eng = E.Engine.for_system(sy, precision=prec)
+            eng.set_history(sets)
+            eng.freeze_history(True)
+            eng.compute(sy.pos, box)
+            eng.autotune(3)
+            for _ in range(a.profile):
+                eng.compute_device_f64(pos_d.data_ptr(), box, True, f_d.data_ptr(), stream)
+            torch.cuda.synchronize()
+            del eng
         # a plain device copy of known size: the scale of the HBM byte counters
         src = torch.ones(CAL_COPY_BYTES // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         torch.mul(src, 2.0, out=dst)   # one elementwise kernel: reads and writes CAL_COPY_BYTES
         torch.cuda.synchronize()
-        log(f"profile pass: {a.profile} mixed evaluations, {eng.iterations()} CG iterations, "
+        log(f"profile pass: {a.profile} evaluations per precision, "
             f"calibration copy {CAL_COPY_BYTES} bytes read + {CAL_COPY_BYTES} bytes written")  # fmt: skip
         return
 
@@ -298,13 +321,16 @@ def main():
     tight.set_history(sets)
     tt, ft, _ = tight.compute(sy.pos, box)
 
-    secs, phases = {}, {}
+    secs, phases, live, dom = {}, {}, {}, {}
     for prec in ("mixed", "double"):
-        secs[prec], phases[prec] = timed_steps(torch, engines[prec], pos_d, f_d, box, a.steps, a.warmup, stream)
+        phases[prec] = phase_table(engines[prec], pos_d, f_d, box, stream, torch, 5)
+        dom[prec] = dominant_pair_phase(phases[prec])
+        secs[prec], live[prec] = timed_steps(torch, engines[prec], pos_d, f_d, box, a.steps, a.warmup, stream,
+                                             dom[prec])  # fmt: skip
         log(f"{prec}: {a.steps / secs[prec]:.2f} evals/s")
     n_it = results["mixed"][2]
-    roof = roofline(phases["mixed"], P, C, sy.n, "mixed", *tuned["mixed"][0])
-    roof64 = roofline(phases["double"], P, C, sy.n, "double", *tuned["double"][0])
+    roof = roofline(dom["mixed"], live["mixed"], P, C, sy.n, "mixed", *tuned["mixed"][0])
+    roof64 = roofline(dom["double"], live["double"], P, C, sy.n, "double", *tuned["double"][0])
 
     value = a.steps / secs["mixed"]
     v64 = a.steps / secs["double"]

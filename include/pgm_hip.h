@@ -178,6 +178,10 @@ int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
 enum { PGM_TIMING_PHASES = 11 };
 int pgm_set_timing(pgm_engine* e, int on);
 int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases);
+/* As pgm_set_timing(e, mask != 0), bracketing only the phases of the mask (bit k: phase k): each bracket
+ * records two events on the engine stream, which leave the GPU idle for a few microseconds, so a timed
+ * production run brackets the one kernel group it measures. */
+int pgm_set_timing_phases(pgm_engine* e, unsigned mask);
 
 #ifdef __cplusplus
 }

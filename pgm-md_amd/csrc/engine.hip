@@ -177,6 +177,8 @@ public:
             HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
             HIPCHK(hipEventCreateWithFlags(&evIn_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evOut_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evSync_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evList_, hipEventDisableTiming));
         });
     }
     ~DeviceEngine() {
@@ -192,6 +194,9 @@ public:
             (void) hipEventDestroy(evIn_);
         if (evOut_)
             (void) hipEventDestroy(evOut_);
+        for (hipEvent_t ev : {evSync_, evList_})
+            if (ev)
+                (void) hipEventDestroy(ev);
         if (st_)
             (void) hipStreamDestroy(st_);
         if (prev >= 0)
@@ -353,10 +358,24 @@ public:
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
             setBoxes(boxes);
             stagePositions(src, pos);
+            // the list check's round trip overlaps the preparation kernels; a stale list (rare) is rebuilt
+            // and the atoms prepared again in the new order
+            const bool checking = haveList_;
             tb(kPhList);
-            checkList();
+            if (checking)
+                startListCheck();
+            else
+                buildList();
             te();
-            if (s_.precision == PGM_MIXED)
+            const bool mixed = s_.precision == PGM_MIXED;
+            mixed ? prep<float>() : prep<double>();
+            if (checking && listStale()) {
+                tb(kPhList);
+                buildList();
+                te();
+                mixed ? prep<float>() : prep<double>();
+            }
+            if (mixed)
                 run<float>(newStep, strainOut != nullptr);
             else
                 run<double>(newStep, strainOut != nullptr);
@@ -377,7 +396,7 @@ public:
             const size_t resBytes = sys_.size()*kResCount*sizeof(double);
             resH_.reserve(resBytes);
             HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, resBytes, hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipStreamSynchronize(st_));
+            waitStream();
             collectTiming();
             assemble(strainOut != nullptr);
             if (fHost) {
@@ -567,8 +586,9 @@ public:
 
     void freezeHistory(bool f) { frozen_ = f; }
 
-    void setTiming(bool on) {
+    void setTiming(bool on, unsigned mask = ~0u) {
         timing_ = on;
+        phaseMask_ = mask;
         resetTiming();
     }
     void resetTiming() {
@@ -617,7 +637,7 @@ private:
 
     pgm_settings s_;
     hipStream_t st_ = nullptr;
-    hipEvent_t evIn_ = nullptr, evOut_ = nullptr;
+    hipEvent_t evIn_ = nullptr, evOut_ = nullptr, evSync_ = nullptr, evList_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1;
     bool polarizable_ = false;
     std::vector<double> qH_, radH_, alphaH_, forcesH_;
@@ -646,6 +666,7 @@ private:
         hipEvent_t a, b;
     };
     bool timing_ = false;
+    unsigned phaseMask_ = ~0u;   // phases bracketed with events while timing (bit k: phase k)
     std::vector<hipEvent_t> evPool_;
     size_t evUsed_ = 0;
     std::vector<Timed> timed_;
@@ -664,8 +685,10 @@ private:
     }
     // Brackets the kernels of one phase (iteration: CG iteration index, -1 outside the CG loop).
     void tb(int phase, int iteration = -1) {
-        if (!timing_)
+        if (!timing_ || !((phaseMask_ >> phase) & 1u)) {
+            openPhase_ = -1;
             return;
+        }
         openPhase_ = phase;
         openIter_ = iteration;
         openEv_ = nextEvent();
@@ -715,7 +738,7 @@ private:
     DevBuf pairOutD_, pmeOutD_, gridTD_, gSD_, gTD_, pdotD_, lamD_, adjBD_, ddotD_, ddotTD_, etotDotD_, gInD_, gradD_;
     int adjIterations_ = 0, lastAdjIterations_ = 0;
     double adjResidual_ = 0;
-    HostBuf resH_, stageH_, scalH_;
+    HostBuf resH_, stageH_, scalH_, listH_;
 
     void validate() {
         const pgm_settings& s = s_;
@@ -1062,34 +1085,45 @@ private:
 
     // ------------------------------------------------------------------------------------ lists
 
-    void checkList() {
-        bool stale = !haveList_;
-        if (!stale) {
-            maxBitsD_.reserve(sizeof(unsigned));
-            HIPCHK(hipMemsetAsync(maxBitsD_.p, 0, sizeof(unsigned), st_));
-            launchMaxDisplacement(st_, n_, posDev_, posBuildD_.as<double>(), boxSel<double>(),
-                                  maxBitsD_.as<unsigned>());
-            scalH_.reserve(64);
-            HIPCHK(hipMemcpyAsync(scalH_.p, maxBitsD_.p, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipStreamSynchronize(st_));
-            float d2;
-            std::memcpy(&d2, scalH_.p, sizeof(float));
-            double boxMove = 0, skin = 1e300;
-            for (const Sys& y : sys_) {
-                double dH = 0;
-                for (int a = 0; a < 3; a++) {
-                    double r2 = 0;
-                    for (int c = 0; c < 3; c++)
-                        r2 += (y.H[a][c]-y.buildBox[a][c])*(y.H[a][c]-y.buildBox[a][c]);
-                    dH += std::sqrt(r2);
-                }
-                boxMove = std::max(boxMove, dH);
-                skin = std::min(skin, y.skinNow);
-            }
-            stale = 2.0*std::sqrt((double) d2)+boxMove > skin;
+    // Host wait for the work enqueued so far on the engine stream, spinning on an event: a blocking
+    // synchronize adds an interrupt's wake-up latency to each of an evaluation's few round trips.
+    void waitEvent(hipEvent_t ev) {
+        hipError_t e;
+        while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
         }
-        if (stale)
-            buildList();
+        HIPCHK(e);
+    }
+    void waitStream() {
+        HIPCHK(hipEventRecord(evSync_, st_));
+        waitEvent(evSync_);
+    }
+
+    // The list check in two halves: the largest displacement since the build is copied to pinned host memory
+    // without waiting (the maximum starts at zero: every preparation clears it for the next check; a value
+    // left over by an interrupted evaluation can only be larger, so at worst it rebuilds early).
+    void startListCheck() {
+        launchMaxDisplacement(st_, n_, posDev_, posBuildD_.as<double>(), boxSel<double>(), maxBitsD_.as<unsigned>());
+        listH_.reserve(64);
+        HIPCHK(hipMemcpyAsync(listH_.p, maxBitsD_.p, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipEventRecord(evList_, st_));
+    }
+    bool listStale() {
+        waitEvent(evList_);
+        float d2;
+        std::memcpy(&d2, listH_.p, sizeof(float));
+        double boxMove = 0, skin = 1e300;
+        for (const Sys& y : sys_) {
+            double dH = 0;
+            for (int a = 0; a < 3; a++) {
+                double r2 = 0;
+                for (int c = 0; c < 3; c++)
+                    r2 += (y.H[a][c]-y.buildBox[a][c])*(y.H[a][c]-y.buildBox[a][c]);
+                dH += std::sqrt(r2);
+            }
+            boxMove = std::max(boxMove, dH);
+            skin = std::min(skin, y.skinNow);
+        }
+        return 2.0*std::sqrt((double) d2)+boxMove > skin;
     }
 
     void buildList() {
@@ -1304,7 +1338,7 @@ private:
             HIPCHK(hipMemcpyAsync(scalH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipMemcpyAsync(scalH_.as<char>()+sizeof(CgState), doneD_.as<int>()+it, sizeof(int),
                                   hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipStreamSynchronize(st_));
+            waitStream();
             int doneNow;
             std::memcpy(&doneNow, scalH_.as<char>()+sizeof(CgState), sizeof(int));
             if (doneNow || it >= s_.max_iter)
@@ -1363,6 +1397,62 @@ private:
         te();
     }
 
+    // Per-evaluation preparation in the current sorted order: positions, charges, radii and permanent dipoles
+    // of the sorted atoms (one fused kernel that also clears the PME brick counters, the result sums and the
+    // next list check's maximum), then the PME bricks and splines.
+    template<class T>
+    void prep() {
+        typedef typename Vec4<T>::type T4;
+        const int n = n_;
+        for (DevBuf* b : {&permD_, &edD_})
+            b->reserve(3*sizeof(double)*n);
+        permTD_.reserve(n*sizeof(T4));
+        posDD_.reserve(n*sizeof(double4));
+        const bool f32 = sizeof(T) == sizeof(float);
+        if (f32) {
+            posFD_.reserve(n*sizeof(float4));
+            scratchD_.reserve(n*sizeof(float));
+        }
+        brickOfD_.reserve(n*sizeof(int));
+        pslotD_.reserve(n*sizeof(int));
+        brickCountD_.reserve((brickTotal_+1)*sizeof(int));
+        brickStartD_.reserve((brickTotal_+1)*sizeof(int));
+        pmeAtomD_.reserve(n*sizeof(int));
+        baseD_.reserve(3*n*sizeof(int));
+        splD_.reserve((size_t) n*9*s_.order*sizeof(T));
+        resD_.reserve(sys_.size()*kResCount*sizeof(double));
+        maxBitsD_.reserve(sizeof(unsigned));
+        tb(kPhPrep);
+        PrepArgs a{};
+        a.n = n;
+        a.order = orderD_.as<int>();
+        a.pos = posDev_;
+        a.qS = qSD_.as<double>();
+        a.rgS = rgSD_.as<double>();
+        a.covStart = covStartD_.as<int>();
+        a.covPartner = covPartnerD_.as<int>();
+        a.covStrength = covStrengthD_.as<double>();
+        a.box = boxSel<double>();
+        a.posD = posDD_.as<double4>();
+        a.posF = f32 ? posFD_.as<float4>() : nullptr;
+        a.rgF = f32 ? scratchD_.as<float>() : nullptr;
+        a.perm = permD_.as<double>();
+        a.permT = permTD_.p;
+        a.zeroI = brickCountD_.as<int>();
+        a.nZeroI = brickTotal_+1;
+        a.zeroD = resD_.as<double>();
+        a.nZeroD = (int) (sys_.size()*kResCount);
+        a.maxBits = maxBitsD_.as<unsigned>();
+        launchPrepAtoms<T>(st_, a);
+        const PmeSel ps = pmeSel();
+        launchPmeBin<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
+                        brickCountD_.as<int>());
+        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_, nullptr);
+        launchPmePlace<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
+                          brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
+        te();
+    }
+
     template<class T>
     void run(bool newStep, bool wantStrain) {
         typedef typename Vec4<T>::type T4;
@@ -1379,46 +1469,12 @@ private:
         for (DevBuf* b : {&permTD_, &pTD_, &dTD_})
             b->reserve(n*sizeof(T4));
         perAtomD_.reserve((size_t) n*11*sizeof(double));
-        // sorted positions (and, for FP32, the FP32 copy and radii)
-        tb(kPhPrep);
-        if (sizeof(T) == sizeof(float)) {
-            posFD_.reserve(n*sizeof(float4));
-            launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
-                                  posFD_.as<float4>());
-            scratchD_.reserve(n*sizeof(float));
-            launchToFloat(st_, n, rgSD_.as<double>(), scratchD_.as<float>());
-        }
-        else
-            launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), bd, posDD_.as<double4>(),
-                                  nullptr);
-        T4* pos = posT<T>();
-        launchPermDipoles(st_, n, orderD_.as<int>(), posDev_, covStartD_.as<int>(), covPartnerD_.as<int>(),
-                          covStrengthD_.as<double>(), bd, permD_.as<double>());
-        launchToVec4<T>(st_, n, permD_.as<double>(), permTD_.as<T4>());
-        // PME bricks and splines
-        const PmeSel ps = pmeSel();
-        brickOfD_.reserve(n*sizeof(int));
-        pslotD_.reserve(n*sizeof(int));
-        brickCountD_.reserve((brickTotal_+1)*sizeof(int));
-        brickStartD_.reserve((brickTotal_+1)*sizeof(int));
-        pmeAtomD_.reserve(n*sizeof(int));
-        baseD_.reserve(3*n*sizeof(int));
-        splD_.reserve((size_t) n*9*s_.order*sizeof(T));
-        HIPCHK(hipMemsetAsync(brickCountD_.p, 0, (brickTotal_+1)*sizeof(int), st_));
-        launchPmeBin<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
-                        brickCountD_.as<int>());
-        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_, nullptr);
-        launchPmePlace<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
-                          brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
-        te();
-        // partial-sum storage
         partD_.reserve((size_t) std::max(partTotal_, 1)*10*sizeof(double));
         cgPartD_.reserve(kPRegions*kCgMaxBlocks*sizeof(double));
-        resD_.reserve(sys_.size()*kResCount*sizeof(double));
         cgD_.reserve(sizeof(CgState));
         doneD_.reserve((s_.max_iter+2)*sizeof(int));
         nbD_.reserve(n*sizeof(int));
-        HIPCHK(hipMemsetAsync(resD_.p, 0, sys_.size()*kResCount*sizeof(double), st_));
+        T4* pos = posT<T>();
         // the initial guess: from the history when there is one, then its first product rides on the pair setup
         const bool induce = polarizable_ && s_.polarizable;
         const GuessCoef gc = guessCoefficients(newStep);
@@ -1923,6 +1979,10 @@ int pgm_set_lanes(pgm_engine* e, int lanes, int fused) {
 
 int pgm_set_timing(pgm_engine* e, int on) {
     return guarded(e, [&](DeviceEngine& g) { g.setTiming(on != 0); });
+}
+
+int pgm_set_timing_phases(pgm_engine* e, unsigned mask) {
+    return guarded(e, [&](DeviceEngine& g) { g.setTiming(mask != 0, mask); });
 }
 
 int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases) {

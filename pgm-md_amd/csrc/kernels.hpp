@@ -69,6 +69,34 @@ void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out);
 void launchPermDipoles(hipStream_t st, int n, const int* order, const double* pos, const int* covStart,
                        const int* covPartner, const double* covStrength, BoxSel<double> box, double* perm);
 
+// The per-evaluation preparation of the sorted atoms in one pass: the wrapped positions with charges (FP64
+// and, when posF is set, FP32 with the FP32 radius parameters rgF), the permanent dipoles (FP64 and as the
+// pair/PME vector type permT), and the clearing of the counters and sums the kernels after it accumulate
+// into (zeroI: nZeroI ints, zeroD: nZeroD doubles, maxBits: the next list check's maximum).
+struct PrepArgs {
+    int n;
+    const int* order;
+    const double* pos;   // caller's order, FP64 xyz
+    const double* qS;    // sorted charges
+    const double* rgS;   // sorted radius parameters
+    const int* covStart;
+    const int* covPartner;
+    const double* covStrength;
+    BoxSel<double> box;
+    double4* posD;
+    float4* posF;
+    float* rgF;
+    double* perm;
+    void* permT;
+    int* zeroI;
+    int nZeroI;
+    double* zeroD;
+    int nZeroD;
+    unsigned* maxBits;
+};
+template<class T>
+void launchPrepAtoms(hipStream_t st, const PrepArgs& a);
+
 // -------------------------------------------------------------------------------------------- pair passes
 
 // Lanes per atom of the pair passes: 8, 16 or 32 (picked per engine by the autotuner, 16 by default).

@@ -224,9 +224,66 @@ __global__ void kPermDipoles(int n, const int* order, const double* pos, const i
     perm[3*s+2] = pz;
 }
 
+template<class T, bool BATCH>
+__global__ void kPrepAtoms(PrepArgs a) {
+    typedef typename Vec4<T>::type T4;
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s < a.nZeroI)
+        a.zeroI[s] = 0;
+    if (s < a.nZeroD)
+        a.zeroD[s] = 0.0;
+    if (s == 0 && a.maxBits)
+        *a.maxBits = 0u;
+    if (s >= a.n)
+        return;
+    const BoxT<double> box = selectBox<BATCH>(a.box, s);
+    const int i = a.order[s];
+    const double xi = a.pos[3*i], yi = a.pos[3*i+1], zi = a.pos[3*i+2];
+    double x = xi, y = yi, z = zi;
+    wrapPosition(x, y, z, box);
+    const double q = a.qS[s];
+    a.posD[s] = make_double4(x, y, z, q);
+    if (a.posF) {
+        a.posF[s] = make_float4((float) x, (float) y, (float) z, (float) q);
+        a.rgF[s] = (float) a.rgS[s];
+    }
+    // p_i = sum_m c_m unit(r_partner - r_i), minimum image (core/pgm_core.cpp:51-62)
+    double px = 0, py = 0, pz = 0;
+    for (int m = a.covStart[i]; m < a.covStart[i+1]; m++) {
+        const int j = a.covPartner[m];
+        double dx = a.pos[3*j]-xi, dy = a.pos[3*j+1]-yi, dz = a.pos[3*j+2]-zi;
+        minImage(dx, dy, dz, box);
+        const double f = a.covStrength[m]/sqrt(dx*dx+dy*dy+dz*dz);
+        px += f*dx;
+        py += f*dy;
+        pz += f*dz;
+    }
+    a.perm[3*s] = px;
+    a.perm[3*s+1] = py;
+    a.perm[3*s+2] = pz;
+    T4 o;
+    o.x = (T) px;
+    o.y = (T) py;
+    o.z = (T) pz;
+    o.w = T(0);
+    static_cast<T4*>(a.permT)[s] = o;
+}
+
 inline int blocksFor(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
 
 }   // namespace
+
+template<class T>
+void launchPrepAtoms(hipStream_t st, const PrepArgs& a) {
+    const int threads = std::max(a.n, std::max(a.nZeroI, a.nZeroD));
+    if (a.box.table)
+        kPrepAtoms<T, true><<<blocksFor(threads, 256), 256, 0, st>>>(a);
+    else
+        kPrepAtoms<T, false><<<blocksFor(threads, 256), 256, 0, st>>>(a);
+}
+
+template void launchPrepAtoms<float>(hipStream_t, const PrepArgs&);
+template void launchPrepAtoms<double>(hipStream_t, const PrepArgs&);
 
 void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
                       int* cellCount, int* subOf) {

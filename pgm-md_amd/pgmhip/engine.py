@@ -61,7 +61,7 @@ EXPORTS = [
     "pgm_compute", "pgm_compute_f32", "pgm_forces", "pgm_compute_device", "pgm_launch_device", "pgm_finish_device",
     "pgm_compute_device_f64", "pgm_iterations", "pgm_residual", "pgm_induced_dipoles", "pgm_permanent_dipoles",
     "pgm_reset_history", "pgm_list_builds", "pgm_num_candidates", "pgm_late_list_checks", "pgm_set_history",
-    "pgm_freeze_history", "pgm_set_timing", "pgm_get_timing", "pgm_num_pairs", "pgm_backward",
+    "pgm_freeze_history", "pgm_set_timing", "pgm_set_timing_phases", "pgm_get_timing", "pgm_num_pairs", "pgm_backward",
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes",
@@ -116,6 +116,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_set_history.argtypes = [vp, C.c_int, vp]
     lib.pgm_freeze_history.argtypes = [vp, C.c_int]
     lib.pgm_set_timing.argtypes = [vp, C.c_int]
+    lib.pgm_set_timing_phases.argtypes = [vp, C.c_uint]
     lib.pgm_num_pairs.argtypes = [vp]
     lib.pgm_num_pairs.restype = C.c_longlong
     lib.pgm_get_timing.argtypes = [vp, vp, vp, C.c_int]
@@ -375,8 +376,13 @@ class Engine:
     def freeze_history(self, frozen=True):
         self._check(self.lib.pgm_freeze_history(self.h, int(bool(frozen))))
 
-    def set_timing(self, on=True):
-        self._check(self.lib.pgm_set_timing(self.h, int(bool(on))))
+    def set_timing(self, on=True, phases=None):
+        """Time every phase (phases None) or only the named ones (pgm_set_timing_phases)."""
+        if phases is None:
+            self._check(self.lib.pgm_set_timing(self.h, int(bool(on))))
+        else:
+            mask = sum(1 << PHASES.index(p) for p in phases) if on else 0
+            self._check(self.lib.pgm_set_timing_phases(self.h, mask))
 
     def timing(self) -> dict:
         """{phase: (total ms, launches counted)} since set_timing."""
