@@ -162,10 +162,11 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
 
 /* No counterpart in the reference: the autotuner of the pair passes.  After an evaluation, times the
  * compaction + setup, (iterations + 1) matrix-vector and final pair passes on the engine's current lists for
- * two variants (compaction fused into the setup pass, or as a pass of its own) and 8, 16 and 32 lanes per
+ * two variants (compaction fused into the setup pass, or as a pass of its own) and 16, 32 and 64 lanes per
  * atom (reps repetitions each), keeps the fastest for later evaluations and returns it in *lanes, *fused
- * (each may be NULL) and the six timings in ms[6] (ms per evaluation, fused 8/16/32 then separate 8/16/32;
- * may be NULL).  pgm_set_lanes sets them directly (lanes 8, 16 or 32; 16 and separate passes by default). */
+ * (each may be NULL) and the six timings in ms[6] (ms per evaluation, fused 16/32/64 then separate
+ * 16/32/64; may be NULL).  pgm_set_lanes sets them directly (lanes 16, 32 or 64; 16 and separate passes by
+ * default). */
 int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms);
 int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
 

@@ -520,8 +520,8 @@ public:
     int lanes() const { return lanes_; }
     bool fused() const { return fused_; }
     void setLanes(int lanes, bool fused) {
-        if (lanes != 8 && lanes != 16 && lanes != 32)
-            throw std::invalid_argument("pgm_hip: pair-pass lanes must be 8, 16 or 32");
+        if (lanes != 16 && lanes != 32 && lanes != 64)
+            throw std::invalid_argument("pgm_hip: pair-pass lanes must be 16, 32 or 64");
         lanes_ = lanes;
         fused_ = fused;
     }
@@ -1531,8 +1531,8 @@ private:
         tb(kPhPairFinal);
         launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(), nbD_.as<int>(),
                            pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(),
-                           edD_.as<double>(), perAtomD_.as<double>(), lanes_);
-        systemSums(perAtomD_.as<double>(), 10, res, kResPair);
+                           edD_.as<double>(), perAtomD_.as<double>(), wantStrain, lanes_);
+        systemSums(perAtomD_.as<double>(), wantStrain ? 10 : 1, res, kResPair);
         te();
         pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
         tb(kPhFinal);
@@ -1571,7 +1571,7 @@ private:
                                    g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes);
             launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(),
                                nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
-                               fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), lanes);
+                               fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), false, lanes);
         };
         once();   // warm-up
         HIPCHK(hipEventRecord(a, st_));

@@ -99,8 +99,8 @@ void launchPrepAtoms(hipStream_t st, const PrepArgs& a);
 
 // -------------------------------------------------------------------------------------------- pair passes
 
-// Lanes per atom of the pair passes: 8, 16 or 32 (picked per engine by the autotuner, 16 by default).
-constexpr int kPairLaneChoices[3] = {8, 16, 32};
+// Lanes per atom of the pair passes: 16, 32 or 64 (picked per engine by the autotuner, 16 by default).
+constexpr int kPairLaneChoices[3] = {16, 32, 64};
 
 // Per evaluation: every atom's candidates compacted in place to the pairs within rc (pj: partner; nb: pairs
 // per atom), in the atom's candidate segment.
@@ -127,14 +127,14 @@ template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxSel<T> box, double* field, const int* skip, int lanes);
-// Energy, forces, field and strain of the pairs at the total dipoles from the stored kernels (FP32: G0, G3 rebuilt
-// from G1, G2 with the radii rg and beta); per atom (sorted order) perAtom[n][10] = energy, strain 0..8 (each
-// pair's half).
+// Energy, forces, field and (strain set) strain of the pairs at the total dipoles from the stored kernels (FP32:
+// G0, G3 rebuilt from G1, G2 with the radii rg and beta); per atom (sorted order) perAtom[n][10] = energy, strain
+// 0..8 with strain, else perAtom[n] = energy (each pair's half).
 template<class T>
 void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
                      const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
-                     double* force, double* field, double* perAtom, int lanes);
+                     double* force, double* field, double* perAtom, bool strain, int lanes);
 
 // Parameter derivatives of the pair energy at fixed dipoles and, with dipDot (and disp) given, their tangents
 // along the displacement disp and the dipole change dipDot, sorted order, each atom from its own visits of its

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec
     const BoxT<T> box = selectBox<BATCH>(bs, s);
     const int shift = (threadIdx.x%64)/L*L;
     const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
-    const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << L)-1ull));
+    const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << (L & 63))-1ull));
     const T4 pi = pos[s];
     const int start = candStart[s], cnt = candStart[s+1]-start;
     int kept = 0;
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
     const BoxT<T> box = selectBox<BATCH>(bs, s);
     const int shift = (threadIdx.x%64)/L*L;
     const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
-    const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << L)-1ull));
+    const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << (L & 63))-1ull));
     const T4 pi = pos[s];
     const T rgi = rg[s];
     const int start = candStart[s], cnt = candStart[s+1]-start;
@@ -378,8 +378,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     }
 }
 
-// One pair of the final pass at fixed total dipoles: energy, dE/dx, field at i, strain (core/pgm_core.cpp:422-440).
-template<class T>
+// One pair of the final pass at fixed total dipoles: energy, dE/dx, field at i and, STRAIN, the strain
+// (core/pgm_core.cpp:422-440).
+template<class T, bool STRAIN>
 __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& di,
                                           const typename Vec4<T>::type& pjv, const typename Vec4<T>::type& dj,
                                           const typename Vec2<T>::type& ga, const typename Vec2<T>::type& gb,
@@ -407,22 +408,25 @@ __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, cons
     f[0] -= (double) gx;
     f[1] -= (double) gy;
     f[2] -= (double) gz;
-    acc[1] += (double) (gx*x);
-    acc[2] += (double) (gx*y);
-    acc[3] += (double) (gx*z);
-    acc[4] += (double) (gy*x);
-    acc[5] += (double) (gy*y);
-    acc[6] += (double) (gy*z);
-    acc[7] += (double) (gz*x);
-    acc[8] += (double) (gz*y);
-    acc[9] += (double) (gz*z);
+    if (STRAIN) {
+        acc[1] += (double) (gx*x);
+        acc[2] += (double) (gx*y);
+        acc[3] += (double) (gx*z);
+        acc[4] += (double) (gy*x);
+        acc[5] += (double) (gy*y);
+        acc[6] += (double) (gy*z);
+        acc[7] += (double) (gz*x);
+        acc[8] += (double) (gz*y);
+        acc[9] += (double) (gz*z);
+    }
     const T c = qj*G1+djx*G2;
     e[0] += (double) (c*x-dj.x*G1);
     e[1] += (double) (c*y-dj.y*G1);
     e[2] += (double) (c*z-dj.z*G1);
 }
 
-template<class T, int L, int U, bool BATCH>
+// perAtom: [n][10] (energy, strain row-major) with STRAIN, else [n] (energy), each half of the atom's pair sums.
+template<class T, int L, int U, bool STRAIN, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const typename Vec4<T>::type* __restrict__ dip,
                                                          const T* __restrict__ rg, T beta,
@@ -436,7 +440,11 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
     typedef typename Vec2<T>::type T2;
     const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
-    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // energy, strain (row-major)
+    constexpr int NACC = STRAIN ? 10 : 1;
+    double acc[NACC];   // energy, strain (row-major)
+#pragma unroll
+    for (int q = 0; q < NACC; q++)
+        acc[q] = 0;
     if (s < n) {
         const BoxT<T> box = selectBox<BATCH>(bs, s);
         const T4 pi = pos[s];
@@ -465,15 +473,15 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                finalTerm<T>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, acc, f, e);
+                finalTerm<T, STRAIN>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, acc, f, e);
         }
         for (; k < cnt; k += L) {
             const int j = pj[start+k];
             T2 gb{};
             if (kStoreG03<T>)
                 gb = g03[start+k];
-            finalTerm<T>(pi, di, pos[j], dip[j], g12[start+k], gb, kStoreG03<T> ? T(0) : rgi+rg[j], beta, box, acc,
-                         f, e);
+            finalTerm<T, STRAIN>(pi, di, pos[j], dip[j], g12[start+k], gb, kStoreG03<T> ? T(0) : rgi+rg[j], beta,
+                                 box, acc, f, e);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -481,7 +489,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             e[c] = groupSum<L>(e[c]);
         }
 #pragma unroll
-        for (int q = 0; q < 10; q++)
+        for (int q = 0; q < NACC; q++)
             acc[q] = groupSum<L>(acc[q]);
         if (lane == 0) {
             force[3*s] = f[0];
@@ -492,8 +500,8 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             field[3*s+2] = e[2];
             // each pair is visited from both atoms: half of the sums
 #pragma unroll
-            for (int q = 0; q < 10; q++)
-                perAtom[10*(size_t) s+q] = 0.5*acc[q];
+            for (int q = 0; q < NACC; q++)
+                perAtom[NACC*(size_t) s+q] = 0.5*acc[q];
         }
     }
 }
@@ -619,7 +627,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
 
 inline int groupsBlocks(int n, int L) { return (int) (((long long) n*L+kPairBlock-1)/kPairBlock); }
 
-// Runs fn with the lane count (8, 16 or 32) and the batch flag as compile-time constants.
+// Runs fn with the lane count (16, 32 or 64) and the batch flag as compile-time constants.
 template<class Fn>
 void dispatchLanes(int lanes, bool batch, Fn fn) {
     auto withBatch = [&](auto L) {
@@ -629,17 +637,17 @@ void dispatchLanes(int lanes, bool batch, Fn fn) {
             fn(L, std::false_type{});
     };
     switch (lanes) {
-    case 8:
-        withBatch(std::integral_constant<int, 8>{});
+    case 16:
+        withBatch(std::integral_constant<int, 16>{});
         break;
     case 32:
         withBatch(std::integral_constant<int, 32>{});
         break;
-    case 16:
-        withBatch(std::integral_constant<int, 16>{});
+    case 64:
+        withBatch(std::integral_constant<int, 64>{});
         break;
     default:
-        throw std::invalid_argument("pgm_hip: pair-pass lanes must be 8, 16 or 32");
+        throw std::invalid_argument("pgm_hip: pair-pass lanes must be 16, 32 or 64");
     }
 }
 
@@ -702,11 +710,14 @@ template<class T>
 void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
                      const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
-                     double* force, double* field, double* perAtom, int lanes) {
+                     double* force, double* field, double* perAtom, bool strain, int lanes) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
-        kPairFinal<T, L(), 2, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, rg, beta, candStart, nb,
-                                                                                pj, g12, g03, box, force, field,
-                                                                                perAtom);
+        if (strain)
+            kPairFinal<T, L(), 2, true, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(
+                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom);
+        else
+            kPairFinal<T, L(), 2, false, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(
+                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom);
     });
 }
 
@@ -746,7 +757,7 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
                                      int);                                                                            \
     template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const T*, T,      \
                                      const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
-                                     BoxSel<T>, double*, double*, double*, int);                                      \
+                                     BoxSel<T>, double*, double*, double*, bool, int);                                \
     template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, T, const Vec4<T>::type*,    \
                                        const Vec4<T>::type*, const Vec4<T>::type*, const int*, const int*,            \
                                        const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxSel<T>, double*);

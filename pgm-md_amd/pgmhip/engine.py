@@ -338,7 +338,7 @@ class Engine:
         lanes, fused = C.c_int(0), C.c_int(0)
         ms = np.zeros(6)
         self._check(self.lib.pgm_autotune(self.h, int(reps), C.byref(lanes), C.byref(fused), _ptr(ms)))
-        table = {(L, f): float(ms[3 * (0 if f else 1) + k]) for f in (True, False) for k, L in enumerate((8, 16, 32))}
+        table = {(L, f): float(ms[3 * (0 if f else 1) + k]) for f in (True, False) for k, L in enumerate((16, 32, 64))}
         return (lanes.value, bool(fused.value)), table
 
     def set_lanes(self, lanes: int, fused: bool = True):

@@ -127,14 +127,14 @@ def test_batch_errors_and_reset():
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_autotune_keeps_results(precision):
-    """Every pair-pass variant (fused or separate compaction, 8/16/32 lanes) gives the same results (FP64: to
+    """Every pair-pass variant (fused or separate compaction, 16/32/64 lanes) gives the same results (FP64: to
     round-off; mixed: to FP32 summation order); the autotuner reports its timings and picks the fastest."""
     sy = S.config3_snapshot(41)
     sy.settings.update(TIGHT)
     eng = E.Engine.for_system(sy, precision=precision)
     t0, f0, _ = eng.compute(sy.pos, sy.box)
     tol_e, tol_f = (1e-13, 1e-10) if precision == "double" else (1e-7, 1e-5)
-    for lanes in (8, 16, 32):
+    for lanes in (16, 32, 64):
         for fused in (True, False):
             eng.set_lanes(lanes, fused)
             t, f, _ = eng.compute(sy.pos, sy.box, new_step=False)
@@ -142,7 +142,7 @@ def test_autotune_keeps_results(precision):
             assert np.abs(f - f0).max() < tol_f * np.abs(f0).max(), (lanes, fused)
     best, ms = eng.autotune(2)
     print(f"autotune {precision} (ms per evaluation in the pair passes):", ms, "->", best)
-    assert best[0] in (8, 16, 32) and all(v > 0 for v in ms.values())
+    assert best[0] in (16, 32, 64) and all(v > 0 for v in ms.values())
     assert ms[best] == min(ms.values())
     t, f, _ = eng.compute(sy.pos, sy.box, new_step=False)
     assert np.abs(f - f0).max() < tol_f * np.abs(f0).max()
