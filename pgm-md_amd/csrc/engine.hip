@@ -1515,14 +1515,16 @@ private:
             tb(kPhCg);
             launchPost<T>(st_, n, s_.peek, alphaSD_.as<double>(), rD_.as<double>(), xD_.as<double>(),
                           permD_.as<double>(), orderD_.as<int>(), gidxD_.as<int>(),
-                          push ? histD_.as<double>()+pushSlot*stride : nullptr, dD_.as<double>(), dTD_.as<T4>());
+                          push ? histD_.as<double>()+pushSlot*stride : nullptr, dD_.as<double>(), rgSD_.as<double>(),
+                          dTD_.as<T4>());
             te();
         }
         else {
             HIPCHK(hipMemsetAsync(xD_.p, 0, v3, st_));
             HIPCHK(hipMemsetAsync(rD_.p, 0, v3, st_));
             launchPost<T>(st_, n, 0.0, alphaSD_.as<double>(), rD_.as<double>(), xD_.as<double>(), permD_.as<double>(),
-                          orderD_.as<int>(), gidxD_.as<int>(), nullptr, dD_.as<double>(), dTD_.as<T4>());
+                          orderD_.as<int>(), gidxD_.as<int>(), nullptr, dD_.as<double>(), rgSD_.as<double>(),
+                          dTD_.as<T4>());
             iterations_ = 0;
             residual_ = 0;
         }

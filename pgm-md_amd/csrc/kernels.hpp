@@ -71,7 +71,7 @@ void launchPermDipoles(hipStream_t st, int n, const int* order, const double* po
 
 // The per-evaluation preparation of the sorted atoms in one pass: the wrapped positions with charges (FP64
 // and, when posF is set, FP32 with the FP32 radius parameters rgF), the permanent dipoles (FP64 and as the
-// pair/PME vector type permT), and the clearing of the counters and sums the kernels after it accumulate
+// pair/PME vector type permT, w = the radius parameter), and the clearing of the counters and sums the kernels after it accumulate
 // into (zeroI: nZeroI ints, zeroD: nZeroD doubles, maxBits: the next list check's maximum).
 struct PrepArgs {
     int n;
@@ -267,10 +267,12 @@ void launchCgUpdate(hipStream_t st, int n, int it, const double* alpha, const do
 template<class T>
 void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, const double* z, double* p,
                        typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done);
-// x += peek alpha r; history push (global order, hist may be NULL); total dipoles d = perm + x.
+// x += peek alpha r; history push (global order, hist may be NULL); total dipoles d = perm + x, also as the pair
+// vector type dT with w = the radius parameter rgS of the (sorted) atom.
 template<class T>
 void launchPost(hipStream_t st, int n, double peek, const double* alpha, const double* r, double* x, const double* perm,
-                const int* order, const int* gidx, double* hist, double* d, typename Vec4<T>::type* dT);
+                const int* order, const int* gidx, double* hist, double* d, const double* rgS,
+                typename Vec4<T>::type* dT);
 // Total field, and per atom perAtom[n][11]: |d|^2, mu^2/(2 alpha), E_rec (x) d (row-major).
 void launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
                         const double* edir, const double* erec, double cself, double* etot, double* perAtom);

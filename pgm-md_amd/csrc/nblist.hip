@@ -265,7 +265,7 @@ __global__ void kPrepAtoms(PrepArgs a) {
     o.x = (T) px;
     o.y = (T) py;
     o.z = (T) pz;
-    o.w = T(0);
+    o.w = (T) a.rgS[s];   // the radius parameter rides along for the pair passes' partner gathers
     static_cast<T4*>(a.permT)[s] = o;
 }
 

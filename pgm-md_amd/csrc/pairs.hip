@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         const T4 d = dip[j];
         const T4 u = DUAL ? dual[j] : d;
         T2 ga, gb;
-        setupTermX<T, DUAL>(xq.x, xq.y, xq.z, xq.w, rgi+rg[j], d, u, beta, ga, gb, e, f);
+        setupTermX<T, DUAL>(xq.x, xq.y, xq.z, xq.w, rgi+d.w, d, u, beta, ga, gb, e, f);
         pj[out] = j;
         g12[out] = ga;
         if (kStoreG03<T>)
@@ -266,7 +266,6 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
     for (; k+(U-1)*L < cnt; k += U*L) {
         int j[U];
         T4 pjv[U], d[U], u[U];
-        T rgj[U];
 #pragma unroll
         for (int q = 0; q < U; q++)
             j[q] = pj[start+k+q*L];
@@ -274,14 +273,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
         for (int q = 0; q < U; q++) {
             pjv[q] = pos[j[q]];
             d[q] = dip[j[q]];
-            rgj[q] = rg[j[q]];
             if (DUAL)
                 u[q] = dual[j[q]];
         }
 #pragma unroll
         for (int q = 0; q < U; q++) {
             T2 ga, gb;
-            setupTerm<T, DUAL>(pi, rgi, pjv[q], rgj[q], d[q], u[q], box, beta, ga, gb, e, f);
+            setupTerm<T, DUAL>(pi, rgi, pjv[q], d[q].w, d[q], u[q], box, beta, ga, gb, e, f);
             g12[start+k+q*L] = ga;
             if (kStoreG03<T>)
                 g03[start+k+q*L] = gb;
@@ -291,7 +289,8 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
         const int j = pj[start+k];
         T2 ga, gb;
         const T4 u = DUAL ? dual[j] : pos[j];
-        setupTerm<T, DUAL>(pi, rgi, pos[j], rg[j], dip[j], u, box, beta, ga, gb, e, f);
+        const T4 dj = dip[j];
+        setupTerm<T, DUAL>(pi, rgi, pos[j], dj.w, dj, u, box, beta, ga, gb, e, f);
         g12[start+k] = ga;
         if (kStoreG03<T>)
             g03[start+k] = gb;
@@ -469,7 +468,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             for (int u = 0; u < U; u++) {
                 pjv[u] = pos[j[u]];
                 dj[u] = dip[j[u]];
-                rgj[u] = kStoreG03<T> ? T(0) : rg[j[u]];
+                rgj[u] = kStoreG03<T> ? T(0) : dj[u].w;
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -480,8 +479,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             T2 gb{};
             if (kStoreG03<T>)
                 gb = g03[start+k];
-            finalTerm<T, STRAIN>(pi, di, pos[j], dip[j], g12[start+k], gb, kStoreG03<T> ? T(0) : rgi+rg[j], beta,
-                                 box, acc, f, e);
+            const T4 dj = dip[j];
+            finalTerm<T, STRAIN>(pi, di, pos[j], dj, g12[start+k], gb, kStoreG03<T> ? T(0) : rgi+dj.w, beta, box, acc,
+                                 f, e);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {

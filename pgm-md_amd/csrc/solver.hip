@@ -221,7 +221,8 @@ __global__ void __launch_bounds__(kCgBlock) kCgDirection(int n, int it, double t
 
 template<class T>
 __global__ void kPost(int n, double peek, const double* alpha, const double* r, double* x, const double* perm,
-                      const int* order, const int* gidx, double* hist, double* d, typename Vec4<T>::type* dT) {
+                      const int* order, const int* gidx, double* hist, double* d, const double* rgS,
+                      typename Vec4<T>::type* dT) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
@@ -243,7 +244,7 @@ __global__ void kPost(int n, double peek, const double* alpha, const double* r, 
     o.x = (T) v[0];
     o.y = (T) v[1];
     o.z = (T) v[2];
-    o.w = T(0);
+    o.w = (T) rgS[s];   // the radius parameter rides along for the pair passes' partner gathers
     dT[s] = o;
 }
 
@@ -434,8 +435,9 @@ void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, c
 
 template<class T>
 void launchPost(hipStream_t st, int n, double peek, const double* alpha, const double* r, double* x, const double* perm,
-                const int* order, const int* gidx, double* hist, double* d, typename Vec4<T>::type* dT) {
-    kPost<T><<<nblk(n, 256), 256, 0, st>>>(n, peek, alpha, r, x, perm, order, gidx, hist, d, dT);
+                const int* order, const int* gidx, double* hist, double* d, const double* rgS,
+                typename Vec4<T>::type* dT) {
+    kPost<T><<<nblk(n, 256), 256, 0, st>>>(n, peek, alpha, r, x, perm, order, gidx, hist, d, rgS, dT);
 }
 
 void launchFinalCombine(hipStream_t st, int n, const double* d, const double* mu, const double* alpha,
@@ -484,7 +486,7 @@ void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, doub
     template void launchCgDirection<T>(hipStream_t, int, int, double, int, const double*, double*, Vec4<T>::type*,  \
                                        double*, CgState*, int*);                                                     \
     template void launchPost<T>(hipStream_t, int, double, const double*, const double*, double*, const double*,     \
-                                const int*, const int*, double*, double*, Vec4<T>::type*);
+                                const int*, const int*, double*, double*, const double*, Vec4<T>::type*);
 PGM_INSTANTIATE_SOLVER(float)
 PGM_INSTANTIATE_SOLVER(double)
 
