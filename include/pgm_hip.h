@@ -184,6 +184,16 @@ int pgm_get_timing(pgm_engine* e, double* ms, long* counts, int nphases);
  * production run brackets the one kernel group it measures. */
 int pgm_set_timing_phases(pgm_engine* e, unsigned mask);
 
+/* No counterpart in the reference: test hooks that evaluate the device's own math on host arrays (they
+ * allocate, copy and synchronize on the current device; for tests, not for production loops).
+ * pgm_eval_erfc_f32: y[i] = erfc(x[i]) as the FP32 pair kernels compute it (the Chebyshev fit of Press et al.,
+ * Numerical Recipes 2nd ed. 6.2, with v_rcp_f32 and exp).  pgm_eval_pair_kernels: out[i][0..3] = G0..G3 of a
+ * pair at squared distance r2[i] with rg_i + rg_j = rg_sum[i] (a = rg_sum^-1/2) and Ewald beta, as the pair
+ * passes of the given precision compute them; out[i][4..5] = G0, G3 rebuilt from G1, G2 as the FP32 final and
+ * tangent passes do (PGM_DOUBLE: G0, G3 again).  Errors: pgm_last_error(NULL). */
+int pgm_eval_erfc_f32(int n, const float* x, float* y);
+int pgm_eval_pair_kernels(int precision, int n, const double* rg_sum, const double* r2, double beta, double* out);
+
 #ifdef __cplusplus
 }
 #endif

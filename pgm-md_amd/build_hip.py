@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libpgm_hip.so")
 EXPORTS_MAP = os.path.join(CSRC, "exports.map")
-SOURCES = ["nblist.hip", "pairs.hip", "pme.hip", "solver.hip", "adjoint.hip", "engine.hip"]
+SOURCES = ["nblist.hip", "pairs.hip", "pme.hip", "solver.hip", "adjoint.hip", "devmath.hip", "engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Werror=shadow", "-Wno-unused-function",

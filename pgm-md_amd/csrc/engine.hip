@@ -1727,6 +1727,14 @@ struct pgm_engine {
 
 namespace {
 thread_local std::string g_createError;
+}   // namespace
+
+namespace pgmhip {
+// The message pgm_last_error(NULL) returns: failures of calls without an engine.
+void setGlobalError(const char* msg) { g_createError = msg; }
+}   // namespace pgmhip
+
+namespace {
 
 template<class Fn>
 int guarded(pgm_engine* e, Fn fn) {

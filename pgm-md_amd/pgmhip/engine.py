@@ -64,7 +64,7 @@ EXPORTS = [
     "pgm_freeze_history", "pgm_set_timing", "pgm_set_timing_phases", "pgm_get_timing", "pgm_num_pairs", "pgm_backward",
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
-    "pgm_autotune", "pgm_set_lanes",
+    "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -133,6 +133,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_backward_batch_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pgm_autotune.argtypes = [vp, C.c_int, vp, vp, vp]
     lib.pgm_set_lanes.argtypes = [vp, C.c_int, C.c_int]
+    lib.pgm_eval_erfc_f32.argtypes = [C.c_int, vp, vp]
+    lib.pgm_eval_pair_kernels.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, vp]
     _ = dp
     _lib = lib
     return lib
@@ -399,6 +401,28 @@ class Engine:
 
     def num_candidates(self) -> int:
         return self.lib.pgm_num_candidates(self.h)
+
+
+def eval_erfc_f32(x) -> np.ndarray:
+    """erfc as the FP32 pair kernels evaluate it on the device (pgm_eval_erfc_f32)."""
+    lib = load() if _lib is None else _lib
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    if lib.pgm_eval_erfc_f32(len(x), _ptr(x), _ptr(y)) != 0:
+        raise PgmError(lib.pgm_last_error(None).decode())
+    return y
+
+
+def eval_pair_kernels(precision: str, rg_sum, r2, beta: float) -> np.ndarray:
+    """(n, 6): G0..G3 of the device pair kernels, then G0, G3 as the FP32 passes rebuild them (pgm_eval_pair_kernels)."""
+    lib = load() if _lib is None else _lib
+    rg_sum = np.ascontiguousarray(rg_sum, dtype=np.float64)
+    r2 = np.ascontiguousarray(r2, dtype=np.float64)
+    out = np.empty((len(r2), 6))
+    prec = {"double": 0, "mixed": 1}[precision]
+    if lib.pgm_eval_pair_kernels(prec, len(r2), _ptr(rg_sum), _ptr(r2), float(beta), _ptr(out)) != 0:
+        raise PgmError(lib.pgm_last_error(None).decode())
+    return out
 
 
 def total_energy(terms: dict) -> float:

@@ -54,6 +54,17 @@ def test_null_handle_calls_fail_cleanly():
     assert lib.pgm_num_pairs(None) == -1
 
 
+def test_device_math_hooks_check_arguments_first():
+    """The device-math test hooks reject bad arguments (message in pgm_last_error(NULL)) and return at once for
+    empty input, before any device call."""
+    lib = E.load()
+    assert lib.pgm_eval_pair_kernels(7, 1, None, None, 1.0, None) != 0
+    assert "precision" in lib.pgm_last_error(None).decode()
+    assert lib.pgm_eval_erfc_f32(-1, None, None) != 0
+    assert "n >= 0" in lib.pgm_last_error(None).decode()
+    assert lib.pgm_eval_erfc_f32(0, None, None) == 0
+
+
 def test_only_the_c_abi_is_exported():
     """No C++ symbol leaves the library: a caller's own classes (e.g. the C++ facade's pgmhip::Engine) cannot
     interpose on the engine's internals."""
