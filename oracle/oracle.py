@@ -73,6 +73,10 @@ def load(kind: str = "port") -> C.CDLL:
         getattr(lib, name).argtypes = [C.c_double, C.c_double, C.c_int, _DP]
     lib.orc_pair_kernels.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _DP]
     lib.orc_bspline_moduli.argtypes = [C.c_int, C.c_int, _DP]
+    lib.orc_gvdw_g.argtypes = [C.c_double, _DP, _DP]
+    lib.orc_gvdw_pair.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _DP, _DP]
+    lib.orc_gvdw_sum.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_double, C.c_int, _IP, _IP,
+                                 _DP, C.c_void_p, C.c_void_p]
     lib.orc_ewald_coefficient.argtypes = [C.c_double, C.c_double]
     lib.orc_ewald_coefficient.restype = C.c_double
     lib.orc_fft3d_r2c.argtypes = [C.c_int, C.c_int, C.c_int, _DP, _DP]
@@ -215,6 +219,36 @@ def erfc_kernels(a, r, nmax=4, kind="port"):
     B = np.zeros(4)
     _check(lib, lib.orc_erfc_kernels(a, r, nmax, B))
     return B[:nmax]
+
+
+def gvdw_g(y, kind="port"):
+    """(G, H) of GVDW at y (core/gvdw_kernel.h:30-55)."""
+    lib = load(kind)
+    G, H = np.zeros(1), np.zeros(1)
+    _check(lib, lib.orc_gvdw_g(float(y), G, H))
+    return float(G[0]), float(H[0])
+
+
+def gvdw_pair(r, bij, A, C6, B, rep, kind="port"):
+    """(energy, dEdR = -(1/r) dU/dr) of one GVDW pair (core/gvdw_kernel.h:58-77)."""
+    lib = load(kind)
+    e, d = np.zeros(1), np.zeros(1)
+    _check(lib, lib.orc_gvdw_pair(float(r), float(bij), float(A), float(C6), float(B), int(rep), e, d))
+    return float(e[0]), float(d[0])
+
+
+def gvdw_sum(pos, box, radius, sqrt_a, sqrt_c6, b, rep, rvdw, skip_i=(), skip_j=(), kind="port"):
+    """GVDW by the combination rule over all minimum-image pairs within rvdw except the skipped ones, brute
+    force: (energy [kJ/mol], forces (n, 3), strain (3, 3) = dE/d eps)."""
+    lib = load(kind)
+    pos = _f64(pos).reshape(-1, 3)
+    n = len(pos)
+    si = np.ascontiguousarray(skip_i, np.int32)
+    sj = np.ascontiguousarray(skip_j, np.int32)
+    e, f, S = np.zeros(1), np.zeros((n, 3)), np.zeros(9)
+    _check(lib, lib.orc_gvdw_sum(n, pos, _f64(box, (3, 3)), _f64(radius), _f64(sqrt_a), _f64(sqrt_c6), _f64(b),
+                                 int(rep), float(rvdw), len(si), si, sj, e, f.ctypes.data, S.ctypes.data))  # fmt: skip
+    return float(e[0]), f, S.reshape(3, 3)
 
 
 def bspline_moduli(K, p, kind="port"):

@@ -148,6 +148,57 @@ inline void minImage(V3& v, const double box[3][3]) {
     }
 }
 
+// ------------------------------------------------------------------------------------------------ GVDW
+
+/* G(y) = F(y)/y^6, F = A^2 + g^2/2, A = erf y - (2y/sqrt(pi))(1 + 2y^2/3) e^{-y^2}, g = (4y^3/(3 sqrt(pi)))
+ * e^{-y^2}, and H(y) = G'(y)/y (core/gvdw_kernel.h:30-55): below y = 0.6 the Taylor series of G in y^2 (its
+ * coefficients are those of (9/2) alpha(s)^2/s^2 + e^{-2s} with A = (2/sqrt(pi)) y alpha(y^2); 15 terms),
+ * the closed form above. */
+void gvdwG(double y, double& G, double& H) {
+    static const double c[15] = {1.0, -2.0, 58.0/25, -188.0/105, 2222.0/2205, -1532.0/3465, 64244.0/405405,
+                                 -162056.0/3378375, 1587734.0/126351225, -211108.0/72747675,
+                                 11902564.0/19860115275.0, -611992.0/5465775315.0,
+                                 755028172.0/39529267903125.0, -213335384.0/71152682225625.0,
+                                 15291334888.0/35078272337233125.0};
+    const double c0 = 8.0/(9.0*kPi), s = y*y;
+    if (y < 0.6) {
+        double g = 0, h = 0;
+        for (int k = 14; k >= 0; k--) {
+            g = g*s+c[k];
+            if (k > 0)
+                h = h*s+2.0*k*c[k];
+        }
+        G = c0*g;
+        H = c0*h;
+        return;
+    }
+    const double e = std::exp(-s), isp = 1.0/kSqrtPi;
+    const double A = std::erf(y)-2.0*isp*y*(1.0+2.0*s/3.0)*e, g = (4.0/3.0)*isp*s*y*e;
+    const double dA = (8.0/3.0)*isp*s*s*e, dg = 4.0*isp*s*(1.0-2.0*s/3.0)*e;
+    const double F = A*A+0.5*g*g, dF = 2.0*A*dA+g*dg, y6 = s*s*s;
+    G = F/y6;
+    H = (dF/y6-6.0*F/(y6*y))/y;
+}
+
+/* U(r) = A exp(-B y^2) (Gaussian) or A exp(-B y) (Slater) - C6 b^6 G(y), y = b r; dEdR = -(1/r) dU/dr
+ * (core/gvdw_kernel.h:58-77); the Slater force at r = 0 is 0. */
+double gvdwPair(double r, double bij, double A, double C6, double B, int rep, double& dEdR) {
+    const double y = bij*r, b2 = bij*bij, b6 = b2*b2*b2;
+    double erep, drep;
+    if (rep == 0) {
+        erep = A*std::exp(-B*y*y);
+        drep = -2.0*B*b2*erep;
+    }
+    else {
+        erep = A*std::exp(-B*y);
+        drep = r > 0 ? -B*bij*erep/r : 0.0;
+    }
+    double G, H;
+    gvdwG(y, G, H);
+    dEdR = C6*b6*b2*H-drep;
+    return erep-C6*b6*G;
+}
+
 // ------------------------------------------------------------------------------------------------ FFT
 
 /* Mixed-radix complex FFT (Stockham autosort over the prime factors of n, radix 4 first), unnormalized,
@@ -1051,6 +1102,59 @@ int orc_pair_kernels(double a, double beta, double r, int nmax, double* G) {
         if (nmax < 1 || nmax > 4)
             throw std::invalid_argument("pgm_oracle: nmax must be 1..4");
         pairKernels(a, beta, r, nmax, G);
+    });
+}
+
+int orc_gvdw_g(double y, double* G, double* H) {
+    return guarded([&] { gvdwG(y, *G, *H); });
+}
+
+int orc_gvdw_pair(double r, double bij, double A, double C6, double B, int rep, double* energy, double* dEdR) {
+    return guarded([&] { *energy = gvdwPair(r, bij, A, C6, B, rep, *dEdR); });
+}
+
+int orc_gvdw_sum(int n, const double* pos, const double* box9, const double* radius, const double* sqrtA,
+                 const double* sqrtC6, const double* b, int rep, double rvdw, int nskip, const int* skipI,
+                 const int* skipJ, double* energy, double* forcesAdd, double* strain9) {
+    return guarded([&] {
+        double box[3][3];
+        std::memcpy(box, box9, 9*sizeof(double));
+        std::vector<std::vector<int>> skip(n);
+        for (int k = 0; k < nskip; k++) {
+            if (skipI[k] < 0 || skipI[k] >= n || skipJ[k] < 0 || skipJ[k] >= n || skipI[k] == skipJ[k])
+                throw std::invalid_argument("pgm_oracle: skipped pair out of range");
+            skip[std::min(skipI[k], skipJ[k])].push_back(std::max(skipI[k], skipJ[k]));
+        }
+        double E = 0, S[9] = {0};
+        for (int i = 0; i < n; i++)
+            for (int j = i+1; j < n; j++) {
+                const double A = sqrtA[i]*sqrtA[j], C6 = sqrtC6[i]*sqrtC6[j];
+                if (A == 0 && C6 == 0)
+                    continue;
+                if (std::find(skip[i].begin(), skip[i].end(), j) != skip[i].end())
+                    continue;
+                V3 x = {pos[3*i]-pos[3*j], pos[3*i+1]-pos[3*j+1], pos[3*i+2]-pos[3*j+2]};
+                minImage(x, box);
+                const double r = std::sqrt(dot3(x, x));
+                if (r >= rvdw)
+                    continue;
+                const double bij = 1.0/std::sqrt(2.0*(radius[i]*radius[i]+radius[j]*radius[j]));
+                double dEdR;
+                E += gvdwPair(r, bij, A, C6, 0.5*(b[i]+b[j]), rep, dEdR);
+                for (int a = 0; a < 3; a++) {
+                    if (forcesAdd) {
+                        forcesAdd[3*i+a] += dEdR*x[a];
+                        forcesAdd[3*j+a] -= dEdR*x[a];
+                    }
+                    for (int c = 0; c < 3; c++)
+                        S[3*a+c] -= dEdR*x[a]*x[c];   // dU/d eps_ac = (dU/dr) x_a x_c / r
+                }
+            }
+        if (energy)
+            *energy = E;
+        if (strain9)
+            for (int k = 0; k < 9; k++)
+                strain9[k] += S[k];
     });
 }
 

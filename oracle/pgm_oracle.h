@@ -66,6 +66,16 @@ int orc_pair_kernels(double a, double beta, double r, int nmax, double* G);
 int orc_bspline_moduli(int K, int p, double* out);
 double orc_ewald_coefficient(double rc, double rtol);
 /* Unnormalized forward r2c (K1 x K2 x (K3/2+1) complex, interleaved re/im) and backward c2r. */
+/* GVDW, the Gaussian-density van der Waals (core/gvdw_kernel.h): G(y) = F(y)/y^6 and H(y) = G'(y)/y; the pair
+ * energy with dEdR = -(1/r) dU/dr (bij = 1/sqrt(2 (R_i^2 + R_j^2)), rep 0 Gaussian / 1 Slater). */
+int orc_gvdw_g(double y, double* G, double* H);
+int orc_gvdw_pair(double r, double bij, double A, double C6, double B, int rep, double* energy, double* dEdR);
+/* GVDW by the combination rule (A_ij = a_i a_j, C6_ij = c_i c_j, B_ij = (b_i + b_j)/2) over all minimum-image
+ * pairs within rvdw except the skipped pairs (skipI[k], skipJ[k]), brute force: energy [kJ/mol], forces added to
+ * forcesAdd (3n), strain S_ab = dE/d eps_ab added to strain9 (each may be NULL). */
+int orc_gvdw_sum(int n, const double* pos, const double* box9, const double* radius, const double* sqrtA,
+                 const double* sqrtC6, const double* b, int rep, double rvdw, int nskip, const int* skipI,
+                 const int* skipJ, double* energy, double* forcesAdd, double* strain9);
 int orc_fft3d_r2c(int K1, int K2, int K3, const double* in, double* out);
 int orc_fft3d_c2r(int K1, int K2, int K3, const double* in, double* out);
 

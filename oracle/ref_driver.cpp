@@ -7,6 +7,7 @@
  */
 #include "pgm_oracle.h"
 
+#include "gvdw_kernel.h"
 #include "pgm_core.h"
 #include "pgm_math.h"
 #include "pocketfft_hdronly.h"
@@ -215,6 +216,21 @@ int orc_erfc_kernels(double a, double r, int nmax, double* B) {
 
 int orc_pair_kernels(double a, double beta, double r, int nmax, double* G) {
     return guarded([&] { pgmcore::ewaldPairKernels(a, beta, r, nmax, G); });
+}
+
+int orc_gvdw_g(double y, double* G, double* H) {
+    return guarded([&] { pgmcore::gvdwG(y, *G, *H); });
+}
+
+int orc_gvdw_pair(double r, double bij, double A, double C6, double B, int rep, double* energy, double* dEdR) {
+    return guarded([&] { *energy = pgmcore::gvdwPair(r, bij, A, C6, B, rep, *dEdR); });
+}
+
+int orc_gvdw_sum(int, const double*, const double*, const double*, const double*, const double*, const double*, int,
+                 double, int, const int*, const int*, double*, double*, double*) {
+    return guarded([&] {
+        throw std::runtime_error("pgm_ref: the reference core has no GVDW sum (core/gvdw_kernel.h has the pair only)");
+    });
 }
 
 int orc_bspline_moduli(int K, int p, double* out) {
