@@ -65,6 +65,15 @@ const char* pgm_last_error(const pgm_engine* e);
  * predictor history (n_global slots). */
 int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, const double* alpha,
                   const int* global_index, int n_global);
+/* Engine::setGvdw (core/cuda/pgm_cuda.h:57-65): GVDW, the Gaussian-density van der Waals of core/gvdw_kernel.h,
+ * by the combination rule (A_ij = a_i a_j, C6_ij = c_i c_j, B_ij = (b_i + b_j)/2 from per-atom sqrt_a, sqrt_c6,
+ * b; zeros: none) for all pairs within rvdw except the skipped pairs (skip_i[k], skip_j[k]), each given once
+ * (exclusions, and pairs with their own parameters: those and the dispersion tail are the caller's).
+ * repulsion: 0 Gaussian, 1 Slater.  Its energy is pgm_terms.gvdw [kJ/mol]; its forces and strain are added to
+ * the electrostatic ones.  After every pgm_set_atoms; rvdw = 0 turns it off.  Unlike the reference, rvdw may
+ * not exceed the electrostatic cutoff (the pair list radius), and pgm_backward refuses an engine with GVDW on. */
+int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
+                 const double* b, int nskip, const int* skip_i, const int* skip_j);
 /* setCovalent (core/cuda/pgm_cuda.h:67): p_atom += strength unit(r_partner - r_atom). */
 int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength);
 

@@ -260,6 +260,58 @@ __device__ __forceinline__ void pairKernels(T a, T beta, T r, T* G) {
         G[k] -= B[k];
 }
 
+// GVDW, the Gaussian-density van der Waals (core/gvdw_kernel.h): G(y) = F(y)/y^6 and H(y) = G'(y)/y with
+// F = A^2 + g^2/2, A = erf y - (2y/sqrt(pi))(1 + 2y^2/3) e^{-y^2}, g = (4y^3/(3 sqrt(pi))) e^{-y^2}.  Below
+// y = 0.6 the Taylor series of G in y^2 (coefficients c_k of G/C0, C0 = 8/(9 pi): exact rationals, from the
+// series of erf and exp), above it the closed form.
+template<class T>
+__device__ __forceinline__ void gvdwG(T y, T& G, T& H) {
+    constexpr double c[15] = {1.0, -2.0, 58.0/25, -188.0/105, 2222.0/2205, -1532.0/3465, 64244.0/405405,
+                              -162056.0/3378375, 1587734.0/126351225, -211108.0/72747675,
+                              11902564.0/19860115275.0, -611992.0/5465775315.0, 755028172.0/39529267903125.0,
+                              -213335384.0/71152682225625.0, 15291334888.0/35078272337233125.0};
+    const T c0 = T(8.0/(9.0*kPi)), s = y*y;
+    if (y < T(0.6)) {
+        T g = T(c[14]), h = T(28.0*c[14]);
+#pragma unroll
+        for (int k = 13; k >= 0; k--) {
+            g = g*s+T(c[k]);
+            if (k > 0)
+                h = h*s+T(2.0*k*c[k]);
+        }
+        G = c0*g;
+        H = c0*h;
+        return;
+    }
+    const T e = expT(-s), isp = T(1.0/kSqrtPi);
+    const T A = erfT(y)-T(2)*isp*y*(T(1)+T(2.0/3.0)*s)*e, g = T(4.0/3.0)*isp*s*y*e;
+    const T dA = T(8.0/3.0)*isp*s*s*e, dg = T(4)*isp*s*(T(1)-T(2.0/3.0)*s)*e;
+    const T F = A*A+T(0.5)*g*g, dF = T(2)*A*dA+g*dg, y6 = s*s*s;
+    G = F/y6;
+    H = (dF/y6-T(6)*F/(y6*y))/y;
+}
+
+// One GVDW pair: U = A exp(-B y^2) (Gaussian) or A exp(-B y) (Slater) - C6 b^6 G(y), y = b r; returns U and
+// dEdR = -(1/r) dU/dr (the force on i is dEdR (r_i - r_j)); the Slater force at r = 0 is 0
+// (core/gvdw_kernel.h:58-77).
+template<class T>
+__device__ __forceinline__ T gvdwPair(T r, T bij, T A, T C6, T B, bool slater, T& dEdR) {
+    const T y = bij*r, b2 = bij*bij, b6 = b2*b2*b2;
+    T erep, drep;
+    if (!slater) {
+        erep = A*expT(-B*y*y);
+        drep = T(-2)*B*b2*erep;
+    }
+    else {
+        erep = A*expT(-B*y);
+        drep = r > T(0) ? -B*bij*erep/r : T(0);
+    }
+    T G, H;
+    gvdwG<T>(y, G, H);
+    dEdR = C6*b6*b2*H-drep;
+    return erep-C6*b6*G;
+}
+
 // Cardinal B-spline weights of order P at fraction fr on the grid points (base - j), j = 0..P-1
 // (core/pgm_core.cpp:131-152): th[j] = M_P(fr + j), dth[j] = M_P'(fr + j), d2th[j] = M_P''(fr + j).
 template<class T, int P>

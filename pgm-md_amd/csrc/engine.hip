@@ -144,7 +144,8 @@ enum {
     kResRec = 10,      // energy, strain[9] (k-space)
     kResComb = 20,     // |d|^2, polarization, E_rec (x) d [9]
     kResCov = 31,      // covalent strain [9]
-    kResCount = 40,
+    kResGvdw = 40,     // GVDW energy
+    kResCount = 41,
 };
 
 }   // namespace
@@ -276,10 +277,68 @@ public:
             upload(alphaD_, alphaH_.data(), n*sizeof(double));
             upload(gidxD_, gi.data(), n*sizeof(int));
             haveList_ = false;   // parameters of the sorted arrays change: rebuild
+            gvdwOn_ = false;     // GVDW is set again after every setAtoms
             if (covN_ < 0)
                 setCovalentImpl(0, nullptr, nullptr, nullptr);
             haveResults_ = false;
         });
+    }
+
+    // GVDW by the combination rule over the pairs within rvdw except the skipped ones (core/cuda/pgm_cuda.h:57-65);
+    // rvdw 0: off.  The pair list's radius is the electrostatic cutoff, so rvdw may not exceed it.
+    void setGvdw(int repulsion, double rvdw, const double* sqrtA, const double* sqrtC6, const double* b, int nskip,
+                 const int* skipI, const int* skipJ) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: setGvdw before setAtoms");
+        if (!(rvdw > 0)) {
+            gvdwOn_ = false;
+            return;
+        }
+        if (repulsion != 0 && repulsion != 1)
+            throw std::invalid_argument("pgm_hip: setGvdw: repulsion must be 0 (Gaussian) or 1 (Slater)");
+        if (rvdw > s_.cutoff)
+            throw std::invalid_argument("pgm_hip: setGvdw: rvdw larger than the cutoff is not supported (the pair "
+                                        "list radius is the electrostatic cutoff)");
+        if (!sqrtA || !sqrtC6 || !b || nskip < 0 || (nskip > 0 && (!skipI || !skipJ)))
+            throw std::invalid_argument("pgm_hip: setGvdw: bad arrays");
+        std::vector<int> start(n_+1, 0), part(2*(size_t) nskip);
+        for (int k = 0; k < nskip; k++) {
+            const int i = skipI[k], j = skipJ[k];
+            if (i < 0 || i >= n_ || j < 0 || j >= n_ || i == j)
+                throw std::invalid_argument("pgm_hip: setGvdw: skipped pair index out of range");
+            if (systemOf(i) != systemOf(j))
+                throw std::invalid_argument("pgm_hip: setGvdw: a skipped pair joins two systems of the batch");
+            start[i+1]++;
+            start[j+1]++;
+        }
+        for (int i = 0; i < n_; i++)
+            start[i+1] += start[i];
+        std::vector<int> fill(start.begin(), start.end()-1);
+        for (int k = 0; k < nskip; k++) {
+            part[fill[skipI[k]]++] = skipJ[k];
+            part[fill[skipJ[k]]++] = skipI[k];
+        }
+        std::vector<double> gv(3*(size_t) n_);
+        for (int i = 0; i < n_; i++) {
+            gv[3*i] = sqrtA[i];
+            gv[3*i+1] = sqrtC6[i];
+            gv[3*i+2] = b[i];
+        }
+        withDevice([&] {
+            HIPCHK(hipStreamSynchronize(st_));
+            upload(gvInD_, gv.data(), gv.size()*sizeof(double));
+            upload(skipStartD_, start.data(), start.size()*sizeof(int));
+            if (part.empty())
+                skipPartD_.reserve(sizeof(int));
+            else
+                upload(skipPartD_, part.data(), part.size()*sizeof(int));
+        });
+        nSkip_ = nskip;
+        rvdw_ = rvdw;
+        gvdwSlater_ = repulsion == 1;
+        gvdwOn_ = true;
+        gvdwReady_ = false;
+        haveResults_ = false;
     }
 
     void setCovalent(int ncov, const int* atom, const int* partner, const double* strength) {
@@ -425,6 +484,8 @@ public:
                   hipStream_t caller) {
         if (n_ <= 0)
             throw std::logic_error("pgm_hip: backward before setAtoms");
+        if (gvdwOn_)
+            throw std::invalid_argument("pgm_hip: backward covers the electrostatics only: set GVDW off (rvdw 0)");
         evaluate(src, pos, boxes, false, nullptr, nullptr, nullptr, 0, 0.0, nullptr, nullptr, caller, false);
         withDevice([&] {
             HIPCHK(hipEventRecord(evIn_, caller));
@@ -666,6 +727,12 @@ private:
         hipEvent_t a, b;
     };
     bool timing_ = false;
+    // GVDW (combination rule), set after setAtoms
+    bool gvdwOn_ = false, gvdwSlater_ = false;
+    bool gvdwReady_ = false;   // the sorted per-atom GVDW parameters are gathered (by prep, for this setting)
+    double rvdw_ = 0;
+    int nSkip_ = 0;
+    DevBuf gvInD_, gvTD_, skipStartD_, skipPartD_, gvdwAtomD_;
     unsigned phaseMask_ = ~0u;   // phases bracketed with events while timing (bit k: phase k)
     std::vector<hipEvent_t> evPool_;
     size_t evUsed_ = 0;
@@ -1397,6 +1464,19 @@ private:
         te();
     }
 
+    template<class T>
+    GvdwArgs<T> gvdwArgs() {
+        GvdwArgs<T> g{};
+        if (!gvdwOn_ || !gvdwReady_)
+            return g;
+        g.gv = gvTD_.as<typename Vec4<T>::type>();
+        g.rvdw2 = (T) (rvdw_*rvdw_);
+        g.slater = gvdwSlater_ ? 1 : 0;
+        g.invKe = (T) (1.0/s_.ke);
+        g.energy = gvdwAtomD_.as<double>();
+        return g;
+    }
+
     // Per-evaluation preparation in the current sorted order: positions, charges, radii and permanent dipoles
     // of the sorted atoms (one fused kernel that also clears the PME brick counters, the result sums and the
     // next list check's maximum), then the PME bricks and splines.
@@ -1443,7 +1523,14 @@ private:
         a.zeroD = resD_.as<double>();
         a.nZeroD = (int) (sys_.size()*kResCount);
         a.maxBits = maxBitsD_.as<unsigned>();
+        if (gvdwOn_) {
+            gvTD_.reserve(n*sizeof(T4));
+            gvdwAtomD_.reserve(n*sizeof(double));
+            a.gvIn = gvInD_.as<double>();
+            a.gvOut = gvTD_.p;
+        }
         launchPrepAtoms<T>(st_, a);
+        gvdwReady_ = gvdwOn_;
         const PmeSel ps = pmeSel();
         launchPmeBin<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                         brickCountD_.as<int>());
@@ -1531,10 +1618,17 @@ private:
         // final stage
         double* res = resD_.as<double>();
         tb(kPhPairFinal);
+        const GvdwArgs<T> ga = gvdwArgs<T>();
         launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(), nbD_.as<int>(),
                            pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(),
-                           edD_.as<double>(), perAtomD_.as<double>(), wantStrain, lanes_);
+                           edD_.as<double>(), perAtomD_.as<double>(), wantStrain, ga, lanes_);
+        if (ga.gv && nSkip_ > 0)
+            launchGvdwSkip<T>(st_, n, orderD_.as<int>(), rankD_.as<int>(), pos, rgT<T>(), skipStartD_.as<int>(),
+                              skipPartD_.as<int>(), bt, (T) (s_.cutoff*s_.cutoff), wantStrain, ga,
+                              fpairD_.as<double>(), perAtomD_.as<double>());
         systemSums(perAtomD_.as<double>(), wantStrain ? 10 : 1, res, kResPair);
+        if (ga.gv)
+            systemSums(gvdwAtomD_.as<double>(), 1, res, kResGvdw);
         te();
         pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
         tb(kPhFinal);
@@ -1573,7 +1667,8 @@ private:
                                    g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes);
             launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(),
                                nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
-                               fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), false, lanes);
+                               fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), false,
+                               gvdwArgs<T>(), lanes);
         };
         once();   // warm-up
         HIPCHK(hipEventRecord(a, st_));
@@ -1699,7 +1794,7 @@ private:
             const double bg = -kPi*y.Qtot*y.Qtot/(2.0*y.volume*s_.beta*s_.beta);
             y.terms.background = ke*bg;
             y.terms.polarization = ke*r[kResComb+1];
-            y.terms.gvdw = 0;
+            y.terms.gvdw = gvdwOn_ ? r[kResGvdw] : 0.0;
             if (wantStrain) {
                 for (int k = 0; k < 9; k++)
                     y.strain[k] = r[kResPair+1+k]+r[kResRec+1+k]+r[kResComb+2+k]+r[kResCov+k];
@@ -1804,6 +1899,11 @@ const char* pgm_last_error(const pgm_engine* e) {
 int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, const double* alpha,
                   const int* global_index, int n_global) {
     return guarded(e, [&](DeviceEngine& g) { g.setAtoms(n, q, radius, alpha, global_index, n_global); });
+}
+
+int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
+                 const double* b, int nskip, const int* skip_i, const int* skip_j) {
+    return guarded(e, [&](DeviceEngine& g) { g.setGvdw(repulsion, rvdw, sqrt_a, sqrt_c6, b, nskip, skip_i, skip_j); });
 }
 
 int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength) {

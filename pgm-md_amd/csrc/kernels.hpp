@@ -93,6 +93,8 @@ struct PrepArgs {
     double* zeroD;
     int nZeroD;
     unsigned* maxBits;
+    const double* gvIn;   // GVDW (sqrt A, sqrt C6, b) per atom, caller's order (nullptr: none)
+    void* gvOut;          // ... as the pair vector type, sorted order
 };
 template<class T>
 void launchPrepAtoms(hipStream_t st, const PrepArgs& a);
@@ -127,6 +129,16 @@ template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxSel<T> box, double* field, const int* skip, int lanes);
+// The GVDW pair term of the final pass (core/gvdw_kernel.h, combination rule; gv == nullptr: none).
+template<class T>
+struct GvdwArgs {
+    const typename Vec4<T>::type* gv;   // per sorted atom (sqrt A, sqrt C6, b, 0)
+    T rvdw2;
+    int slater;                         // repulsion: 0 Gaussian, 1 Slater
+    T invKe;                            // forces and strain in units of ke, as the electrostatics
+    double* energy;                     // per sorted atom: half of its pairs' GVDW energy [kJ/mol]
+};
+
 // Energy, forces, field and (strain set) strain of the pairs at the total dipoles from the stored kernels (FP32:
 // G0, G3 rebuilt from G1, G2 with the radii rg and beta); per atom (sorted order) perAtom[n][10] = energy, strain
 // 0..8 with strain, else perAtom[n] = energy (each pair's half).
@@ -134,7 +146,14 @@ template<class T>
 void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
                      const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
-                     double* force, double* field, double* perAtom, bool strain, int lanes);
+                     double* force, double* field, double* perAtom, bool strain, const GvdwArgs<T>& gvdw,
+                     int lanes);
+// After launchPairFinal with GVDW: the GVDW terms of the skipped pairs (per atom CSR of partners in the caller's
+// order, both directions) subtracted from the atoms' forces, GVDW energies and (strain) strain rows.
+template<class T>
+void launchGvdwSkip(hipStream_t st, int n, const int* order, const int* rank, const typename Vec4<T>::type* pos,
+                    const T* rg, const int* skipStart, const int* skipPartner, BoxSel<T> box, T rc2, bool strain,
+                    const GvdwArgs<T>& gvdw, double* force, double* perAtom);
 
 // Parameter derivatives of the pair energy at fixed dipoles and, with dipDot (and disp) given, their tangents
 // along the displacement disp and the dipole change dipDot, sorted order, each atom from its own visits of its

@@ -267,6 +267,14 @@ __global__ void kPrepAtoms(PrepArgs a) {
     o.z = (T) pz;
     o.w = (T) a.rgS[s];   // the radius parameter rides along for the pair passes' partner gathers
     static_cast<T4*>(a.permT)[s] = o;
+    if (a.gvIn) {
+        T4 g;
+        g.x = (T) a.gvIn[3*i];
+        g.y = (T) a.gvIn[3*i+1];
+        g.z = (T) a.gvIn[3*i+2];
+        g.w = T(0);
+        static_cast<T4*>(a.gvOut)[s] = g;
+    }
 }
 
 inline int blocksFor(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }

@@ -377,21 +377,53 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     }
 }
 
-// One pair of the final pass at fixed total dipoles: energy, dE/dx, field at i and, STRAIN, the strain
-// (core/pgm_core.cpp:422-440).
+// The GVDW pair term of one visit of pair (i, j) (combination rule: A = a_i a_j, C6 = c_i c_j, B = (b_i + b_j)/2
+// from gv = (a, c, b), b_ij = (rg_i + rg_j)^-1/2): energy into eg, force on i and, STRAIN, the strain into f and
+// acc in units of ke (invKe = 1/ke), as the electrostatic terms (core/cuda/pgm_cuda.cu:398-414).  sign = -1
+// removes exactly what sign = +1 added: the skipped pairs.
 template<class T, bool STRAIN>
+__device__ __forceinline__ void gvdwTerm(T x, T y, T z, T r2, const typename Vec4<T>::type& gvi,
+                                         const typename Vec4<T>::type& gvj, T rgSum, T rvdw2, bool slater,
+                                         T invKe, double sign, double& eg, double* acc, double* f) {
+    const T A = gvi.x*gvj.x, C6 = gvi.y*gvj.y;
+    if (!(r2 < rvdw2) || (A == T(0) && C6 == T(0)))
+        return;
+    T dEdR;
+    const T u = gvdwPair<T>(sqrt(r2), rsqrtT(rgSum), A, C6, T(0.5)*(gvi.z+gvj.z), slater, dEdR);
+    const T fs = dEdR*invKe;
+    eg += sign*(double) u;
+    f[0] += sign*(double) (fs*x);
+    f[1] += sign*(double) (fs*y);
+    f[2] += sign*(double) (fs*z);
+    if (STRAIN) {
+        const T xs[3] = {x, y, z};
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                acc[1+3*a+c] -= sign*(double) (fs*xs[a]*xs[c]);
+    }
+}
+
+// One pair of the final pass at fixed total dipoles: energy, dE/dx, field at i and, STRAIN, the strain
+// (core/pgm_core.cpp:422-440); GVDW adds the pair's GVDW term.
+template<class T, bool STRAIN, bool GVDW>
 __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& di,
                                           const typename Vec4<T>::type& pjv, const typename Vec4<T>::type& dj,
                                           const typename Vec2<T>::type& ga, const typename Vec2<T>::type& gb,
-                                          T rgSum, T beta, const BoxT<T>& box, double* acc, double* f, double* e) {
+                                          T rgSum, T beta, const BoxT<T>& box, double* acc, double* f, double* e,
+                                          const typename Vec4<T>::type& gvi, const typename Vec4<T>::type& gvj,
+                                          const GvdwArgs<T>& gva, double& eg) {
     T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
     minImage(x, y, z, box);
+    if (GVDW)
+        gvdwTerm<T, STRAIN>(x, y, z, x*x+y*y+z*z, gvi, gvj, rgSum, gva.rvdw2, gva.slater != 0, gva.invKe, 1.0, eg,
+                            acc, f);
     const T G1 = ga.x, G2 = ga.y;
     T G0, G3;
     if constexpr (kStoreG03<T>) {
         G0 = gb.x;
         G3 = gb.y;
-        (void) rgSum;
         (void) beta;
     }
     else
@@ -424,8 +456,9 @@ __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, cons
     e[2] += (double) (c*z-dj.z*G1);
 }
 
-// perAtom: [n][10] (energy, strain row-major) with STRAIN, else [n] (energy), each half of the atom's pair sums.
-template<class T, int L, int U, bool STRAIN, bool BATCH>
+// perAtom: [n][10] (energy, strain row-major) with STRAIN, else [n] (energy), each half of the atom's pair sums;
+// GVDW: the GVDW energy per atom into gva.energy.
+template<class T, int L, int U, bool STRAIN, bool GVDW, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                          const typename Vec4<T>::type* __restrict__ dip,
                                                          const T* __restrict__ rg, T beta,
@@ -434,12 +467,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
                                                          const typename Vec2<T>::type* __restrict__ g12,
                                                          const typename Vec2<T>::type* __restrict__ g03, BoxSel<T> bs,
                                                          double* __restrict__ force, double* __restrict__ field,
-                                                         double* __restrict__ perAtom) {
+                                                         double* __restrict__ perAtom, GvdwArgs<T> gva) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     constexpr int NACC = STRAIN ? 10 : 1;
+    double eg = 0;
     double acc[NACC];   // energy, strain (row-major)
 #pragma unroll
     for (int q = 0; q < NACC; q++)
@@ -448,14 +482,15 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
         const BoxT<T> box = selectBox<BATCH>(bs, s);
         const T4 pi = pos[s];
         const T4 di = dip[s];
-        const T rgi = kStoreG03<T> ? T(0) : rg[s];
+        const T rgi = (kStoreG03<T> && !GVDW) ? T(0) : rg[s];
+        const T4 gvi = GVDW ? gva.gv[s] : di;
         const int start = candStart[s], cnt = nb[s];
         double f[3] = {0, 0, 0}, e[3] = {0, 0, 0};
         int k = lane;
         for (; k+(U-1)*L < cnt; k += U*L) {
             int j[U];
             T2 ga[U], gb[U];
-            T4 pjv[U], dj[U];
+            T4 pjv[U], dj[U], gvj[U];
             T rgj[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -468,11 +503,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             for (int u = 0; u < U; u++) {
                 pjv[u] = pos[j[u]];
                 dj[u] = dip[j[u]];
-                rgj[u] = kStoreG03<T> ? T(0) : dj[u].w;
+                rgj[u] = (kStoreG03<T> && !GVDW) ? T(0) : dj[u].w;
+                gvj[u] = GVDW ? gva.gv[j[u]] : dj[u];
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                finalTerm<T, STRAIN>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, acc, f, e);
+                finalTerm<T, STRAIN, GVDW>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, acc, f, e, gvi,
+                                           gvj[u], gva, eg);
         }
         for (; k < cnt; k += L) {
             const int j = pj[start+k];
@@ -480,8 +517,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             if (kStoreG03<T>)
                 gb = g03[start+k];
             const T4 dj = dip[j];
-            finalTerm<T, STRAIN>(pi, di, pos[j], dj, g12[start+k], gb, kStoreG03<T> ? T(0) : rgi+dj.w, beta, box, acc,
-                                 f, e);
+            const T4 gvj = GVDW ? gva.gv[j] : dj;
+            finalTerm<T, STRAIN, GVDW>(pi, di, pos[j], dj, g12[start+k], gb, (kStoreG03<T> && !GVDW) ? T(0) : rgi+dj.w,
+                                       beta, box, acc, f, e, gvi, gvj, gva, eg);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -491,6 +529,8 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
 #pragma unroll
         for (int q = 0; q < NACC; q++)
             acc[q] = groupSum<L>(acc[q]);
+        if (GVDW)
+            eg = groupSum<L>(eg);
         if (lane == 0) {
             force[3*s] = f[0];
             force[3*s+1] = f[1];
@@ -502,8 +542,53 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
 #pragma unroll
             for (int q = 0; q < NACC; q++)
                 perAtom[NACC*(size_t) s+q] = 0.5*acc[q];
+            if (GVDW)
+                gva.energy[s] = 0.5*eg;
         }
     }
+}
+
+// The GVDW terms of the skipped pairs (exclusions) taken out again after the final pass: per sorted atom, over its
+// skipped partners (caller's order CSR), each pair the final pass included (within rc, so in the compacted list,
+// and within rvdw) is evaluated by the same code from the same operands and subtracted from the atom's own force,
+// GVDW energy and strain rows (no atomics: every visit changes only its own atom's rows).
+template<class T, bool STRAIN, bool BATCH>
+__global__ void kGvdwSkip(int n, const int* __restrict__ order, const int* __restrict__ rank,
+                          const typename Vec4<T>::type* __restrict__ pos, const T* __restrict__ rg,
+                          const int* __restrict__ skipStart, const int* __restrict__ skipPartner, BoxSel<T> bs, T rc2,
+                          GvdwArgs<T> gva, double* __restrict__ force, double* __restrict__ perAtom) {
+    typedef typename Vec4<T>::type T4;
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    const int i = order[s];
+    const int m0 = skipStart[i], m1 = skipStart[i+1];
+    if (m0 == m1)
+        return;
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
+    const T4 pi = pos[s];
+    const T4 gvi = gva.gv[s];
+    const T rgi = rg[s];
+    double eg = 0, f[3] = {0, 0, 0}, acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = m0; m < m1; m++) {
+        const int sj = rank[skipPartner[m]];
+        const T4 pjv = pos[sj];
+        T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+        minImage(x, y, z, box);
+        const T r2 = x*x+y*y+z*z;
+        if (!(r2 < rc2))
+            continue;
+        gvdwTerm<T, STRAIN>(x, y, z, r2, gvi, gva.gv[sj], rgi+rg[sj], gva.rvdw2, gva.slater != 0, gva.invKe, -1.0, eg,
+                            acc, f);
+    }
+    force[3*s] += f[0];
+    force[3*s+1] += f[1];
+    force[3*s+2] += f[2];
+    gva.energy[s] += 0.5*eg;
+    if (STRAIN)
+#pragma unroll
+        for (int q = 1; q < 10; q++)
+            perAtom[10*(size_t) s+q] += 0.5*acc[q];
 }
 
 // Parameter derivatives of the pair energy at fixed dipoles d and their tangents along a displacement (x' = g_i - g_j
@@ -710,15 +795,35 @@ template<class T>
 void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
                      const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
-                     double* force, double* field, double* perAtom, bool strain, int lanes) {
+                     double* force, double* field, double* perAtom, bool strain, const GvdwArgs<T>& gvdw,
+                     int lanes) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        auto go = [&](auto S, auto G) {
+            kPairFinal<T, L(), 2, S(), G(), B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(
+                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom, gvdw);
+        };
+        const bool g = gvdw.gv != nullptr;
         if (strain)
-            kPairFinal<T, L(), 2, true, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(
-                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom);
+            g ? go(std::true_type{}, std::true_type{}) : go(std::true_type{}, std::false_type{});
         else
-            kPairFinal<T, L(), 2, false, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(
-                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom);
+            g ? go(std::false_type{}, std::true_type{}) : go(std::false_type{}, std::false_type{});
     });
+}
+
+template<class T>
+void launchGvdwSkip(hipStream_t st, int n, const int* order, const int* rank, const typename Vec4<T>::type* pos,
+                    const T* rg, const int* skipStart, const int* skipPartner, BoxSel<T> box, T rc2, bool strain,
+                    const GvdwArgs<T>& gvdw, double* force, double* perAtom) {
+    const int blocks = (n+255)/256;
+    auto go = [&](auto S, auto B) {
+        kGvdwSkip<T, S(), B()><<<blocks, 256, 0, st>>>(n, order, rank, pos, rg, skipStart, skipPartner, box, rc2,
+                                                        gvdw, force, perAtom);
+    };
+    const bool b = box.table != nullptr;
+    if (strain)
+        b ? go(std::true_type{}, std::true_type{}) : go(std::true_type{}, std::false_type{});
+    else
+        b ? go(std::false_type{}, std::true_type{}) : go(std::false_type{}, std::false_type{});
 }
 
 template<class T>
@@ -757,7 +862,9 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
                                      int);                                                                            \
     template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const T*, T,      \
                                      const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
-                                     BoxSel<T>, double*, double*, double*, bool, int);                                \
+                                     BoxSel<T>, double*, double*, double*, bool, const GvdwArgs<T>&, int);            \
+    template void launchGvdwSkip<T>(hipStream_t, int, const int*, const int*, const Vec4<T>::type*, const T*,         \
+                                    const int*, const int*, BoxSel<T>, T, bool, const GvdwArgs<T>&, double*, double*); \
     template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, T, const Vec4<T>::type*,    \
                                        const Vec4<T>::type*, const Vec4<T>::type*, const int*, const int*,            \
                                        const int*, const Vec2<T>::type*, const Vec2<T>::type*, BoxSel<T>, double*);

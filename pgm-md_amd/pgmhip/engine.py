@@ -64,7 +64,7 @@ EXPORTS = [
     "pgm_freeze_history", "pgm_set_timing", "pgm_set_timing_phases", "pgm_get_timing", "pgm_num_pairs", "pgm_backward",
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
-    "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels",
+    "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -134,6 +134,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_autotune.argtypes = [vp, C.c_int, vp, vp, vp]
     lib.pgm_set_lanes.argtypes = [vp, C.c_int, C.c_int]
     lib.pgm_eval_erfc_f32.argtypes = [C.c_int, vp, vp]
+    lib.pgm_set_gvdw.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp, C.c_int, vp, vp]
     lib.pgm_eval_pair_kernels.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, vp]
     _ = dp
     _lib = lib
@@ -280,6 +281,16 @@ class Engine:
         rc = self.lib.pgm_backward_device(self.h, pos_ptr, _ptr(box), g_ptr, float(dl_denergy), gq_ptr, gr_ptr, ga_ptr,
                                           gc_ptr, stream)  # fmt: skip
         self._check(rc)
+
+    def set_gvdw(self, repulsion: int, rvdw: float, sqrt_a, sqrt_c6, b, skip_i=(), skip_j=()):
+        """GVDW by the combination rule within rvdw, except the skipped pairs (pgm_set_gvdw); rvdw 0: off."""
+        sa, sc, bb = (_f64(v).reshape(-1) for v in (sqrt_a, sqrt_c6, b))
+        si = np.ascontiguousarray(skip_i, np.int32)
+        sj = np.ascontiguousarray(skip_j, np.int32)
+        if len(si) != len(sj):
+            raise PgmError("skip_i and skip_j differ in length")
+        self._check(self.lib.pgm_set_gvdw(self.h, int(repulsion), float(rvdw), _ptr(sa), _ptr(sc), _ptr(bb), len(si),
+                                          _ptr(si) if len(si) else None, _ptr(sj) if len(sj) else None))  # fmt: skip
 
     def set_batch(self, atom_offsets, grids=None):
         """Partition the atoms into independent systems (pgm_set_batch); grids (nsys, 3) or None."""
