@@ -157,6 +157,35 @@ def roofline(dom, live, P, C, N, precision, lanes, fused):
     }  # fmt: skip
 
 
+def gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, steps, warmup, log):
+    """The same workload with GVDW on (pgm_set_gvdw; Gaussian repulsion and dispersion on O, dispersion on H,
+    the intramolecular pairs skipped, rvdw = the cutoff): evals/s, the final pair pass (live HIP events), the GVDW
+    energy and the mixed-vs-FP64 force difference."""
+    n = sy.n
+    o = np.arange(0, n, 3)
+    sa, sc = np.zeros(n), np.zeros(n)
+    sa[o] = math.sqrt(422.0 * 4.184)
+    sc[o] = math.sqrt(594.825035 * 4.184e-6)
+    sc[o + 1] = sc[o + 2] = 0.3 * sc[o]
+    b = np.full(n, 0.9453)
+    si, sj = np.concatenate([o, o, o + 1]), np.concatenate([o + 1, o + 2, o + 2])
+    out, forces = {}, {}
+    for prec in ("mixed", "double"):
+        eng = E.Engine.for_system(sy, precision=prec)
+        eng.set_history(sets)
+        eng.freeze_history(True)
+        eng.set_gvdw(0, sy.settings["cutoff"], sa, sc, b, si, sj)
+        t, forces[prec], _ = eng.compute(sy.pos, sy.box)
+        (lanes, fused), _ = eng.autotune(3)
+        dt, live = timed_steps(torch, eng, pos_d, f_d, sy.box, steps, warmup, stream, "pair_final")
+        out[prec] = {"evals_per_s": round(steps / dt, 3), "pair_final_ms": round(live["ms_per_eval"], 4),
+                     "gvdw_kj_mol": t["gvdw"], "lanes": lanes, "fused_setup": fused}  # fmt: skip
+        log(f"gvdw {prec}: {steps / dt:.2f} evals/s")
+    out["force_relrms_mixed_vs_fp64"] = E.rel_rms(forces["mixed"], forces["double"])
+    out["energy_rel_mixed_vs_fp64"] = out["mixed"]["gvdw_kj_mol"] / out["double"]["gvdw_kj_mol"] - 1
+    return out
+
+
 def snapshot_batch(S, nsnap=64):
     """The uneven snapshot batch as one engine's arrays: offsets, grids, boxes, per-atom parameters, covalent."""
     systems = [S.config3_snapshot(k) for k in range(nsnap)]
@@ -257,6 +286,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-snapshot adjoint batch")
+    ap.add_argument("--no-gvdw", action="store_true", help="skip the GVDW-on timing of the same workload")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--profile", type=int, default=0, metavar="N",
                     help="profiler pass: N evaluations of the workload per precision (each engine autotuned "
@@ -373,6 +403,9 @@ def main():
         out["accuracy"]["oracle_cg_iterations"] = it_o
     else:
         out["cpu_baseline"] = None
+    if not a.no_gvdw:
+        out["gvdw"] = gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)
+        out["gvdw"]["pair_final_ms_without"] = {p: round(phases[p]["pair_final"]["ms_per_eval"], 4) for p in phases}
     if not a.no_batch:
         out["adjoint_batch"] = adjoint_batch(torch, E, S, a.batch_reps, log)
     print(json.dumps(out))
