@@ -797,6 +797,8 @@ private:
     DevBuf gridD_, cplxD_, inflD_, twD_, bmodD_;
     DevBuf cgPartD_, partD_, resD_, cgD_, doneD_, histD_, maxBitsD_;
     DevBuf perAtomD_, chunkSumD_;
+    DevBuf fftGroupsD_;     // the merged transform launches' group table
+    FftLaunch fftL_{};
     // systems tables (batch)
     DevBuf sysOfD_, boxTabD_, boxTabFD_, pmeTabD_, brickSysD_, groupBoxD_, chunkStartD_, sysChunkD_;
     DevBuf recStartD_, recSysD_, qBackD_, wED_;
@@ -996,6 +998,38 @@ private:
                 recSys.push_back(G.members[k]);
             }
         }
+        // the merged transform launches: every group's share of each stage
+        std::vector<FftGroup> fg;
+        fftL_ = FftLaunch{};
+        int boxOff = 0;
+        for (Group& G : groups_) {
+            FftGroup f{};
+            f.g = G.g;
+            f.ngrid = (int) G.members.size();
+            f.realOff = G.realOff;
+            f.cplxOff = G.cplxOff;
+            f.twOff = (int) G.twOff;
+            f.partOff = G.partOff;
+            f.boxOff = boxOff;
+            int b[3];
+            size_t l[3];
+            if (mixed)
+                fftGroupShape<float>(f, b, l);
+            else
+                fftGroupShape<double>(f, b, l);
+            for (int k = 0; k < 3; k++) {
+                f.blk0[k] = fftL_.blocks[k];
+                fftL_.blocks[k] += b[k];
+                fftL_.lds[k] = std::max(fftL_.lds[k], l[k]);
+            }
+            if (b[2] != G.blocksPerGrid*f.ngrid)
+                throw std::logic_error("pgm_hip: transform block counts disagree");
+            boxOff += f.ngrid;
+            fg.push_back(f);
+        }
+        upload(fftGroupsD_, fg.data(), fg.size()*sizeof(FftGroup));
+        fftL_.groups = fftGroupsD_.as<FftGroup>();
+        fftL_.ngroups = (int) fg.size();
         brickTotal_ = 0;
         for (Sys& y : sys_) {
             y.brickOff = brickTotal_;
@@ -1338,14 +1372,9 @@ private:
         te();
         tb(kPhFft, it);
         const BoxT<double> one = makeBox<double>(sys_[0].H, sys_[0].recip);
-        int boxOff = 0;
-        for (const Group& G : groups_) {
-            const BoxT<double>* boxes = batch_ ? groupBoxD_.as<BoxT<double>>()+boxOff : nullptr;
-            launchPmeConvolve<T>(st_, G.g, (int) G.members.size(), grid+G.realOff, cplxD_.as<T2>()+G.cplxOff,
-                                 inflD_.as<T>()+G.cplxOff, twD_.as<T2>()+G.twOff, one, boxes, s_.beta, energyMode,
-                                 strain ? 1 : 0, partD_.as<double>()+10*(size_t) G.partOff, skip);
-            boxOff += (int) G.members.size();
-        }
+        launchPmeConvolve<T>(st_, fftL_, grid, cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), one,
+                             batch_ ? groupBoxD_.as<BoxT<double>>() : nullptr, s_.beta, energyMode, strain ? 1 : 0,
+                             partD_.as<double>(), skip);
         if (energyMode)
             launchRangeSums(st_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
                             kResCount, res+kResRec);

@@ -217,14 +217,35 @@ void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* p
                   const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, T* grid,
                   const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
                   const typename Vec4<T>::type* disp = nullptr);
-// ngrid stacked grids of one shape g (contiguous real and half-spectrum grids and influence functions): forward
-// transform, energy (and k-space strain) with the influence function, convolution, back transform.  boxes:
-// per stacked grid (nullptr: box for all).  partials: per block [nblk][10] (energy, strain 0..8) when energy !=
-// 0, the blocks of grid k being [k nblk/ngrid, (k+1) nblk/ngrid); returns nblk.
+// One grid shape's share of a merged transform launch: every shape of a batch goes through each stage (z, y,
+// x + convolution, y, z) in one launch; a block finds its shape by its block index.
+struct FftGroup {
+    PmeGrid g;
+    int ngrid;                   // stacked grids of this shape
+    long long realOff, cplxOff;  // first element of its real grids / complex grids and influence functions
+    int twOff;                   // its twiddles
+    int partOff;                 // first row of its x-pass partial sums
+    int boxOff;                  // its first system's box in the batch box table
+    int pz, zy, zx;              // lines per block of the z, y and x stages
+    int blk0[3];                 // its first block in the z, y and x launches
+};
+struct FftLaunch {
+    const FftGroup* groups;      // device table
+    int ngroups;
+    int blocks[3];               // z, y, x launch sizes
+    size_t lds[3];               // dynamic LDS of the z, y, x stages (the largest shape's)
+};
+// The shape of group g's stages (lines per block) and its blocks per stage, as the transforms use them.
 template<class T>
-int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Vec2<T>::type* cplx, const T* influence,
-                      const typename Vec2<T>::type* twiddles, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                      int energy, int strain, double* partials, const int* skip);
+void fftGroupShape(FftGroup& G, int blocks[3], size_t lds[3]);
+// Forward transforms, influence function (energy and strain partials when asked), inverse transforms of all
+// grids of all groups; real: the stacked real grids, cplx/influence/tw: the stacked complex arrays.  boxes: per
+// stacked grid (nullptr: box for all).  partials: [.][10] (energy, strain 0..8) per x block, the blocks of a
+// group's grid k being rows partOff + [k, k+1) x blocks per grid.
+template<class T>
+void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
+                       const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
+                       int energy, int strain, double* partials, const int* skip);
 // Blocks of the x pass per grid (the rows of its energy partials) when ngrid grids of shape g are stacked.
 template<class T>
 int convolveBlocksPerGrid(PmeGrid g, int ngrid);

@@ -339,23 +339,36 @@ __device__ __forceinline__ void loadTwiddles(typename Vec2<T>::type* dst, const 
         dst[i] = src[i];
 }
 
+// The group (grid shape) of this block in a merged launch of stage 0 (z), 1 (y) or 2 (x).
+__device__ __forceinline__ int fftGroupOf(const FftGroup* groups, int ngroups, int stage) {
+    int gi = 0;
+    for (int k = 1; k < ngroups; k++)
+        if ((int) blockIdx.x >= groups[k].blk0[stage])
+            gi = k;
+    return gi;
+}
+
 // z: two real lines per complex FFT, unpacked into their half spectra.
 template<class T, bool INVERSE>
-__global__ void __launch_bounds__(kBlock) kFftZ(PmeGrid g, int ngrid, T* __restrict__ real,
-                                                typename Vec2<T>::type* __restrict__ cplx,
-                                                const typename Vec2<T>::type* __restrict__ twAll, int pairsPerBlock,
-                                                const int* skip) {
+__global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ groups, int ngroups,
+                                                T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
+                                                const typename Vec2<T>::type* __restrict__ twAll, const int* skip) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
         return;
+    const FftGroup& grp = groups[fftGroupOf(groups, ngroups, 0)];
+    const PmeGrid& g = grp.g;
+    const int ngrid = grp.ngrid, pairsPerBlock = grp.pz, blk = (int) blockIdx.x-grp.blk0[0];
+    T* __restrict__ real = realAll+grp.realOff;
+    T2* __restrict__ cplx = cplxAll+grp.cplxOff;
     const FftPlan& p = g.plan[2];
     const int K = g.K[2], Kh = g.K3h, nlines = ngrid*g.K[0]*g.K[1];
     T2* tw = reinterpret_cast<T2*>(smemRaw);
     T2* b0 = tw+K;
     T2* b1 = b0+pairsPerBlock*K;
-    loadTwiddles<T>(tw, twAll+g.K[0]+g.K[1], K);
-    const int pair0 = blockIdx.x*pairsPerBlock;
+    loadTwiddles<T>(tw, twAll+grp.twOff+g.K[0]+g.K[1], K);
+    const int pair0 = blk*pairsPerBlock;
     const int npairs = min(pairsPerBlock, (nlines+1)/2-pair0);
     if (!INVERSE) {
         for (int i = threadIdx.x; i < npairs*K; i += blockDim.x) {
@@ -425,21 +438,25 @@ __global__ void __launch_bounds__(kBlock) kFftZ(PmeGrid g, int ngrid, T* __restr
 
 // y: lines of K2 for fixed (x, z), a tile of consecutive z per block.
 template<class T, bool INVERSE>
-__global__ void __launch_bounds__(kBlock) kFftY(PmeGrid g, typename Vec2<T>::type* __restrict__ cplx,
-                                                const typename Vec2<T>::type* __restrict__ twAll, int zt,
-                                                const int* skip) {
+__global__ void __launch_bounds__(kBlock) kFftY(const FftGroup* __restrict__ groups, int ngroups,
+                                                typename Vec2<T>::type* __restrict__ cplxAll,
+                                                const typename Vec2<T>::type* __restrict__ twAll, const int* skip) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
         return;
+    const FftGroup& grp = groups[fftGroupOf(groups, ngroups, 1)];
+    const PmeGrid& g = grp.g;
+    const int zt = grp.zy, blk = (int) blockIdx.x-grp.blk0[1];
+    T2* __restrict__ cplx = cplxAll+grp.cplxOff;
     const int K = g.K[1], Kh = g.K3h;
     const int ntile = (Kh+zt-1)/zt;
-    const int x = blockIdx.x/ntile, z0 = (blockIdx.x%ntile)*zt;
+    const int x = blk/ntile, z0 = (blk%ntile)*zt;
     const int nl = min(zt, Kh-z0);
     T2* tw = reinterpret_cast<T2*>(smemRaw);
     T2* b0 = tw+K;
     T2* b1 = b0+zt*K;
-    loadTwiddles<T>(tw, twAll+g.K[0], K);
+    loadTwiddles<T>(tw, twAll+grp.twOff+g.K[0], K);
     for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
         const int y = i/nl, l = i-y*nl;
         b0[l*K+y] = cplx[((size_t) x*K+y)*Kh+z0+l];
@@ -454,9 +471,10 @@ __global__ void __launch_bounds__(kBlock) kFftY(PmeGrid g, typename Vec2<T>::typ
 
 // x: forward, influence function (energy, strain), back; a tile of consecutive z for fixed y per block.
 template<class T>
-__global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>::type* __restrict__ cplxAll,
+__global__ void __launch_bounds__(kBlock) kFftXConv(const FftGroup* __restrict__ groups, int ngroups,
+                                                    typename Vec2<T>::type* __restrict__ cplxAll,
                                                     const T* __restrict__ inflAll,
-                                                    const typename Vec2<T>::type* __restrict__ twAll, int zt,
+                                                    const typename Vec2<T>::type* __restrict__ twAll,
                                                     BoxT<double> boxOne, const BoxT<double>* boxes, double beta,
                                                     int energy, int strain, double* __restrict__ partials,
                                                     const int* skip) {
@@ -465,20 +483,23 @@ __global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>:
     __shared__ double red[kBlock/64][10];
     if (skip && *skip)
         return;
+    const FftGroup& grp = groups[fftGroupOf(groups, ngroups, 2)];
+    const PmeGrid& g = grp.g;
+    const int zt = grp.zx, blk = (int) blockIdx.x-grp.blk0[2];
     const int K = g.K[0], K2 = g.K[1], K3 = g.K[2], Kh = g.K3h;
     const int ntile = (Kh+zt-1)/zt;
     // stacked grids of one shape: block -> (grid, y, z tile)
-    const int gi = blockIdx.x/(K2*ntile), lb = blockIdx.x-gi*(K2*ntile);
-    const size_t goff = (size_t) gi*K*K2*Kh;
+    const int gi = blk/(K2*ntile), lb = blk-gi*(K2*ntile);
+    const size_t goff = (size_t) grp.cplxOff+(size_t) gi*K*K2*Kh;
     typename Vec2<T>::type* __restrict__ cplx = cplxAll+goff;
     const T* __restrict__ infl = inflAll+goff;
-    const BoxT<double>& box = boxes ? boxes[gi] : boxOne;
+    const BoxT<double>& box = boxes ? boxes[grp.boxOff+gi] : boxOne;
     const int y = lb/ntile, z0 = (lb%ntile)*zt;
     const int nl = min(zt, Kh-z0);
     T2* tw = reinterpret_cast<T2*>(smemRaw);
     T2* b0 = tw+K;
     T2* b1 = b0+zt*K;
-    loadTwiddles<T>(tw, twAll, K);
+    loadTwiddles<T>(tw, twAll+grp.twOff, K);
     for (int i = threadIdx.x; i < nl*K; i += blockDim.x) {
         const int xx = i/nl, l = i-xx*nl;
         b0[l*K+xx] = cplx[((size_t) xx*K2+y)*Kh+z0+l];
@@ -533,7 +554,7 @@ __global__ void __launch_bounds__(kBlock) kFftXConv(PmeGrid g, typename Vec2<T>:
             if (threadIdx.x < nq)
                 for (int w = 0; w < kBlock/64; w++)
                     v += red[w][threadIdx.x];
-            partials[(size_t) blockIdx.x*10+threadIdx.x] = v;
+            partials[((size_t) grp.partOff+blk)*10+threadIdx.x] = v;
         }
     }
 }
@@ -872,29 +893,35 @@ void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* p
 }
 
 template<class T>
-int launchPmeConvolve(hipStream_t st, PmeGrid g, int ngrid, T* real, typename Vec2<T>::type* cplx, const T* influence,
-                      const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                      int energy, int strain, double* partials, const int* skip) {
+void fftGroupShape(FftGroup& G, int blocks[3], size_t lds[3]) {
     typedef typename Vec2<T>::type T2;
+    const PmeGrid& g = G.g;
+    const int ngrid = G.ngrid;
     const int nlines = ngrid*g.K[0]*g.K[1];
-    // z forward
-    const int pz = linesPerBlock<T>(g.K[2], (nlines+1)/2);
-    const int zBlocks = nblk((nlines+1)/2, pz);
-    const size_t ldsZ = (size_t) (g.K[2]+2*pz*g.K[2])*sizeof(T2);
-    kFftZ<T, false><<<zBlocks, kBlock, ldsZ, st>>>(g, ngrid, real, cplx, tw, pz, skip);
-    // y: the stacked grids are one grid of ngrid K0 x-planes
-    const int zy = linesPerBlock<T>(g.K[1], (long long) ngrid*g.K[0]*g.K3h);
-    const int yBlocks = ngrid*g.K[0]*((g.K3h+zy-1)/zy);
-    const size_t ldsY = (size_t) (g.K[1]+2*zy*g.K[1])*sizeof(T2);
-    kFftY<T, false><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
-    const int zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
-    const int xBlocks = ngrid*g.K[1]*((g.K3h+zx-1)/zx);
-    const size_t ldsX = (size_t) (g.K[0]+2*zx*g.K[0])*sizeof(T2);
-    kFftXConv<T><<<xBlocks, kBlock, ldsX, st>>>(g, cplx, influence, tw, zx, box, boxes, beta, energy, strain,
-                                                partials, skip);
-    kFftY<T, true><<<yBlocks, kBlock, ldsY, st>>>(g, cplx, tw, zy, skip);
-    kFftZ<T, true><<<zBlocks, kBlock, ldsZ, st>>>(g, ngrid, real, cplx, tw, pz, skip);
-    return xBlocks;
+    // z: pairs of real lines; y: the stacked grids are one grid of ngrid K0 x-planes; x: lines of K0
+    G.pz = linesPerBlock<T>(g.K[2], (nlines+1)/2);
+    blocks[0] = nblk((nlines+1)/2, G.pz);
+    lds[0] = (size_t) (g.K[2]+2*G.pz*g.K[2])*sizeof(T2);
+    G.zy = linesPerBlock<T>(g.K[1], (long long) ngrid*g.K[0]*g.K3h);
+    blocks[1] = ngrid*g.K[0]*((g.K3h+G.zy-1)/G.zy);
+    lds[1] = (size_t) (g.K[1]+2*G.zy*g.K[1])*sizeof(T2);
+    G.zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
+    blocks[2] = ngrid*g.K[1]*((g.K3h+G.zx-1)/G.zx);
+    lds[2] = (size_t) (g.K[0]+2*G.zx*g.K[0])*sizeof(T2);
+}
+
+template<class T>
+void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
+                       const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
+                       int energy, int strain, double* partials, const int* skip) {
+    const FftGroup* G = L.groups;
+    const int ng = L.ngroups;
+    kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
+    kFftY<T, false><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
+    kFftXConv<T><<<L.blocks[2], kBlock, L.lds[2], st>>>(G, ng, cplx, influence, tw, box, boxes, beta, energy, strain,
+                                                        partials, skip);
+    kFftY<T, true><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
+    kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
 }
 
 template<class T>
@@ -959,8 +986,10 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
                                   const Vec4<T>::type*, const Vec4<T>::type*);                                         \
     template void launchGatherTangent<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,         \
                                          const T*, const Vec4<T>::type*, double*);                                     \
-    template int launchPmeConvolve<T>(hipStream_t, PmeGrid, int, T*, Vec2<T>::type*, const T*, const Vec2<T>::type*,   \
-                                      BoxT<double>, const BoxT<double>*, double, int, int, double*, const int*);       \
+    template void fftGroupShape<T>(FftGroup&, int*, size_t*);                                                       \
+    template void launchPmeConvolve<T>(hipStream_t, const FftLaunch&, T*, Vec2<T>::type*, const T*,                    \
+                                       const Vec2<T>::type*, BoxT<double>, const BoxT<double>*, double, int, int,     \
+                                       double*, const int*);                                                          \
     template void launchGather<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,                \
                                   const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
