@@ -130,8 +130,8 @@ __device__ __forceinline__ void waveSync() {
 
 // Compaction and setup in one pass.  Each group of L lanes tests L candidates per step (one per lane) and
 // ballot-compacts those within rc, with their minimum-image displacement and charge, into its LDS buffer of 2L
-// entries; whenever L entries are buffered, all L lanes evaluate them (kernels, fields) and write (j, G1, G2)
-// as L contiguous entries of the atom's segment.  No compacted index list goes through HBM, the partner
+// entries; whenever a group of the wavefront has L entries buffered, every group evaluates up to L of its own
+// (kernels, fields) and writes (j, G1, G2) as contiguous entries of its atom's segment.  No compacted index list goes through HBM, the partner
 // positions are gathered once, and the transcendental part runs with every lane busy.
 template<class T, int L, bool DUAL, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
@@ -199,15 +199,19 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         }
         buffered += __popcll(m);
         waveSync();
-        if (buffered >= L) {
-            evaluate(g0+lane, start+kept+lane);
-            kept += L;
-            const int rest = buffered-L;
+        // when any group of the wavefront has L entries, every group flushes up to L: the wavefront evaluates
+        // once for all its groups instead of once per group
+        if (__ballot(lane == 0 && buffered >= L) != 0ull) {
+            const int nflush = min(buffered, L);
+            if (lane < nflush)
+                evaluate(g0+lane, start+kept+lane);
+            kept += nflush;
+            const int rest = buffered-nflush;
             int tj = 0;
             T4 tx;
             if (lane < rest) {
-                tj = bufJ[g0+L+lane];
-                tx = bufX[g0+L+lane];
+                tj = bufJ[g0+nflush+lane];
+                tx = bufX[g0+nflush+lane];
             }
             waveSync();
             if (lane < rest) {
