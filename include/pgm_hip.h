@@ -70,7 +70,7 @@ int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, c
  * b; zeros: none) for all pairs within rvdw except the skipped pairs (skip_i[k], skip_j[k]), each given once
  * (exclusions, and pairs with their own parameters: those and the dispersion tail are the caller's).
  * repulsion: 0 Gaussian, 1 Slater.  Its energy is pgm_terms.gvdw [kJ/mol]; its forces and strain are added to
- * the electrostatic ones.  After every pgm_set_atoms; rvdw = 0 turns it off.  Unlike the reference, rvdw may
+ * the electrostatic ones.  After every pgm_set_atoms and pgm_set_batch; rvdw = 0 turns it off.  Unlike the reference, rvdw may
  * not exceed the electrostatic cutoff (the pair list radius), and pgm_backward refuses an engine with GVDW on. */
 int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
                  const double* b, int nskip, const int* skip_i, const int* skip_j);

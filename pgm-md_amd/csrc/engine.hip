@@ -393,6 +393,7 @@ public:
                     sys_[s].grid[a] = grids ? grids[3*s+a] : s_.grid[a];
             }
             batch_ = k > 1;
+            gvdwOn_ = false;   // GVDW (its skipped pairs are checked per system) is set again after set_batch
             layoutDirty_ = true;
             haveList_ = false;
             haveResults_ = false;

@@ -133,3 +133,46 @@ def test_gvdw_errors():
     eng.compute(sy.pos, sy.box)
     with pytest.raises(E.PgmError, match="GVDW"):
         eng.backward(sy.pos, sy.box, np.zeros((sy.n, 3)), 1.0)
+
+
+def test_batch_with_gvdw_matches_single_systems():
+    """GVDW in a batch (per-atom parameters and skipped pairs concatenated with the atom offsets): every system's
+    GVDW energy, forces and other terms equal its single-system evaluation."""
+    import dataclasses
+
+    systems = [S.config3_snapshot(11), S.config3_snapshot(23), S.config3_snapshot(37)]
+    for sy in systems:
+        sy.settings.update(tol=1e-12, max_iter=2000, predictor=0, peek=0.0)
+    rvdw = 0.8
+
+    def gv(sy):
+        n = sy.n
+        o = np.arange(0, n, 3)
+        sa, sc = np.zeros(n), np.zeros(n)
+        sa[o] = np.sqrt(422.0 * KCAL)
+        sc[o] = np.sqrt(594.825035 * KCAL * 1e-6)
+        sc[o + 1] = sc[o + 2] = 0.3 * sc[o]
+        return sa, sc, np.full(n, 0.9453), np.concatenate([o, o, o + 1]), np.concatenate([o + 1, o + 2, o + 2])
+
+    off = np.concatenate([[0], np.cumsum([s.n for s in systems])]).astype(np.int32)
+    cat = lambda k: np.concatenate([getattr(s, k) for s in systems])  # noqa: E731
+    st = systems[0].settings
+    keys = ("cutoff", "beta", "order", "tol", "max_iter", "peek", "predictor", "ke", "skin", "polarizable")
+    eng = E.Engine(precision="double", grid=tuple(st["grid"]), **{k: st[k] for k in keys})
+    eng.set_atoms(cat("q"), cat("radius"), cat("alpha"))
+    eng.set_covalent(np.concatenate([s.cov_atom + off[i] for i, s in enumerate(systems)]),
+                     np.concatenate([s.cov_partner + off[i] for i, s in enumerate(systems)]), cat("cov_strength"))
+    eng.set_batch(off, [s.settings["grid"] for s in systems])
+    parts = [gv(s) for s in systems]
+    eng.set_gvdw(0, rvdw, *[np.concatenate([p[k] for p in parts]) for k in range(3)],
+                 np.concatenate([p[3] + off[i] for i, p in enumerate(parts)]),
+                 np.concatenate([p[4] + off[i] for i, p in enumerate(parts)]))  # fmt: skip
+    terms, f = eng.compute_batch(np.concatenate([s.pos for s in systems]), np.stack([s.box for s in systems]))
+    for k, sy in enumerate(systems):
+        single = E.Engine.for_system(dataclasses.replace(sy), precision="double")
+        single.set_gvdw(0, rvdw, *parts[k])
+        ts, fs, _ = single.compute(sy.pos, sy.box)
+        assert abs(terms[k]["gvdw"] / ts["gvdw"] - 1) < 1e-11, k
+        assert abs(E.total_energy(terms[k]) / E.total_energy(ts) - 1) < 1e-11, k
+        fk = f[off[k] : off[k + 1]]
+        assert np.abs(fk - fs).max() < 1e-8 * np.abs(fs).max(), k
