@@ -180,6 +180,10 @@ public:
             HIPCHK(hipEventCreateWithFlags(&evOut_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evSync_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evList_, hipEventDisableTiming));
+            HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&evFork_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evJoin_, hipEventDisableTiming));
+            pmeSt_ = st_;
         });
     }
     ~DeviceEngine() {
@@ -195,9 +199,13 @@ public:
             (void) hipEventDestroy(evIn_);
         if (evOut_)
             (void) hipEventDestroy(evOut_);
-        for (hipEvent_t ev : {evSync_, evList_})
+        if (st2_)
+            (void) hipStreamSynchronize(st2_);
+        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_})
             if (ev)
                 (void) hipEventDestroy(ev);
+        if (st2_)
+            (void) hipStreamDestroy(st2_);
         if (st_)
             (void) hipStreamDestroy(st_);
         if (prev >= 0)
@@ -700,6 +708,9 @@ private:
     pgm_settings s_;
     hipStream_t st_ = nullptr;
     hipEvent_t evIn_ = nullptr, evOut_ = nullptr, evSync_ = nullptr, evList_ = nullptr;
+    // the PME solves of an evaluation run on st2_ beside the pair passes on st_ (fork / join by events)
+    hipStream_t st2_ = nullptr, pmeSt_ = nullptr;
+    hipEvent_t evFork_ = nullptr, evJoin_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1;
     bool polarizable_ = false;
     std::vector<double> qH_, radH_, alphaH_, forcesH_;
@@ -1360,6 +1371,31 @@ private:
 
     // Spread + transforms (per grid shape) into grid (the stacked real grids): the convolved potential grids.
     // energyMode: per-system k-space energy (and strain) into res[s][kResRec..].
+    // PME beside the pair passes unless PME phases are being timed (their brackets are on the engine stream).
+    bool overlapPme() const {
+        constexpr unsigned pmeMask = (1u << kPhSpread) | (1u << kPhFft) | (1u << kPhGather);
+        return !(timing_ && (phaseMask_ & pmeMask));
+    }
+    // The PME stream of one evaluation, back to the engine stream when the scope ends (also on an error).
+    struct PmeStreamScope {
+        DeviceEngine& e;
+        PmeStreamScope(DeviceEngine& eng, bool overlap) : e(eng) { e.pmeSt_ = overlap ? e.st2_ : e.st_; }
+        ~PmeStreamScope() { e.pmeSt_ = e.st_; }
+    };
+    // st2_ starts after everything enqueued on st_ so far; st_ continues after everything enqueued on st2_.
+    void fork() {
+        if (pmeSt_ == st_)
+            return;
+        HIPCHK(hipEventRecord(evFork_, st_));
+        HIPCHK(hipStreamWaitEvent(st2_, evFork_, 0));
+    }
+    void join() {
+        if (pmeSt_ == st_)
+            return;
+        HIPCHK(hipEventRecord(evJoin_, st2_));
+        HIPCHK(hipStreamWaitEvent(st_, evJoin_, 0));
+    }
+
     template<class T>
     void pmeGrids(const typename Vec4<T>::type* dip, bool withQ, const typename Vec4<T>::type* posq, int energyMode,
                   bool strain, T* grid, const int* skip, double* res, int it,
@@ -1367,17 +1403,17 @@ private:
         typedef typename Vec2<T>::type T2;
         const PmeSel ps = pmeSel();
         tb(kPhSpread, it);
-        launchZero<T>(st_, grid, (size_t) realTotal_, skip);
-        launchSpread<T>(st_, ps, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
+        launchZero<T>(pmeSt_, grid, (size_t) realTotal_, skip);
+        launchSpread<T>(pmeSt_, ps, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
                         posq, withQ ? 1 : 0, grid, skip, dipDot, disp);
         te();
         tb(kPhFft, it);
         const BoxT<double> one = makeBox<double>(sys_[0].H, sys_[0].recip);
-        launchPmeConvolve<T>(st_, fftL_, grid, cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), one,
+        launchPmeConvolve<T>(pmeSt_, fftL_, grid, cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), one,
                              batch_ ? groupBoxD_.as<BoxT<double>>() : nullptr, s_.beta, energyMode, strain ? 1 : 0,
                              partD_.as<double>(), skip);
         if (energyMode)
-            launchRangeSums(st_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
+            launchRangeSums(pmeSt_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
                             kResCount, res+kResRec);
         te();
     }
@@ -1387,7 +1423,7 @@ private:
                   bool strain, double* field, double* force, const int* skip, double* res, int it = -1) {
         pmeGrids<T>(dip, withQ, posq, energyMode, strain, gridD_.as<T>(), skip, res, it);
         tb(kPhGather, it);
-        launchGather<T>(st_, pmeSel(), n_, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), gridD_.as<T>(), dip,
+        launchGather<T>(pmeSt_, pmeSel(), n_, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), gridD_.as<T>(), dip,
                         posq, force ? 1 : 0, field, force, skip);
         te();
     }
@@ -1395,11 +1431,13 @@ private:
     template<class T>
     void matvec(const typename Vec4<T>::type* posT, const typename Vec4<T>::type* dipT, const int* skip,
                 int it = -1) {
+        fork();
+        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
         tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                            g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_);
         te();
-        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
+        join();
     }
 
     // Jacobi-preconditioned CG on the polarizable rows of (alpha^-1 - T) x = b from the guess x, whose product
@@ -1608,15 +1646,18 @@ private:
         }
         // field of the charges and permanent dipoles (and T x0): compaction + pair setup (fused or as two
         // passes, the autotuner's choice) + PME
-        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
+        PmeStreamScope pmeScope(*this, overlapPme());
+        fork();
         pmeSolve<T>(permTD_.as<T4>(), true, pos, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
+        if (dual)   // the guess's reciprocal field (its direct field rides on the setup pass)
+            pmeSolve<T>(pTD_.as<T4>(), false, pos, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
+        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
+        join();
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
         // induced dipoles
         int pushSlot = -1;
         if (induce) {
-            if (dual)
-                pmeSolve<T>(pTD_.as<T4>(), false, pos, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
-            else {
+            if (!dual) {
                 tb(kPhCg);
                 launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
                             alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
@@ -1647,6 +1688,8 @@ private:
         }
         // final stage
         double* res = resD_.as<double>();
+        fork();
+        pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
         tb(kPhPairFinal);
         const GvdwArgs<T> ga = gvdwArgs<T>();
         launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(), nbD_.as<int>(),
@@ -1660,7 +1703,7 @@ private:
         if (ga.gv)
             systemSums(gvdwAtomD_.as<double>(), 1, res, kResGvdw);
         te();
-        pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
+        join();
         tb(kPhFinal);
         launchFinalCombine(st_, n, dD_.as<double>(), xD_.as<double>(), alphaSD_.as<double>(), edD_.as<double>(),
                            erD_.as<double>(), cself, etotD_.as<double>(), perAtomD_.as<double>());
