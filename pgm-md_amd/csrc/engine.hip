@@ -1431,12 +1431,14 @@ private:
     template<class T>
     void matvec(const typename Vec4<T>::type* posT, const typename Vec4<T>::type* dipT, const int* skip,
                 int it = -1) {
+        // the fork is recorded before the pair pass is enqueued (the PME stream waits for nothing later), and the
+        // long pair pass is enqueued first, so the host's PME launches do not delay it
         fork();
-        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
         tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                            g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_);
         te();
+        pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
         join();
     }
 
@@ -1648,10 +1650,10 @@ private:
         // passes, the autotuner's choice) + PME
         PmeStreamScope pmeScope(*this, overlapPme());
         fork();
+        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
         pmeSolve<T>(permTD_.as<T4>(), true, pos, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         if (dual)   // the guess's reciprocal field (its direct field rides on the setup pass)
             pmeSolve<T>(pTD_.as<T4>(), false, pos, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
-        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
         join();
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
         // induced dipoles
@@ -1689,7 +1691,6 @@ private:
         // final stage
         double* res = resD_.as<double>();
         fork();
-        pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
         tb(kPhPairFinal);
         const GvdwArgs<T> ga = gvdwArgs<T>();
         launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(), nbD_.as<int>(),
@@ -1703,6 +1704,7 @@ private:
         if (ga.gv)
             systemSums(gvdwAtomD_.as<double>(), 1, res, kResGvdw);
         te();
+        pmeSolve<T>(dTD_.as<T4>(), true, pos, 1, wantStrain, erD_.as<double>(), frecD_.as<double>(), nullptr, res);
         join();
         tb(kPhFinal);
         launchFinalCombine(st_, n, dD_.as<double>(), xD_.as<double>(), alphaSD_.as<double>(), edD_.as<double>(),
