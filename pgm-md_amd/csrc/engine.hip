@@ -1785,7 +1785,10 @@ private:
             upload(wED_, wE, sys_.size()*sizeof(double));
         // the potential / field of the charges and total dipoles (and their tangents for the displacement g and
         // the dipole change ddT): pair pass over the stored kernels, the PME grids and their tangent grids
+        // (the PME part on the second stream beside the pair pass, as in the forward evaluation)
+        PmeStreamScope pmeScope(*this, overlapPme());
         auto tangentPass = [&](const T4* ddT) {
+            fork();
             launchPairTangent<T>(st_, n, pos, rgT<T>(), (T) s_.beta, dTD_.as<T4>(), ddT, ddT ? gTD_.as<T4>() : nullptr,
                                  candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                                  g12D_.as<typename Vec2<T>::type>(), g03D_.as<typename Vec2<T>::type>(), bt,
@@ -1794,9 +1797,10 @@ private:
             if (ddT)
                 pmeGrids<T>(dTD_.as<T4>(), true, pos, 0, false, gridTD_.as<T>(), nullptr, nullptr, -1, ddT,
                             gTD_.as<T4>());
-            launchGatherTangent<T>(st_, pmeSel(), n, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
+            launchGatherTangent<T>(pmeSt_, pmeSel(), n, pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(),
                                    gridD_.as<T>(), ddT ? gridTD_.as<T>() : nullptr, ddT ? gTD_.as<T4>() : nullptr,
                                    pmeOutD_.as<double>());
+            join();
         };
         adjIterations_ = 0;
         adjResidual_ = 0;
