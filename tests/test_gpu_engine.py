@@ -226,3 +226,31 @@ def test_large_water_properties():
     assert abs(_energy(td) / eo - 1) < 1e-11
     assert rel_rms(fd, fo) < 1e-10
     assert rel_rms(fm, fo) < MIXED_FORCE_RELRMS
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_pme_beside_pair_passes_same_results(precision):
+    """The PME solves run on a second stream beside the pair passes, or on the engine stream while the PME phases
+    are timed. The kernels and their inputs are the same either way; only the order in which the spread's brick
+    tiles are added into the grid (global atomics) varies, from run to run in either mode, which moves FP64
+    results by ulps and FP32 grids by their rounding. A missing fork/join dependency would read half-written
+    fields: errors of order one, far above these bounds (FP64 1e-12, mixed 1e-6 relative)."""
+    sy = S.config0()
+    sy.settings.update(tol=1e-8, max_iter=200, predictor=0, peek=0.0)
+    g = np.random.default_rng(2).normal(0.0, 1e-3, (sy.n, 3))
+    out = []
+    for timed in (False, True, False):
+        eng = E.Engine.for_system(sy, precision=precision)
+        eng.set_timing(timed)
+        t, f, S3 = eng.compute(sy.pos, sy.box, strain=True)
+        gr = eng.backward(sy.pos, sy.box, g, 0.5)
+        out.append((t, f, S3, gr))
+    tol = 1e-12 if precision == "double" else 1e-6
+    t0, f0, S0, g0 = out[0]
+    for t, f, S3, gr in out[1:]:
+        for k in t:
+            assert abs(t[k] - t0[k]) <= tol * abs(t0[k]) + 1e-12, k
+        assert np.abs(f - f0).max() <= tol * np.abs(f0).max()
+        assert np.abs(S3 - S0).max() <= tol * np.abs(S0).max()
+        for k in gr:
+            assert np.abs(gr[k] - g0[k]).max() <= tol * np.abs(g0[k]).max(), k
