@@ -74,6 +74,14 @@ int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, c
  * not exceed the electrostatic cutoff (the pair list radius), and pgm_backward refuses an engine with GVDW on. */
 int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
                  const double* b, int nskip, const int* skip_i, const int* skip_j);
+/* The CG preconditioner, the reference CUDA engine's PGM_CUDA_PRECOND and PGM_CUDA_JACOBI_SELF options
+ * (core/cuda/pgm_cuda.cu:1753-1760, precondK :1526-1588): radius > 0 [nm, at most the cutoff] applies the
+ * short-range M = D + D T_s D, D = diag(alpha), T_s the direct-space dipole tensor of the pairs within radius
+ * (from the stored pair kernels; Jacobi everywhere when an atom has more than 48 of them); radius 0 is Jacobi,
+ * with jacobi_self != 0 alpha/(1 - alpha c_self).  The tolerance stays on max |alpha r|; the solution is the same,
+ * the iteration count changes.  Unlike the reference, which uses it only when the predicted guess rides on the
+ * pair setup, it applies to every solve, the adjoint's included.  Kept across pgm_set_atoms; default Jacobi. */
+int pgm_set_preconditioner(pgm_engine* e, double radius, int jacobi_self);
 /* setCovalent (core/cuda/pgm_cuda.h:67): p_atom += strength unit(r_partner - r_atom). */
 int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength);
 

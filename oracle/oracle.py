@@ -58,6 +58,9 @@ def load(kind: str = "port") -> C.CDLL:
     lib.orc_induced.argtypes = [C.c_void_p, _DP]
     lib.orc_permanent.argtypes = [C.c_void_p, _DP]
     lib.orc_iterations.argtypes = [C.c_void_p]
+    lib.orc_set_preconditioner.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    lib.orc_precondition.argtypes = [C.c_void_p, _DP, _DP]
+    lib.orc_preconditioner_on.argtypes = [C.c_void_p]
     lib.orc_residual.argtypes = [C.c_void_p]
     lib.orc_residual.restype = C.c_double
     lib.orc_strain.argtypes = [C.c_void_p, _DP]
@@ -198,6 +201,19 @@ class Oracle:
 
     def num_pairs(self) -> int:
         return self.lib.orc_num_pairs(self.h)
+
+    def set_preconditioner(self, radius: float, jacobi_self: bool = False):
+        """CG preconditioner: short-range M = D + D T_s D within radius (0: Jacobi); Jacobi with the self term."""
+        _check(self.lib, self.lib.orc_set_preconditioner(self.h, float(radius), int(bool(jacobi_self))))
+
+    def precondition(self, r):
+        """z = M r for a 3n vector r (after set_geometry or compute)."""
+        z = np.zeros((self.n, 3))
+        _check(self.lib, self.lib.orc_precondition(self.h, _f64(r, (self.n, 3)), z))
+        return z
+
+    def preconditioner_on(self) -> bool:
+        return self.lib.orc_preconditioner_on(self.h) == 1
 
 
 def pair_kernels(a, beta, r, nmax=4, kind="port"):

@@ -380,6 +380,15 @@ struct Engine {
         double G[4];
     };
     std::vector<Pair> pairs;
+    // CG preconditioner: the short-range one, M = D + D T_s D over the pairs within preRadius (0: Jacobi, M = D),
+    // at most kPreCap such pairs per atom or Jacobi everywhere (core/cuda/pgm_cuda.cu:162,1526-1588); Jacobi with
+    // the self term of the diagonal, alpha/(1 - alpha c_self) (:1461-1468,1758-1760).  Both are options of the
+    // reference's CUDA engine only; core/pgm_core.cpp is Jacobi.
+    static constexpr int kPreCap = 48;
+    double preRadius = 0;
+    bool jacobiSelf = false;
+    bool preOn = false;
+    std::vector<std::vector<int> > preList;   // per atom: its short-range pairs (indices into pairs)
     // PME
     std::vector<int> gidx[3];
     std::vector<double> th[3], dth[3], d2th[3];
@@ -699,8 +708,60 @@ struct Engine {
         return E;
     }
 
-    /* Jacobi-PCG with flexible Polak-Ribiere beta on (alpha^-1 - T) mu = E_perm; predictor, peek and
-     * history (core/pgm_core.cpp:297-401). */
+    // The short-range pairs per atom; the preconditioner is on when the radius is set and no atom has more than
+    // kPreCap of them (the pairs counted whatever the partners' polarizabilities, core/cuda/pgm_cuda.cu:1139-1154).
+    void buildPreconditioner() {
+        preOn = false;
+        preList.assign(n, {});
+        if (!(preRadius > 0))
+            return;
+        const double r2 = preRadius*preRadius;
+        for (size_t m = 0; m < pairs.size(); m++)
+            if (dot3(pairs[m].x, pairs[m].x) < r2) {
+                preList[pairs[m].i].push_back((int) m);
+                preList[pairs[m].j].push_back((int) m);
+            }
+        preOn = true;
+        for (const auto& l : preList)
+            if ((int) l.size() > kPreCap)
+                preOn = false;
+    }
+
+    /* z = M r: z_i = alpha_i (r_i + sum_j T_ij alpha_j r_j) over the short-range partners j with alpha_j > 0,
+     * T_ij u = -u G1 + (u.x) x G2 (core/cuda/pgm_cuda.cu:1536-1569); off: z_i = alpha_i r_i, or with the self
+     * term alpha_i/(1 - alpha_i c_self) r_i (:1467). */
+    void precondition(const std::vector<V3>& r, std::vector<V3>& z) const {
+        const double cs = s.periodic ? 4.0*s.beta*s.beta*s.beta/(3.0*kSqrtPi) : 0.0;
+        z.assign(n, V3{0, 0, 0});
+        for (int i = 0; i < n; i++) {
+            const double a = alpha[i];
+            if (!preOn) {
+                const double f = jacobiSelf ? a/(1.0-a*cs) : a;
+                for (int k = 0; k < 3; k++)
+                    z[i][k] = f*r[i][k];
+                continue;
+            }
+            V3 e = {0, 0, 0};
+            if (a > 0)
+                for (int m : preList[i]) {
+                    const Pair& pr = pairs[m];
+                    const int j = (pr.i == i ? pr.j : pr.i);
+                    const double aj = alpha[j];
+                    if (aj == 0)
+                        continue;
+                    const V3 u = {aj*r[j][0], aj*r[j][1], aj*r[j][2]};
+                    const double ux = dot3(u, pr.x);
+                    for (int k = 0; k < 3; k++)
+                        e[k] += -u[k]*pr.G[1]+ux*pr.x[k]*pr.G[2];
+                }
+            for (int k = 0; k < 3; k++)
+                z[i][k] = a*(r[i][k]+e[k]);
+        }
+    }
+
+    /* PCG with flexible Polak-Ribiere beta on (alpha^-1 - T) mu = E_perm (Jacobi unless a preconditioner is
+     * set; the tolerance is on max |alpha r| with every preconditioner, core/cuda/pgm_cuda.cu:1511); predictor,
+     * peek and history (core/pgm_core.cpp:297-401). */
     void solve() {
         const std::vector<double> zeroQ(n, 0.0);
         const std::vector<V3> b = field(q, perm);
@@ -753,26 +814,26 @@ struct Engine {
                     m = std::max(m, std::fabs(alpha[i]*r[i][k]));
             return m/norm;
         };
+        buildPreconditioner();
         const std::vector<V3> Ax = applyA(x);
         std::vector<V3> r(n), z(n), dir;
         for (int i = 0; i < n; i++)
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < 3; k++)
                 r[i][k] = (alpha[i] > 0 ? b[i][k]-Ax[i][k] : 0.0);
-                z[i][k] = alpha[i]*r[i][k];
-            }
+        precondition(r, z);
         dir = z;
         double rz = vdot(r, z), err = errOf(r);
         int it = 0;
         while (err > s.tol && it < s.maxIter) {
             const std::vector<V3> Ap = applyA(dir);
             const double step = rz/std::max(vdot(dir, Ap), 1e-300);
-            std::vector<V3> rn(n), zn(n);
+            std::vector<V3> rn(n), zn;
             for (int i = 0; i < n; i++)
                 for (int k = 0; k < 3; k++) {
                     x[i][k] += step*dir[i][k];
                     rn[i][k] = r[i][k]-step*Ap[i][k];
-                    zn[i][k] = alpha[i]*rn[i][k];
                 }
+            precondition(rn, zn);
             double bnum = 0;
             for (int i = 0; i < n; i++)
                 for (int k = 0; k < 3; k++)
@@ -1080,6 +1141,34 @@ int orc_field(void* h, const double* q, const double* d, double* out) {
 }
 
 long long orc_num_pairs(void* h) { return h ? (long long) static_cast<Engine*>(h)->pairs.size() : -1; }
+
+int orc_set_preconditioner(void* h, double radius, int jacobiSelf) {
+    return guarded([&] {
+        Engine* e = E(h);
+        if (!(radius >= 0) || (e->s.periodic && radius > e->s.cutoff))
+            throw std::invalid_argument("pgm_oracle: the preconditioner radius must be in [0, cutoff]");
+        e->preRadius = radius;
+        e->jacobiSelf = jacobiSelf != 0;
+    });
+}
+
+int orc_precondition(void* h, const double* r, double* z) {
+    return guarded([&] {
+        Engine* e = E(h);
+        if (e->pos.empty())
+            throw std::logic_error("pgm_oracle: precondition before set_geometry");
+        e->buildPreconditioner();
+        std::vector<V3> rv(e->n), zv;
+        for (int i = 0; i < e->n; i++)
+            rv[i] = V3{r[3*i], r[3*i+1], r[3*i+2]};
+        e->precondition(rv, zv);
+        for (int i = 0; i < e->n; i++)
+            for (int k = 0; k < 3; k++)
+                z[3*i+k] = zv[i][k];
+    });
+}
+
+int orc_preconditioner_on(void* h) { return h ? (int) static_cast<Engine*>(h)->preOn : -1; }
 
 int orc_erf_kernels(double a, double r, int nmax, double* B) {
     return guarded([&] {

@@ -58,6 +58,13 @@ int orc_set_geometry(void* h, const double* pos, const double* box9);
 /* Field [e/nm^2] of charges q (n) and dipoles d (3n) at every atom, after orc_set_geometry. */
 int orc_field(void* h, const double* q, const double* d, double* E);
 long long orc_num_pairs(void* h);
+/* CG preconditioner (options of the reference's CUDA engine, core/cuda/pgm_cuda.cu:1526-1588,1753-1760): the
+ * short-range M = D + D T_s D over the pairs within radius (0: Jacobi; off when an atom has more than 48 such
+ * pairs), Jacobi with the self term alpha/(1 - alpha c_self).  orc_precondition applies M (after set_geometry);
+ * orc_preconditioner_on: 1 when the short-range M was used (the last solve or orc_precondition). */
+int orc_set_preconditioner(void* h, double radius, int jacobi_self);
+int orc_precondition(void* h, const double* r, double* z);
+int orc_preconditioner_on(void* h);
 
 /* Numerics (core/pgm_math.h). */
 int orc_erf_kernels(double a, double r, int nmax, double* B);

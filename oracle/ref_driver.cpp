@@ -206,6 +206,17 @@ int orc_field(void* h, const double* q, const double* d, double* out) {
 
 long long orc_num_pairs(void*) { return -1; }   // pgmcore::Engine keeps its pairs private
 
+// The preconditioners are options of the reference's CUDA engine only; pgmcore::Engine is Jacobi.
+int orc_set_preconditioner(void*, double, int) {
+    g_err = "pgm_ref: pgmcore::Engine has no preconditioner option";
+    return -1;
+}
+int orc_precondition(void*, const double*, double*) {
+    g_err = "pgm_ref: pgmcore::Engine has no preconditioner option";
+    return -1;
+}
+int orc_preconditioner_on(void*) { return 0; }
+
 int orc_erf_kernels(double a, double r, int nmax, double* B) {
     return guarded([&] { pgmcore::erfKernels(a, r, nmax, B); });
 }

@@ -349,6 +349,18 @@ public:
         haveResults_ = false;
     }
 
+    // CG preconditioner (options of the reference's CUDA engine, core/cuda/pgm_cuda.cu:1753-1760): the short-range
+    // M = D + D T_s D over the pairs within radius (0: Jacobi), at most kPreCap per atom or Jacobi everywhere;
+    // Jacobi with the self term of the diagonal, alpha/(1 - alpha c_self).  Kept across setAtoms.
+    void setPreconditioner(double radius, bool jacobiSelf) {
+        if (!(radius >= 0) || radius > s_.cutoff)
+            throw std::invalid_argument("pgm_hip: the preconditioner radius must be in [0, cutoff]");
+        preRadius_ = radius;
+        jacobiSelf_ = jacobiSelf;
+        preReady_ = false;
+        haveResults_ = false;
+    }
+
     void setCovalent(int ncov, const int* atom, const int* partner, const double* strength) {
         if (n_ <= 0)
             throw std::logic_error("pgm_hip: setCovalent before setAtoms");
@@ -745,6 +757,11 @@ private:
     double rvdw_ = 0;
     int nSkip_ = 0;
     DevBuf gvInD_, gvTD_, skipStartD_, skipPartD_, gvdwAtomD_;
+    // CG preconditioner
+    double preRadius_ = 0;
+    bool jacobiSelf_ = false;
+    bool preReady_ = false;   // the short list of this evaluation's pairs is built (by run)
+    DevBuf srCountD_, srJD_, srGD_, srOverD_;
     unsigned phaseMask_ = ~0u;   // phases bracketed with events while timing (bit k: phase k)
     std::vector<hipEvent_t> evPool_;
     size_t evUsed_ = 0;
@@ -1448,12 +1465,23 @@ private:
     template<class T>
     int cgSolve(const typename Vec4<T>::type* posT, const double* b, double* x, int& lastIts, double& err) {
         typedef typename Vec4<T>::type T4;
+        typedef typename Vec2<T>::type T2;
         const int n = n_;
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        const Precond pc{preReady_, (!preReady_ && jacobiSelf_) ? cself : 0.0};
+        const BoxSel<T> bt = boxSel<T>();
+        auto precond = [&](int it, bool init) {
+            if (pc.pre)
+                launchPrecond<T>(st_, n, it, init, alphaSD_.as<double>(), rD_.as<double>(), apD_.as<double>(), posT,
+                                 bt, srCountD_.as<int>(), srJD_.as<int>(), srGD_.as<T2>(), srOverD_.as<int>(),
+                                 zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(),
+                                 doneD_.as<int>());
+        };
         tb(kPhCg);
         launchCgInit<T>(st_, n, alphaSD_.as<double>(), b, x, tdD_.as<double>(), trD_.as<double>(), cself, s_.tol,
                         s_.max_iter, rD_.as<double>(), zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(),
-                        cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+                        cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>(), pc);
+        precond(0, true);
         te();
         int it = 0;
         int chunk = std::max(1, lastIts);
@@ -1467,7 +1495,8 @@ private:
                 launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
                            trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
                 launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(), x,
-                               rD_.as<double>(), zD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
+                               rD_.as<double>(), zD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>(), pc);
+                precond(it, false);
                 launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(),
                                      pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
                 te();
@@ -1651,6 +1680,20 @@ private:
         PmeStreamScope pmeScope(*this, overlapPme());
         fork();
         pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
+        preReady_ = false;
+        if (induce && preRadius_ > 0) {   // the preconditioner's short list, from the stored kernels
+            srCountD_.reserve(n*sizeof(int));
+            srJD_.reserve((size_t) n*kPreCap*sizeof(int));
+            srGD_.reserve((size_t) n*kPreCap*sizeof(T2));
+            srOverD_.reserve(sizeof(int));
+            tb(kPhCg);
+            launchZero<int>(st_, srOverD_.as<int>(), 1, nullptr);
+            launchShortList<T>(st_, n, pos, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
+                               bt, (T) (preRadius_*preRadius_), srCountD_.as<int>(), srJD_.as<int>(),
+                               srGD_.as<T2>(), srOverD_.as<int>());
+            te();
+            preReady_ = true;
+        }
         pmeSolve<T>(permTD_.as<T4>(), true, pos, 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         if (dual)   // the guess's reciprocal field (its direct field rides on the setup pass)
             pmeSolve<T>(pTD_.as<T4>(), false, pos, 0, false, trD_.as<double>(), nullptr, nullptr, nullptr);
@@ -1983,6 +2026,10 @@ int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, c
 int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
                  const double* b, int nskip, const int* skip_i, const int* skip_j) {
     return guarded(e, [&](DeviceEngine& g) { g.setGvdw(repulsion, rvdw, sqrt_a, sqrt_c6, b, nskip, skip_i, skip_j); });
+}
+
+int pgm_set_preconditioner(pgm_engine* e, double radius, int jacobi_self) {
+    return guarded(e, [&](DeviceEngine& g) { g.setPreconditioner(radius, jacobi_self != 0); });
 }
 
 int pgm_set_covalent(pgm_engine* e, int ncov, const int* atom, const int* partner, const double* strength) {

@@ -101,6 +101,9 @@ void launchPrepAtoms(hipStream_t st, const PrepArgs& a);
 
 // -------------------------------------------------------------------------------------------- pair passes
 
+// Short-range pairs per atom kept for the CG preconditioner (core/cuda/pgm_cuda.cu:162).
+constexpr int kPreCap = 48;
+
 // Lanes per atom of the pair passes: 16, 32 or 64 (picked per engine by the autotuner, 16 by default).
 constexpr int kPairLaneChoices[3] = {16, 32, 64};
 
@@ -129,6 +132,12 @@ template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxSel<T> box, double* field, const int* skip, int lanes);
+// The CG preconditioner's short-range list: per atom, its compacted pairs within the radius (srJ partner, srG the
+// stored G1, G2; at most kPreCap, counted in srCount); *over set when an atom has more (cleared by the caller).
+template<class T>
+void launchShortList(hipStream_t st, int n, const typename Vec4<T>::type* pos, const int* candStart, const int* nb,
+                     const int* pj, const typename Vec2<T>::type* g12, BoxSel<T> box, T rpre2, int* srCount, int* srJ,
+                     typename Vec2<T>::type* srG, int* over);
 // The GVDW pair term of the final pass (core/gvdw_kernel.h, combination rule; gv == nullptr: none).
 template<class T>
 struct GvdwArgs {
@@ -296,14 +305,29 @@ void launchCombineB(hipStream_t st, int n, const double* ed, const double* er, c
 template<class T>
 void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type* out);
 int cgBlocks(int n);
+// The CG's preconditioner: Jacobi z = alpha/(1 - alpha cj) r (cj = 0, or the self coefficient), or, with a short
+// list (pre), the short-range z = alpha (r + T_s alpha r), computed by launchPrecond after launchCgInit and after
+// each launchCgUpdate.
+struct Precond {
+    bool pre;
+    double cj;
+};
 template<class T>
 void launchCgInit(hipStream_t st, int n, const double* alpha, const double* b, const double* x, const double* td,
                   const double* tr, double cself, double tol, int maxIter, double* r, double* z, double* p,
-                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done);
+                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, Precond pc);
 void launchCgAp(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* td, const double* tr,
                 double cself, double* ap, double* partials, const int* done);
 void launchCgUpdate(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* ap, double* x,
-                    double* r, double* z, double* partials, const int* done);
+                    double* r, double* z, double* partials, const int* done, Precond pc);
+// z = alpha (r + sum_j T_ij alpha_j r_j) over the short list (partners with alpha_j > 0; M = D when *over), its
+// sums into the CG partials: init (after launchCgInit): p = z, r.z; else (after launchCgUpdate of iteration it):
+// -step z.Ap and the new r.z (core/cuda/pgm_cuda.cu:1526-1588).
+template<class T>
+void launchPrecond(hipStream_t st, int n, int it, bool init, const double* alpha, const double* r, const double* ap,
+                   const typename Vec4<T>::type* pos, BoxSel<T> box, const int* srCount, const int* srJ,
+                   const typename Vec2<T>::type* srG, const int* over, double* z, double* p,
+                   typename Vec4<T>::type* pT, double* partials, const int* done);
 template<class T>
 void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, const double* z, double* p,
                        typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done);

@@ -714,6 +714,56 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
     }
 }
 
+// The short-range list of the CG preconditioner: one wavefront per atom over its compacted pairs keeps, in
+// pair order, those within the preconditioner radius (partner, G1, G2), at most kPreCap; an atom with more sets
+// *over, which turns the preconditioner off everywhere (the pairs count whatever the partners' polarizabilities;
+// core/cuda/pgm_cuda.cu:1139-1154,1178-1179).
+template<class T, bool BATCH>
+__global__ void __launch_bounds__(kPairBlock) kShortList(int n, const typename Vec4<T>::type* __restrict__ pos,
+                                                         const int* __restrict__ candStart,
+                                                         const int* __restrict__ nb, const int* __restrict__ pj,
+                                                         const typename Vec2<T>::type* __restrict__ g12,
+                                                         BoxSel<T> bs, T rpre2, int* __restrict__ srCount,
+                                                         int* __restrict__ srJ,
+                                                         typename Vec2<T>::type* __restrict__ srG, int* over) {
+    typedef typename Vec4<T>::type T4;
+    const int gid = blockIdx.x*blockDim.x+threadIdx.x;
+    const int s = gid/64, lane = gid%64;
+    if (s >= n)   // uniform per wavefront (64 lanes per atom, blocks of whole wavefronts)
+        return;
+    const BoxT<T> box = selectBox<BATCH>(bs, s);
+    const T4 pi = pos[s];
+    const int start = candStart[s], cnt = nb[s];
+    const unsigned long long below = (1ull << lane)-1ull;
+    int kept = 0;
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+        const int k = k0+lane;
+        bool in = false;
+        int j = 0;
+        if (k < cnt) {
+            j = pj[start+k];
+            const T4 pjv = pos[j];
+            T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+            minImage(x, y, z, box);
+            in = x*x+y*y+z*z < rpre2;
+        }
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            const int q = kept+__popcll(m & below);
+            if (q < kPreCap) {
+                srJ[(size_t) s*kPreCap+q] = j;
+                srG[(size_t) s*kPreCap+q] = g12[start+k];
+            }
+        }
+        kept += __popcll(m);
+    }
+    if (lane == 0) {
+        srCount[s] = min(kept, kPreCap);
+        if (kept > kPreCap)
+            atomicOr(over, 1);
+    }
+}
+
 inline int groupsBlocks(int n, int L) { return (int) (((long long) n*L+kPairBlock-1)/kPairBlock); }
 
 // Runs fn with the lane count (16, 32 or 64) and the batch flag as compile-time constants.
@@ -852,7 +902,22 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
                                                                                  candStart, nb, pj, g12, g03, box, out);
 }
 
+template<class T>
+void launchShortList(hipStream_t st, int n, const typename Vec4<T>::type* pos, const int* candStart, const int* nb,
+                     const int* pj, const typename Vec2<T>::type* g12, BoxSel<T> box, T rpre2, int* srCount, int* srJ,
+                     typename Vec2<T>::type* srG, int* over) {
+    const int blocks = groupsBlocks(n, 64);
+    if (box.table != nullptr)
+        kShortList<T, true><<<blocks, kPairBlock, 0, st>>>(n, pos, candStart, nb, pj, g12, box, rpre2, srCount, srJ,
+                                                          srG, over);
+    else
+        kShortList<T, false><<<blocks, kPairBlock, 0, st>>>(n, pos, candStart, nb, pj, g12, box, rpre2, srCount, srJ,
+                                                           srG, over);
+}
+
 #define PGM_INSTANTIATE_PAIRS(T)                                                                                      \
+    template void launchShortList<T>(hipStream_t, int, const Vec4<T>::type*, const int*, const int*, const int*,     \
+                                     const Vec2<T>::type*, BoxSel<T>, T, int*, int*, Vec2<T>::type*, int*);          \
     template void launchPairCompact<T>(hipStream_t, int, const Vec4<T>::type*, const int*, const int*, int*, int*,   \
                                        BoxSel<T>, T, int);                                                            \
     template void launchPairSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,         \

@@ -71,44 +71,51 @@ __global__ void kToVec4(int n, const double* v, typename Vec4<T>::type* out) {
     out[s] = o;
 }
 
-// r = b - A x (polarizable rows), z = alpha r, p = z; partials of sum |alpha b|, r.z, max |alpha r|.
+// r = b - A x (polarizable rows); partials of sum |alpha b| and max |alpha r|.  Jacobi (not pc.pre): z = p =
+// alpha/(1 - alpha cj) r and the partials of r.z; with the short-range preconditioner kPrecond<true> does that.
 template<class T>
 __global__ void __launch_bounds__(kCgBlock) kCgInit(int n, const double* alpha, const double* b, const double* x,
                                                     const double* td, const double* tr, double cself, double* r,
                                                     double* z, double* p, typename Vec4<T>::type* pT,
-                                                    double* partials) {
+                                                    double* partials, Precond pc) {
     __shared__ double smem[40];
     double nrm = 0, rz = 0, mx = 0;
     for (int s = blockIdx.x*blockDim.x+threadIdx.x; s < n; s += gridDim.x*blockDim.x) {
         const double a = alpha[s];
+        const double f = a/(1.0-a*pc.cj);
         double pv[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int k = 3*s+c;
             const double ax = (a > 0 ? x[k]/a-(td[k]+tr[k]+cself*x[k]) : 0.0);
             const double rv = (a > 0 ? b[k]-ax : 0.0);
-            const double zv = a*rv;
+            const double zv = f*rv;
             r[k] = rv;
-            z[k] = zv;
-            p[k] = zv;
-            pv[c] = zv;
             nrm += fabs(a*b[k]);
-            rz += rv*zv;
-            mx = fmax(mx, fabs(zv));
+            mx = fmax(mx, fabs(a*rv));   // the tolerance is on |alpha r| with every preconditioner
+            if (!pc.pre) {
+                z[k] = zv;
+                p[k] = zv;
+                rz += rv*zv;
+            }
+            pv[c] = zv;
         }
-        typename Vec4<T>::type o;
-        o.x = (T) pv[0];
-        o.y = (T) pv[1];
-        o.z = (T) pv[2];
-        o.w = T(0);
-        pT[s] = o;
+        if (!pc.pre) {
+            typename Vec4<T>::type o;
+            o.x = (T) pv[0];
+            o.y = (T) pv[1];
+            o.z = (T) pv[2];
+            o.w = T(0);
+            pT[s] = o;
+        }
     }
     nrm = blockSum(nrm, smem);
     rz = blockSum(rz, smem);
     mx = blockMax(mx, smem);
     if (threadIdx.x == 0) {
         partials[kPNorm*kCgMaxBlocks+blockIdx.x] = nrm;
-        partials[kPRz0*kCgMaxBlocks+blockIdx.x] = rz;
+        if (!pc.pre)
+            partials[kPRz0*kCgMaxBlocks+blockIdx.x] = rz;
         partials[kPMax*kCgMaxBlocks+blockIdx.x] = mx;
     }
 }
@@ -145,9 +152,11 @@ __global__ void __launch_bounds__(kCgBlock) kCgAp(int n, const double* alpha, co
         partials[kPpAp*kCgMaxBlocks+blockIdx.x] = pap;
 }
 
+// x += step p, r' = r - step Ap; partials of max |alpha r'| and, Jacobi (not pc.pre), with z' = alpha/(1 - alpha
+// cj) r', of z'.(r' - r) and r'.z' (with the short-range preconditioner kPrecond<false> does those).
 __global__ void __launch_bounds__(kCgBlock) kCgUpdate(int n, int it, const double* alpha, const double* p,
                                                       const double* ap, double* x, double* r, double* z,
-                                                      double* partials, const int* skip) {
+                                                      double* partials, const int* skip, Precond pc) {
     __shared__ double smem[40];
     if (*skip)
         return;
@@ -161,20 +170,108 @@ __global__ void __launch_bounds__(kCgBlock) kCgUpdate(int n, int it, const doubl
         x[k] += step*p[k];
         const double rold = r[k];
         const double rn = rold-step*ap[k];
-        const double zn = a*rn;
-        bnum += zn*(rn-rold);
-        rzn += rn*zn;
-        mx = fmax(mx, fabs(zn));
+        mx = fmax(mx, fabs(a*rn));
         r[k] = rn;
-        z[k] = zn;
+        if (!pc.pre) {
+            const double zn = a/(1.0-a*pc.cj)*rn;
+            bnum += zn*(rn-rold);
+            rzn += rn*zn;
+            z[k] = zn;
+        }
     }
     bnum = blockSum(bnum, smem);
     rzn = blockSum(rzn, smem);
     mx = blockMax(mx, smem);
     if (threadIdx.x == 0) {
-        partials[kPBnum*kCgMaxBlocks+blockIdx.x] = bnum;
-        partials[((it & 1) ? kPRz0 : kPRz1)*kCgMaxBlocks+blockIdx.x] = rzn;
+        if (!pc.pre) {
+            partials[kPBnum*kCgMaxBlocks+blockIdx.x] = bnum;
+            partials[((it & 1) ? kPRz0 : kPRz1)*kCgMaxBlocks+blockIdx.x] = rzn;
+        }
         partials[kPMax*kCgMaxBlocks+blockIdx.x] = mx;
+    }
+}
+
+// The short-range preconditioner, z_s = alpha_s (r_s + sum_j T_sj alpha_j r_j), T_sj u = -u G1 + (u.x) x G2 over
+// the atom's short list (partners with alpha_j > 0; *over: z_s = alpha_s r_s everywhere), FP64 sums of the
+// pair terms from the stored kernels and the positions of the pair type (core/cuda/pgm_cuda.cu:1533-1570).
+// INIT (after kCgInit): p = z and its pair-type copy, partials of r.z; else (after kCgUpdate of iteration it):
+// partials of -step z.Ap (= z.(r' - r)) and r.z.  One thread per atom, the CG kernels' grid.
+template<class T, bool INIT, bool BATCH>
+__global__ void __launch_bounds__(kCgBlock) kPrecond(int n, int it, const double* alpha, const double* r,
+                                                     const double* ap, const typename Vec4<T>::type* pos,
+                                                     BoxSel<T> bs, const int* srCount, const int* srJ,
+                                                     const typename Vec2<T>::type* srG, const int* over, double* z,
+                                                     double* p, typename Vec4<T>::type* pT, double* partials,
+                                                     const int* skip) {
+    typedef typename Vec4<T>::type T4;
+    typedef typename Vec2<T>::type T2;
+    __shared__ double smem[40];
+    if (!INIT && *skip)
+        return;
+    const int nb = gridDim.x;
+    double step = 0;
+    if (!INIT) {
+        const double pap = reduceRegion<false>(partials+kPpAp*kCgMaxBlocks, nb, smem);
+        const double rz = reduceRegion<false>(partials+((it & 1) ? kPRz1 : kPRz0)*kCgMaxBlocks, nb, smem);
+        step = rz/fmax(pap, 1e-300);
+    }
+    const bool useT = (*over == 0);
+    double rz = 0, bnum = 0;
+    for (int s = blockIdx.x*blockDim.x+threadIdx.x; s < n; s += gridDim.x*blockDim.x) {
+        const double a = alpha[s];
+        double e[3] = {0, 0, 0};
+        if (useT && a > 0) {
+            const BoxT<T> box = selectBox<BATCH>(bs, s);
+            const T4 pi = pos[s];
+            const int cnt = srCount[s];
+            for (int q = 0; q < cnt; q++) {
+                const int j = srJ[(size_t) s*kPreCap+q];
+                const double aj = alpha[j];
+                if (aj == 0)
+                    continue;
+                const T2 g = srG[(size_t) s*kPreCap+q];
+                const T4 pj = pos[j];
+                T x = pi.x-pj.x, y = pi.y-pj.y, zz = pi.z-pj.z;
+                minImage(x, y, zz, box);
+                const double xd[3] = {(double) x, (double) y, (double) zz};
+                const double u[3] = {aj*r[3*j], aj*r[3*j+1], aj*r[3*j+2]};
+                const double ux = u[0]*xd[0]+u[1]*xd[1]+u[2]*xd[2];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    e[c] += -u[c]*(double) g.x+ux*xd[c]*(double) g.y;
+            }
+        }
+        double zv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int k = 3*s+c;
+            zv[c] = a*(r[k]+e[c]);
+            z[k] = zv[c];
+            rz += r[k]*zv[c];
+            if (INIT)
+                p[k] = zv[c];
+            else
+                bnum -= step*zv[c]*ap[k];
+        }
+        if (INIT) {
+            T4 o;
+            o.x = (T) zv[0];
+            o.y = (T) zv[1];
+            o.z = (T) zv[2];
+            o.w = T(0);
+            pT[s] = o;
+        }
+    }
+    rz = blockSum(rz, smem);
+    if (!INIT)
+        bnum = blockSum(bnum, smem);
+    if (threadIdx.x == 0) {
+        if (INIT)
+            partials[kPRz0*kCgMaxBlocks+blockIdx.x] = rz;
+        else {
+            partials[kPBnum*kCgMaxBlocks+blockIdx.x] = bnum;
+            partials[((it & 1) ? kPRz0 : kPRz1)*kCgMaxBlocks+blockIdx.x] = rz;
+        }
     }
 }
 
@@ -411,9 +508,9 @@ void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type
 template<class T>
 void launchCgInit(hipStream_t st, int n, const double* alpha, const double* b, const double* x, const double* td,
                   const double* tr, double cself, double tol, int maxIter, double* r, double* z, double* p,
-                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done) {
+                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, Precond pc) {
     const int nb = cgBlocks(n);
-    kCgInit<T><<<nb, kCgBlock, 0, st>>>(n, alpha, b, x, td, tr, cself, r, z, p, pT, partials);
+    kCgInit<T><<<nb, kCgBlock, 0, st>>>(n, alpha, b, x, td, tr, cself, r, z, p, pT, partials, pc);
     kCgInitCheck<<<1, kCgBlock, 0, st>>>(n, nb, tol, maxIter, partials, cg, done);
 }
 
@@ -423,8 +520,25 @@ void launchCgAp(hipStream_t st, int n, int it, const double* alpha, const double
 }
 
 void launchCgUpdate(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* ap, double* x,
-                    double* r, double* z, double* partials, const int* done) {
-    kCgUpdate<<<cgBlocks(n), kCgBlock, 0, st>>>(n, it, alpha, p, ap, x, r, z, partials, done+it);
+                    double* r, double* z, double* partials, const int* done, Precond pc) {
+    kCgUpdate<<<cgBlocks(n), kCgBlock, 0, st>>>(n, it, alpha, p, ap, x, r, z, partials, done+it, pc);
+}
+
+template<class T>
+void launchPrecond(hipStream_t st, int n, int it, bool init, const double* alpha, const double* r, const double* ap,
+                   const typename Vec4<T>::type* pos, BoxSel<T> box, const int* srCount, const int* srJ,
+                   const typename Vec2<T>::type* srG, const int* over, double* z, double* p,
+                   typename Vec4<T>::type* pT, double* partials, const int* done) {
+    const int nb = cgBlocks(n);
+    auto go = [&](auto I, auto B) {
+        kPrecond<T, I(), B()><<<nb, kCgBlock, 0, st>>>(n, it, alpha, r, ap, pos, box, srCount, srJ, srG, over, z, p,
+                                                       pT, partials, done+it);
+    };
+    const bool b = box.table != nullptr;
+    if (init)
+        b ? go(std::true_type{}, std::true_type{}) : go(std::true_type{}, std::false_type{});
+    else
+        b ? go(std::false_type{}, std::true_type{}) : go(std::false_type{}, std::false_type{});
 }
 
 template<class T>
@@ -482,7 +596,10 @@ void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, doub
     template void launchToVec4<T>(hipStream_t, int, const double*, Vec4<T>::type*);                                 \
     template void launchCgInit<T>(hipStream_t, int, const double*, const double*, const double*, const double*,     \
                                   const double*, double, double, int, double*, double*, double*, Vec4<T>::type*,     \
-                                  double*, CgState*, int*);                                                          \
+                                  double*, CgState*, int*, Precond);                                                 \
+    template void launchPrecond<T>(hipStream_t, int, int, bool, const double*, const double*, const double*,         \
+                                   const Vec4<T>::type*, BoxSel<T>, const int*, const int*, const Vec2<T>::type*,     \
+                                   const int*, double*, double*, Vec4<T>::type*, double*, const int*);               \
     template void launchCgDirection<T>(hipStream_t, int, int, double, int, const double*, double*, Vec4<T>::type*,  \
                                        double*, CgState*, int*);                                                     \
     template void launchPost<T>(hipStream_t, int, double, const double*, const double*, double*, const double*,     \

@@ -65,6 +65,7 @@ EXPORTS = [
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
+    "pgm_set_preconditioner",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -135,6 +136,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_set_lanes.argtypes = [vp, C.c_int, C.c_int]
     lib.pgm_eval_erfc_f32.argtypes = [C.c_int, vp, vp]
     lib.pgm_set_gvdw.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp, C.c_int, vp, vp]
+    lib.pgm_set_preconditioner.argtypes = [vp, C.c_double, C.c_int]
     lib.pgm_eval_pair_kernels.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, vp]
     _ = dp
     _lib = lib
@@ -291,6 +293,11 @@ class Engine:
             raise PgmError("skip_i and skip_j differ in length")
         self._check(self.lib.pgm_set_gvdw(self.h, int(repulsion), float(rvdw), _ptr(sa), _ptr(sc), _ptr(bb), len(si),
                                           _ptr(si) if len(si) else None, _ptr(sj) if len(sj) else None))  # fmt: skip
+
+    def set_preconditioner(self, radius: float, jacobi_self: bool = False):
+        """CG preconditioner (pgm_set_preconditioner): short-range M = D + D T_s D within radius [nm] (0: Jacobi);
+        jacobi_self: Jacobi with the self term, alpha/(1 - alpha c_self)."""
+        self._check(self.lib.pgm_set_preconditioner(self.h, float(radius), int(bool(jacobi_self))))
 
     def set_batch(self, atom_offsets, grids=None):
         """Partition the atoms into independent systems (pgm_set_batch); grids (nsys, 3) or None."""
