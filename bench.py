@@ -15,7 +15,9 @@ pair pass + PME energy/forces, covalent chain rule, reductions.  The predictor h
 step solves the same problem from the same guess.  The pair list is built once (positions do not move).
 
 Prints one JSON line: value = mixed-precision evals/s; the FP64 path of the same engine, the force errors,
-the per-kernel-group timing, the roofline of the dominant kernel group and the CPU baseline ride along.
+the per-kernel-group timing, the roofline of the dominant kernel group and the CPU baseline ride along, and
+(unless --no-large) the 999,000-atom box: both paths' evals/s, device memory and the mixed path's error against
+the FP64 oracle's evaluation of that box.
 """
 
 from __future__ import annotations
@@ -186,6 +188,47 @@ def gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, steps, warmup, log):
     return out
 
 
+def large_box(torch, E, S, steps, warmup, log):
+    """The 999,000-atom box (333,000 waters, 21.54 nm, 216^3 PME, tau 1e-5) timed like the main workload (mu4
+    guess from 4 prior sets, history frozen) in both precisions, with the mixed path's error against the FP64
+    oracle's evaluation of the same box (tests/golden/water999k_oracle.npz, tau 1e-10: its 3,000-atom sample)."""
+    sy = S.config4()
+    sets = prior_dipole_sets(sy, E)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    pos_d = torch.from_numpy(sy.pos.copy()).to(dev)
+    f_d = torch.zeros((sy.n, 3), dtype=torch.float64, device=dev)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "water999k_oracle.npz"))
+    out = {"workload": "water999k: 333,000 rigid waters (999,000 atoms), 21.54 nm cubic box (liquid density), rc "
+                       "0.9 nm, PME 216^3 order 6, tau 1e-5, mu4 guess from 4 seeded prior dipole sets, peek 0.65",
+           "atoms": sy.n}  # fmt: skip
+    for prec in ("mixed", "double"):
+        free0 = torch.cuda.mem_get_info()[0]
+        eng = E.Engine.for_system(sy, precision=prec)
+        eng.set_history(sets)
+        eng.freeze_history(True)
+        t, f, _ = eng.compute(sy.pos, sy.box)
+        used = (free0 - torch.cuda.mem_get_info()[0]) / 2**30
+        (lanes, fused), _ = eng.autotune(3)
+        for _ in range(warmup):
+            eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        fs = g["forces_sample"]
+        err = float(np.sqrt(((f[g["sample"]] - fs) ** 2).sum() / (fs**2).sum()))
+        out[prec] = {"evals_per_s": round(steps / dt, 3), "cg_iterations": eng.iterations(), "device_gib": round(used, 2),
+                     "lanes": lanes, "fused_setup": fused, "force_relrms_vs_oracle_tau1e-10": err,
+                     "energy_rel_vs_oracle_tau1e-10": E.total_energy(t) / float(g["terms"].sum()) - 1}  # fmt: skip
+        log(f"999k {prec}: {steps / dt:.2f} evals/s, {used:.1f} GiB, force error {err:.2e}")
+        del eng
+    out["mixed_over_fp64"] = round(out["mixed"]["evals_per_s"] / out["double"]["evals_per_s"], 3)
+    return out
+
+
 def snapshot_batch(S, nsnap=64):
     """The uneven snapshot batch as one engine's arrays: offsets, grids, boxes, per-atom parameters, covalent."""
     systems = [S.config3_snapshot(k) for k in range(nsnap)]
@@ -288,6 +331,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-snapshot adjoint batch")
     ap.add_argument("--no-gvdw", action="store_true", help="skip the GVDW-on timing of the same workload")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-large", action="store_true", help="skip the 999,000-atom box")
     ap.add_argument("--profile", type=int, default=0, metavar="N",
                     help="profiler pass: N evaluations of the workload per precision (each engine autotuned "
                          "first) and one calibration copy of "
@@ -408,6 +452,11 @@ def main():
         out["gvdw"]["pair_final_ms_without"] = {p: round(phases[p]["pair_final"]["ms_per_eval"], 4) for p in phases}
     if not a.no_batch:
         out["adjoint_batch"] = adjoint_batch(torch, E, S, a.batch_reps, log)
+    if not a.no_large:
+        for prec in list(engines):
+            del engines[prec]
+        del tight
+        out["large_box"] = large_box(torch, E, S, max(5, a.steps // 5), 2, log)
     print(json.dumps(out))
 
 
