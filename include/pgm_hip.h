@@ -74,6 +74,14 @@ int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, c
  * not exceed the electrostatic cutoff (the pair list radius), and pgm_backward refuses an engine with GVDW on. */
 int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
                  const double* b, int nskip, const int* skip_i, const int* skip_j);
+/* Engine::setCandidates (core/cuda/pgm_cuda.h:68-73): the caller's pair candidates instead of the engine's own
+ * list, until the next pgm_set_atoms.  Segment s (nseg of them) holds the pairs (seg_i[s], cand_j[k]),
+ * k in [seg_begin[s], seg_begin[s+1]) (seg_begin: nseg + 1 entries), with x = r_i + t.box - r_j for the integer
+ * image t = seg_shift[3s..3s+2] (|t_k| <= 7); every pair at most once; pairs beyond the cutoff are skipped at
+ * every evaluation.  Positions are used as given (not wrapped into the box), the list is never rebuilt, and the
+ * engine keeps the caller's atom order.  One system of at most 2^20 atoms; not with pgm_set_batch or GVDW. */
+int pgm_set_candidates(pgm_engine* e, int nseg, const int* seg_i, const int* seg_shift, const int* seg_begin,
+                       const int* cand_j);
 /* The CG preconditioner, the reference CUDA engine's PGM_CUDA_PRECOND and PGM_CUDA_JACOBI_SELF options
  * (core/cuda/pgm_cuda.cu:1753-1760, precondK :1526-1588): radius > 0 [nm, at most the cutoff] applies the
  * short-range M = D + D T_s D, D = diag(alpha), T_s the direct-space dipole tensor of the pairs within radius

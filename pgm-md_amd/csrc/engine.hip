@@ -21,9 +21,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -286,6 +288,7 @@ public:
             upload(gidxD_, gi.data(), n*sizeof(int));
             haveList_ = false;   // parameters of the sorted arrays change: rebuild
             gvdwOn_ = false;     // GVDW is set again after every setAtoms
+            extList_ = false;    // and so is a caller's pair list
             if (covN_ < 0)
                 setCovalentImpl(0, nullptr, nullptr, nullptr);
             haveResults_ = false;
@@ -298,6 +301,8 @@ public:
                  const int* skipI, const int* skipJ) {
         if (n_ <= 0)
             throw std::logic_error("pgm_hip: setGvdw before setAtoms");
+        if (extList_ && rvdw > 0)
+            throw std::invalid_argument("pgm_hip: GVDW with a caller's pair list (setCandidates) is not supported");
         if (!(rvdw > 0)) {
             gvdwOn_ = false;
             return;
@@ -346,6 +351,64 @@ public:
         gvdwSlater_ = repulsion == 1;
         gvdwOn_ = true;
         gvdwReady_ = false;
+        haveResults_ = false;
+    }
+
+    // The caller's pair candidates instead of the engine's own list, until the next setAtoms
+    // (Engine::setCandidates, core/cuda/pgm_cuda.h:70-73, core/cuda/pgm_cuda.cu:2694-2737): segment s holds the
+    // pairs (segI[s], candJ[k]), k in [segBegin[s], segBegin[s+1]), with x = r_i + t.box - r_j,
+    // t = segShift[3s..3s+2] integer box vectors (|t_k| <= 7); every pair at most once; those beyond the cutoff
+    // are skipped at every evaluation.  Both directions go into per-atom segments in the caller's order (no
+    // spatial sort), each entry packing its image (extEntry).  Positions are then used as given, not wrapped.
+    void setCandidates(int nseg, const int* segI, const int* segShift, const int* segBegin, const int* candJ) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: setCandidates before setAtoms");
+        if (batch_)
+            throw std::invalid_argument("pgm_hip: setCandidates: one system only (not with set_batch)");
+        if (gvdwOn_)
+            throw std::invalid_argument("pgm_hip: GVDW with a caller's pair list (setCandidates) is not supported");
+        if (n_ > (int) kExtIndexMask+1)
+            throw std::invalid_argument("pgm_hip: setCandidates: at most 2^20 atoms");
+        if (nseg < 0 || (nseg > 0 && (!segI || !segShift || !segBegin)))
+            throw std::invalid_argument("pgm_hip: setCandidates: missing arrays");
+        const int n = n_;
+        std::vector<int> count(n+1, 0);
+        for (int sg = 0; sg < nseg; sg++) {
+            const int i = segI[sg];
+            if (i < 0 || i >= n)
+                throw std::invalid_argument("pgm_hip: setCandidates: atom index out of range");
+            if (segBegin[sg+1] < segBegin[sg] || segBegin[sg] < 0)
+                throw std::invalid_argument("pgm_hip: setCandidates: segment bounds not increasing");
+            for (int a = 0; a < 3; a++)
+                if (std::abs(segShift[3*sg+a]) > 7)
+                    throw std::invalid_argument("pgm_hip: setCandidates: periodic shift beyond 7 box vectors");
+            if (segBegin[sg+1] > segBegin[sg] && !candJ)
+                throw std::invalid_argument("pgm_hip: setCandidates: missing arrays");
+            for (int k = segBegin[sg]; k < segBegin[sg+1]; k++) {
+                const int j = candJ[k];
+                if (j < 0 || j >= n)
+                    throw std::invalid_argument("pgm_hip: setCandidates: atom index out of range");
+                if (j == i && segShift[3*sg] == 0 && segShift[3*sg+1] == 0 && segShift[3*sg+2] == 0)
+                    throw std::invalid_argument("pgm_hip: setCandidates: an atom paired with itself");
+                count[i+1]++;
+                count[j+1]++;
+            }
+        }
+        for (int i = 0; i < n; i++)
+            count[i+1] += count[i];
+        std::vector<int> fill(count.begin(), count.end()-1), cand(std::max(count[n], 1));
+        for (int sg = 0; sg < nseg; sg++) {
+            const int i = segI[sg], *t = segShift+3*sg;
+            for (int k = segBegin[sg]; k < segBegin[sg+1]; k++) {
+                const int j = candJ[k];
+                cand[fill[i]++] = extEntry(j, t[0], t[1], t[2]);
+                cand[fill[j]++] = extEntry(i, -t[0], -t[1], -t[2]);
+            }
+        }
+        extStartH_.swap(count);
+        extCandH_.swap(cand);
+        extList_ = true;
+        haveList_ = false;
         haveResults_ = false;
     }
 
@@ -414,6 +477,7 @@ public:
             }
             batch_ = k > 1;
             gvdwOn_ = false;   // GVDW (its skipped pairs are checked per system) is set again after set_batch
+            extList_ = false;
             layoutDirty_ = true;
             haveList_ = false;
             haveResults_ = false;
@@ -440,11 +504,11 @@ public:
             stagePositions(src, pos);
             // the list check's round trip overlaps the preparation kernels; a stale list (rare) is rebuilt
             // and the atoms prepared again in the new order
-            const bool checking = haveList_;
+            const bool checking = haveList_ && !extList_;   // a caller's list is the caller's to keep
             tb(kPhList);
             if (checking)
                 startListCheck();
-            else
+            else if (!haveList_)
                 buildList();
             te();
             const bool mixed = s_.precision == PGM_MIXED;
@@ -466,7 +530,7 @@ public:
                                 fcovD_.as<double>(), s_.ke, fDev, fixed, stride, fixedScale);
             if (fHost || keepForces) {
                 forcesOutD_.reserve(3*sizeof(double)*n_);
-                HIPCHK(hipMemsetAsync(forcesOutD_.p, 0, 3*sizeof(double)*n_, st_));
+                launchZero<double>(st_, forcesOutD_.as<double>(), 3*(size_t) n_, nullptr);
                 launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
                                 fcovD_.as<double>(), s_.ke, forcesOutD_.as<double>(), nullptr, 0, 0.0);
                 stageH_.reserve(3*sizeof(double)*n_);
@@ -757,6 +821,9 @@ private:
     double rvdw_ = 0;
     int nSkip_ = 0;
     DevBuf gvInD_, gvTD_, skipStartD_, skipPartD_, gvdwAtomD_;
+    // the caller's pair list (setCandidates): per-atom segments of packed entries, caller's order
+    bool extList_ = false;
+    std::vector<int> extStartH_, extCandH_;
     // CG preconditioner
     double preRadius_ = 0;
     bool jacobiSelf_ = false;
@@ -1155,6 +1222,7 @@ private:
     template<class T>
     BoxSel<T> boxSel() const {
         BoxSel<T> b{};
+        b.ext = extList_ ? 1 : 0;
         if (batch_) {
             b.table = sizeof(T) == sizeof(float) ? reinterpret_cast<const BoxT<T>*>(boxTabFD_.p)
                                                  : reinterpret_cast<const BoxT<T>*>(boxTabD_.p);
@@ -1256,7 +1324,46 @@ private:
         return 2.0*std::sqrt((double) d2)+boxMove > skin;
     }
 
+    // The caller's pair list in place of the engine's own: the caller's order kept (identity permutation), its
+    // per-atom segments uploaded as the candidates; no displacement checks follow.
+    void buildExternalList() {
+        const int n = n_;
+        std::vector<int> ident(n);
+        std::iota(ident.begin(), ident.end(), 0);
+        upload(orderD_, ident.data(), n*sizeof(int));
+        upload(rankD_, ident.data(), n*sizeof(int));
+        qSD_.reserve(n*sizeof(double));
+        alphaSD_.reserve(n*sizeof(double));
+        rgSD_.reserve(n*sizeof(double));
+        launchPermute(st_, n, orderD_.as<int>(), qD_.as<double>(), qSD_.as<double>());
+        launchPermute(st_, n, orderD_.as<int>(), alphaD_.as<double>(), alphaSD_.as<double>());
+        launchPermute(st_, n, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
+        posDD_.reserve(n*sizeof(double4));
+        launchGatherPositions(st_, n, orderD_.as<int>(), posDev_, qSD_.as<double>(), boxSel<double>(),
+                              posDD_.as<double4>(), nullptr);
+        const int total = extStartH_[n];
+        nCand_ = total;
+        upload(candStartD_, extStartH_.data(), (n+1)*sizeof(int));
+        const size_t cap = (size_t) total+64;
+        const size_t g2 = (s_.precision == PGM_MIXED ? sizeof(float2) : sizeof(double2));
+        candD_.reserve(cap*sizeof(int));
+        HIPCHK(hipMemcpyAsync(candD_.p, extCandH_.data(), std::max(total, 1)*sizeof(int), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        pjD_.reserve(cap*sizeof(int));
+        g12D_.reserve(cap*g2);
+        if (s_.precision == PGM_DOUBLE)
+            g03D_.reserve(cap*g2);
+        for (Sys& y : sys_)
+            std::memcpy(y.buildBox, y.H, sizeof(y.H));
+        haveList_ = true;
+        listBuilds_++;
+    }
+
     void buildList() {
+        if (extList_) {
+            buildExternalList();
+            return;
+        }
         const int n = n_;
         // cells at least rl/2 wide, +-2 stencil (all distinct cells when fewer than 5 per dimension), per system
         cellTotal_ = 0;
@@ -1290,7 +1397,7 @@ private:
         rankD_.reserve(n*sizeof(int));
         sortedCellD_.reserve(n*sizeof(int));
         subOfD_.reserve(n*sizeof(int));
-        HIPCHK(hipMemsetAsync(cellCountD_.p, 0, cellTotal_*sizeof(int), st_));
+        launchZero<int>(st_, cellCountD_.as<int>(), (size_t) cellTotal_, nullptr);
         for (const Sys& y : sys_) {
             const int ns = y.a1-y.a0;
             const BoxT<double> bd = makeBox<double>(y.H, y.recip);
@@ -1723,8 +1830,8 @@ private:
             te();
         }
         else {
-            HIPCHK(hipMemsetAsync(xD_.p, 0, v3, st_));
-            HIPCHK(hipMemsetAsync(rD_.p, 0, v3, st_));
+            launchZero<double>(st_, xD_.as<double>(), 3*(size_t) n, nullptr);
+            launchZero<double>(st_, rD_.as<double>(), 3*(size_t) n, nullptr);
             launchPost<T>(st_, n, 0.0, alphaSD_.as<double>(), rD_.as<double>(), xD_.as<double>(), permD_.as<double>(),
                           orderD_.as<int>(), gidxD_.as<int>(), nullptr, dD_.as<double>(), rgSD_.as<double>(),
                           dTD_.as<T4>());
@@ -2026,6 +2133,11 @@ int pgm_set_atoms(pgm_engine* e, int n, const double* q, const double* radius, c
 int pgm_set_gvdw(pgm_engine* e, int repulsion, double rvdw, const double* sqrt_a, const double* sqrt_c6,
                  const double* b, int nskip, const int* skip_i, const int* skip_j) {
     return guarded(e, [&](DeviceEngine& g) { g.setGvdw(repulsion, rvdw, sqrt_a, sqrt_c6, b, nskip, skip_i, skip_j); });
+}
+
+int pgm_set_candidates(pgm_engine* e, int nseg, const int* seg_i, const int* seg_shift, const int* seg_begin,
+                       const int* cand_j) {
+    return guarded(e, [&](DeviceEngine& g) { g.setCandidates(nseg, seg_i, seg_shift, seg_begin, cand_j); });
 }
 
 int pgm_set_preconditioner(pgm_engine* e, double radius, int jacobi_self) {

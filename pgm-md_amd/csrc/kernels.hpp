@@ -13,11 +13,14 @@ namespace pgmhip {
 // The box of an atom.  A batch packs several independent periodic systems into one set of arrays: the atoms
 // of a system are contiguous, with the same index range in the caller's and in the sorted order, so one
 // system index per atom index serves both orders.  table == nullptr: a single system with box.
+// ext != 0: the pair list is the caller's (pgm_set_candidates), whose entries carry their periodic images;
+// positions are then used as given, not wrapped into the box.
 template<class T>
 struct BoxSel {
     BoxT<T> box;
     const BoxT<T>* table;
     const int* sysOf;
+    int ext;
 };
 
 template<bool BATCH, class T>
@@ -25,6 +28,35 @@ __device__ __forceinline__ BoxT<T> selectBox(const BoxSel<T>& b, int idx) {
     if (BATCH)
         return b.table[b.sysOf[idx]];
     return b.box;
+}
+
+// An entry of a caller's pair list: partner j in the low kExtIndexBits bits, the image t (|t_k| <= 7 box
+// vectors) in the 12 above, code = (t0 + 8) | (t1 + 8) << 4 | (t2 + 8) << 8, for x = r_i - r_j + t.box
+// (core/cuda/pgm_cuda.cu:2699-2713 packs the shifts alike).  Entries of the engine's own list are plain j.
+constexpr int kExtIndexBits = 20;
+constexpr unsigned kExtIndexMask = (1u << kExtIndexBits)-1u;
+
+__host__ __device__ __forceinline__ int extEntry(int j, int t0, int t1, int t2) {
+    return (int) ((unsigned) j | ((unsigned) (t0+8) | (unsigned) (t1+8) << 4 | (unsigned) (t2+8) << 8)
+                                     << kExtIndexBits);
+}
+
+__device__ __forceinline__ int entryAtom(int v, int ext) {
+    return ext ? (int) ((unsigned) v & kExtIndexMask) : v;
+}
+
+// x = r_i - r_j of a list entry v: the minimum image, or, in a caller's list, plus the entry's image.
+template<class T>
+__device__ __forceinline__ void pairImage(T& x, T& y, T& z, const BoxT<T>& b, int v, int ext) {
+    if (ext) {
+        const unsigned c = (unsigned) v >> kExtIndexBits;
+        const T t0 = T((int) (c & 15u)-8), t1 = T((int) ((c >> 4) & 15u)-8), t2 = T((int) ((c >> 8) & 15u)-8);
+        x += t0*b.ax+t1*b.bx+t2*b.cx;
+        y += t1*b.by+t2*b.cy;
+        z += t2*b.cz;
+    }
+    else
+        minImage(x, y, z, b);
 }
 
 // -------------------------------------------------------------------------------------------- pair list

@@ -187,7 +187,8 @@ __global__ void kGatherPositions(int n, const int* order, const double* pos, con
     const BoxT<double> box = selectBox<BATCH>(bs, s);
     const int i = order[s];
     double x = pos[3*i], y = pos[3*i+1], z = pos[3*i+2];
-    wrapPosition(x, y, z, box);
+    if (!bs.ext)   // a caller's pair list carries images relative to the positions as given
+        wrapPosition(x, y, z, box);
     const double q = qS[s];
     posD[s] = make_double4(x, y, z, q);
     if (posF)
@@ -240,7 +241,8 @@ __global__ void kPrepAtoms(PrepArgs a) {
     const int i = a.order[s];
     const double xi = a.pos[3*i], yi = a.pos[3*i+1], zi = a.pos[3*i+2];
     double x = xi, y = yi, z = zi;
-    wrapPosition(x, y, z, box);
+    if (!a.box.ext)
+        wrapPosition(x, y, z, box);
     const double q = a.qS[s];
     a.posD[s] = make_double4(x, y, z, q);
     if (a.posF) {

@@ -50,10 +50,10 @@ __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec
         bool in = false;
         int j = 0;
         if (k < cnt) {
-            j = cand[start+k];
-            const T4 pjv = pos[j];
+            j = cand[start+k];   // the raw entry: a caller's list carries the image in it
+            const T4 pjv = pos[entryAtom(j, bs.ext)];
             T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-            minImage(x, y, z, box);
+            pairImage(x, y, z, box, j, bs.ext);
             in = x*x+y*y+z*z < rc2;
         }
         const unsigned long long m = (__ballot(in) >> shift) & gmask;
@@ -69,10 +69,10 @@ __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec
 template<class T, bool DUAL>
 __device__ __forceinline__ void setupTerm(const typename Vec4<T>::type& pi, T rgi, const typename Vec4<T>::type& pjv,
                                           T rgj, const typename Vec4<T>::type& d, const typename Vec4<T>::type& u,
-                                          const BoxT<T>& box, T beta, typename Vec2<T>::type& ga,
+                                          const BoxT<T>& box, int v, int ext, T beta, typename Vec2<T>::type& ga,
                                           typename Vec2<T>::type& gb, double* e, double* f) {
     T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-    minImage(x, y, z, box);
+    pairImage(x, y, z, box, v, ext);
     T G[4];
     if constexpr (sizeof(T) == sizeof(float))
         pairKernelsF<4>(rgi+rgj, beta, x*x+y*y+z*z, G);
@@ -94,7 +94,7 @@ __device__ __forceinline__ void setupTerm(const typename Vec4<T>::type& pi, T rg
     gb.y = G[3];
 }
 
-// setupTerm from the minimum-image displacement x = r_i - r_j and the partner's charge.
+// setupTerm from the displacement x = r_i - r_j (the pair's image) and the partner's charge.
 template<class T, bool DUAL>
 __device__ __forceinline__ void setupTermX(T x, T y, T z, T qj, T rgSum, const typename Vec4<T>::type& d,
                                            const typename Vec4<T>::type& u, T beta, typename Vec2<T>::type& ga,
@@ -164,13 +164,14 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
     double e[3] = {0, 0, 0}, f[3] = {0, 0, 0};
     int kept = 0, buffered = 0;
     auto evaluate = [&](int slot, int out) {
-        const int j = bufJ[slot];
+        const int v = bufJ[slot];   // raw entry (stored as is); j its atom
+        const int j = entryAtom(v, bs.ext);
         const T4 xq = bufX[slot];
         const T4 d = dip[j];
         const T4 u = DUAL ? dual[j] : d;
         T2 ga, gb;
         setupTermX<T, DUAL>(xq.x, xq.y, xq.z, xq.w, rgi+d.w, d, u, beta, ga, gb, e, f);
-        pj[out] = j;
+        pj[out] = v;
         g12[out] = ga;
         if (kStoreG03<T>)
             g03[out] = gb;
@@ -181,10 +182,10 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         int j = 0;
         T4 xq;
         if (k < cnt) {
-            j = cand[start+k];
-            const T4 pjv = pos[j];
+            j = cand[start+k];   // raw entry
+            const T4 pjv = pos[entryAtom(j, bs.ext)];
             T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-            minImage(x, y, z, box);
+            pairImage(x, y, z, box, j, bs.ext);
             in = x*x+y*y+z*z < rc2;
             xq.x = x;
             xq.y = y;
@@ -268,11 +269,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
     double e[3] = {0, 0, 0}, f[3] = {0, 0, 0};
     int k = lane;
     for (; k+(U-1)*L < cnt; k += U*L) {
-        int j[U];
+        int v[U], j[U];
         T4 pjv[U], d[U], u[U];
 #pragma unroll
-        for (int q = 0; q < U; q++)
-            j[q] = pj[start+k+q*L];
+        for (int q = 0; q < U; q++) {
+            v[q] = pj[start+k+q*L];
+            j[q] = entryAtom(v[q], bs.ext);
+        }
 #pragma unroll
         for (int q = 0; q < U; q++) {
             pjv[q] = pos[j[q]];
@@ -283,18 +286,18 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
 #pragma unroll
         for (int q = 0; q < U; q++) {
             T2 ga, gb;
-            setupTerm<T, DUAL>(pi, rgi, pjv[q], d[q].w, d[q], u[q], box, beta, ga, gb, e, f);
+            setupTerm<T, DUAL>(pi, rgi, pjv[q], d[q].w, d[q], u[q], box, v[q], bs.ext, beta, ga, gb, e, f);
             g12[start+k+q*L] = ga;
             if (kStoreG03<T>)
                 g03[start+k+q*L] = gb;
         }
     }
     for (; k < cnt; k += L) {
-        const int j = pj[start+k];
+        const int v = pj[start+k], j = entryAtom(v, bs.ext);
         T2 ga, gb;
         const T4 u = DUAL ? dual[j] : pos[j];
         const T4 dj = dip[j];
-        setupTerm<T, DUAL>(pi, rgi, pos[j], dj.w, dj, u, box, beta, ga, gb, e, f);
+        setupTerm<T, DUAL>(pi, rgi, pos[j], dj.w, dj, u, box, v, bs.ext, beta, ga, gb, e, f);
         g12[start+k] = ga;
         if (kStoreG03<T>)
             g03[start+k] = gb;
@@ -320,9 +323,10 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
 template<class T>
 __device__ __forceinline__ void fieldTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& pj,
                                           const typename Vec4<T>::type& d, const typename Vec2<T>::type& g,
-                                          const BoxT<T>& box, double& ex, double& ey, double& ez) {
+                                          const BoxT<T>& box, int v, int ext, double& ex, double& ey,
+                                          double& ez) {
     T x = pi.x-pj.x, y = pi.y-pj.y, z = pi.z-pj.z;
-    minImage(x, y, z, box);
+    pairImage(x, y, z, box, v, ext);
     const T c2 = (d.x*x+d.y*y+d.z*z)*g.y;
     ex += (double) (c2*x-d.x*g.x);
     ey += (double) (c2*y-d.y*g.x);
@@ -350,26 +354,27 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     double ex = 0, ey = 0, ez = 0;
     int k = lane;
     for (; k+(U-1)*L < cnt; k += U*L) {
-        int j[U];
+        int v[U];
         T2 g[U];
         T4 pjv[U], d[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            j[u] = pj[start+k+u*L];
+            v[u] = pj[start+k+u*L];
             g[u] = g12[start+k+u*L];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            pjv[u] = pos[j[u]];
-            d[u] = dip[j[u]];
+            const int j = entryAtom(v[u], bs.ext);
+            pjv[u] = pos[j];
+            d[u] = dip[j];
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
-            fieldTerm<T>(pi, pjv[u], d[u], g[u], box, ex, ey, ez);
+            fieldTerm<T>(pi, pjv[u], d[u], g[u], box, v[u], bs.ext, ex, ey, ez);
     }
     for (; k < cnt; k += L) {
-        const int j = pj[start+k];
-        fieldTerm<T>(pi, pos[j], dip[j], g12[start+k], box, ex, ey, ez);
+        const int v = pj[start+k], j = entryAtom(v, bs.ext);
+        fieldTerm<T>(pi, pos[j], dip[j], g12[start+k], box, v, bs.ext, ex, ey, ez);
     }
     ex = groupSum<L>(ex);
     ey = groupSum<L>(ey);
@@ -415,11 +420,11 @@ template<class T, bool STRAIN, bool GVDW>
 __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& di,
                                           const typename Vec4<T>::type& pjv, const typename Vec4<T>::type& dj,
                                           const typename Vec2<T>::type& ga, const typename Vec2<T>::type& gb,
-                                          T rgSum, T beta, const BoxT<T>& box, double* acc, double* f, double* e,
-                                          const typename Vec4<T>::type& gvi, const typename Vec4<T>::type& gvj,
-                                          const GvdwArgs<T>& gva, double& eg) {
+                                          T rgSum, T beta, const BoxT<T>& box, int v, int ext, double* acc, double* f,
+                                          double* e, const typename Vec4<T>::type& gvi,
+                                          const typename Vec4<T>::type& gvj, const GvdwArgs<T>& gva, double& eg) {
     T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-    minImage(x, y, z, box);
+    pairImage(x, y, z, box, v, ext);
     if (GVDW)
         gvdwTerm<T, STRAIN>(x, y, z, x*x+y*y+z*z, gvi, gvj, rgSum, gva.rvdw2, gva.slater != 0, gva.invKe, 1.0, eg,
                             acc, f);
@@ -492,13 +497,14 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
         double f[3] = {0, 0, 0}, e[3] = {0, 0, 0};
         int k = lane;
         for (; k+(U-1)*L < cnt; k += U*L) {
-            int j[U];
+            int v[U], j[U];
             T2 ga[U], gb[U];
             T4 pjv[U], dj[U], gvj[U];
             T rgj[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                j[u] = pj[start+k+u*L];
+                v[u] = pj[start+k+u*L];
+                j[u] = entryAtom(v[u], bs.ext);
                 ga[u] = g12[start+k+u*L];
                 if (kStoreG03<T>)
                     gb[u] = g03[start+k+u*L];
@@ -512,18 +518,18 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                finalTerm<T, STRAIN, GVDW>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, acc, f, e, gvi,
-                                           gvj[u], gva, eg);
+                finalTerm<T, STRAIN, GVDW>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, v[u], bs.ext,
+                                           acc, f, e, gvi, gvj[u], gva, eg);
         }
         for (; k < cnt; k += L) {
-            const int j = pj[start+k];
+            const int v = pj[start+k], j = entryAtom(v, bs.ext);
             T2 gb{};
             if (kStoreG03<T>)
                 gb = g03[start+k];
             const T4 dj = dip[j];
             const T4 gvj = GVDW ? gva.gv[j] : dj;
             finalTerm<T, STRAIN, GVDW>(pi, di, pos[j], dj, g12[start+k], gb, (kStoreG03<T> && !GVDW) ? T(0) : rgi+dj.w,
-                                       beta, box, acc, f, e, gvi, gvj, gva, eg);
+                                       beta, box, v, bs.ext, acc, f, e, gvi, gvj, gva, eg);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -636,13 +642,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairTangent(int n, const typename
     // phi, E[3], R, phi', E'[3], R'
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = lane; k < cnt; k += L) {
-        const int j = pj[start+k];
+        const int ent = pj[start+k], j = entryAtom(ent, bs.ext);
         const T2 ga = g12[start+k];
         const T G1 = ga.x, G2 = ga.y;
         const T4 pjv = pos[j];
         const T4 dj = dip[j];
         T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-        minImage(x, y, z, box);
+        pairImage(x, y, z, box, ent, bs.ext);
         T G0, G3;
         if constexpr (kStoreG03<T>) {
             const T2 gb = g03[start+k];
@@ -741,10 +747,10 @@ __global__ void __launch_bounds__(kPairBlock) kShortList(int n, const typename V
         bool in = false;
         int j = 0;
         if (k < cnt) {
-            j = pj[start+k];
-            const T4 pjv = pos[j];
+            j = pj[start+k];   // the raw entry, kept as is (the preconditioner decodes it)
+            const T4 pjv = pos[entryAtom(j, bs.ext)];
             T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
-            minImage(x, y, z, box);
+            pairImage(x, y, z, box, j, bs.ext);
             in = x*x+y*y+z*z < rpre2;
         }
         const unsigned long long m = __ballot(in);

@@ -225,14 +225,14 @@ __global__ void __launch_bounds__(kCgBlock) kPrecond(int n, int it, const double
             const T4 pi = pos[s];
             const int cnt = srCount[s];
             for (int q = 0; q < cnt; q++) {
-                const int j = srJ[(size_t) s*kPreCap+q];
+                const int v = srJ[(size_t) s*kPreCap+q], j = entryAtom(v, bs.ext);
                 const double aj = alpha[j];
                 if (aj == 0)
                     continue;
                 const T2 g = srG[(size_t) s*kPreCap+q];
                 const T4 pj = pos[j];
                 T x = pi.x-pj.x, y = pi.y-pj.y, zz = pi.z-pj.z;
-                minImage(x, y, zz, box);
+                pairImage(x, y, zz, box, v, bs.ext);
                 const double xd[3] = {(double) x, (double) y, (double) zz};
                 const double u[3] = {aj*r[3*j], aj*r[3*j+1], aj*r[3*j+2]};
                 const double ux = u[0]*xd[0]+u[1]*xd[1]+u[2]*xd[2];

@@ -65,7 +65,7 @@ EXPORTS = [
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
-    "pgm_set_preconditioner",
+    "pgm_set_preconditioner", "pgm_set_candidates",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -137,6 +137,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_eval_erfc_f32.argtypes = [C.c_int, vp, vp]
     lib.pgm_set_gvdw.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp, C.c_int, vp, vp]
     lib.pgm_set_preconditioner.argtypes = [vp, C.c_double, C.c_int]
+    lib.pgm_set_candidates.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.pgm_eval_pair_kernels.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, vp]
     _ = dp
     _lib = lib
@@ -293,6 +294,18 @@ class Engine:
             raise PgmError("skip_i and skip_j differ in length")
         self._check(self.lib.pgm_set_gvdw(self.h, int(repulsion), float(rvdw), _ptr(sa), _ptr(sc), _ptr(bb), len(si),
                                           _ptr(si) if len(si) else None, _ptr(sj) if len(sj) else None))  # fmt: skip
+
+    def set_candidates(self, seg_i, seg_shift, seg_begin, cand_j):
+        """The caller's pair list (pgm_set_candidates): segment s = pairs (seg_i[s], cand_j[k]),
+        k in [seg_begin[s], seg_begin[s+1]), image x = r_i + seg_shift[s] . box - r_j."""
+        si = np.ascontiguousarray(seg_i, np.int32)
+        sh = np.ascontiguousarray(seg_shift, np.int32).reshape(-1)
+        sb = np.ascontiguousarray(seg_begin, np.int32)
+        cj = np.ascontiguousarray(cand_j, np.int32)
+        if len(sb) != len(si) + 1 or len(sh) != 3 * len(si):
+            raise ValueError("seg_begin needs nseg + 1 entries and seg_shift 3 per segment")
+        self._check(self.lib.pgm_set_candidates(self.h, len(si), _ptr(si) if len(si) else None,
+                                                _ptr(sh) if len(sh) else None, _ptr(sb), _ptr(cj) if len(cj) else None))
 
     def set_preconditioner(self, radius: float, jacobi_self: bool = False):
         """CG preconditioner (pgm_set_preconditioner): short-range M = D + D T_s D within radius [nm] (0: Jacobi);
