@@ -91,8 +91,15 @@ public:
         check(pgm_set_atoms(e_, n, q, radius, alpha, globalIndex, numGlobal));
         n_ = n;
     }
+    void setGvdw(int repulsion, double rvdw, const double* sqrtA, const double* sqrtC6, const double* b, int nskip,
+                 const int* skipI, const int* skipJ) {
+        check(pgm_set_gvdw(e_, repulsion, rvdw, sqrtA, sqrtC6, b, nskip, skipI, skipJ));
+    }
     void setCovalent(int ncov, const int* atom, const int* partner, const double* strength) {
         check(pgm_set_covalent(e_, ncov, atom, partner, strength));
+    }
+    void setCandidates(int nseg, const int* segI, const int* segShift, const int* segBegin, const int* candJ) {
+        check(pgm_set_candidates(e_, nseg, segI, segShift, segBegin, candJ));
     }
     Terms compute(const double* pos, const double box[3][3], bool newStep, double* f, double* strain) {
         pgm_terms t;

@@ -113,6 +113,55 @@ int main() {
     const bool okM = rel < 5e-5 && std::fabs(t.total()/eo-1) < 1e-6;
     printf("Engine mixed computeF: force rel-RMS %.2e, energy rel %.2e (%s)\n", rel, t.total()/eo-1, okM ? "ok" : "FAIL");
     fails += !okM;
+    // pgmcuda-style engine in FP64 over the caller's pair list (setCandidates): every pair within rc + 0.05 nm at
+    // its shortest image, found among the 125 images t in [-2, 2]^3, one segment per pair
+    std::vector<int> segI, segShift, segBegin, candJ;
+    for (int i = 0; i < n; i++)
+        for (int j = i+1; j < n; j++) {
+            double best = 1e30;
+            int bt[3] = {0, 0, 0};
+            for (int t0 = -2; t0 <= 2; t0++)
+                for (int t1 = -2; t1 <= 2; t1++)
+                    for (int t2 = -2; t2 <= 2; t2++) {
+                        double r2 = 0;
+                        for (int k = 0; k < 3; k++) {
+                            const double x = pos[i][k]-pos[j][k]+t0*box[0][k]+t1*box[1][k]+t2*box[2][k];
+                            r2 += x*x;
+                        }
+                        if (r2 < best) {
+                            best = r2;
+                            bt[0] = t0;
+                            bt[1] = t1;
+                            bt[2] = t2;
+                        }
+                    }
+            if (best < (s.cutoff+0.05)*(s.cutoff+0.05)) {
+                segBegin.push_back((int) candJ.size());
+                segI.push_back(i);
+                segShift.insert(segShift.end(), {bt[0], bt[1], bt[2]});
+                candJ.push_back(j);
+            }
+        }
+    segBegin.push_back((int) candJ.size());
+    pgmhip::Settings d = g;
+    d.mixed = false;
+    d.tol = 1e-12;
+    d.maxIter = 1000;
+    d.peek = 0;
+    d.predictor = 0;
+    pgmhip::Engine ext(d);
+    ext.setAtoms(n, top.q.data(), top.radius.data(), top.alpha.data(), nullptr, n);
+    ext.setCovalent((int) top.covAtom.size(), top.covAtom.data(), top.covPartner.data(), top.covStrength.data());
+    ext.setCandidates((int) segI.size(), segI.data(), segShift.data(), segBegin.data(), candJ.data());
+    std::vector<double> fe(3*n, 0.0);
+    const pgmhip::Terms te = ext.compute(&pos[0][0], bb, true, fe.data(), nullptr);
+    double dfe = 0;
+    for (int k = 0; k < 3*n; k++)
+        dfe = std::max(dfe, std::fabs(fe[k]-fo[k]));
+    const bool okX = std::fabs(te.total()-eo) <= 1e-11*std::fabs(eo) && dfe <= 1e-9*fmax;
+    printf("Engine FP64 over setCandidates (%zu pairs): E %.10f oracle %.10f, max|dF| %.2e (%s)\n", candJ.size(),
+           te.total(), eo, dfe, okX ? "ok" : "FAIL");
+    fails += !okX;
     orc_destroy(o);
     printf(fails ? "FAILED\n" : "PASSED\n");
     return fails;
