@@ -115,6 +115,10 @@ int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int 
 int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                       int stride, double fixed_scale, void* stream);
 int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid);
+/* waitDevice (core/cuda/pgm_cuda.h:105-111): instead of pgm_finish_device, the caller's stream waits on the
+ * GPU for the forces of the last launch (no host wait).  This engine checks the pair list inside the launch,
+ * so no check is ever late (pgm_late_list_checks stays 0). */
+int pgm_wait_device(pgm_engine* e, void* stream);
 
 /* No counterpart in the reference: compute on FP64 device arrays, for callers that keep the system in HBM
  * (positions 3n doubles in the caller's order; forces added to f_dev, 3n doubles, may be NULL). */

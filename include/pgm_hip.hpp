@@ -131,6 +131,9 @@ public:
         terms = detail::fromC(t);
         return valid != 0;
     }
+    void waitDevice(void* stream) { check(pgm_wait_device(e_, stream)); }
+    // computeF copies the caller's array (nothing is registered): nothing to release.
+    void releasePositions() {}
     int iterations() const { return pgm_iterations(e_); }
     double residual() const { return pgm_residual(e_); }
     void inducedDipoles(double* mu) const { check(pgm_induced_dipoles(e_, mu)); }

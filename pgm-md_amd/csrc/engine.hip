@@ -692,6 +692,10 @@ public:
         return total/2;
     }
     long lateListChecks() const { return 0; }
+    // The caller's stream waits on the GPU for the last evaluation's outputs (no host wait).
+    void waitOn(hipStream_t stream) {
+        withDevice([&] { HIPCHK(hipStreamWaitEvent(stream, evOut_, 0)); });
+    }
     const std::vector<double>& forces() const { return forcesH_; }
 
     void dipoles(bool induced, double* out) {
@@ -2200,6 +2204,10 @@ int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
         if (valid)
             *valid = 1;
     });
+}
+
+int pgm_wait_device(pgm_engine* e, void* stream) {
+    return guarded(e, [&](DeviceEngine& g) { g.waitOn(static_cast<hipStream_t>(stream)); });
 }
 
 int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,

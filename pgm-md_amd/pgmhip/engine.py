@@ -65,7 +65,7 @@ EXPORTS = [
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
-    "pgm_set_preconditioner", "pgm_set_candidates",
+    "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -138,6 +138,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_set_gvdw.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp, C.c_int, vp, vp]
     lib.pgm_set_preconditioner.argtypes = [vp, C.c_double, C.c_int]
     lib.pgm_set_candidates.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    lib.pgm_wait_device.argtypes = [vp, vp]
     lib.pgm_eval_pair_kernels.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, vp]
     _ = dp
     _lib = lib
@@ -259,6 +260,23 @@ class Engine:
         )
         self._check(rc)
         return t.as_dict()
+
+    def launch_device(self, posq_ptr: int, box, new_step, fixed_ptr: int, stride: int, scale: float, stream: int = 0):
+        """pgm_launch_device: compute_device in two parts; finish_device or wait_device follows."""
+        box = _f64(box, (3, 3))
+        self._check(self.lib.pgm_launch_device(self.h, posq_ptr, _ptr(box), int(bool(new_step)), fixed_ptr, stride,
+                                               scale, stream))  # fmt: skip
+
+    def finish_device(self):
+        """pgm_finish_device: (terms, valid) of the last launch."""
+        t = Terms()
+        valid = C.c_int(0)
+        self._check(self.lib.pgm_finish_device(self.h, C.byref(t), C.byref(valid)))
+        return t.as_dict(), bool(valid.value)
+
+    def wait_device(self, stream: int):
+        """pgm_wait_device: the stream waits on the GPU for the last launch's forces."""
+        self._check(self.lib.pgm_wait_device(self.h, stream))
 
     def backward(self, pos, box, dl_dforces=None, dl_denergy=0.0) -> dict:
         """Gradients of a loss L(E, F) with respect to the per-atom parameters (pgm_backward).

@@ -182,6 +182,41 @@ def test_compute_f32_and_device_paths():
     assert np.abs(fd.cpu().numpy() - fo).max() <= 1e-9 * np.abs(fo).max()
 
 
+def test_launch_finish_and_wait():
+    """launchDevice + finishDevice, and launchDevice + waitDevice on another stream (core/cuda/pgm_cuda.h:95-111):
+    the forces of compute_device, read on the waiting stream without a host wait."""
+    torch = pytest.importorskip("torch")
+    sy = S.config0()
+    dev = torch.device("cuda:0")
+    posq = torch.zeros((sy.n, 4), dtype=torch.float32, device=dev)
+    posq[:, :3] = torch.from_numpy(sy.pos.astype(np.float32)).to(dev)
+    stride, scale = sy.n, 2.0**32
+    ref = E.Engine.for_system(sy, precision="mixed")
+    f0 = torch.zeros(3 * stride, dtype=torch.int64, device=dev)
+    t0 = ref.compute_device(posq.data_ptr(), sy.box, True, f0.data_ptr(), stride, scale,
+                            torch.cuda.current_stream().cuda_stream)  # fmt: skip
+    torch.cuda.synchronize()
+    eng = E.Engine.for_system(sy, precision="mixed")
+    f1 = torch.zeros_like(f0)
+    eng.launch_device(posq.data_ptr(), sy.box, True, f1.data_ptr(), stride, scale,
+                      torch.cuda.current_stream().cuda_stream)  # fmt: skip
+    t1, valid = eng.finish_device()
+    torch.cuda.synchronize()
+    assert valid and abs(_energy(t1) / _energy(t0) - 1) < 1e-8
+    side = torch.cuda.Stream()
+    f2 = torch.zeros_like(f0)
+    eng2 = E.Engine.for_system(sy, precision="mixed")
+    eng2.launch_device(posq.data_ptr(), sy.box, True, f2.data_ptr(), stride, scale,
+                       torch.cuda.current_stream().cuda_stream)  # fmt: skip
+    eng2.wait_device(side.cuda_stream)
+    with torch.cuda.stream(side):
+        f2d = f2.to(torch.float64) / scale  # ordered after the forces on the waiting stream
+    side.synchronize()
+    ref_f = f0.to(torch.float64) / scale
+    for f in (f1.to(torch.float64) / scale, f2d):
+        assert float(((f - ref_f) ** 2).sum().sqrt() / (ref_f**2).sum().sqrt()) < 1e-6
+
+
 def test_errors_are_reported():
     sy = S.config0()
     with pytest.raises(E.PgmError):
