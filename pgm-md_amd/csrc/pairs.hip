@@ -176,14 +176,25 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         if (kStoreG03<T>)
             g03[out] = gb;
     };
+    // software pipeline: the candidate index two chunks ahead and the position one chunk ahead are in flight
+    // while a chunk is tested (and a flush evaluated)
+    int jNext = (lane < cnt) ? cand[start+lane] : 0;
+    int jAfter = (L+lane < cnt) ? cand[start+L+lane] : 0;
+    T4 pNext;
+    if (lane < cnt)
+        pNext = pos[entryAtom(jNext, bs.ext)];
     for (int first = 0; first < cnt; first += L) {
         const int k = first+lane;
         bool in = false;
-        int j = 0;
+        const int j = jNext;
+        const T4 pjv = pNext;
+        jNext = jAfter;
+        if (k+L < cnt)
+            pNext = pos[entryAtom(jNext, bs.ext)];
+        if (k+2*L < cnt)
+            jAfter = cand[start+k+2*L];
         T4 xq;
         if (k < cnt) {
-            j = cand[start+k];   // raw entry
-            const T4 pjv = pos[entryAtom(j, bs.ext)];
             T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
             pairImage(x, y, z, box, j, bs.ext);
             in = x*x+y*y+z*z < rc2;
