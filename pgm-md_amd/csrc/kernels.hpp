@@ -208,10 +208,23 @@ void launchPairTangent(hipStream_t st, int n, const typename Vec4<T>::type* pos,
 
 // -------------------------------------------------------------------------------------------- PME
 
+// a / d for 0 <= a < 2^20 and 1 <= d < 2^12 by one multiply: floor(a * ceil(2^32 / d) / 2^32) is exact there
+// (the rounding of the reciprocal contributes a * (ceil - 2^32/d) / 2^32 < a * d / 2^32 < 1/d).  Replaces the
+// ~20-instruction integer divisions of the FFT index arithmetic.
+struct FastDiv {
+    unsigned long long magic;
+    __host__ __device__ __forceinline__ int div(int a) const {
+        return (int) (((unsigned long long) (unsigned) a*magic) >> 32);
+    }
+};
+inline FastDiv fastDiv(int d) { return FastDiv{((1ull << 32)+(unsigned long long) d-1)/(unsigned long long) d}; }
+
+// Stages of a mixed-radix FFT of length K; for each stage the divisions by m = K/R and by Ns = R_0...R_{s-1}.
 struct FftPlan {
     int K;
     int nstage;
     int radix[10];
+    FastDiv byM[10], byNs[10];
 };
 FftPlan makePlan(int K);
 

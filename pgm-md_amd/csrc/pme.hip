@@ -34,6 +34,14 @@ FftPlan makePlan(int K) {
     if (m != 1)
         throw std::runtime_error("pgm_hip: PME grid dimension " + std::to_string(K)+
                                  " must factor into 2, 3, 5 and 7");
+    if (K >= 4096)
+        throw std::runtime_error("pgm_hip: PME grid dimension " + std::to_string(K)+" above 4095");
+    int ns = 1;
+    for (int st = 0; st < p.nstage; st++) {
+        p.byM[st] = fastDiv(K/p.radix[st]);
+        p.byNs[st] = fastDiv(ns);
+        ns *= p.radix[st];
+    }
     return p;
 }
 
@@ -98,10 +106,11 @@ __device__ typename Vec2<T>::type* ldsFft(typename Vec2<T>::type* src, typename 
     int Ns = 1;
     for (int st = 0; st < p.nstage; st++) {
         const int R = p.radix[st], m = K/R, span = K/(Ns*R);
+        const FastDiv byM = p.byM[st], byNs = p.byNs[st];
         for (int idx = threadIdx.x; idx < nlines*m; idx += blockDim.x) {
-            const int l = idx/m, j = idx-l*m, kk = j%Ns;
+            const int l = byM.div(idx), j = idx-l*m, jq = byNs.div(j), kk = j-jq*Ns;
             const T2* in = src+l*K;
-            T2* out = dst+l*K+(j/Ns)*Ns*R+kk;
+            T2* out = dst+l*K+jq*Ns*R+kk;
             T2 v[7];
             for (int r = 0; r < R; r++) {
                 const T2 a = in[j+r*m];
