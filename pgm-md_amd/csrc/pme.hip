@@ -829,9 +829,11 @@ inline int nblk(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
 
 template<class T>
 int linesPerBlock(int K, long long lines) {
-    // LDS: twiddles + two line buffers within 48 KB; and enough blocks to fill 256 CUs twice over
+    // LDS: twiddles + two line buffers within 48 KB; and enough blocks to fill 256 CUs twice over.  The y and x
+    // passes tile a row of K3h half-spectrum lines: a cap at or above K3h keeps a row in one block (no short
+    // tail tile), and more lines per block keep more of the 256 threads busy in each radix stage.
     const int budget = (int) (48*1024/(2*sizeof(typename Vec2<T>::type)))-K;   // elements per buffer
-    const int byLds = std::max(1, std::min(16, budget/(2*K)));
+    const int byLds = std::max(1, std::min(64, budget/(2*K)));
     const int byFill = (int) std::max(1LL, lines/512);
     return std::min(byLds, byFill);
 }
