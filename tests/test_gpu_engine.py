@@ -2,7 +2,7 @@
 
 Tolerances: the FP64 path is held to the oracle at the level of FP64 round-off and solver tolerance (the
 pGM-JAX agreement of the reference, docs/openmm.md:102-111, is 1e-12 rel energy and 2e-10 kJ/mol/nm); the
-mixed path to the north-star bound of 5e-5 relative-RMS force error and 1e-6 relative energy.
+mixed path to its bound of 5e-5 relative-RMS force error and 1e-6 relative energy.
 """
 
 import numpy as np
