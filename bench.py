@@ -291,11 +291,19 @@ def adjoint_batch(torch, E, S, reps, log):
             step()
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
+        # one more step with every kernel group bracketed (in line, so not the timed figure): where a batch goes
+        eng.set_timing(True)
+        step()
+        torch.cuda.synchronize()
+        tab = eng.timing()
+        eng.set_timing(False)
+        phases = {k: round(v[0], 3) for k, v in tab.items() if v[1] > 0}
         gq = np.concatenate([g[k] for k in ("q", "radius", "alpha")])
         gq0 = np.concatenate([ref[1][k] for k in ("q", "radius", "alpha")])
         out[prec] = {
             "snapshot_gradients_per_s": round(64 / dt, 2), "ms_per_batch": round(1000 * dt, 3),
             "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "fused_setup": fused,
+            "phases_ms_per_batch_in_line": phases,
             "autotune_ms": {f"{k[0]}{'f' if k[1] else 's'}": round(v, 4) for k, v in tune.items()},
             "force_relrms_vs_fp64": E.rel_rms(f, ref[0]),
             "gradient_maxrel_vs_fp64": float(np.abs(gq - gq0).max() / np.abs(gq0).max()),
