@@ -542,10 +542,15 @@ public:
                 HIPCHK(hipMemcpyAsync(stageH_.p, forcesOutD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost, st_));
             }
             te();
+            // the evaluation's positions, for a backward at the same positions (which then skips evaluating)
+            lastPosD_.reserve(3*sizeof(double)*n_);
+            HIPCHK(hipMemcpyAsync(lastPosD_.p, posDev_, 3*sizeof(double)*n_, hipMemcpyDeviceToDevice, st_));
             const size_t resBytes = sys_.size()*kResCount*sizeof(double);
             resH_.reserve(resBytes);
             HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, resBytes, hipMemcpyDeviceToHost, st_));
             waitStream();
+            lastBoxes_.assign(boxes, boxes+9*sys_.size());
+            adjointReady_ = true;
             collectTiming();
             assemble(strainOut != nullptr);
             if (fHost) {
@@ -566,9 +571,51 @@ public:
         });
     }
 
+    // True when an evaluation finished at bitwise these positions and boxes and nothing has changed the engine's
+    // state since (parameters, settings, the pair passes' timing).  Compares on the device, ordered after the
+    // caller's stream; one small read back.
+    bool sameAsLastEvaluation(PosSource src, const void* pos, const double* boxes, hipStream_t caller) {
+        if (!haveResults_ || !adjointReady_ || lastBoxes_.size() != 9*sys_.size() ||
+            !std::equal(lastBoxes_.begin(), lastBoxes_.end(), boxes))
+            return false;
+        int differs = 1;
+        withDevice([&] {
+            HIPCHK(hipEventRecord(evIn_, caller));
+            HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
+            const size_t bytes = 3*sizeof(double)*n_;
+            const double* cand = nullptr;
+            if (src == PosSource::DeviceF64)
+                cand = static_cast<const double*>(pos);
+            else {
+                cmpPosD_.reserve(bytes);
+                if (src == PosSource::DeviceF4)
+                    launchF4ToDouble(st_, n_, static_cast<const float4*>(pos), cmpPosD_.as<double>());
+                else {
+                    stageH_.reserve(bytes);
+                    double* d = stageH_.as<double>();
+                    if (src == PosSource::HostF64)
+                        std::memcpy(d, pos, bytes);
+                    else
+                        for (int k = 0; k < 3*n_; k++)
+                            d[k] = static_cast<const float*>(pos)[k];
+                    HIPCHK(hipMemcpyAsync(cmpPosD_.p, stageH_.p, bytes, hipMemcpyHostToDevice, st_));
+                }
+                cand = cmpPosD_.as<double>();
+            }
+            cmpFlagD_.reserve(sizeof(int));
+            launchZero<int>(st_, cmpFlagD_.as<int>(), 1, nullptr);
+            launchDiffers(st_, 3LL*n_, cand, lastPosD_.as<double>(), cmpFlagD_.as<int>());
+            cmpH_.reserve(64);
+            HIPCHK(hipMemcpyAsync(cmpH_.p, cmpFlagD_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+            waitStream();
+            std::memcpy(&differs, cmpH_.p, sizeof(int));
+        });
+        return differs == 0;
+    }
+
     // Parameter gradients of a loss L(E, F) at the given positions: evaluates there (same step: the last solution
-    // as the guess), then the adjoint.  g: dL/dF (caller's order, host or device, may be null), wE: dL/dE per
-    // system.
+    // as the guess) unless the last evaluation was at exactly these positions and boxes with nothing changed
+    // since, then the adjoint.  g: dL/dF (caller's order, host or device, may be null), wE: dL/dE per system.
     void backward(PosSource src, const void* pos, const double* boxes, const double* g, bool gDevice,
                   const double* wE, double* gq, double* gR, double* galpha, double* gc, bool outDevice,
                   hipStream_t caller) {
@@ -576,7 +623,11 @@ public:
             throw std::logic_error("pgm_hip: backward before setAtoms");
         if (gvdwOn_)
             throw std::invalid_argument("pgm_hip: backward covers the electrostatics only: set GVDW off (rvdw 0)");
-        evaluate(src, pos, boxes, false, nullptr, nullptr, nullptr, 0, 0.0, nullptr, nullptr, caller, false);
+        if (sameAsLastEvaluation(src, pos, boxes, caller))
+            posDev_ = lastPosD_.as<double>();   // the adjoint reads the evaluation's positions from the engine's copy
+        else
+            evaluate(src, pos, boxes, false, nullptr, nullptr, nullptr, 0, 0.0, nullptr, nullptr, caller, false);
+        adjointReady_ = false;   // the adjoint's solve reuses the CG buffers: a next backward evaluates again
         withDevice([&] {
             HIPCHK(hipEventRecord(evIn_, caller));
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
@@ -913,7 +964,11 @@ private:
     DevBuf pairOutD_, pmeOutD_, gridTD_, gSD_, gTD_, pdotD_, lamD_, adjBD_, ddotD_, ddotTD_, etotDotD_, gInD_, gradD_;
     int adjIterations_ = 0, lastAdjIterations_ = 0;
     double adjResidual_ = 0;
-    HostBuf resH_, stageH_, scalH_, listH_;
+    HostBuf resH_, stageH_, scalH_, listH_, cmpH_;
+    // the last evaluation's positions and boxes, for a backward at the same positions
+    DevBuf lastPosD_, cmpPosD_, cmpFlagD_;
+    std::vector<double> lastBoxes_;
+    bool adjointReady_ = false;   // the stored state is that evaluation's (cleared by the pair passes' timing)
 
     void validate() {
         const pgm_settings& s = s_;
@@ -1906,6 +1961,7 @@ private:
         hipEvent_t a = nextEvent(), b = nextEvent();
         const bool wasTiming = timing_;
         timing_ = false;
+        adjointReady_ = false;   // the timed passes overwrite the evaluation's pair-pass outputs
         auto once = [&] {
             pairSetup<T>(lanes, fused, pTD_.as<T4>());
             for (int k = 0; k <= iterations_; k++)

@@ -97,6 +97,8 @@ void launchPermute(hipStream_t st, int n, const int* order, const double* in, do
 void launchToFloat(hipStream_t st, int n, const double* in, float* out);
 // Caller-order float4 (xyz, w ignored) or float xyz positions to FP64 xyz.
 void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out);
+// *differs = 1 when the count doubles of a and b are not all equal (*differs cleared by the caller).
+void launchDiffers(hipStream_t st, long long count, const double* a, const double* b, int* differs);
 // Permanent dipoles of the sorted atoms from the covalent CSR (caller's order, positions in FP64).
 void launchPermDipoles(hipStream_t st, int n, const int* order, const double* pos, const int* covStart,
                        const int* covPartner, const double* covStrength, BoxSel<double> box, double* perm);

@@ -5,6 +5,8 @@
 // candidate scans with ballot compaction into full per-atom lists.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace pgmhip {
 
 namespace {
@@ -377,6 +379,21 @@ __global__ void kF4ToDouble(int n, const float4* posq, double* out) {
 
 void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out) {
     kF4ToDouble<<<(n+255)/256, 256, 0, st>>>(n, posq, out);
+}
+
+// *differs = 1 when a and b differ anywhere (count doubles; *differs cleared by the caller).  Every writer writes
+// the same value, so plain stores suffice.
+__global__ void kDiffers(long long count, const double* a, const double* b, int* differs) {
+    for (long long i = (long long) blockIdx.x*blockDim.x+threadIdx.x; i < count; i += (long long) gridDim.x*blockDim.x)
+        if (a[i] != b[i]) {
+            *differs = 1;
+            return;
+        }
+}
+
+void launchDiffers(hipStream_t st, long long count, const double* a, const double* b, int* differs) {
+    const int blocks = (int) std::min<long long>(1024, (count+255)/256);
+    kDiffers<<<std::max(1, blocks), 256, 0, st>>>(count, a, b, differs);
 }
 }   // namespace pgmhip
 

@@ -216,3 +216,45 @@ def test_backward_errors():
     eng = E.Engine(precision="double")
     with pytest.raises(E.PgmError, match="backward before setAtoms"):
         eng.backward(np.zeros((0, 3)), np.eye(3), None, 1.0)
+
+
+def test_backward_after_an_evaluation_at_the_same_positions():
+    """A backward at the positions and box of the engine's last evaluation reuses that evaluation (no second
+    forward solve): its gradients equal those of a backward that evaluates first (a fresh engine), also when
+    the device positions come in another buffer with the same values; at other positions it evaluates again."""
+    torch = pytest.importorskip("torch")
+    sy = S.config0()
+    sy.settings.update(TIGHT)
+    g = np.random.default_rng(8).normal(0.0, 1e-3, (sy.n, 3))
+
+    def fresh(pos):
+        return E.Engine.for_system(sy, precision="double").backward(pos, sy.box, g, 0.3)
+
+    def close(a, b, tol=1e-11):
+        for k in a:
+            assert np.abs(a[k] - b[k]).max() <= tol * np.abs(b[k]).max(), k
+
+    ref = fresh(sy.pos)
+    eng = E.Engine.for_system(sy, precision="double")
+    eng.compute(sy.pos, sy.box)
+    close(eng.backward(sy.pos, sy.box, g, 0.3), ref)
+    # device positions in a buffer other than the evaluated one, same values
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    p1 = torch.from_numpy(sy.pos.copy()).to(dev)
+    f1 = torch.zeros((sy.n, 3), dtype=torch.float64, device=dev)
+    eng.compute_device_f64(p1.data_ptr(), sy.box, True, f1.data_ptr(), stream)
+    p2 = p1.clone()
+    del p1
+    gd = torch.from_numpy(g).to(dev)
+    outs = [torch.zeros(sy.n, dtype=torch.float64, device=dev) for _ in range(3)]
+    gc = torch.zeros(len(sy.cov_atom), dtype=torch.float64, device=dev)
+    eng.backward_device(p2.data_ptr(), sy.box, gd.data_ptr(), 0.3, *[t.data_ptr() for t in outs], gc.data_ptr(),
+                        stream)  # fmt: skip
+    torch.cuda.synchronize()
+    close({"q": outs[0].cpu().numpy(), "radius": outs[1].cpu().numpy(), "alpha": outs[2].cpu().numpy(),
+           "cov_strength": gc.cpu().numpy()}, ref)  # fmt: skip
+    # other positions: evaluated there first
+    moved = sy.pos + np.random.default_rng(9).normal(0.0, 2e-3, sy.pos.shape)
+    eng.compute(sy.pos, sy.box)
+    close(eng.backward(moved, sy.box, g, 0.3), fresh(moved), 1e-9)  # other guesses: the solver tolerance
