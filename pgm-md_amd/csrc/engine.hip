@@ -1183,13 +1183,13 @@ private:
             f.twOff = (int) G.twOff;
             f.partOff = G.partOff;
             f.boxOff = boxOff;
-            int b[3];
-            size_t l[3];
+            int b[4];
+            size_t l[4];
             if (mixed)
                 fftGroupShape<float>(f, b, l);
             else
                 fftGroupShape<double>(f, b, l);
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < 4; k++) {
                 f.blk0[k] = fftL_.blocks[k];
                 fftL_.blocks[k] += b[k];
                 fftL_.lds[k] = std::max(fftL_.lds[k], l[k]);
@@ -1199,6 +1199,8 @@ private:
             boxOff += f.ngrid;
             fg.push_back(f);
         }
+        // z and y fused when the x-planes alone give enough blocks (many small grids) and a plane fits the LDS
+        fftL_.zyFused = (fftL_.blocks[3] >= 512 && fftL_.lds[3] <= 48*1024) ? 1 : 0;
         upload(fftGroupsD_, fg.data(), fg.size()*sizeof(FftGroup));
         fftL_.groups = fftGroupsD_.as<FftGroup>();
         fftL_.ngroups = (int) fg.size();

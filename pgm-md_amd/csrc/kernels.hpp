@@ -283,17 +283,18 @@ struct FftGroup {
     int partOff;                 // first row of its x-pass partial sums
     int boxOff;                  // its first system's box in the batch box table
     int pz, zy, zx;              // lines per block of the z, y and x stages
-    int blk0[3];                 // its first block in the z, y and x launches
+    int blk0[4];                 // its first block in the z, y, x and fused z+y launches
 };
 struct FftLaunch {
     const FftGroup* groups;      // device table
     int ngroups;
-    int blocks[3];               // z, y, x launch sizes
-    size_t lds[3];               // dynamic LDS of the z, y, x stages (the largest shape's)
+    int blocks[4];               // z, y, x and fused z+y launch sizes
+    size_t lds[4];               // dynamic LDS of those stages (the largest shape's)
+    int zyFused;                 // z and y in one pass per x-plane (many small grids), else apart
 };
 // The shape of group g's stages (lines per block) and its blocks per stage, as the transforms use them.
 template<class T>
-void fftGroupShape(FftGroup& G, int blocks[3], size_t lds[3]);
+void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]);
 // Forward transforms, influence function (energy and strain partials when asked), inverse transforms of all
 // grids of all groups; real: the stacked real grids, cplx/influence/tw: the stacked complex arrays.  boxes: per
 // stacked grid (nullptr: box for all).  partials: [.][10] (energy, strain 0..8) per x block, the blocks of a

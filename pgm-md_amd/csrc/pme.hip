@@ -478,6 +478,109 @@ __global__ void __launch_bounds__(kBlock) kFftY(const FftGroup* __restrict__ gro
     }
 }
 
+// z and y of one x-plane per block (many small grids): forward, the plane's real rows two per complex transform
+// along z (as kFftZ), their half spectra unpacked transposed (a line of K1 per z), then the y transforms; inverse
+// the other way round.  The plane goes through global memory once each way instead of twice.
+template<class T, bool INVERSE>
+__global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ groups, int ngroups,
+                                                 T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
+                                                 const typename Vec2<T>::type* __restrict__ twAll, const int* skip) {
+    typedef typename Vec2<T>::type T2;
+    extern __shared__ unsigned char smemRaw[];
+    if (skip && *skip)
+        return;
+    const FftGroup& grp = groups[fftGroupOf(groups, ngroups, 3)];
+    const PmeGrid& g = grp.g;
+    const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2], Kh = g.K3h;
+    const int plane = (int) blockIdx.x-grp.blk0[3];   // grid * K0 + x
+    T* __restrict__ real = realAll+grp.realOff+(size_t) plane*K1*K2;
+    T2* __restrict__ cplx = cplxAll+grp.cplxOff+(size_t) plane*K1*Kh;
+    (void) K0;
+    const int npairs = (K1+1)/2;
+    const int S = max(npairs*K2, Kh*K1);
+    T2* twY = reinterpret_cast<T2*>(smemRaw);
+    T2* twZ = twY+K1;
+    T2* bufA = twZ+K2;
+    T2* bufB = bufA+S;
+    loadTwiddles<T>(twY, twAll+grp.twOff+g.K[0], K1);
+    loadTwiddles<T>(twZ, twAll+grp.twOff+g.K[0]+K1, K2);
+    if (!INVERSE) {
+        for (int i = threadIdx.x; i < npairs*K2; i += blockDim.x) {
+            const int l = i/K2, k = i-l*K2;
+            const int ya = 2*l, yb = ya+1;
+            T2 z;
+            z.x = real[(size_t) ya*K2+k];
+            z.y = (yb < K1 ? real[(size_t) yb*K2+k] : T(0));
+            bufA[l*K2+k] = z;
+        }
+        __syncthreads();
+        T2* res = ldsFft<T>(bufA, bufB, twZ, g.plan[2], npairs, false);
+        T2* other = (res == bufA ? bufB : bufA);
+        for (int i = threadIdx.x; i < npairs*Kh; i += blockDim.x) {
+            const int l = i/Kh, k = i-l*Kh;
+            const T2 Z = res[l*K2+k], Zc = res[l*K2+(K2-k)%K2];
+            T2 X, Y;
+            X.x = T(0.5)*(Z.x+Zc.x);
+            X.y = T(0.5)*(Z.y-Zc.y);
+            Y.x = T(0.5)*(Z.y+Zc.y);
+            Y.y = T(-0.5)*(Z.x-Zc.x);
+            other[k*K1+2*l] = X;
+            if (2*l+1 < K1)
+                other[k*K1+2*l+1] = Y;
+        }
+        __syncthreads();
+        T2* res2 = ldsFft<T>(other, res, twY, g.plan[1], Kh, false);
+        for (int i = threadIdx.x; i < K1*Kh; i += blockDim.x) {
+            const int y = i/Kh, z = i-y*Kh;
+            cplx[(size_t) y*Kh+z] = res2[z*K1+y];
+        }
+    }
+    else {
+        for (int i = threadIdx.x; i < K1*Kh; i += blockDim.x) {
+            const int y = i/Kh, z = i-y*Kh;
+            bufA[z*K1+y] = cplx[(size_t) y*Kh+z];
+        }
+        __syncthreads();
+        T2* res = ldsFft<T>(bufA, bufB, twY, g.plan[1], Kh, true);
+        T2* other = (res == bufA ? bufB : bufA);
+        for (int i = threadIdx.x; i < npairs*K2; i += blockDim.x) {
+            const int l = i/K2, k = i-l*K2;
+            const int ya = 2*l, yb = ya+1;
+            const bool lower = k < Kh;
+            const int kk = lower ? k : K2-k;
+            T2 X = res[kk*K1+ya];
+            T2 Y;
+            Y.x = Y.y = T(0);
+            if (yb < K1)
+                Y = res[kk*K1+yb];
+            if (kk == 0 || (K2%2 == 0 && kk == K2/2)) {
+                X.y = T(0);
+                Y.y = T(0);
+            }
+            T2 z;
+            if (lower) {
+                z.x = X.x-Y.y;
+                z.y = X.y+Y.x;
+            }
+            else {
+                z.x = X.x+Y.y;
+                z.y = Y.x-X.y;
+            }
+            other[l*K2+k] = z;
+        }
+        __syncthreads();
+        T2* res2 = ldsFft<T>(other, res, twZ, g.plan[2], npairs, true);
+        for (int i = threadIdx.x; i < npairs*K2; i += blockDim.x) {
+            const int l = i/K2, k = i-l*K2;
+            const int ya = 2*l, yb = ya+1;
+            const T2 z = res2[l*K2+k];
+            real[(size_t) ya*K2+k] = z.x;
+            if (yb < K1)
+                real[(size_t) yb*K2+k] = z.y;
+        }
+    }
+}
+
 // x: forward, influence function (energy, strain), back; a tile of consecutive z for fixed y per block.
 template<class T>
 __global__ void __launch_bounds__(kBlock) kFftXConv(const FftGroup* __restrict__ groups, int ngroups,
@@ -904,7 +1007,7 @@ void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* p
 }
 
 template<class T>
-void fftGroupShape(FftGroup& G, int blocks[3], size_t lds[3]) {
+void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]) {
     typedef typename Vec2<T>::type T2;
     const PmeGrid& g = G.g;
     const int ngrid = G.ngrid;
@@ -919,6 +1022,10 @@ void fftGroupShape(FftGroup& G, int blocks[3], size_t lds[3]) {
     G.zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
     blocks[2] = ngrid*g.K[1]*((g.K3h+G.zx-1)/G.zx);
     lds[2] = (size_t) (g.K[0]+2*G.zx*g.K[0])*sizeof(T2);
+    // fused z+y: one block per x-plane; twiddles of y and z and two plane buffers
+    const int S = std::max(((g.K[1]+1)/2)*g.K[2], g.K3h*g.K[1]);
+    blocks[3] = ngrid*g.K[0];
+    lds[3] = (size_t) (g.K[1]+g.K[2]+2*S)*sizeof(T2);
 }
 
 template<class T>
@@ -927,12 +1034,20 @@ void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec
                        int energy, int strain, double* partials, const int* skip) {
     const FftGroup* G = L.groups;
     const int ng = L.ngroups;
-    kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
-    kFftY<T, false><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
+    if (L.zyFused)
+        kFftZY<T, false><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip);
+    else {
+        kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
+        kFftY<T, false><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
+    }
     kFftXConv<T><<<L.blocks[2], kBlock, L.lds[2], st>>>(G, ng, cplx, influence, tw, box, boxes, beta, energy, strain,
                                                         partials, skip);
-    kFftY<T, true><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
-    kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
+    if (L.zyFused)
+        kFftZY<T, true><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip);
+    else {
+        kFftY<T, true><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
+        kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
+    }
 }
 
 template<class T>
