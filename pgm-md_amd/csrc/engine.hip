@@ -502,6 +502,7 @@ public:
             throw std::invalid_argument("pgm_hip: compute needs positions and a box");
         if (strainOut && sys_.size() > 1)
             throw std::invalid_argument("pgm_hip: the strain derivative is per system: not available for a batch");
+        adjointReady_ = false;   // set again only when this evaluation completes
         withDevice([&] {
             HIPCHK(hipEventRecord(evIn_, caller));
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
