@@ -931,14 +931,24 @@ __global__ void kZero(T* p, size_t count, const int* skip) {
 inline int nblk(long long threads, int bs) { return (int) ((threads+bs-1)/bs); }
 
 template<class T>
-int linesPerBlock(int K, long long lines) {
-    // LDS: twiddles + two line buffers within 48 KB; and enough blocks to fill 256 CUs twice over.  The y and x
-    // passes tile a row of K3h half-spectrum lines: a cap at or above K3h keeps a row in one block (no short
-    // tail tile), and more lines per block keep more of the 256 threads busy in each radix stage.
-    const int budget = (int) (48*1024/(2*sizeof(typename Vec2<T>::type)))-K;   // elements per buffer
-    const int byLds = std::max(1, std::min(64, budget/(2*K)));
+int linesPerBlock(int K, long long lines, int ldsBytes) {
+    // LDS: twiddles + two line buffers within ldsBytes; and enough blocks to fill 256 CUs twice over.  More
+    // lines per block keep more of the 256 threads busy in each radix stage.
+    const int elems = (int) (ldsBytes/sizeof(typename Vec2<T>::type));
+    const int byLds = std::max(1, std::min(64, (elems-K)/(2*K)));
     const int byFill = (int) std::max(1LL, lines/512);
     return std::min(byLds, byFill);
+}
+
+// The y and x passes tile a row of K3h half-spectrum lines and read each tile as runs of consecutive z: up to
+// 64 KB of LDS (less the x pass's static reduction buffer) gives runs of whole 128-byte lines where the grid
+// allows (16 complex floats or 8 complex doubles at K = 216, against 6 and 3 within 24 KB), and the tiles are
+// balanced over the row, with no short tail tile.
+template<class T>
+int tileLines(int K, int Kh, long long lines) {
+    const int L = linesPerBlock<T>(K, lines, 64*1024-512);
+    const int ntile = (Kh+L-1)/L;
+    return (Kh+ntile-1)/ntile;
 }
 
 }   // namespace
@@ -1013,13 +1023,13 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]) {
     const int ngrid = G.ngrid;
     const int nlines = ngrid*g.K[0]*g.K[1];
     // z: pairs of real lines; y: the stacked grids are one grid of ngrid K0 x-planes; x: lines of K0
-    G.pz = linesPerBlock<T>(g.K[2], (nlines+1)/2);
+    G.pz = linesPerBlock<T>(g.K[2], (nlines+1)/2, 24*1024);   // contiguous lines: small tiles, more blocks
     blocks[0] = nblk((nlines+1)/2, G.pz);
     lds[0] = (size_t) (g.K[2]+2*G.pz*g.K[2])*sizeof(T2);
-    G.zy = linesPerBlock<T>(g.K[1], (long long) ngrid*g.K[0]*g.K3h);
+    G.zy = tileLines<T>(g.K[1], g.K3h, (long long) ngrid*g.K[0]*g.K3h);
     blocks[1] = ngrid*g.K[0]*((g.K3h+G.zy-1)/G.zy);
     lds[1] = (size_t) (g.K[1]+2*G.zy*g.K[1])*sizeof(T2);
-    G.zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
+    G.zx = tileLines<T>(g.K[0], g.K3h, (long long) ngrid*g.K[1]*g.K3h);
     blocks[2] = ngrid*g.K[1]*((g.K3h+G.zx-1)/G.zx);
     lds[2] = (size_t) (g.K[0]+2*G.zx*g.K[0])*sizeof(T2);
     // fused z+y: one block per x-plane; twiddles of y and z and two plane buffers
@@ -1052,7 +1062,7 @@ void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec
 
 template<class T>
 int convolveBlocksPerGrid(PmeGrid g, int ngrid) {
-    const int zx = linesPerBlock<T>(g.K[0], (long long) ngrid*g.K[1]*g.K3h);
+    const int zx = tileLines<T>(g.K[0], g.K3h, (long long) ngrid*g.K[1]*g.K3h);
     return g.K[1]*((g.K3h+zx-1)/zx);
 }
 
