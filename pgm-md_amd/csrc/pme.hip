@@ -95,60 +95,126 @@ __device__ __forceinline__ T2 rotQuarter(T2 a, bool inverse) {
     return r;
 }
 
+// cos(2 pi k/R) and sin(2 pi k/R) for R = 5 or 7 and 0 < k < R (cos(2 pi/5) = (sqrt 5 - 1)/4, and so on).
+__host__ __device__ constexpr double cosTwoPi(int R, int k) {
+    const int q = k <= R/2 ? k : R-k;
+    return R == 5 ? (q == 1 ? 0.30901699437494742410 : -0.80901699437494742410)
+                  : (q == 1 ? 0.62348980185873353053 : q == 2 ? -0.22252093395631440429 : -0.90096886790241912624);
+}
+__host__ __device__ constexpr double sinTwoPi(int R, int k) {
+    const int q = k <= R/2 ? k : R-k;
+    const double s = R == 5 ? (q == 1 ? 0.95105651629515357212 : 0.58778525229247312917)
+                            : (q == 1 ? 0.78183148246802980871 : q == 2 ? 0.97492791218182360702 : 0.43388373911755812048);
+    return k <= R/2 ? s : -s;
+}
+
+// One radix-R stage of a Stockham FFT over nlines lines of length K in LDS (line l at buf[l*K]): butterfly j of
+// line l takes in[j + r*m], r < R (m = K/R), each times w^(r*kk*span) (kk = j mod Ns, span = K/(Ns*R)), and
+// writes out[(j div Ns)*Ns*R + kk + s*Ns], s < R.  R is a compile-time constant, so the R inputs stay in
+// registers: with a runtime radix the array was indexed at run time and lived in per-lane scratch memory.
+template<int R, class T>
+__device__ __forceinline__ void fftStage(const typename Vec2<T>::type* src, typename Vec2<T>::type* dst,
+                                         const typename Vec2<T>::type* tw, int K, int Ns, FastDiv byM, FastDiv byNs,
+                                         int nlines, bool inverse) {
+    typedef typename Vec2<T>::type T2;
+    const int m = K/R, span = K/(Ns*R);
+    for (int idx = threadIdx.x; idx < nlines*m; idx += blockDim.x) {
+        const int l = byM.div(idx), j = idx-l*m, jq = byNs.div(j), kk = j-jq*Ns;
+        const T2* in = src+l*K;
+        T2* out = dst+l*K+jq*Ns*R+kk;
+        T2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const T2 a = in[j+r*m];
+            v[r] = (r == 0 ? a : cmul(a, conjIf(tw[r*kk*span], inverse)));
+        }
+        if constexpr (R == 2) {
+            out[0] = cadd(v[0], v[1]);
+            out[Ns] = csub(v[0], v[1]);
+        }
+        else if constexpr (R == 4) {
+            const T2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+            const T2 s13 = cadd(v[1], v[3]), d13 = rotQuarter(csub(v[1], v[3]), inverse);
+            out[0] = cadd(s02, s13);
+            out[Ns] = cadd(d02, d13);
+            out[2*Ns] = csub(s02, s13);
+            out[3*Ns] = csub(d02, d13);
+        }
+        else if constexpr (R == 3) {
+            const T c = T(-0.5), sn = (inverse ? T(0.86602540378443864676) : T(-0.86602540378443864676));
+            const T2 sum = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+            T2 t;
+            t.x = v[0].x+c*sum.x;
+            t.y = v[0].y+c*sum.y;
+            out[0] = cadd(v[0], sum);
+            T2 u;
+            u.x = -sn*d.y;
+            u.y = sn*d.x;
+            out[Ns] = cadd(t, u);
+            out[2*Ns] = csub(t, u);
+        }
+        else {
+            // radix 5 and 7 in closed form: with a_r = v_r + v_{R-r}, b_r = v_r - v_{R-r} (r = 1..h, h = (R-1)/2),
+            // X_s = v_0 + sum a_r cos(2 pi rs/R) -/+ i sum b_r sin(2 pi rs/R) and X_{R-s} the other sign
+            // (forward/inverse); the cosines and sines fold to constants
+            constexpr int h = (R-1)/2;
+            T2 a[h+1], b[h+1];
+            T2 x0 = v[0];
+#pragma unroll
+            for (int r = 1; r <= h; r++) {
+                a[r] = cadd(v[r], v[R-r]);
+                b[r] = csub(v[r], v[R-r]);
+                x0 = cadd(x0, a[r]);
+            }
+            out[0] = x0;
+#pragma unroll
+            for (int s = 1; s <= h; s++) {
+                T2 A = v[0], B;
+                B.x = B.y = T(0);
+#pragma unroll
+                for (int r = 1; r <= h; r++) {
+                    const T c = T(cosTwoPi(R, (r*s)%R)), sn = T(sinTwoPi(R, (r*s)%R));
+                    A.x += c*a[r].x;
+                    A.y += c*a[r].y;
+                    B.x += sn*b[r].x;
+                    B.y += sn*b[r].y;
+                }
+                const T2 rb = rotQuarter(B, inverse);
+                out[s*Ns] = cadd(A, rb);
+                out[(R-s)*Ns] = csub(A, rb);
+            }
+        }
+    }
+}
+
 // Mixed-radix Stockham FFT of nlines lines of length K held in LDS (line l at buf[l*K]); tw[j] =
 // e^{-2 pi i j/K} in LDS.  Returns the buffer that holds the result (natural order).
 template<class T>
-__device__ typename Vec2<T>::type* ldsFft(typename Vec2<T>::type* src, typename Vec2<T>::type* dst,
-                                          const typename Vec2<T>::type* tw, const FftPlan& p, int nlines,
-                                          bool inverse) {
+__device__ __forceinline__ typename Vec2<T>::type* ldsFft(typename Vec2<T>::type* src, typename Vec2<T>::type* dst,
+                                                          const typename Vec2<T>::type* tw, const FftPlan& p,
+                                                          int nlines, bool inverse) {
     typedef typename Vec2<T>::type T2;
     const int K = p.K;
     int Ns = 1;
     for (int st = 0; st < p.nstage; st++) {
-        const int R = p.radix[st], m = K/R, span = K/(Ns*R);
+        const int R = p.radix[st];
         const FastDiv byM = p.byM[st], byNs = p.byNs[st];
-        for (int idx = threadIdx.x; idx < nlines*m; idx += blockDim.x) {
-            const int l = byM.div(idx), j = idx-l*m, jq = byNs.div(j), kk = j-jq*Ns;
-            const T2* in = src+l*K;
-            T2* out = dst+l*K+jq*Ns*R+kk;
-            T2 v[7];
-            for (int r = 0; r < R; r++) {
-                const T2 a = in[j+r*m];
-                v[r] = (r == 0 ? a : cmul(a, conjIf(tw[r*kk*span], inverse)));
-            }
-            if (R == 2) {
-                out[0] = cadd(v[0], v[1]);
-                out[Ns] = csub(v[0], v[1]);
-            }
-            else if (R == 4) {
-                const T2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-                const T2 s13 = cadd(v[1], v[3]), d13 = rotQuarter(csub(v[1], v[3]), inverse);
-                out[0] = cadd(s02, s13);
-                out[Ns] = cadd(d02, d13);
-                out[2*Ns] = csub(s02, s13);
-                out[3*Ns] = csub(d02, d13);
-            }
-            else if (R == 3) {
-                const T c = T(-0.5), sn = (inverse ? T(0.86602540378443864676) : T(-0.86602540378443864676));
-                const T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-                T2 t;
-                t.x = v[0].x+c*s.x;
-                t.y = v[0].y+c*s.y;
-                out[0] = cadd(v[0], s);
-                T2 u;
-                u.x = -sn*d.y;
-                u.y = sn*d.x;
-                out[Ns] = cadd(t, u);
-                out[2*Ns] = csub(t, u);
-            }
-            else {
-                for (int s = 0; s < R; s++) {
-                    T2 acc = v[0];
-                    for (int r = 1; r < R; r++)
-                        acc = cadd(acc, cmul(v[r], conjIf(tw[((r*s)%R)*m], inverse)));
-                    out[s*Ns] = acc;
-                }
-            }
+        switch (R) {   // makePlan's radices: 4, 2, 3, 5, 7
+        case 2:
+            fftStage<2, T>(src, dst, tw, K, Ns, byM, byNs, nlines, inverse);
+            break;
+        case 3:
+            fftStage<3, T>(src, dst, tw, K, Ns, byM, byNs, nlines, inverse);
+            break;
+        case 4:
+            fftStage<4, T>(src, dst, tw, K, Ns, byM, byNs, nlines, inverse);
+            break;
+        case 5:
+            fftStage<5, T>(src, dst, tw, K, Ns, byM, byNs, nlines, inverse);
+            break;
+        default:
+            fftStage<7, T>(src, dst, tw, K, Ns, byM, byNs, nlines, inverse);
+            break;
         }
         __syncthreads();
         T2* t = src;

@@ -7,6 +7,8 @@ systems of one and two molecules).
   oracle's to round-off, the non-polarizable atoms carry no induced dipole, mixed within its bound.
 - One water and two waters in a 2 nm box (the cell list's fewer-than-5-cells path, a nearly empty grid): FP64
   equal to the oracle.
+- PME grids with factors 5 and 7 and odd, unequal dimensions (35^3, 28 x 42 x 49, 40 x 30 x 50; the other tests'
+  grids are 2^a 3^b or 20, 30, 40): FP64 equal to the oracle on the same grid, mixed within its bound.
 - A batch of a one-water box and an ordinary snapshot equals its single evaluations.
 """
 
@@ -68,6 +70,18 @@ def test_one_and_two_molecules(nmol):
     sy = _tight(S.water_box(nmol, 2.0, 2, 5, 0.05, f"w{nmol}", S.base_settings(grid=(20, 20, 20))))
     eng, o = _against_oracle(sy, e_tol=1e-10, f_tol=1e-8)
     assert eng.num_pairs() == o.num_pairs()
+
+
+@pytest.mark.parametrize("grid", [(35, 35, 35), (28, 42, 49), (40, 30, 50)])
+def test_grids_with_factors_5_and_7(grid):
+    """The transforms' radix-5 and radix-7 stages and odd, unequal dimensions: FP64 equal to the oracle on the
+    same grid, mixed within the force bound."""
+    sy = _tight(S.config0())
+    sy.settings.update(grid=grid)
+    _against_oracle(sy)
+    m = _tight(dataclasses.replace(sy))
+    m.settings.update(tol=1e-6)
+    _against_oracle(m, "mixed", 1e-6, 5e-5)
 
 
 def test_batch_with_a_one_molecule_box():
