@@ -340,6 +340,9 @@ def main():
     ap.add_argument("--no-gvdw", action="store_true", help="skip the GVDW-on timing of the same workload")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-large", action="store_true", help="skip the 999,000-atom box")
+    ap.add_argument("--large-only", action="store_true",
+                    help="profiler pass over the 999,000-atom box alone (both precisions, --steps timed "
+                         "evaluations each); prints its figures, not the JSON line")  # fmt: skip
     ap.add_argument("--profile", type=int, default=0, metavar="N",
                     help="profiler pass: N evaluations of the workload per precision (each engine autotuned "
                          "first) and one calibration copy of "
@@ -357,6 +360,10 @@ def main():
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if a.large_only:
+        print(json.dumps(large_box(torch, E, S, a.steps, a.warmup, log)))
+        return
 
     sy = S.config2()
     log(f"workload {sy.name}: {sy.n} atoms")
