@@ -781,14 +781,25 @@ __global__ void __launch_bounds__(kBlock) kGather(PmeSel ps, int n, const int* _
         x0 += (x0 < 0 ? K0 : 0);
         x1 += (x1 < 0 ? K1 : 0);
         const T* row = phi+((size_t) x0*K1+x1)*K2;
+        T ph[P];
+        if (B2 >= P-1) {   // no wrap along z: the P points are contiguous and load as a few wide loads
+            const T* run = row+B2-(P-1);
+#pragma unroll
+            for (int j = 0; j < P; j++)
+                ph[P-1-j] = run[j];
+        }
+        else {
+#pragma unroll
+            for (int j3 = 0; j3 < P; j3++)
+                ph[j3] = row[g3[j3]];
+        }
         T st = 0, su = 0, sv = 0;
 #pragma unroll
         for (int j3 = 0; j3 < P; j3++) {
-            const T ph = row[g3[j3]];
-            st += t3[j3]*ph;
-            su += u3[j3]*ph;
+            st += t3[j3]*ph[j3];
+            su += u3[j3]*ph[j3];
             if (FINAL)
-                sv += v3[j3]*ph;
+                sv += v3[j3]*ph[j3];
         }
         const T t1 = o[j1], u1 = o[P+j1], t2 = o[3*P+j2], u2 = o[4*P+j2];
         f[0] += (double) (u1*t2*st);
@@ -882,17 +893,33 @@ __global__ void __launch_bounds__(kBlock) kGatherTangent(PmeSel ps, int n, const
         x0 += (x0 < 0 ? K0 : 0);
         x1 += (x1 < 0 ? K1 : 0);
         const size_t rowOff = ((size_t) x0*K1+x1)*K2;
+        T ph[P], pd[P];
+        if (B2 >= P-1) {   // no wrap along z: contiguous points, a few wide loads (as in kGather)
+            const size_t run = rowOff+B2-(P-1);
+#pragma unroll
+            for (int j = 0; j < P; j++) {
+                ph[P-1-j] = phi[run+j];
+                if (TAN)
+                    pd[P-1-j] = phiDot[run+j];
+            }
+        }
+        else {
+#pragma unroll
+            for (int j3 = 0; j3 < P; j3++) {
+                ph[j3] = phi[rowOff+g3[j3]];
+                if (TAN)
+                    pd[j3] = phiDot[rowOff+g3[j3]];
+            }
+        }
         T st = 0, su = 0, sv = 0, dt = 0, du = 0;
 #pragma unroll
         for (int j3 = 0; j3 < P; j3++) {
-            const T ph = phi[rowOff+g3[j3]];
-            st += t3[j3]*ph;
-            su += u3[j3]*ph;
+            st += t3[j3]*ph[j3];
+            su += u3[j3]*ph[j3];
             if (TAN) {
-                sv += v3[j3]*ph;
-                const T pd = phiDot[rowOff+g3[j3]];
-                dt += t3[j3]*pd;
-                du += u3[j3]*pd;
+                sv += v3[j3]*ph[j3];
+                dt += t3[j3]*pd[j3];
+                du += u3[j3]*pd[j3];
             }
         }
         const T t1 = o[j1], u1 = o[P+j1], v1 = o[2*P+j1], t2 = o[3*P+j2], u2 = o[4*P+j2], v2 = o[5*P+j2];
