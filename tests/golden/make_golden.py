@@ -5,7 +5,7 @@ Inputs (data files of the reference's tests):
   tests/data/ref_w512_{q,qp,qi,qpi}_novdw.npz  pGM-JAX FP64 results for the four levels (positions, box,
   energy, forces, induced dipoles, iterations)
 The fixture keeps those arrays as they are, plus the parameters in engine units (read with
-pgmhip.prmtop, which follows openmm/python/pgm_prmtop.py:54-115).  Settings of the fixture:
+pgmhip.prmtop, which reads the sections openmm/python/pgm_prmtop.py:54-115 reads).  Settings of the fixture:
 tests/openmm_tests.sh:18-22 and tests/ref_pgmjax.py:36-40 (cutoff 0.9, beta 3.5, 36^3, order 6, tau 1e-10,
 no predictor, no peek, ke 138.935458).
 """

@@ -123,3 +123,43 @@ def test_parameter_types_round_trip(golden):
     assert np.array_equal(ty2["q"][ty2["atom_type"]], params["q"])
     with pytest.raises(ValueError):
         parameter_types(params, names[:-1])
+
+
+def test_prmtop_field_layouts_and_errors(tmp_path):
+    """%FORMAT specifications as AMBER writes them, comments between %FLAG and %FORMAT, and the checks of
+    pgm_parameters (missing sections, counts that do not add up, partners out of range)."""
+    L = prmtop.FieldLayout.parse
+    assert L("(10I8)") == prmtop.FieldLayout(10, "i", 8)
+    assert L("(5E16.8)") == prmtop.FieldLayout(5, "f", 16)
+    assert L("(20a4)") == prmtop.FieldLayout(20, "a", 4)
+    assert L("(i5)") == prmtop.FieldLayout(1, "i", 5)
+    assert L("(3F12.6)  ") == prmtop.FieldLayout(3, "f", 12)
+    for bad in ("10I8", "(10X8)", "(10I)", "()"):
+        with pytest.raises(ValueError):
+            L(bad)
+    assert L("(4I3)").fields("  1 -2  3   ") == ["1", "-2", "3"]
+
+    def write(name, counts, partners, strengths, nq=2):
+        sec = lambda flag, fmt, body: f"%FLAG {flag}\n%COMMENT a comment\n%FORMAT({fmt})\n{body}\n"  # noqa: E731
+        text = "%VERSION  VERSION_STAMP = V0001.000\n"
+        text += sec("POL_GAUSS_COVALENT_POINTERS_LIST", "10I8", "".join(f"{c:8d}" for c in counts))
+        text += sec("POL_GAUSS_COVALENT_ATOMS_LIST", "10I8", "".join(f"{c:8d}" for c in partners))
+        text += sec("POL_GAUSS_COVALENT_DIPOLES_LIST", "5E16.8", "".join(f"{c:16.8E}" for c in strengths))
+        for flag in ("POL_GAUSS_MONOPOLES_LIST", "POL_GAUSS_RADII_LIST", "POL_GAUSS_POLARIZABILITY_LIST"):
+            text += sec(flag, "5E16.8", "".join(f"{0.5:16.8E}" for _ in range(nq)))
+        p = tmp_path / name
+        p.write_text(text)
+        return str(p)
+
+    par = prmtop.pgm_parameters(write("ok.prmtop", [1, 0], [2], [0.25]))
+    np.testing.assert_array_equal(par["cov_atom"], [0])
+    np.testing.assert_array_equal(par["cov_partner"], [1])
+    assert par["cov_strength"][0] == 0.25 * 0.1 and par["alpha"][0] == 0.5 * 1e-3
+    with pytest.raises(ValueError, match="add to"):
+        prmtop.pgm_parameters(write("count.prmtop", [2, 0], [2], [0.25]))
+    with pytest.raises(ValueError, match="outside"):
+        prmtop.pgm_parameters(write("range.prmtop", [1, 0], [3], [0.25]))
+    with pytest.raises(ValueError, match="entries for"):
+        prmtop.pgm_parameters(write("short.prmtop", [1, 0, 0], [2], [0.25]))
+    secs = prmtop.read_sections(write("all.prmtop", [1, 0], [2], [0.25]), ("POL_GAUSS_RADII_LIST", "NOT_THERE"))
+    assert list(secs) == ["POL_GAUSS_RADII_LIST"] and secs["POL_GAUSS_RADII_LIST"].dtype == np.float64
