@@ -60,6 +60,13 @@ KERNEL_OF_PHASE = {"pair_compact": "kCompact", "pair_setup": "kPairSetup", "pair
 # Algorithmic flops (add/mul/sub 1, FMA 2; exp/erfc/sqrt/rsqrt/rint not counted) per half pair within the
 # cutoff (P), per atom (N) and per grid: geometry 26, G1+G2 32, G0..G3 44, dual field body 76, dipole
 # matrix-vector body 36, final body 108; splines 300, spread 1750, field gather 1300, final gather 3950.
+# Per launch of a pair pass (each pair visited from both atoms counts once): setup (compaction fused or not)
+# 26 + 32 + 76 = 134 P, field 26 + 36 = 62 P, final 26 + 44 + 108 = 178 P; compaction alone is the geometry of
+# the setup's count.
+PAIR_FLOPS_PER_P = {"pair_setup": 134, "pair_field": 62, "pair_final": 178, "pair_compact": 0}
+VALU_NONPACKED_TFLOPS = 78.6  # FP32 without packed instructions (this build disables them) = the FP64 rate
+# Compulsory HBM bytes per evaluation: N (196 + 280 n) + (n + 3)(20 K + 32 Kc) (positions, parameters, forces,
+# dipoles, 40 B of dipole/field traffic per field pass, 10 FP64 3-vector streams per CG iteration, grids).
 
 
 def fft_flops(K):
@@ -71,6 +78,17 @@ def fft_flops(K):
 def flops_per_eval(P, N, K, n):
     """F(n) = P (312 + 62 n) + N (12,100 + 3,050 n) + (n + 3)(5 K log2 K + 8 Kc)."""
     return P * (312 + 62 * n) + N * (12100 + 3050 * n) + (n + 3) * fft_flops(K)
+
+
+def compulsory_bytes(N, K, n):
+    Kt = K[0] * K[1] * K[2]
+    Kc = K[0] * K[1] * (K[2] // 2 + 1)
+    return N * (196 + 280 * n) + (n + 3) * (20 * Kt + 32 * Kc)
+
+
+def adjoint_flops(P, N, K, n, m):
+    """F_adj = F(n) + m (62 P + 3,050 N + 5 K log2 K + 8 Kc) + 2 (178 P + 3,950 N)."""
+    return flops_per_eval(P, N, K, n) + m * (62 * P + 3050 * N + fft_flops(K)) + 2 * (178 * P + 3950 * N)
 
 
 def prior_dipole_sets(sy, E, nset=4, step=0.002, seed=22):
@@ -140,22 +158,42 @@ def traffic_table(path=TRAFFIC_FILE):
 
 
 def roofline(dom, live, P, C, N, precision, lanes, fused):
-    """Achieved algorithmic HBM bandwidth of the pair pass `dom` (the one with the most time per evaluation),
-    from its live launch average `live` over the timed steps."""
+    """The pair pass `dom` (the one with the most time per evaluation) against the VALU rate, the bound of the
+    path: its algorithmic flops per launch (PAIR_FLOPS_PER_P) over its live launch average `live` in the timed
+    steps, against the MI355X FP32 (packed, 157.3 TFLOP/s) or FP64 (78.6) peak.  `traffic` is the HBM bytes per
+    launch measured by the counters; `stored_list_bytes` the part of them that is this implementation's stored
+    list and kernels (per entry of the full list and per candidate), which is no algorithmic traffic."""
     E = 2 * P
     table = dict(PAIR_BYTES, pair_setup=PAIR_BYTES_FUSED_SETUP) if fused else PAIR_BYTES
     kernels = dict(KERNEL_OF_PHASE, pair_setup="kCompactSetup") if fused else KERNEL_OF_PHASE
     per_launch = {ph: b[precision][0] * E + b[precision][1] * N + b[precision][2] * C for ph, b in table.items()}
+    stored = {ph: b[precision][0] * E + b[precision][2] * C for ph, b in table.items()}
     t = live["launch_avg_us"] * 1e-6
-    achieved = per_launch[dom] / t / 1e9
+    flops = PAIR_FLOPS_PER_P[dom] * P
+    achieved = flops / t / 1e12
+    peak = PEAK_TFLOPS[precision]
     real = "float" if precision == "mixed" else "double"
     prefix = f"{kernels[dom]}<{real}, {lanes},"
     traffic = next((v for k, v in traffic_table().items() if k.startswith(prefix)), None)
     return {
-        "bound": "hbm", "kernel": f"{kernels[dom]}<{real}, {lanes}>", "phase": dom,
-        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic, "algorithmic_bytes": per_launch[dom], "launch_avg_us": round(live["launch_avg_us"], 2),
+        "bound": "valu", "kernel": f"{kernels[dom]}<{real}, {lanes}>", "phase": dom,
+        "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+        "frac_of_nonpacked_fp32": round(achieved / VALU_NONPACKED_TFLOPS, 4) if precision == "mixed" else None,
+        "flops_per_launch": flops, "traffic": traffic, "stored_list_bytes": stored[dom],
+        "hbm_achieved_gbs": round(per_launch[dom] / t / 1e9, 1), "launch_avg_us": round(live["launch_avg_us"], 2),
         "launches_per_eval": live["launches"], "traffic_source": TRAFFIC_FILE if traffic else None,
+    }  # fmt: skip
+
+
+def eval_roofline(P, N, K, n, evals_per_s, precision):
+    """The whole evaluation against the VALU peak (F(n) flops per evaluation) and its compulsory HBM bytes."""
+    f = flops_per_eval(P, N, K, n)
+    peak = PEAK_TFLOPS[precision]
+    b = compulsory_bytes(N, K, n)
+    return {
+        "flops_per_eval": f, "achieved_tflops": round(f * evals_per_s / 1e12, 3), "peak_tflops": peak,
+        "frac": round(f * evals_per_s / 1e12 / peak, 4), "compulsory_bytes_per_eval": b,
+        "compulsory_hbm_frac": round(b * evals_per_s / 1e9 / HBM_PEAK_GBS, 4),
     }  # fmt: skip
 
 
@@ -229,6 +267,91 @@ def large_box(torch, E, S, steps, warmup, log):
     return out
 
 
+def md_like(torch, E, sy, sets, steps, log, precision="mixed", step_nm=0.002, seed=31):
+    """MD-like steps of the 96k workload through the asynchronous device boundary (pgm_launch_device +
+    pgm_finish_device, float4 positions, fixed-point forces, as an OpenMM caller): the atoms move every step along
+    a seeded velocity field (step_nm per step for the mean atom), every step is a new step (the mu4 history
+    follows the trajectory from the 4 prior sets), the pair list is rebuilt when the displacement check asks.
+    Reports evals/s with the rebuilds in the timed region, the builds, repeats (valid = 0) and the mean CG
+    iterations, and what one list build costs (the list phase of a separate pass with every rebuild bracketed)."""
+    n = sy.n
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0.0, 1.0, sy.pos.shape)
+    v /= np.sqrt((v * v).sum(1)).mean()
+    dev = torch.device("cuda:0")
+    posq = torch.zeros((steps + 1, n, 4), dtype=torch.float32, device=dev)
+    for k in range(steps + 1):
+        posq[k, :, :3] = torch.from_numpy((sy.pos + k * step_nm * v).astype(np.float32)).to(dev)
+    stride, scale = n, 2.0**32
+    fixed = torch.zeros(3 * stride, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for timed in (False, True):
+        eng = E.Engine.for_system(sy, precision=precision)
+        eng.set_history(sets)
+        eng.set_lanes(*md_like.lanes[precision])
+        if timed:
+            eng.set_timing(True, phases=["list"])
+        its, repeats = [], 0
+        builds0 = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps + 1):
+            if k == 1:   # step 0 builds the list and takes the first (host-checked) solve: outside the timed region
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                builds0 = eng.list_builds()
+            while True:
+                fixed.zero_()
+                eng.launch_device(posq[k].data_ptr(), sy.box, True, fixed.data_ptr(), stride, scale, stream)
+                _, valid = eng.finish_device()
+                if valid:
+                    break
+                repeats += 1
+            its.append(eng.iterations())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if timed:
+            tab = eng.timing()
+            ms, cnt = tab["list"]
+            out["list_build_ms"] = round(ms / max(cnt, 1), 3)
+        else:
+            builds = eng.list_builds() - builds0
+            out.update({"evals_per_s": round(steps / dt, 3), "steps": steps, "step_nm": step_nm,
+                        "list_builds": builds, "repeated_launches": repeats,
+                        "late_list_checks": eng.late_list_checks(), "mean_cg_iterations": round(float(np.mean(its[1:])), 3)})  # fmt: skip
+            log(f"md-like {precision}: {steps / dt:.2f} evals/s, {builds} list builds in {steps} steps, {repeats} repeats")
+        del eng
+    return out
+
+
+md_like.lanes = {}
+
+
+def mu3_variant(torch, E, sy, sets, pos_d, f_d, stream, steps, warmup, log):
+    """The main workload with the quadratic mu3 guess from 3 prior sets (predictor 1), both precisions."""
+    out = {}
+    for prec in ("mixed", "double"):
+        eng = E.Engine.for_system(sy, precision=prec, predictor=1)
+        eng.set_history(sets[:3])
+        eng.freeze_history(True)
+        eng.set_lanes(*md_like.lanes[prec])
+        eng.compute(sy.pos, sy.box)
+        for _ in range(warmup):
+            eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[prec] = {"evals_per_s": round(steps / dt, 3), "cg_iterations": eng.iterations()}
+        log(f"mu3 {prec}: {steps / dt:.2f} evals/s, {eng.iterations()} CG iterations")
+        del eng
+    out["mixed_over_fp64"] = round(out["mixed"]["evals_per_s"] / out["double"]["evals_per_s"], 3)
+    return out
+
+
 def snapshot_batch(S, nsnap=64):
     """The uneven snapshot batch as one engine's arrays: offsets, grids, boxes, per-atom parameters, covalent."""
     systems = [S.config3_snapshot(k) for k in range(nsnap)]
@@ -241,6 +364,17 @@ def snapshot_batch(S, nsnap=64):
         "cov_atom": np.concatenate([s.cov_atom + off[i] for i, s in enumerate(systems)]),
         "cov_partner": np.concatenate([s.cov_partner + off[i] for i, s in enumerate(systems)]),
     }  # fmt: skip
+
+
+def batch_flops(b, P, n, m):
+    """Sum over the snapshots of F_adj (SURVEY's adjoint count), with the batch's total half pairs P split over
+    the snapshots in proportion to their atoms (all snapshots have the same density)."""
+    N = int(b["off"][-1])
+    total = 0.0
+    for s in b["systems"]:
+        Ps = P * s.n / N
+        total += adjoint_flops(Ps, s.n, s.settings["grid"], n, m)
+    return total
 
 
 def batch_engine(E, b, precision, tol):
@@ -266,10 +400,18 @@ def adjoint_batch(torch, E, S, reps, log):
     f_d = torch.zeros((n, 3), dtype=torch.float64, device=dev)
     g_d = torch.from_numpy(np.random.default_rng(9).normal(0.0, 1e-3, (n, 3))).to(dev)
     grads = [torch.empty(m, dtype=torch.float64, device=dev) for m in (n, n, n, len(b["cov_atom"]))]
+    # the parameters a fit would set at each step: per-type scalings of q, R, alpha and the covalent strengths
+    # (two sets, alternated), through set_atoms + set_covalent (the pair list is kept: same atoms, same order)
+    prng = np.random.default_rng(12)
+    params = []
+    for _ in range(2):
+        f = 1.0 + 0.01 * prng.standard_normal(4)
+        params.append((b["q"] * f[0], b["radius"] * f[1], b["alpha"] * f[2], b["cov_strength"] * f[3]))
     out, ref = {}, None
     for prec in ("double", "mixed"):
         eng = batch_engine(E, b, prec, 1e-8)
         _, f = eng.compute_batch(b["pos"], b["boxes"])
+        P_batch = eng.num_pairs()
         (lanes, fused), tune = eng.autotune(3)
         fwd_it = eng.iterations()
         g = eng.backward_batch(b["pos"], b["boxes"], g_d.cpu().numpy(), None)
@@ -277,8 +419,14 @@ def adjoint_batch(torch, E, S, reps, log):
             ref = (f, g)
         adj_it = eng.adjoint_iterations()
 
+        counter = [0]
+
         def step():
             # a fitting step: new parameters, so the forward solve starts from the alpha E guess (no history)
+            q, r, al, cs = params[counter[0] % 2]
+            counter[0] += 1
+            eng.set_atoms(q, r, al)
+            eng.set_covalent(b["cov_atom"], b["cov_partner"], cs)
             eng.reset_history()
             eng.compute_batch_device(pos_d.data_ptr(), b["boxes"], True, f_d.data_ptr(), stream)
             eng.backward_batch_device(pos_d.data_ptr(), b["boxes"], g_d.data_ptr(), None,
@@ -300,8 +448,13 @@ def adjoint_batch(torch, E, S, reps, log):
         phases = {k: round(v[0], 3) for k, v in tab.items() if v[1] > 0}
         gq = np.concatenate([g[k] for k in ("q", "radius", "alpha")])
         gq0 = np.concatenate([ref[1][k] for k in ("q", "radius", "alpha")])
+        fl = batch_flops(b, P_batch, fwd_it, adj_it)
+        peak = PEAK_TFLOPS[prec]
         out[prec] = {
             "snapshot_gradients_per_s": round(64 / dt, 2), "ms_per_batch": round(1000 * dt, 3),
+            "roofline": {"bound": "valu", "flops_per_batch": fl, "achieved": round(fl / dt / 1e12, 3), "peak": peak,
+                         "unit": "TFLOP/s", "frac": round(fl / dt / 1e12 / peak, 4)},
+            "list_builds": eng.list_builds(),
             "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "fused_setup": fused,
             "phases_ms_per_batch_in_line": phases,
             "autotune_ms": {f"{k[0]}{'f' if k[1] else 's'}": round(v, 4) for k, v in tune.items()},
@@ -312,22 +465,63 @@ def adjoint_batch(torch, E, S, reps, log):
     return {"snapshots": 64, "atoms": n, "tol": 1e-8, **out}
 
 
-def cpu_baseline(sy, sets, budget_s=60.0):
-    """The FP64 CPU oracle (single thread) on the same workload and guess: one full evaluation."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return next(ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        return "unknown"
+
+
+def sub_box(sy, S):
+    """A 1/8 sample of the workload: the waters whose oxygen lies in [0, L/2)^3, as a periodic box of edge L/2
+    at the same density and settings (PME grid at the same spacing)."""
+    L = sy.box[0, 0]
+    o = np.arange(0, sy.n, 3)
+    keep = o[np.all((sy.pos[o] % L) < 0.5 * L, axis=1)]
+    idx = np.sort(np.concatenate([keep, keep + 1, keep + 2]))
+    remap = -np.ones(sy.n, np.int64)
+    remap[idx] = np.arange(len(idx))
+    cov = np.isin(sy.cov_atom, idx)
+    st = dict(sy.settings, grid=tuple(k // 2 for k in sy.settings["grid"]))
+    return S.System(f"{sy.name}-eighth", sy.pos[idx] % L, sy.box * 0.5, sy.q[idx], sy.radius[idx], sy.alpha[idx],
+                    remap[sy.cov_atom[cov]], remap[sy.cov_partner[cov]], sy.cov_strength[cov], st)  # fmt: skip
+
+
+def cpu_baseline(sy, sets, S):
+    """The reference's own FP64 engine, core/pgm_core.cpp (oracle/_ref, built from the reference sources),
+    single-threaded as it is, on a bounded sample of the workload: one evaluation of its 1/8 sub-box (12,000
+    atoms) from the reference's default guess (alpha E, no history), scaled by the atom ratio to the full box.
+    That scaling favours the reference (its pair search is O(N^2)).  Falls back to this build's FP64 oracle
+    (oracle/pgm_oracle.cpp) where the reference library was not built.  Returns (baseline, oracle energy and
+    forces of the full workload from the GPU's guess, their CG iterations)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    sub = sub_box(sy, S)
+    try:
+        r, kind = O.Oracle(sub, kind="reference"), "reference"
+    except (OSError, RuntimeError):
+        r, kind = O.Oracle(sub), "port"
+    t0 = time.perf_counter()
+    r.compute(sub.pos, sub.box)
+    dt = time.perf_counter() - t0
+    src = "core/pgm_core.cpp (oracle/_ref)" if kind == "reference" else "oracle/pgm_oracle.cpp"
+    cb = {
+        "value": round(sub.n / sy.n / dt, 6), "unit": "evals/s", "cores": 1, "kind": kind,
+        "sample": f"one FP64 evaluation of the workload's 1/8 sub-box ({sub.n} atoms, {sub.box[0, 0]:.2f} nm, "
+                  f"PME {sub.settings['grid'][0]}^3, tau 1e-4, the reference's default alpha E guess: "
+                  f"{r.iterations()} CG iterations) by {src} on 1 host thread in {dt:.2f} s, scaled by "
+                  f"{sub.n}/{sy.n} to the full box",
+        "cpu": cpu_model(), "nproc": os.cpu_count(),
+    }  # fmt: skip
+    del r
+    # the checker: this build's oracle on the full workload from the GPU engines' guess
     o = O.Oracle(sy)
     for mu in sets[::-1]:
         o.push_history(mu)
-    t0 = time.perf_counter()
     e, f = o.compute(sy.pos, sy.box)
-    dt = time.perf_counter() - t0
-    return {
-        "value": round(1.0 / dt, 5), "unit": "evals/s", "cores": 1, "kind": "port",
-        "sample": f"one complete FP64 evaluation of the same 96,000-atom workload and guess on 1 host thread "
-                  f"({dt:.1f} s, {o.iterations()} CG iterations; oracle/pgm_oracle.cpp)",
-    }, e, f, o.iterations()  # fmt: skip
+    return cb, e, f, o.iterations()
 
 
 def main():
@@ -340,6 +534,8 @@ def main():
     ap.add_argument("--no-gvdw", action="store_true", help="skip the GVDW-on timing of the same workload")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-large", action="store_true", help="skip the 999,000-atom box")
+    ap.add_argument("--no-md", action="store_true", help="skip the MD-like steps and the mu3 variant")
+    ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--large-only", action="store_true",
                     help="profiler pass over the 999,000-atom box alone (both precisions, --steps timed "
                          "evaluations each); prints its figures, not the JSON line")  # fmt: skip
@@ -403,6 +599,7 @@ def main():
         t, f, _ = eng.compute(sy.pos, box)  # checked evaluation (host arrays)
         results[prec] = (t, f, eng.iterations())
         tuned[prec] = eng.autotune(3)
+        md_like.lanes[prec] = tuned[prec][0]
         log(f"{prec}: {eng.iterations()} CG iterations, E = {E.total_energy(t):.6f}, pair passes {tuned[prec][0]}")
     P = engines["double"].num_pairs()
     C = engines["double"].num_candidates()
@@ -451,10 +648,12 @@ def main():
         "algorithmic_tflops": round(flops_per_eval(P, sy.n, sy.settings["grid"], n_it) * value / 1e12, 3),
         "phases_ms_per_eval": {p: {k: round(v["ms_per_eval"], 4) for k, v in phases[p].items()} for p in phases},
         "roofline": roof,
+        "eval_roofline": {"mixed": eval_roofline(P, sy.n, sy.settings["grid"], n_it, value, "mixed"),
+                          "double": eval_roofline(P, sy.n, sy.settings["grid"], results["double"][2], v64, "double")},
     }  # fmt: skip
     if not a.no_cpu_baseline:
-        log("CPU baseline (FP64 oracle, 1 thread) ...")
-        cb, eo, fo, it_o = cpu_baseline(sy, sets)
+        log("CPU baseline (the reference's pgm_core.cpp, 1 thread) ...")
+        cb, eo, fo, it_o = cpu_baseline(sy, sets, S)
         out["cpu_baseline"] = cb
         out["accuracy"]["force_relrms_fp64_gpu_vs_oracle"] = E.rel_rms(fd, fo)
         out["accuracy"]["force_relrms_mixed_vs_oracle"] = E.rel_rms(fm, fo)
@@ -462,6 +661,9 @@ def main():
         out["accuracy"]["oracle_cg_iterations"] = it_o
     else:
         out["cpu_baseline"] = None
+    if not a.no_md:
+        out["md_like"] = md_like(torch, E, sy, sets, a.md_steps, log)
+        out["mu3"] = mu3_variant(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)
     if not a.no_gvdw:
         out["gvdw"] = gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)
         out["gvdw"]["pair_final_ms_without"] = {p: round(phases[p]["pair_final"]["ms_per_eval"], 4) for p in phases}

@@ -110,15 +110,27 @@ int pgm_forces(pgm_engine* e, double* out);
  * whose work so far comes first. */
 int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                        int stride, double fixed_scale, void* stream, pgm_terms* terms);
-/* launchDevice / finishDevice (core/cuda/pgm_cuda.h:102-104): enqueue, then wait; valid = 0 when the pair
- * list turned out to be stale for these positions (forces incomplete; the list is rebuilt: repeat). */
+/* launchDevice / finishDevice (core/cuda/pgm_cuda.h:95-104): pgm_compute_device in two parts, the caller doing
+ * other work on the GPU in between.  pgm_launch_device enqueues the whole evaluation behind the work so far on
+ * stream and returns without waiting for the GPU: the pair list is checked on the device, the host predicting
+ * from the last step's displacement and largest move whether it will hold (rebuilt before the launch if not);
+ * the CG iterations enqueued are the most the last 16 solves needed, each skipped on the device once the solve
+ * has converged.  Exceptions that wait: a list build, and the first launch of an engine or a repeat after a
+ * short solve, whose CG checks convergence on the host between chunks of iterations.  The forces are added to
+ * force_fixed in every case.  pgm_finish_device waits for the launch and sets valid = 0 when its list turned
+ * out to be stale for these positions (the list is rebuilt now) or its solve needed more iterations than were
+ * enqueued: the forces added are then not valid, and the caller repeats the launch with the same positions and
+ * new_step (the same guess and history entry: an invalid launch does not advance the history). */
 int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int new_step, long long* force_fixed,
                       int stride, double fixed_scale, void* stream);
 int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid);
 /* waitDevice (core/cuda/pgm_cuda.h:105-111): instead of pgm_finish_device, the caller's stream waits on the
- * GPU for the forces of the last launch (no host wait).  This engine checks the pair list inside the launch,
- * so no check is ever late (pgm_late_list_checks stays 0). */
+ * GPU for the forces of the launch (no host wait).  Its completion happens at the next call of any kind: a
+ * stale list or a short solve found then is counted by pgm_late_list_checks (the forces were used). */
 int pgm_wait_device(pgm_engine* e, void* stream);
+/* No counterpart in the reference: 1 while the GPU work of a launch is still running, 0 when it has completed
+ * or no launch is in flight (a poll without waiting, for callers that schedule other host work). */
+int pgm_launch_pending(const pgm_engine* e);
 
 /* No counterpart in the reference: compute on FP64 device arrays, for callers that keep the system in HBM
  * (positions 3n doubles in the caller's order; forces added to f_dev, 3n doubles, may be NULL). */
@@ -194,10 +206,15 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
  * two variants (compaction fused into the setup pass, or as a pass of its own) and 16, 32 and 64 lanes per
  * atom (reps repetitions each), keeps the fastest for later evaluations and returns it in *lanes, *fused
  * (each may be NULL) and the six timings in ms[6] (ms per evaluation, fused 16/32/64 then separate
- * 16/32/64; may be NULL).  pgm_set_lanes sets them directly (lanes 16, 32 or 64; 16 and separate passes by
- * default). */
+ * 16/32/64, each the faster of the two field-pass forms in FP32; may be NULL).  pgm_set_lanes sets them
+ * directly (lanes 16, 32 or 64; 16 and separate passes by default). */
 int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms);
 int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
+/* No counterpart in the reference: the FP32 (PGM_MIXED) field pass of the CG products with its pair kernels
+ * evaluated on the fly (1) or read from the kernels the setup pass stored (0); pgm_autotune times both and keeps
+ * the faster.  pgm_field_otf: the current choice. */
+int pgm_set_field_otf(pgm_engine* e, int on);
+int pgm_field_otf(const pgm_engine* e);
 
 /* No counterpart in the reference (whose engine prints host wall clocks under PGM_CUDA_TIMING,
  * core/cuda/pgm_cuda.cu:1814-1817): HIP-event timing of the kernel groups of every evaluation, for

@@ -72,6 +72,10 @@ struct DevBuf {
     U* as() const {
         return static_cast<U*>(p);
     }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 // Pinned host staging that only grows.
@@ -190,7 +194,13 @@ public:
             HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
             HIPCHK(hipEventCreateWithFlags(&evFork_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evJoin_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evDone_, hipEventDisableTiming));
             pmeSt_ = st_;
+            // pinned words read back by the evaluations (allocated here: a launch makes no allocation that
+            // could wait for the GPU)
+            listH_.reserve(64);
+            cgH_.reserve(64);
+            scalH_.reserve(64);
         });
     }
     ~DeviceEngine() {
@@ -208,7 +218,7 @@ public:
             (void) hipEventDestroy(evOut_);
         if (st2_)
             (void) hipStreamSynchronize(st2_);
-        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_})
+        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_, evDone_})
             if (ev)
                 (void) hipEventDestroy(ev);
         if (st2_)
@@ -257,10 +267,38 @@ public:
         }
         withDevice([&] {
             HIPCHK(hipStreamSynchronize(st_));
-            if (n != n_ || ng != nGlobal_) {
+            const bool resized = n != n_ || ng != nGlobal_;
+            if (resized) {
                 histSlots_.clear();
                 haveList_ = false;
                 lastIterations_ = 0;
+            }
+            // The same atoms again (new parameters, or another order named by the global indices): the list over
+            // the spatially sorted atoms is kept and renamed (core/cuda/pgm_cuda.cu:2519-2533,2586-2632).
+            // newOf[old caller index] = new caller index of the same global atom.
+            std::vector<int> newOf;
+            if (!resized && haveList_ && !extList_ && (int) gidxH_.size() == n) {
+                std::vector<int> localOf(ng, -1);
+                for (int i = 0; i < n; i++)
+                    localOf[gi[i]] = i;
+                newOf.resize(n);
+                std::vector<char> used(n, 0);
+                std::vector<int> sysOfAtom;
+                if (batch_) {   // a permutation must keep every atom in its system of a batch
+                    sysOfAtom.resize(n);
+                    for (size_t y = 0; y < sys_.size(); y++)
+                        for (int i = sys_[y].a0; i < sys_[y].a1; i++)
+                            sysOfAtom[i] = (int) y;
+                }
+                for (int i = 0; i < n && !newOf.empty(); i++) {
+                    const int k = localOf[gidxH_[i]];
+                    if (k < 0 || used[k] || (batch_ && k != i && sysOfAtom[k] != sysOfAtom[i]))
+                        newOf.clear();
+                    else {
+                        used[k] = 1;
+                        newOf[i] = k;
+                    }
+                }
             }
             if (n != n_ || sys_.empty()) {
                 // a new atom count: one system (a batch partition is set again with pgm_set_batch)
@@ -291,7 +329,10 @@ public:
             upload(rgD_, rg.data(), n*sizeof(double));
             upload(alphaD_, alphaH_.data(), n*sizeof(double));
             upload(gidxD_, gi.data(), n*sizeof(int));
-            haveList_ = false;   // parameters of the sorted arrays change: rebuild
+            if (newOf.empty())
+                haveList_ = false;
+            else
+                renameList(newOf);
             gvdwOn_ = false;     // GVDW is set again after every setAtoms
             extList_ = false;    // and so is a caller's pair list
             if (covN_ < 0)
@@ -503,6 +544,7 @@ public:
         if (strainOut && sys_.size() > 1)
             throw std::invalid_argument("pgm_hip: the strain derivative is per system: not available for a batch");
         adjointReady_ = false;   // set again only when this evaluation completes
+        prevValid_ = false;      // (a next launch's move prediction starts over)
         withDevice([&] {
             HIPCHK(hipEventRecord(evIn_, caller));
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
@@ -570,6 +612,161 @@ public:
             HIPCHK(hipEventRecord(evOut_, st_));
             HIPCHK(hipStreamWaitEvent(caller, evOut_, 0));
         });
+    }
+
+    // launchDevice (core/cuda/pgm_cuda.cu:2754-2774): the whole evaluation enqueued behind the caller's stream, no
+    // host wait.  The pair list is checked on the device and read by the completion; before the launch the host
+    // predicts from the last step's displacement and largest move whether the list will hold and rebuilds it if
+    // not (core/cuda/pgm_cuda.cu:1912-1928).  The CG iterations enqueued are the most that the last 16 solves
+    // needed, each behind its device flag; the first launch of an engine (no such count yet), and a repeat after
+    // a solve that needed more, wait on the host between chunks of iterations instead.  The forces are added to
+    // the fixed-point buffer in any case; the completion says whether they are valid.
+    void launchDevice(const void* posq4, const double* boxes, bool newStep, long long* fixed, int stride,
+                      double fixedScale, hipStream_t caller) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: launch before setAtoms");
+        if (!posq4 || !boxes || !fixed)
+            throw std::invalid_argument("pgm_hip: launch needs positions, a box and a force buffer");
+        settle();
+        adjointReady_ = false;
+        withDevice([&] {
+            HIPCHK(hipEventRecord(evIn_, caller));
+            HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
+            setBoxes(boxes);
+            stagePositions(PosSource::DeviceF4, posq4);
+            // (a repeat is at the positions of the launch it repeats: no move ahead to predict)
+            const double ahead = repeatNext_ ? 0.0 : 2.0*std::sqrt(lastS2_);
+            repeatNext_ = false;
+            bool built = false;
+            if (!haveList_ || (!extList_ && staleFor(lastD2_, ahead))) {
+                tb(kPhList);
+                buildList();
+                te();
+                built = true;
+            }
+            pendingChecking_ = !extList_ && !built;
+            pendingPrev_ = pendingChecking_ && prevValid_;
+            if (pendingChecking_)
+                startListCheck(pendingPrev_);
+            posPrevD_.reserve(3*sizeof(double)*n_);
+            HIPCHK(hipMemcpyAsync(posPrevD_.p, posDev_, 3*sizeof(double)*n_, hipMemcpyDeviceToDevice, st_));
+            prevValid_ = !extList_;
+            const bool mixed = s_.precision == PGM_MIXED;
+            mixed ? prep<float>() : prep<double>();
+            asyncEval_ = true;
+            asyncCg_ = !syncCgNext_ && numRecent_ > 0;
+            asyncPredict_ = predictedIterations();
+            syncCgNext_ = false;
+            pendingPush_ = -1;
+            try {
+                mixed ? run<float>(newStep, false) : run<double>(newStep, false);
+            }
+            catch (...) {
+                asyncEval_ = asyncCg_ = false;
+                throw;
+            }
+            const bool cgAhead = asyncCg_;
+            asyncEval_ = asyncCg_ = false;
+            if (!cgAhead)
+                asyncStop_ = -1;   // the solve completed on the host: iterations_, residual_ are final
+            tb(kPhFinal);
+            launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(), fcovD_.as<double>(),
+                            s_.ke, nullptr, fixed, stride, fixedScale);
+            te();
+            const size_t resBytes = sys_.size()*kResCount*sizeof(double);
+            resH_.reserve(resBytes);
+            HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, resBytes, hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipEventRecord(evDone_, st_));
+            inFlight_ = true;
+        });
+    }
+
+    // finishDevice (core/cuda/pgm_cuda.cu:2782-2788,2932-2959): waits for the launch; false when its pair list
+    // was stale for those positions (the list is rebuilt now) or its solve needed more iterations than were
+    // enqueued: the forces added are not valid, and the caller repeats the launch (same positions, same guess
+    // and history entry, which the completion has not advanced).
+    bool finishDevice() {
+        if (!inFlight_)
+            return true;
+        bool valid = true;
+        withDevice([&] { valid = complete(true); });
+        return valid;
+    }
+
+    // waitDevice (core/cuda/pgm_cuda.cu:2777-2780): the caller's stream waits on the GPU for the launch's forces.
+    void waitDevice(hipStream_t stream) {
+        withDevice([&] {
+            if (inFlight_)
+                HIPCHK(hipStreamWaitEvent(stream, evDone_, 0));
+            else
+                waitOn(stream);
+        });
+    }
+
+    bool launchPending() const {
+        return inFlight_ && hipEventQuery(evDone_) == hipErrorNotReady;
+    }
+
+    // The completion of a launch that nobody finished (waitDevice): at the next call of any kind.  Its forces
+    // were used, so a stale list or a short solve found now only counts as late (lateListChecks).
+    void settle() {
+        if (!inFlight_)
+            return;
+        withDevice([&] { (void) complete(false); });
+    }
+
+    bool complete(bool canRepeat) {
+        inFlight_ = false;
+        waitEvent(evDone_);
+        bool stale = false;
+        if (pendingChecking_) {
+            float d2, s2;
+            std::memcpy(&d2, listH_.p, sizeof(float));
+            std::memcpy(&s2, listH_.as<char>()+sizeof(float), sizeof(float));
+            if (pendingPrev_)
+                lastS2_ = s2;
+            stale = staleFor(d2, 0.0);
+            lastD2_ = stale ? 0.0 : d2;
+        }
+        else
+            lastD2_ = 0.0;   // a list built for this launch
+        int its = iterations_;
+        double err = residual_;
+        bool shortSolve = false;
+        if (asyncStop_ >= 0 && polarizable_ && s_.polarizable) {
+            CgState cs;
+            int done;
+            std::memcpy(&cs, cgH_.p, sizeof(CgState));
+            std::memcpy(&done, cgH_.as<char>()+sizeof(CgState), sizeof(int));
+            its = cs.iterations;
+            err = cs.err;
+            shortSolve = !done && asyncStop_ < s_.max_iter;
+            noteIterations(shortSolve ? asyncStop_+1 : its);
+        }
+        if (stale) {
+            tb(kPhList);
+            buildList();
+            te();
+            lastD2_ = 0.0;
+        }
+        if (stale || shortSolve) {
+            if (canRepeat) {
+                repeats_++;
+                repeatNext_ = true;
+                syncCgNext_ = shortSolve;
+                collectTiming();
+                return false;
+            }
+            lateChecks_++;
+        }
+        commitPush(pendingPush_);
+        pendingPush_ = -1;
+        iterations_ = its;
+        lastIterations_ = its;
+        residual_ = err;
+        collectTiming();
+        assemble(false);
+        return true;
     }
 
     // True when an evaluation finished at bitwise these positions and boxes and nothing has changed the engine's
@@ -698,27 +895,41 @@ public:
         if (reps < 1)
             throw std::invalid_argument("pgm_hip: autotune needs reps >= 1");
         int best = lanes_;
-        bool bestFused = fused_;
+        bool bestFused = fused_, bestOtf = fieldOtf_;
         withDevice([&] {
             double bestMs = 1e300;
+            const bool mixed = s_.precision == PGM_MIXED;
             for (int v = 0; v < 2; v++) {
                 const bool fused = v == 0;
                 for (int c = 0; c < 3; c++) {
                     const int lanes = kPairLaneChoices[c];
-                    const double ms = s_.precision == PGM_MIXED ? timePairPasses<float>(lanes, fused, reps)
-                                                                : timePairPasses<double>(lanes, fused, reps);
-                    tuneMs_[3*v+c] = ms;
-                    if (ms < bestMs) {
-                        bestMs = ms;
-                        best = lanes;
-                        bestFused = fused;
+                    tuneMs_[3*v+c] = 1e300;
+                    // FP32 also tries the field pass with its kernels evaluated on the fly
+                    for (int o = 0; o < (mixed ? 2 : 1); o++) {
+                        const bool otf = o == 1;
+                        const double ms = mixed ? timePairPasses<float>(lanes, fused, reps, otf)
+                                                : timePairPasses<double>(lanes, fused, reps, false);
+                        tuneMs_[3*v+c] = std::min(tuneMs_[3*v+c], ms);
+                        if (ms < bestMs) {
+                            bestMs = ms;
+                            best = lanes;
+                            bestFused = fused;
+                            bestOtf = otf;
+                        }
                     }
                 }
             }
             lanes_ = best;
             fused_ = bestFused;
+            fieldOtf_ = bestOtf;
         });
         return best;
+    }
+    bool fieldOtf() const { return fieldOtf_; }
+    void setFieldOtf(bool on) {
+        if (on && s_.precision != PGM_MIXED)
+            throw std::invalid_argument("pgm_hip: the on-the-fly field pass is an FP32 (PGM_MIXED) variant");
+        fieldOtf_ = on;
     }
     int lanes() const { return lanes_; }
     bool fused() const { return fused_; }
@@ -733,6 +944,7 @@ public:
     int iterations() const { return iterations_; }
     double residual() const { return residual_; }
     long listBuilds() const { return listBuilds_; }
+    long listRenames() const { return listRenames_; }
     long long numCandidates() const { return nCand_; }
     // Pairs within the cutoff at the last evaluation (each counted once).
     long long numPairs() {
@@ -748,7 +960,8 @@ public:
         });
         return total/2;
     }
-    long lateListChecks() const { return 0; }
+    long lateListChecks() const { return lateChecks_; }
+    long repeatedLaunches() const { return repeats_; }
     // The caller's stream waits on the GPU for the last evaluation's outputs (no host wait).
     void waitOn(hipStream_t stream) {
         withDevice([&] { HIPCHK(hipStreamWaitEvent(stream, evOut_, 0)); });
@@ -861,7 +1074,18 @@ private:
     int brick_ = kBrickSmall;   // edge of the spreading bricks, chosen by buildLayout
     // list state
     bool haveList_ = false;
-    long listBuilds_ = 0;
+    long listBuilds_ = 0, listRenames_ = 0;
+    // the asynchronous launch (launchDevice / finishDevice): its state between the two calls
+    bool inFlight_ = false, asyncEval_ = false, asyncCg_ = false, syncCgNext_ = false, repeatNext_ = false;
+    bool pendingChecking_ = false, pendingPrev_ = false, prevValid_ = false;
+    int asyncPredict_ = 0, asyncStop_ = 0, pendingPush_ = -1;
+    double lastD2_ = 0, lastS2_ = 0;   // squared: displacement since the build at the last step, the last move
+    long lateChecks_ = 0, repeats_ = 0;
+    int recentIts_[16] = {0}, numRecent_ = 0, recentPos_ = 0;
+    hipEvent_t evDone_ = nullptr;
+    DevBuf posPrevD_;
+    HostBuf cgH_;
+    DevBuf newOfD_, order2D_, rank2D_, build2D_;
     long long nCand_ = 0;
     int lastIterations_ = 0, iterations_ = 0;
     double residual_ = 0;
@@ -870,6 +1094,7 @@ private:
     bool haveResults_ = false;
     int lanes_ = 16;
     bool fused_ = false;
+    bool fieldOtf_ = false;   // FP32: the field pass evaluates its kernels instead of reading the stored ones
     double tuneMs_[6] = {0, 0, 0, 0, 0, 0};
     // kernel timing
     struct Timed {
@@ -952,7 +1177,7 @@ private:
     DevBuf permD_, permTD_, edD_, erD_, bD_, xD_, rD_, zD_, pD_, apD_, tdD_, trD_, pTD_, dD_, dTD_, etotD_;
     DevBuf fpairD_, frecD_, fcovD_, forcesOutD_, tmpD_;
     DevBuf brickOfD_, pslotD_, brickCountD_, brickStartD_, pmeAtomD_, baseD_, splD_;
-    DevBuf gridD_, cplxD_, inflD_, twD_, bmodD_;
+    DevBuf gridD_, gridFixedD_, cplxD_, inflD_, twD_, bmodD_;
     DevBuf cgPartD_, partD_, resD_, cgD_, doneD_, histD_, maxBitsD_;
     DevBuf perAtomD_, chunkSumD_;
     DevBuf fftGroupsD_;     // the merged transform launches' group table
@@ -1001,6 +1226,26 @@ private:
         if (bytes)
             HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    // The list kept over a renaming of the caller's indices (newOf: old -> new): order, rank and the build
+    // positions renamed, the sorted parameters gathered again from the new arrays.
+    void renameList(const std::vector<int>& newOf) {
+        const int n = n_;
+        upload(newOfD_, newOf.data(), n*sizeof(int));
+        order2D_.reserve(n*sizeof(int));
+        rank2D_.reserve(n*sizeof(int));
+        build2D_.reserve(3*n*sizeof(double));
+        launchRename(st_, n, newOfD_.as<int>(), orderD_.as<int>(), rankD_.as<int>(), posBuildD_.as<double>(),
+                     order2D_.as<int>(), rank2D_.as<int>(), build2D_.as<double>());
+        orderD_.swap(order2D_);
+        rankD_.swap(rank2D_);
+        posBuildD_.swap(build2D_);
+        launchPermute(st_, n, orderD_.as<int>(), qD_.as<double>(), qSD_.as<double>());
+        launchPermute(st_, n, orderD_.as<int>(), alphaD_.as<double>(), alphaSD_.as<double>());
+        launchPermute(st_, n, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
+        HIPCHK(hipStreamSynchronize(st_));
+        listRenames_++;
     }
 
     int systemOf(int atom) const {
@@ -1213,6 +1458,7 @@ private:
         // device arrays of the layout
         const size_t tsz = mixed ? sizeof(float) : sizeof(double), t2 = 2*tsz;
         gridD_.reserve((size_t) realTotal_*tsz);
+        gridFixedD_.reserve((size_t) realTotal_*sizeof(long long));
         cplxD_.reserve((size_t) cplxTotal_*t2);
         inflD_.reserve((size_t) cplxTotal_*tsz);
         twD_.reserve((size_t) twTotal_*t2);
@@ -1378,16 +1624,25 @@ private:
     // The list check in two halves: the largest displacement since the build is copied to pinned host memory
     // without waiting (the maximum starts at zero: every preparation clears it for the next check; a value
     // left over by an interrupted evaluation can only be larger, so at worst it rebuilds early).
-    void startListCheck() {
+    // withPrev: also the largest move since the previous launch's positions (posPrevD_), into the second word.
+    void startListCheck(bool withPrev = false) {
         launchMaxDisplacement(st_, n_, posDev_, posBuildD_.as<double>(), boxSel<double>(), maxBitsD_.as<unsigned>());
+        if (withPrev)
+            launchMaxDisplacement(st_, n_, posDev_, posPrevD_.as<double>(), boxSel<double>(),
+                                  maxBitsD_.as<unsigned>()+1);
         listH_.reserve(64);
-        HIPCHK(hipMemcpyAsync(listH_.p, maxBitsD_.p, sizeof(unsigned), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipMemcpyAsync(listH_.p, maxBitsD_.p, 2*sizeof(unsigned), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipEventRecord(evList_, st_));
     }
     bool listStale() {
         waitEvent(evList_);
         float d2;
         std::memcpy(&d2, listH_.p, sizeof(float));
+        return staleFor(d2, 0.0);
+    }
+    // A list built for displacements below skin/2 is stale when twice the largest displacement since the build
+    // (plus ahead, a predicted further move) and the box change exceed the skin (core/cuda/pgm_cuda.cu:1898-1928).
+    bool staleFor(double d2, double ahead) const {
         double boxMove = 0, skin = 1e300;
         for (const Sys& y : sys_) {
             double dH = 0;
@@ -1400,7 +1655,7 @@ private:
             boxMove = std::max(boxMove, dH);
             skin = std::min(skin, y.skinNow);
         }
-        return 2.0*std::sqrt((double) d2)+boxMove > skin;
+        return 2.0*(std::sqrt(d2)+ahead)+boxMove > skin;
     }
 
     // The caller's pair list in place of the engine's own: the caller's order kept (identity permutation), its
@@ -1605,16 +1860,22 @@ private:
                   const typename Vec4<T>::type* dipDot = nullptr, const typename Vec4<T>::type* disp = nullptr) {
         typedef typename Vec2<T>::type T2;
         const PmeSel ps = pmeSel();
+        // the charge/dipole grids through the fixed-point accumulator (order-independent), a tangent density
+        // (dipDot) straight into grid
+        long long* fixed = dipDot ? nullptr : gridFixedD_.as<long long>();
         tb(kPhSpread, it);
-        launchZero<T>(pmeSt_, grid, (size_t) realTotal_, skip);
+        if (fixed)
+            launchZero<long long>(pmeSt_, fixed, (size_t) realTotal_, skip);
+        else
+            launchZero<T>(pmeSt_, grid, (size_t) realTotal_, skip);
         launchSpread<T>(pmeSt_, ps, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
-                        posq, withQ ? 1 : 0, grid, skip, dipDot, disp);
+                        posq, withQ ? 1 : 0, grid, fixed, skip, dipDot, disp);
         te();
         tb(kPhFft, it);
         const BoxT<double> one = makeBox<double>(sys_[0].H, sys_[0].recip);
         launchPmeConvolve<T>(pmeSt_, fftL_, grid, cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), one,
                              batch_ ? groupBoxD_.as<BoxT<double>>() : nullptr, s_.beta, energyMode, strain ? 1 : 0,
-                             partD_.as<double>(), skip);
+                             partD_.as<double>(), skip, fixed);
         if (energyMode)
             launchRangeSums(pmeSt_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
                             kResCount, res+kResRec);
@@ -1639,7 +1900,7 @@ private:
         fork();
         tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                           g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_);
+                           g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_, fieldOtf_, (T) s_.beta);
         te();
         pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
         join();
@@ -1670,23 +1931,25 @@ private:
         precond(0, true);
         te();
         int it = 0;
+        if (asyncCg_) {
+            // enqueued ahead, no host wait: the predicted iterations, then the state and the flag that says
+            // whether they sufficed, read by the launch's completion
+            const int stop = std::min(s_.max_iter, asyncPredict_);
+            for (; it < stop; it++)
+                cgIteration<T>(posT, x, it, pc);
+            cgH_.reserve(64);
+            HIPCHK(hipMemcpyAsync(cgH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipMemcpyAsync(cgH_.as<char>()+sizeof(CgState), doneD_.as<int>()+stop, sizeof(int),
+                                  hipMemcpyDeviceToHost, st_));
+            asyncStop_ = stop;
+            return -1;
+        }
         int chunk = std::max(1, lastIts);
         scalH_.reserve(64);
         while (true) {
             const int stop = std::min(s_.max_iter, it+chunk);
-            for (; it < stop; it++) {
-                const int* skip = doneD_.as<int>()+it;
-                matvec<T>(posT, pTD_.as<T4>(), skip, it);
-                tb(kPhCg, it);
-                launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(),
-                           trD_.as<double>(), cself, apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
-                launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(), x,
-                               rD_.as<double>(), zD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>(), pc);
-                precond(it, false);
-                launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(),
-                                     pTD_.as<T4>(), cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
-                te();
-            }
+            for (; it < stop; it++)
+                cgIteration<T>(posT, x, it, pc);
             HIPCHK(hipMemcpyAsync(scalH_.p, cgD_.p, sizeof(CgState), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipMemcpyAsync(scalH_.as<char>()+sizeof(CgState), doneD_.as<int>()+it, sizeof(int),
                                   hipMemcpyDeviceToHost, st_));
@@ -1701,7 +1964,45 @@ private:
         std::memcpy(&cs, scalH_.p, sizeof(CgState));
         lastIts = cs.iterations;
         err = cs.err;
+        noteIterations(cs.iterations);
         return cs.iterations;
+    }
+
+    // One CG iteration behind the device flag done[it] (every kernel returns at once when it is set).
+    template<class T>
+    void cgIteration(const typename Vec4<T>::type* posT, double* x, int it, const Precond& pc) {
+        typedef typename Vec4<T>::type T4;
+        typedef typename Vec2<T>::type T2;
+        const int n = n_;
+        const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        const int* skip = doneD_.as<int>()+it;
+        matvec<T>(posT, pTD_.as<T4>(), skip, it);
+        tb(kPhCg, it);
+        launchCgAp(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), tdD_.as<double>(), trD_.as<double>(), cself,
+                   apD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>());
+        launchCgUpdate(st_, n, it, alphaSD_.as<double>(), pD_.as<double>(), apD_.as<double>(), x, rD_.as<double>(),
+                       zD_.as<double>(), cgPartD_.as<double>(), doneD_.as<int>(), pc);
+        if (pc.pre)
+            launchPrecond<T>(st_, n, it, false, alphaSD_.as<double>(), rD_.as<double>(), apD_.as<double>(), posT,
+                             boxSel<T>(), srCountD_.as<int>(), srJD_.as<int>(), srGD_.as<T2>(), srOverD_.as<int>(),
+                             zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(),
+                             doneD_.as<int>());
+        launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(),
+                             cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+        te();
+    }
+
+    // The iterations the last 16 solves needed: their maximum is what an asynchronous launch enqueues.
+    void noteIterations(int need) {
+        recentIts_[recentPos_] = need;
+        recentPos_ = (recentPos_+1)%16;
+        numRecent_ = std::min(numRecent_+1, 16);
+    }
+    int predictedIterations() const {
+        int m = 1;
+        for (int k = 0; k < numRecent_; k++)
+            m = std::max(m, recentIts_[k]);
+        return m;
     }
 
     // Per-system sums of per-atom rows [n][width] into res[s][slot..slot+width).
@@ -1786,7 +2087,7 @@ private:
         baseD_.reserve(3*n*sizeof(int));
         splD_.reserve((size_t) n*9*s_.order*sizeof(T));
         resD_.reserve(sys_.size()*kResCount*sizeof(double));
-        maxBitsD_.reserve(sizeof(unsigned));
+        maxBitsD_.reserve(2*sizeof(unsigned));
         tb(kPhPrep);
         PrepArgs a{};
         a.n = n;
@@ -1896,7 +2197,9 @@ private:
                 te();
                 matvec<T>(pos, pTD_.as<T4>(), nullptr);
             }
-            iterations_ = cgSolve<T>(pos, bD_.as<double>(), xD_.as<double>(), lastIterations_, residual_);
+            const int its = cgSolve<T>(pos, bD_.as<double>(), xD_.as<double>(), lastIterations_, residual_);
+            if (its >= 0)   // (else enqueued ahead: known when the launch completes)
+                iterations_ = its;
             const int h = (int) histSlots_.size();
             const bool push = !frozen_ && (newStep || h == 0);
             if (push)
@@ -1945,17 +2248,24 @@ private:
                              perAtomD_.as<double>());
         systemSums(perAtomD_.as<double>(), 9, res, kResCov);
         te();
-        if (pushSlot >= 0) {
-            histSlots_.push_front(pushSlot);
-            while (histSlots_.size() > 4)
-                histSlots_.pop_back();
-        }
+        if (asyncEval_)
+            pendingPush_ = pushSlot;   // committed when the launch completes valid
+        else
+            commitPush(pushSlot);
         haveResults_ = true;
+    }
+
+    void commitPush(int slot) {
+        if (slot < 0)
+            return;
+        histSlots_.push_front(slot);
+        while (histSlots_.size() > 4)
+            histSlots_.pop_back();
     }
 
     // Milliseconds per evaluation spent in the pair passes at the given lane count (current lists and dipoles).
     template<class T>
-    double timePairPasses(int lanes, bool fused, int reps) {
+    double timePairPasses(int lanes, bool fused, int reps, bool otf) {
         typedef typename Vec4<T>::type T4;
         typedef typename Vec2<T>::type T2;
         const int n = n_;
@@ -1969,7 +2279,7 @@ private:
             pairSetup<T>(lanes, fused, pTD_.as<T4>());
             for (int k = 0; k <= iterations_; k++)
                 launchPairField<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                                   g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes);
+                                   g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes, otf, (T) s_.beta);
             launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(),
                                nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
                                fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), false,
@@ -2141,10 +2451,13 @@ void setGlobalError(const char* msg) { g_createError = msg; }
 namespace {
 
 template<class Fn>
-int guarded(pgm_engine* e, Fn fn) {
+// settle: first complete a launch still in flight (every call but launch, finish and wait)
+int guarded(pgm_engine* e, Fn fn, bool settle = true) {
     if (!e || !e->eng)
         return -2;
     try {
+        if (settle)
+            e->eng->settle();
         fn(*e->eng);
         e->eng->lastError.clear();
         return 0;
@@ -2268,23 +2581,25 @@ int pgm_launch_device(pgm_engine* e, const void* posq4, const double* box, int n
                       int stride, double fixed_scale, void* stream) {
     return guarded(e, [&](DeviceEngine& g) {
         requireSingle(g);
-        g.evaluate(PosSource::DeviceF4, posq4, box, new_step != 0, nullptr, nullptr, force_fixed, stride, fixed_scale,
-                   nullptr, nullptr, static_cast<hipStream_t>(stream), false);
-    });
+        g.launchDevice(posq4, box, new_step != 0, force_fixed, stride, fixed_scale, static_cast<hipStream_t>(stream));
+    }, false);
 }
 
 int pgm_finish_device(pgm_engine* e, pgm_terms* terms, int* valid) {
     return guarded(e, [&](DeviceEngine& g) {
+        const bool ok = g.finishDevice();
         if (terms)
             *terms = g.lastTerms();
         if (valid)
-            *valid = 1;
-    });
+            *valid = ok ? 1 : 0;
+    }, false);
 }
 
 int pgm_wait_device(pgm_engine* e, void* stream) {
-    return guarded(e, [&](DeviceEngine& g) { g.waitOn(static_cast<hipStream_t>(stream)); });
+    return guarded(e, [&](DeviceEngine& g) { g.waitDevice(static_cast<hipStream_t>(stream)); }, false);
 }
+
+int pgm_launch_pending(const pgm_engine* e) { return (e && e->eng && e->eng->launchPending()) ? 1 : 0; }
 
 int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,
                            double* strain, void* stream, pgm_terms* terms) {
@@ -2408,6 +2723,11 @@ int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms) {
 int pgm_set_lanes(pgm_engine* e, int lanes, int fused) {
     return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes, fused != 0); });
 }
+
+int pgm_set_field_otf(pgm_engine* e, int on) {
+    return guarded(e, [&](DeviceEngine& g) { g.setFieldOtf(on != 0); });
+}
+int pgm_field_otf(const pgm_engine* e) { return (e && e->eng) ? (e->eng->fieldOtf() ? 1 : 0) : -1; }
 
 int pgm_set_timing(pgm_engine* e, int on) {
     return guarded(e, [&](DeviceEngine& g) { g.setTiming(on != 0); });

@@ -94,6 +94,9 @@ void launchMaxDisplacement(hipStream_t st, int n, const double* pos, const doubl
 void launchGatherPositions(hipStream_t st, int n, const int* order, const double* pos, const double* qS,
                            BoxSel<double> box, double4* posD, float4* posF);
 void launchPermute(hipStream_t st, int n, const int* order, const double* in, double* out);
+// The same atoms under new caller indices (newOf: old -> new): order, rank and the build positions renamed.
+void launchRename(hipStream_t st, int n, const int* newOf, const int* order, const int* rank, const double* build,
+                  int* order2, int* rank2, double* build2);
 void launchToFloat(hipStream_t st, int n, const double* in, float* out);
 // Caller-order float4 (xyz, w ignored) or float xyz positions to FP64 xyz.
 void launchF4ToDouble(hipStream_t st, int n, const float4* posq, double* out);
@@ -161,11 +164,12 @@ void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type*
                             const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
                             typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, double* field,
                             double* fieldDual, int lanes);
-// The dipole part of a CG matrix-vector product from the stored G1, G2.
+// The dipole part of a CG matrix-vector product from the stored G1, G2 or, otf (FP32 only), with the kernels
+// evaluated on the fly from the positions and the radius parameters in the dipoles' w.
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     BoxSel<T> box, double* field, const int* skip, int lanes);
+                     BoxSel<T> box, double* field, const int* skip, int lanes, bool otf = false, T beta = T(0));
 // The CG preconditioner's short-range list: per atom, its compacted pairs within the radius (srJ partner, srG the
 // stored G1, G2; at most kPreCap, counted in srCount); *over set when an atom has more (cleared by the caller).
 template<class T>
@@ -266,13 +270,35 @@ void launchPmeBin(hipStream_t st, int n, const double4* posD, PmeSel ps, int* br
 template<class T>
 void launchPmePlace(hipStream_t st, int n, const double4* posD, PmeSel ps, const int* brickOf, const int* slot,
                     const int* brickStart, int* pmeAtom, int* base, T* spl);
-// Spread charges (withQ) and dipoles of the sorted atoms onto the real grids (zeroed by the caller) or, with
-// dipDot, the change of that density for displacements disp and dipole changes dipDot.
+// Spread charges (withQ) and dipoles of the sorted atoms onto the 64-bit fixed-point grids fixed (zeroed by the
+// caller; GridFixed<T>::scale units per grid unit, order-independent) or, with dipDot, the change of that
+// density for displacements disp and dipole changes dipDot onto the real grids grid (zeroed by the caller).
 template<class T>
 void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
                   const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, T* grid,
-                  const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
+                  long long* fixed, const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
                   const typename Vec4<T>::type* disp = nullptr);
+// The PME charge and dipole grids in 64-bit fixed point (units per grid unit): FP32 2^32, the reference's scale
+// (core/cuda/pgm_cuda.cu:52), for |grid values| < 2^31; FP64 2^40 (resolution 9.1e-13) for |grid values| < 2^23.
+template<class T>
+struct GridFixed;
+template<>
+struct GridFixed<float> {
+    static constexpr double scale = 4294967296.0, inv = 1.0/4294967296.0;
+};
+template<>
+struct GridFixed<double> {
+    static constexpr double scale = 1099511627776.0, inv = 1.0/1099511627776.0;
+};
+template<class T>
+__device__ __forceinline__ long long toFixed(T v) {
+    return __double2ll_rn((double) v*GridFixed<T>::scale);
+}
+template<class T>
+__device__ __forceinline__ T fromFixed(long long v) {
+    return (T) ((double) v*GridFixed<T>::inv);
+}
+
 // One grid shape's share of a merged transform launch: every shape of a batch goes through each stage (z, y,
 // x + convolution, y, z) in one launch; a block finds its shape by its block index.
 struct FftGroup {
@@ -298,11 +324,12 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]);
 // Forward transforms, influence function (energy and strain partials when asked), inverse transforms of all
 // grids of all groups; real: the stacked real grids, cplx/influence/tw: the stacked complex arrays.  boxes: per
 // stacked grid (nullptr: box for all).  partials: [.][10] (energy, strain 0..8) per x block, the blocks of a
-// group's grid k being rows partOff + [k, k+1) x blocks per grid.
+// group's grid k being rows partOff + [k, k+1) x blocks per grid.  fixed: the forward transforms read the
+// fixed-point grids (as the spread left them) instead of real; the inverse transforms write real.
 template<class T>
 void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
                        const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                       int energy, int strain, double* partials, const int* skip);
+                       int energy, int strain, double* partials, const int* skip, const long long* fixed);
 // Blocks of the x pass per grid (the rows of its energy partials) when ngrid grids of shape g are stacked.
 template<class T>
 int convolveBlocksPerGrid(PmeGrid g, int ngrid);

@@ -197,6 +197,21 @@ __global__ void kGatherPositions(int n, const int* order, const double* pos, con
         posF[s] = make_float4((float) x, (float) y, (float) z, (float) q);
 }
 
+// The same atoms under new caller indices (newOf: old caller index -> new): the sorted order and the list over
+// it stay; order, rank and the positions at the build (caller's order) are renamed into order2, rank2, build2.
+__global__ void kRename(int n, const int* newOf, const int* order, const int* rank, const double* build,
+                        int* order2, int* rank2, double* build2) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n)
+        return;
+    const int k = newOf[i];
+    order2[i] = newOf[order[i]];   // i read as a sorted index here
+    rank2[k] = rank[i];
+    build2[3*k] = build[3*i];
+    build2[3*k+1] = build[3*i+1];
+    build2[3*k+2] = build[3*i+2];
+}
+
 __global__ void kPermute(int n, const int* order, const double* in, double* out) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s < n)
@@ -346,6 +361,11 @@ void launchGatherPositions(hipStream_t st, int n, const int* order, const double
         kGatherPositions<true><<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, qS, box, posD, posF);
     else
         kGatherPositions<false><<<blocksFor(n, 256), 256, 0, st>>>(n, order, pos, qS, box, posD, posF);
+}
+
+void launchRename(hipStream_t st, int n, const int* newOf, const int* order, const int* rank, const double* build,
+                  int* order2, int* rank2, double* build2) {
+    kRename<<<blocksFor(n, 256), 256, 0, st>>>(n, newOf, order, rank, build, order2, rank2, build2);
 }
 
 void launchPermute(hipStream_t st, int n, const int* order, const double* in, double* out) {

@@ -397,6 +397,68 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     }
 }
 
+// The FP32 field pass with the kernels evaluated on the fly (no stored kernels read): per entry the partner's
+// index, position and dipole (w = its radius parameter rg_j; rg_i rides in the atom's own dipole w), G1 and G2
+// from r^2 and rg_i + rg_j (pairKernelsF, as the setup pass evaluates them).
+template<int L, int U, bool BATCH>
+__global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4* __restrict__ pos,
+                                                            const float4* __restrict__ dip,
+                                                            const int* __restrict__ candStart,
+                                                            const int* __restrict__ nb, const int* __restrict__ pj,
+                                                            BoxSel<float> bs, float beta, double* __restrict__ field,
+                                                            const int* skip) {
+    if (skip && *skip)
+        return;
+    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
+    const int s = gid/L, lane = gid%L;
+    if (s >= n)
+        return;
+    const BoxT<float> box = selectBox<BATCH>(bs, s);
+    const float4 pi = pos[s];
+    const float rgi = dip[s].w;
+    const int start = candStart[s], cnt = nb[s];
+    double ex = 0, ey = 0, ez = 0;
+    auto term = [&](const float4& pjv, const float4& d, int v) {
+        float x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
+        pairImage(x, y, z, box, v, bs.ext);
+        float G[3];
+        pairKernelsF<3>(rgi+d.w, beta, x*x+y*y+z*z, G);
+        const float c2 = (d.x*x+d.y*y+d.z*z)*G[2];
+        ex += (double) (c2*x-d.x*G[1]);
+        ey += (double) (c2*y-d.y*G[1]);
+        ez += (double) (c2*z-d.z*G[1]);
+    };
+    int k = lane;
+    for (; k+(U-1)*L < cnt; k += U*L) {
+        int v[U];
+        float4 pjv[U], d[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            v[u] = pj[start+k+u*L];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = entryAtom(v[u], bs.ext);
+            pjv[u] = pos[j];
+            d[u] = dip[j];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            term(pjv[u], d[u], v[u]);
+    }
+    for (; k < cnt; k += L) {
+        const int v = pj[start+k], j = entryAtom(v, bs.ext);
+        term(pos[j], dip[j], v);
+    }
+    ex = groupSum<L>(ex);
+    ey = groupSum<L>(ey);
+    ez = groupSum<L>(ez);
+    if (lane == 0) {
+        field[3*s] = ex;
+        field[3*s+1] = ey;
+        field[3*s+2] = ez;
+    }
+}
+
 // The GVDW pair term of one visit of pair (i, j) (combination rule: A = a_i a_j, C6 = c_i c_j, B = (b_i + b_j)/2
 // from gv = (a, c, b), b_ij = (rg_i + rg_j)^-1/2): energy into eg, force on i and, STRAIN, the strain into f and
 // acc in units of ke (invKe = 1/ke), as the electrostatic terms (core/cuda/pgm_cuda.cu:398-414).  sign = -1
@@ -855,8 +917,15 @@ void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type*
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     BoxSel<T> box, double* field, const int* skip, int lanes) {
+                     BoxSel<T> box, double* field, const int* skip, int lanes, bool otf, T beta) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
+        if constexpr (sizeof(T) == sizeof(float)) {
+            if (otf) {
+                kPairFieldOtf<L(), 4, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb,
+                                                                                       pj, box, beta, field, skip);
+                return;
+            }
+        }
         kPairField<T, L(), 4, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12,
                                                                                 box, field, skip);
     });
@@ -945,7 +1014,7 @@ void launchShortList(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                                             Vec2<T>::type*, BoxSel<T>, T, T, double*, double*, int);                  \
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
                                      const int*, const int*, const Vec2<T>::type*, BoxSel<T>, double*, const int*,    \
-                                     int);                                                                            \
+                                     int, bool, T);                                                                   \
     template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const T*, T,      \
                                      const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
                                      BoxSel<T>, double*, double*, double*, bool, const GvdwArgs<T>&, int);            \

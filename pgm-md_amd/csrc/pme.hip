@@ -300,33 +300,42 @@ __global__ void kPmePlace(int n, const double4* posD, PmeSel ps, const int* bric
 // dipole changes by d'_s: the density q theta(r - r_s) + d . grad_s theta moves to first order by
 // (q g + d') . grad_s theta + g^T (grad_s grad_s theta) d, i.e. a dipole q g + d' and a symmetric second moment
 // (g d^T + d g^T)/2 on the second spline derivatives.
+//
+// The charge and dipole grids (not TAN) accumulate in 64-bit fixed point, GridFixed<T>::scale units per grid
+// unit, in the LDS tile and in the global grid: integer additions commute, so the grid, and everything after
+// it, is the same bit for bit whatever the order of the atomics (the reference's 2^32 fixed-point grid,
+// core/cuda/pgm_cuda.cu:52,67-69,804-843).  Each contribution is rounded to the fixed-point unit once.  TAN
+// grids, whose scale follows the caller's cotangent, accumulate in FP64 in the tile and in T in the grid.
 template<class T, int P, bool TAN, bool BATCH>
 __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restrict__ brickStart,
                                                   const int* __restrict__ pmeAtom, const int* __restrict__ baseArr,
                                                   const T* __restrict__ spl,
                                                   const typename Vec4<T>::type* __restrict__ dip,
                                                   const typename Vec4<T>::type* __restrict__ posq, int withQ,
-                                                  T* __restrict__ gridAll, const int* skip,
+                                                  T* __restrict__ gridAll, long long* __restrict__ fixedAll,
+                                                  const int* skip,
                                                   const typename Vec4<T>::type* __restrict__ dipDot,
                                                   const typename Vec4<T>::type* __restrict__ disp) {
     typedef typename Vec4<T>::type T4;
-    // The tile accumulates in FP64 for both precisions: ds_add_f64 is several times faster than ds_add_f32 on
-    // gfx950, and the FP64 sums lose nothing.
+    // The TAN tile accumulates in FP64 for both precisions: ds_add_f64 is several times faster than ds_add_f32
+    // on gfx950, and the FP64 sums lose nothing.
     extern __shared__ unsigned char smemRaw[];
     double* tile = reinterpret_cast<double*>(smemRaw);
+    unsigned long long* tileI = reinterpret_cast<unsigned long long*>(smemRaw);
     if (skip && *skip)
         return;
     const int bid = blockIdx.x;
     const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.brickSys[bid] : 0);
     const PmeGrid& g = BATCH ? se->g : ps.g;
     const BoxT<double>& box = BATCH ? se->box : ps.box;
-    T* __restrict__ grid = gridAll+(BATCH ? se->gridOff : 0);
+    T* __restrict__ grid = TAN ? gridAll+(BATCH ? se->gridOff : 0) : nullptr;
+    long long* __restrict__ gridI = TAN ? nullptr : fixedAll+(BATCH ? se->gridOff : 0);
     const int lb = bid-(BATCH ? se->brickOff : 0);
     const int b2 = lb%g.nb[2], b1 = (lb/g.nb[2])%g.nb[1], b0 = lb/(g.nb[2]*g.nb[1]);
     const int E = g.brick+P-1;
     const int lo0 = b0*g.brick-(P-1), lo1 = b1*g.brick-(P-1), lo2 = b2*g.brick-(P-1);
     for (int i = threadIdx.x; i < E*E*E; i += blockDim.x)
-        tile[i] = 0.0;
+        tileI[i] = 0ull;
     __syncthreads();
     T rc[3][3];
 #pragma unroll
@@ -383,16 +392,22 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restri
                 A = (q*t1+e[0]*u1)*t2+e[1]*t1*u2;
                 C = e[2]*t1*t2;
             }
-            double* row = tile+(l0*E+(l1b-j2))*E+l2b;
+            const int r0 = (l0*E+(l1b-j2))*E+l2b;
+            if (TAN) {
 #pragma unroll
-            for (int j3 = 0; j3 < P; j3++)
-                atomicAdd(row-j3, (double) (TAN ? A*t3[j3]+C*u3[j3]+V*v3[j3] : A*t3[j3]+C*u3[j3]));
+                for (int j3 = 0; j3 < P; j3++)
+                    atomicAdd(tile+r0-j3, (double) (A*t3[j3]+C*u3[j3]+V*v3[j3]));
+            }
+            else {
+#pragma unroll
+                for (int j3 = 0; j3 < P; j3++)
+                    atomicAdd(tileI+r0-j3, (unsigned long long) toFixed<T>(A*t3[j3]+C*u3[j3]));
+            }
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < E*E*E; i += blockDim.x) {
-        const T v = (T) tile[i];
-        if (v == T(0))
+        if (TAN ? tile[i] == 0.0 : tileI[i] == 0ull)
             continue;
         const int x2 = i%E, x1 = (i/E)%E, x0 = i/(E*E);
         int g0 = lo0+x0, g1 = lo1+x1, g2 = lo2+x2;
@@ -402,7 +417,11 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restri
         g1 -= (g1 >= g.K[1] ? g.K[1] : 0);
         g2 += (g2 < 0 ? g.K[2] : 0);
         g2 -= (g2 >= g.K[2] ? g.K[2] : 0);
-        unsafeAtomicAdd(grid+((size_t) g0*g.K[1]+g1)*g.K[2]+g2, v);
+        const size_t at = ((size_t) g0*g.K[1]+g1)*g.K[2]+g2;
+        if (TAN)
+            unsafeAtomicAdd(grid+at, (T) tile[i]);
+        else
+            atomicAdd(reinterpret_cast<unsigned long long*>(gridI+at), tileI[i]);
     }
 }
 
@@ -424,10 +443,12 @@ __device__ __forceinline__ int fftGroupOf(const FftGroup* groups, int ngroups, i
 }
 
 // z: two real lines per complex FFT, unpacked into their half spectra.
+// Forward: from the fixed-point grid when fixedAll is set, else from the real grid.
 template<class T, bool INVERSE>
 __global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ groups, int ngroups,
                                                 T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
-                                                const typename Vec2<T>::type* __restrict__ twAll, const int* skip) {
+                                                const typename Vec2<T>::type* __restrict__ twAll, const int* skip,
+                                                const long long* __restrict__ fixedAll) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
@@ -446,12 +467,19 @@ __global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ gro
     const int pair0 = blk*pairsPerBlock;
     const int npairs = min(pairsPerBlock, (nlines+1)/2-pair0);
     if (!INVERSE) {
+        const long long* fixed = fixedAll ? fixedAll+grp.realOff : nullptr;
         for (int i = threadIdx.x; i < npairs*K; i += blockDim.x) {
             const int l = i/K, k = i-l*K;
             const int la = 2*(pair0+l), lb = la+1;
             T2 z;
-            z.x = real[(size_t) la*K+k];
-            z.y = (lb < nlines ? real[(size_t) lb*K+k] : T(0));
+            if (fixed) {
+                z.x = fromFixed<T>(fixed[(size_t) la*K+k]);
+                z.y = (lb < nlines ? fromFixed<T>(fixed[(size_t) lb*K+k]) : T(0));
+            }
+            else {
+                z.x = real[(size_t) la*K+k];
+                z.y = (lb < nlines ? real[(size_t) lb*K+k] : T(0));
+            }
             b0[l*K+k] = z;
         }
     }
@@ -550,7 +578,8 @@ __global__ void __launch_bounds__(kBlock) kFftY(const FftGroup* __restrict__ gro
 template<class T, bool INVERSE>
 __global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ groups, int ngroups,
                                                  T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
-                                                 const typename Vec2<T>::type* __restrict__ twAll, const int* skip) {
+                                                 const typename Vec2<T>::type* __restrict__ twAll, const int* skip,
+                                                 const long long* __restrict__ fixedAll) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
@@ -571,12 +600,19 @@ __global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ gr
     loadTwiddles<T>(twY, twAll+grp.twOff+g.K[0], K1);
     loadTwiddles<T>(twZ, twAll+grp.twOff+g.K[0]+K1, K2);
     if (!INVERSE) {
+        const long long* fixed = fixedAll ? fixedAll+grp.realOff+(size_t) plane*K1*K2 : nullptr;
         for (int i = threadIdx.x; i < npairs*K2; i += blockDim.x) {
             const int l = i/K2, k = i-l*K2;
             const int ya = 2*l, yb = ya+1;
             T2 z;
-            z.x = real[(size_t) ya*K2+k];
-            z.y = (yb < K1 ? real[(size_t) yb*K2+k] : T(0));
+            if (fixed) {
+                z.x = fromFixed<T>(fixed[(size_t) ya*K2+k]);
+                z.y = (yb < K1 ? fromFixed<T>(fixed[(size_t) yb*K2+k]) : T(0));
+            }
+            else {
+                z.x = real[(size_t) ya*K2+k];
+                z.y = (yb < K1 ? real[(size_t) yb*K2+k] : T(0));
+            }
             bufA[l*K2+k] = z;
         }
         __syncthreads();
@@ -1096,16 +1132,17 @@ void launchPmePlace(hipStream_t st, int n, const double4* posD, PmeSel ps, const
 template<class T>
 void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
                   const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, T* grid,
-                  const int* skip, const typename Vec4<T>::type* dipDot, const typename Vec4<T>::type* disp) {
+                  long long* fixed, const int* skip, const typename Vec4<T>::type* dipDot,
+                  const typename Vec4<T>::type* disp) {
     const int E = ps.brick+ps.order-1;
     const size_t lds = (size_t) E*E*E*sizeof(double);
     dispatchOrder(ps.order, ps.table != nullptr, [&](auto P, auto B) {
         if (dipDot)
             kSpread<T, P(), true, B()><<<ps.nbrick, kBlock, lds, st>>>(ps, brickStart, pmeAtom, base, spl, dip, posq,
-                                                                       withQ, grid, skip, dipDot, disp);
+                                                                       withQ, grid, nullptr, skip, dipDot, disp);
         else
             kSpread<T, P(), false, B()><<<ps.nbrick, kBlock, lds, st>>>(ps, brickStart, pmeAtom, base, spl, dip,
-                                                                        posq, withQ, grid, skip, dip, dip);
+                                                                        posq, withQ, nullptr, fixed, skip, dip, dip);
     });
 }
 
@@ -1134,22 +1171,22 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]) {
 template<class T>
 void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
                        const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                       int energy, int strain, double* partials, const int* skip) {
+                       int energy, int strain, double* partials, const int* skip, const long long* fixed) {
     const FftGroup* G = L.groups;
     const int ng = L.ngroups;
     if (L.zyFused)
-        kFftZY<T, false><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip);
+        kFftZY<T, false><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip, fixed);
     else {
-        kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
+        kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip, fixed);
         kFftY<T, false><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
     }
     kFftXConv<T><<<L.blocks[2], kBlock, L.lds[2], st>>>(G, ng, cplx, influence, tw, box, boxes, beta, energy, strain,
                                                         partials, skip);
     if (L.zyFused)
-        kFftZY<T, true><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip);
+        kFftZY<T, true><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip, nullptr);
     else {
         kFftY<T, true><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
-        kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip);
+        kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip, nullptr);
     }
 }
 
@@ -1211,14 +1248,14 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
     template void launchPmePlace<T>(hipStream_t, int, const double4*, PmeSel, const int*, const int*, const int*,      \
                                     int*, int*, T*);                                                                   \
     template void launchSpread<T>(hipStream_t, PmeSel, const int*, const int*, const int*, const T*,                   \
-                                  const Vec4<T>::type*, const Vec4<T>::type*, int, T*, const int*,                     \
+                                  const Vec4<T>::type*, const Vec4<T>::type*, int, T*, long long*, const int*,         \
                                   const Vec4<T>::type*, const Vec4<T>::type*);                                         \
     template void launchGatherTangent<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,         \
                                          const T*, const Vec4<T>::type*, double*);                                     \
     template void fftGroupShape<T>(FftGroup&, int*, size_t*);                                                       \
     template void launchPmeConvolve<T>(hipStream_t, const FftLaunch&, T*, Vec2<T>::type*, const T*,                    \
                                        const Vec2<T>::type*, BoxT<double>, const BoxT<double>*, double, int, int,     \
-                                       double*, const int*);                                                          \
+                                       double*, const int*, const long long*);                                        \
     template void launchGather<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,                \
                                   const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
@@ -1229,5 +1266,6 @@ PGM_INSTANTIATE_PME(float)
 PGM_INSTANTIATE_PME(double)
 template void launchZero<int>(hipStream_t, int*, size_t, const int*);
 template void launchZero<unsigned>(hipStream_t, unsigned*, size_t, const int*);
+template void launchZero<long long>(hipStream_t, long long*, size_t, const int*);
 
 }   // namespace pgmhip

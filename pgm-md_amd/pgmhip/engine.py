@@ -65,7 +65,8 @@ EXPORTS = [
     "pgm_backward_device", "pgm_adjoint_iterations", "pgm_adjoint_residual", "pgm_set_batch", "pgm_num_systems",
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
-    "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device",
+    "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device", "pgm_launch_pending", "pgm_set_field_otf",
+    "pgm_field_otf",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -114,6 +115,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_num_candidates.restype = C.c_longlong
     lib.pgm_late_list_checks.argtypes = [vp]
     lib.pgm_late_list_checks.restype = C.c_long
+    lib.pgm_launch_pending.argtypes = [vp]
+    lib.pgm_set_field_otf.argtypes = [vp, C.c_int]
+    lib.pgm_field_otf.argtypes = [vp]
     lib.pgm_set_history.argtypes = [vp, C.c_int, vp]
     lib.pgm_freeze_history.argtypes = [vp, C.c_int]
     lib.pgm_set_timing.argtypes = [vp, C.c_int]
@@ -278,6 +282,13 @@ class Engine:
         """pgm_wait_device: the stream waits on the GPU for the last launch's forces."""
         self._check(self.lib.pgm_wait_device(self.h, stream))
 
+    def launch_pending(self) -> bool:
+        """pgm_launch_pending: the GPU work of the launch is still running."""
+        return bool(self.lib.pgm_launch_pending(self.h))
+
+    def late_list_checks(self) -> int:
+        return self.lib.pgm_late_list_checks(self.h)
+
     def backward(self, pos, box, dl_dforces=None, dl_denergy=0.0) -> dict:
         """Gradients of a loss L(E, F) with respect to the per-atom parameters (pgm_backward).
 
@@ -394,6 +405,13 @@ class Engine:
 
     def set_lanes(self, lanes: int, fused: bool = True):
         self._check(self.lib.pgm_set_lanes(self.h, int(lanes), int(bool(fused))))
+
+    def set_field_otf(self, on: bool = True):
+        """FP32 field pass with its kernels evaluated on the fly (pgm_set_field_otf)."""
+        self._check(self.lib.pgm_set_field_otf(self.h, int(bool(on))))
+
+    def field_otf(self) -> bool:
+        return self.lib.pgm_field_otf(self.h) == 1
 
     def adjoint_iterations(self) -> int:
         return self.lib.pgm_adjoint_iterations(self.h)
