@@ -1922,12 +1922,12 @@ private:
                 launchPrecond<T>(st_, n, it, init, alphaSD_.as<double>(), rD_.as<double>(), apD_.as<double>(), posT,
                                  bt, srCountD_.as<int>(), srJD_.as<int>(), srGD_.as<T2>(), srOverD_.as<int>(),
                                  zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(),
-                                 doneD_.as<int>());
+                                 doneD_.as<int>(), rgT<T>());
         };
         tb(kPhCg);
         launchCgInit<T>(st_, n, alphaSD_.as<double>(), b, x, tdD_.as<double>(), trD_.as<double>(), cself, s_.tol,
                         s_.max_iter, rD_.as<double>(), zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(),
-                        cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>(), pc);
+                        cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>(), pc, rgT<T>());
         precond(0, true);
         te();
         int it = 0;
@@ -1986,9 +1986,9 @@ private:
             launchPrecond<T>(st_, n, it, false, alphaSD_.as<double>(), rD_.as<double>(), apD_.as<double>(), posT,
                              boxSel<T>(), srCountD_.as<int>(), srJD_.as<int>(), srGD_.as<T2>(), srOverD_.as<int>(),
                              zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(), cgPartD_.as<double>(),
-                             doneD_.as<int>());
+                             doneD_.as<int>(), rgT<T>());
         launchCgDirection<T>(st_, n, it, s_.tol, s_.max_iter, zD_.as<double>(), pD_.as<double>(), pTD_.as<T4>(),
-                             cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>());
+                             cgPartD_.as<double>(), cgD_.as<CgState>(), doneD_.as<int>(), rgT<T>());
         te();
     }
 
@@ -2159,7 +2159,7 @@ private:
             tb(kPhCg);
             launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
                         alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
-            launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
+            launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>(), rgT<T>());
             te();
         }
         // field of the charges and permanent dipoles (and T x0): compaction + pair setup (fused or as two
@@ -2193,7 +2193,7 @@ private:
                 tb(kPhCg);
                 launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), histD_.as<double>(), stride, gc,
                             alphaSD_.as<double>(), bD_.as<double>(), xD_.as<double>());
-                launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>());
+                launchToVec4<T>(st_, n, xD_.as<double>(), pTD_.as<T4>(), rgT<T>());
                 te();
                 matvec<T>(pos, pTD_.as<T4>(), nullptr);
             }
@@ -2358,7 +2358,7 @@ private:
                                  pdotD_.as<double>(), cself, adjBD_.as<double>());
                 launchGuess(st_, n, orderD_.as<int>(), gidxD_.as<int>(), nullptr, 0, GuessCoef{},
                             alphaSD_.as<double>(), adjBD_.as<double>(), lamD_.as<double>());
-                launchToVec4<T>(st_, n, lamD_.as<double>(), pTD_.as<T4>());
+                launchToVec4<T>(st_, n, lamD_.as<double>(), pTD_.as<T4>(), rgT<T>());
                 matvec<T>(pos, pTD_.as<T4>(), nullptr);
                 adjIterations_ = cgSolve<T>(pos, adjBD_.as<double>(), lamD_.as<double>(), lastAdjIterations_,
                                             adjResidual_);

@@ -378,7 +378,7 @@ void launchGuess(hipStream_t st, int n, const int* order, const int* gidx, const
 void launchCombineB(hipStream_t st, int n, const double* ed, const double* er, const double* perm, double cself,
                     double* b);
 template<class T>
-void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type* out);
+void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type* out, const T* rgw = nullptr);
 int cgBlocks(int n);
 // The CG's preconditioner: Jacobi z = alpha/(1 - alpha cj) r (cj = 0, or the self coefficient), or, with a short
 // list (pre), the short-range z = alpha (r + T_s alpha r), computed by launchPrecond after launchCgInit and after
@@ -390,7 +390,8 @@ struct Precond {
 template<class T>
 void launchCgInit(hipStream_t st, int n, const double* alpha, const double* b, const double* x, const double* td,
                   const double* tr, double cself, double tol, int maxIter, double* r, double* z, double* p,
-                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, Precond pc);
+                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, Precond pc,
+                  const T* rgw = nullptr);
 void launchCgAp(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* td, const double* tr,
                 double cself, double* ap, double* partials, const int* done);
 void launchCgUpdate(hipStream_t st, int n, int it, const double* alpha, const double* p, const double* ap, double* x,
@@ -402,10 +403,10 @@ template<class T>
 void launchPrecond(hipStream_t st, int n, int it, bool init, const double* alpha, const double* r, const double* ap,
                    const typename Vec4<T>::type* pos, BoxSel<T> box, const int* srCount, const int* srJ,
                    const typename Vec2<T>::type* srG, const int* over, double* z, double* p,
-                   typename Vec4<T>::type* pT, double* partials, const int* done);
+                   typename Vec4<T>::type* pT, double* partials, const int* done, const T* rgw = nullptr);
 template<class T>
 void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, const double* z, double* p,
-                       typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done);
+                       typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, const T* rgw = nullptr);
 // x += peek alpha r; history push (global order, hist may be NULL); total dipoles d = perm + x, also as the pair
 // vector type dT with w = the radius parameter rgS of the (sorted) atom.
 template<class T>

@@ -58,8 +58,10 @@ __global__ void kCombineB(int n, const double* ed, const double* er, const doubl
         b[k] = ed[k]+er[k]+cself*perm[k];
 }
 
+// The pair vector type of a 3-vector per atom, w = rgw[s] (the radius parameter, read by the FP32 field pass
+// that evaluates its kernels on the fly) or 0.
 template<class T>
-__global__ void kToVec4(int n, const double* v, typename Vec4<T>::type* out) {
+__global__ void kToVec4(int n, const double* v, typename Vec4<T>::type* out, const T* rgw) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (s >= n)
         return;
@@ -67,7 +69,7 @@ __global__ void kToVec4(int n, const double* v, typename Vec4<T>::type* out) {
     o.x = (T) v[3*s];
     o.y = (T) v[3*s+1];
     o.z = (T) v[3*s+2];
-    o.w = T(0);
+    o.w = rgw ? rgw[s] : T(0);
     out[s] = o;
 }
 
@@ -77,7 +79,7 @@ template<class T>
 __global__ void __launch_bounds__(kCgBlock) kCgInit(int n, const double* alpha, const double* b, const double* x,
                                                     const double* td, const double* tr, double cself, double* r,
                                                     double* z, double* p, typename Vec4<T>::type* pT,
-                                                    double* partials, Precond pc) {
+                                                    double* partials, Precond pc, const T* rgw) {
     __shared__ double smem[40];
     double nrm = 0, rz = 0, mx = 0;
     for (int s = blockIdx.x*blockDim.x+threadIdx.x; s < n; s += gridDim.x*blockDim.x) {
@@ -105,7 +107,7 @@ __global__ void __launch_bounds__(kCgBlock) kCgInit(int n, const double* alpha, 
             o.x = (T) pv[0];
             o.y = (T) pv[1];
             o.z = (T) pv[2];
-            o.w = T(0);
+            o.w = rgw ? rgw[s] : T(0);
             pT[s] = o;
         }
     }
@@ -202,7 +204,7 @@ __global__ void __launch_bounds__(kCgBlock) kPrecond(int n, int it, const double
                                                      BoxSel<T> bs, const int* srCount, const int* srJ,
                                                      const typename Vec2<T>::type* srG, const int* over, double* z,
                                                      double* p, typename Vec4<T>::type* pT, double* partials,
-                                                     const int* skip) {
+                                                     const int* skip, const T* rgw) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     __shared__ double smem[40];
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(kCgBlock) kPrecond(int n, int it, const double
             o.x = (T) zv[0];
             o.y = (T) zv[1];
             o.z = (T) zv[2];
-            o.w = T(0);
+            o.w = rgw ? rgw[s] : T(0);
             pT[s] = o;
         }
     }
@@ -278,7 +280,8 @@ __global__ void __launch_bounds__(kCgBlock) kPrecond(int n, int it, const double
 template<class T>
 __global__ void __launch_bounds__(kCgBlock) kCgDirection(int n, int it, double tol, int maxIter, const double* z,
                                                          double* p, typename Vec4<T>::type* pT,
-                                                         const double* partials, CgState* cg, int* done) {
+                                                         const double* partials, CgState* cg, int* done,
+                                                         const T* rgw) {
     __shared__ double smem[40];
     if (done[it]) {
         // finished earlier: the flag of the next iteration (enqueued ahead) must say so too
@@ -302,7 +305,7 @@ __global__ void __launch_bounds__(kCgBlock) kCgDirection(int n, int it, double t
         o.x = (T) v[0];
         o.y = (T) v[1];
         o.z = (T) v[2];
-        o.w = T(0);
+        o.w = rgw ? rgw[s] : T(0);
         pT[s] = o;
     }
     if (blockIdx.x == 0) {
@@ -501,16 +504,16 @@ void launchCombineB(hipStream_t st, int n, const double* ed, const double* er, c
 }
 
 template<class T>
-void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type* out) {
-    kToVec4<T><<<nblk(n, 256), 256, 0, st>>>(n, v, out);
+void launchToVec4(hipStream_t st, int n, const double* v, typename Vec4<T>::type* out, const T* rgw) {
+    kToVec4<T><<<nblk(n, 256), 256, 0, st>>>(n, v, out, rgw);
 }
 
 template<class T>
 void launchCgInit(hipStream_t st, int n, const double* alpha, const double* b, const double* x, const double* td,
                   const double* tr, double cself, double tol, int maxIter, double* r, double* z, double* p,
-                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, Precond pc) {
+                  typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, Precond pc, const T* rgw) {
     const int nb = cgBlocks(n);
-    kCgInit<T><<<nb, kCgBlock, 0, st>>>(n, alpha, b, x, td, tr, cself, r, z, p, pT, partials, pc);
+    kCgInit<T><<<nb, kCgBlock, 0, st>>>(n, alpha, b, x, td, tr, cself, r, z, p, pT, partials, pc, rgw);
     kCgInitCheck<<<1, kCgBlock, 0, st>>>(n, nb, tol, maxIter, partials, cg, done);
 }
 
@@ -528,11 +531,11 @@ template<class T>
 void launchPrecond(hipStream_t st, int n, int it, bool init, const double* alpha, const double* r, const double* ap,
                    const typename Vec4<T>::type* pos, BoxSel<T> box, const int* srCount, const int* srJ,
                    const typename Vec2<T>::type* srG, const int* over, double* z, double* p,
-                   typename Vec4<T>::type* pT, double* partials, const int* done) {
+                   typename Vec4<T>::type* pT, double* partials, const int* done, const T* rgw) {
     const int nb = cgBlocks(n);
     auto go = [&](auto I, auto B) {
         kPrecond<T, I(), B()><<<nb, kCgBlock, 0, st>>>(n, it, alpha, r, ap, pos, box, srCount, srJ, srG, over, z, p,
-                                                       pT, partials, done+it);
+                                                       pT, partials, done+it, rgw);
     };
     const bool b = box.table != nullptr;
     if (init)
@@ -543,8 +546,8 @@ void launchPrecond(hipStream_t st, int n, int it, bool init, const double* alpha
 
 template<class T>
 void launchCgDirection(hipStream_t st, int n, int it, double tol, int maxIter, const double* z, double* p,
-                       typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done) {
-    kCgDirection<T><<<cgBlocks(n), kCgBlock, 0, st>>>(n, it, tol, maxIter, z, p, pT, partials, cg, done);
+                       typename Vec4<T>::type* pT, double* partials, CgState* cg, int* done, const T* rgw) {
+    kCgDirection<T><<<cgBlocks(n), kCgBlock, 0, st>>>(n, it, tol, maxIter, z, p, pT, partials, cg, done, rgw);
 }
 
 template<class T>
@@ -593,15 +596,15 @@ void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, doub
 }
 
 #define PGM_INSTANTIATE_SOLVER(T)                                                                                     \
-    template void launchToVec4<T>(hipStream_t, int, const double*, Vec4<T>::type*);                                 \
+    template void launchToVec4<T>(hipStream_t, int, const double*, Vec4<T>::type*, const T*);                       \
     template void launchCgInit<T>(hipStream_t, int, const double*, const double*, const double*, const double*,     \
                                   const double*, double, double, int, double*, double*, double*, Vec4<T>::type*,     \
-                                  double*, CgState*, int*, Precond);                                                 \
+                                  double*, CgState*, int*, Precond, const T*);                                       \
     template void launchPrecond<T>(hipStream_t, int, int, bool, const double*, const double*, const double*,         \
                                    const Vec4<T>::type*, BoxSel<T>, const int*, const int*, const Vec2<T>::type*,     \
-                                   const int*, double*, double*, Vec4<T>::type*, double*, const int*);               \
+                                   const int*, double*, double*, Vec4<T>::type*, double*, const int*, const T*);     \
     template void launchCgDirection<T>(hipStream_t, int, int, double, int, const double*, double*, Vec4<T>::type*,  \
-                                       double*, CgState*, int*);                                                     \
+                                       double*, CgState*, int*, const T*);                                           \
     template void launchPost<T>(hipStream_t, int, double, const double*, const double*, double*, const double*,     \
                                 const int*, const int*, double*, double*, const double*, Vec4<T>::type*);
 PGM_INSTANTIATE_SOLVER(float)
