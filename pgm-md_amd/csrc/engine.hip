@@ -1458,7 +1458,7 @@ private:
         // device arrays of the layout
         const size_t tsz = mixed ? sizeof(float) : sizeof(double), t2 = 2*tsz;
         gridD_.reserve((size_t) realTotal_*tsz);
-        gridFixedD_.reserve((size_t) realTotal_*sizeof(long long));
+        gridFixedD_.reserve((size_t) realTotal_*sizeof(long long)*(s_.precision == PGM_MIXED ? 1 : 2));
         cplxD_.reserve((size_t) cplxTotal_*t2);
         inflD_.reserve((size_t) cplxTotal_*tsz);
         twD_.reserve((size_t) twTotal_*t2);
@@ -1571,6 +1571,7 @@ private:
         p.nbrick = brickTotal_;
         p.order = s_.order;
         p.brick = brick_;
+        p.fixedStride = (size_t) realTotal_;
         return p;
     }
 
@@ -1865,7 +1866,7 @@ private:
         long long* fixed = dipDot ? nullptr : gridFixedD_.as<long long>();
         tb(kPhSpread, it);
         if (fixed)
-            launchZero<long long>(pmeSt_, fixed, (size_t) realTotal_, skip);
+            launchZero<long long>(pmeSt_, fixed, (size_t) realTotal_*GridFixed<T>::words, skip);
         else
             launchZero<T>(pmeSt_, grid, (size_t) realTotal_, skip);
         launchSpread<T>(pmeSt_, ps, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
@@ -1875,7 +1876,7 @@ private:
         const BoxT<double> one = makeBox<double>(sys_[0].H, sys_[0].recip);
         launchPmeConvolve<T>(pmeSt_, fftL_, grid, cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), one,
                              batch_ ? groupBoxD_.as<BoxT<double>>() : nullptr, s_.beta, energyMode, strain ? 1 : 0,
-                             partD_.as<double>(), skip, fixed);
+                             partD_.as<double>(), skip, fixed, (size_t) realTotal_);
         if (energyMode)
             launchRangeSums(pmeSt_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
                             kResCount, res+kResRec);

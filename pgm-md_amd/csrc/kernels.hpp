@@ -262,6 +262,7 @@ struct PmeSel {
     int nbrick;             // bricks of all systems
     int order;
     int brick;
+    size_t fixedStride;     // FP64 fixed-point grids: the second words' offset (the stacked real grids' size)
 };
 
 // Binning of the sorted atoms into spreading bricks and their spline weights, in brick order.
@@ -270,33 +271,48 @@ void launchPmeBin(hipStream_t st, int n, const double4* posD, PmeSel ps, int* br
 template<class T>
 void launchPmePlace(hipStream_t st, int n, const double4* posD, PmeSel ps, const int* brickOf, const int* slot,
                     const int* brickStart, int* pmeAtom, int* base, T* spl);
-// Spread charges (withQ) and dipoles of the sorted atoms onto the 64-bit fixed-point grids fixed (zeroed by the
-// caller; GridFixed<T>::scale units per grid unit, order-independent) or, with dipDot, the change of that
-// density for displacements disp and dipole changes dipDot onto the real grids grid (zeroed by the caller).
+// Spread charges (withQ) and dipoles of the sorted atoms onto the fixed-point grids fixed (GridFixed<T>::words
+// words per point, the second words of all points after the first words of all points; zeroed by the caller;
+// order-independent) or, with dipDot, the change of that density for displacements disp and dipole changes dipDot
+// onto the real grids grid (zeroed by the caller).
 template<class T>
 void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* pmeAtom, const int* base, const T* spl,
                   const typename Vec4<T>::type* dip, const typename Vec4<T>::type* posq, int withQ, T* grid,
                   long long* fixed, const int* skip, const typename Vec4<T>::type* dipDot = nullptr,
                   const typename Vec4<T>::type* disp = nullptr);
-// The PME charge and dipole grids in 64-bit fixed point (units per grid unit): FP32 2^32, the reference's scale
-// (core/cuda/pgm_cuda.cu:52), for |grid values| < 2^31; FP64 2^40 (resolution 9.1e-13) for |grid values| < 2^23.
+// The PME charge and dipole grids in 64-bit fixed point.  FP32: one word per point, 2^32 units per grid unit, the
+// reference's scale (core/cuda/pgm_cuda.cu:52), for |grid values| < 2^31.  FP64: two words per point, the value
+// split into hi = rint(v 2^24) and lo = rint((v - hi 2^-24) 2^56): each contribution is rounded at 2^-57 (below
+// FP64's own rounding of values of order one), and the sum reads back exactly for |grid values| < 2^29.  The two
+// words are accumulated apart (the lo words of up to 2^22 contributions cannot overflow) and combined once.
 template<class T>
 struct GridFixed;
 template<>
 struct GridFixed<float> {
+    static constexpr int words = 1;
     static constexpr double scale = 4294967296.0, inv = 1.0/4294967296.0;
 };
 template<>
 struct GridFixed<double> {
-    static constexpr double scale = 1099511627776.0, inv = 1.0/1099511627776.0;
+    static constexpr int words = 2;
+    static constexpr double scale = 16777216.0, inv = 1.0/16777216.0;           // hi: 2^24
+    static constexpr double loScale = 72057594037927936.0, loInv = 1.0/72057594037927936.0;   // lo: 2^56
 };
 template<class T>
-__device__ __forceinline__ long long toFixed(T v) {
-    return __double2ll_rn((double) v*GridFixed<T>::scale);
+__device__ __forceinline__ void toFixed(T v, long long& hi, long long& lo) {
+    hi = __double2ll_rn((double) v*GridFixed<T>::scale);
+    if constexpr (GridFixed<T>::words == 2)
+        lo = __double2ll_rn(fma(-(double) hi, GridFixed<T>::inv, (double) v)*GridFixed<T>::loScale);
+    else
+        lo = 0;
 }
+// The value of a point whose words are at p[0] and (FP64) p[stride].
 template<class T>
-__device__ __forceinline__ T fromFixed(long long v) {
-    return (T) ((double) v*GridFixed<T>::inv);
+__device__ __forceinline__ T fromFixed(const long long* p, size_t stride) {
+    if constexpr (GridFixed<T>::words == 2)
+        return (T) ((double) p[0]*GridFixed<T>::inv+(double) p[stride]*GridFixed<T>::loInv);
+    else
+        return (T) ((double) p[0]*GridFixed<T>::inv);
 }
 
 // One grid shape's share of a merged transform launch: every shape of a batch goes through each stage (z, y,
@@ -329,7 +345,8 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]);
 template<class T>
 void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
                        const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                       int energy, int strain, double* partials, const int* skip, const long long* fixed);
+                       int energy, int strain, double* partials, const int* skip, const long long* fixed,
+                       size_t fixedStride);
 // Blocks of the x pass per grid (the rows of its energy partials) when ngrid grids of shape g are stacked.
 template<class T>
 int convolveBlocksPerGrid(PmeGrid g, int ngrid);

@@ -319,9 +319,10 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restri
     typedef typename Vec4<T>::type T4;
     // The TAN tile accumulates in FP64 for both precisions: ds_add_f64 is several times faster than ds_add_f32
     // on gfx950, and the FP64 sums lose nothing.
+    constexpr int W = GridFixed<T>::words;
     extern __shared__ unsigned char smemRaw[];
     double* tile = reinterpret_cast<double*>(smemRaw);
-    unsigned long long* tileI = reinterpret_cast<unsigned long long*>(smemRaw);
+    unsigned long long* tileI = reinterpret_cast<unsigned long long*>(smemRaw);   // W words per point, apart
     if (skip && *skip)
         return;
     const int bid = blockIdx.x;
@@ -332,9 +333,9 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restri
     long long* __restrict__ gridI = TAN ? nullptr : fixedAll+(BATCH ? se->gridOff : 0);
     const int lb = bid-(BATCH ? se->brickOff : 0);
     const int b2 = lb%g.nb[2], b1 = (lb/g.nb[2])%g.nb[1], b0 = lb/(g.nb[2]*g.nb[1]);
-    const int E = g.brick+P-1;
+    const int E = g.brick+P-1, E3 = E*E*E;
     const int lo0 = b0*g.brick-(P-1), lo1 = b1*g.brick-(P-1), lo2 = b2*g.brick-(P-1);
-    for (int i = threadIdx.x; i < E*E*E; i += blockDim.x)
+    for (int i = threadIdx.x; i < (TAN ? 1 : W)*E3; i += blockDim.x)
         tileI[i] = 0ull;
     __syncthreads();
     T rc[3][3];
@@ -400,14 +401,19 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restri
             }
             else {
 #pragma unroll
-                for (int j3 = 0; j3 < P; j3++)
-                    atomicAdd(tileI+r0-j3, (unsigned long long) toFixed<T>(A*t3[j3]+C*u3[j3]));
+                for (int j3 = 0; j3 < P; j3++) {
+                    long long hi, lo;
+                    toFixed<T>(A*t3[j3]+C*u3[j3], hi, lo);
+                    atomicAdd(tileI+r0-j3, (unsigned long long) hi);
+                    if (W == 2)
+                        atomicAdd(tileI+E3+r0-j3, (unsigned long long) lo);
+                }
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < E*E*E; i += blockDim.x) {
-        if (TAN ? tile[i] == 0.0 : tileI[i] == 0ull)
+    for (int i = threadIdx.x; i < E3; i += blockDim.x) {
+        if (TAN ? tile[i] == 0.0 : (tileI[i] == 0ull && (W == 1 || tileI[E3+i] == 0ull)))
             continue;
         const int x2 = i%E, x1 = (i/E)%E, x0 = i/(E*E);
         int g0 = lo0+x0, g1 = lo1+x1, g2 = lo2+x2;
@@ -420,8 +426,11 @@ __global__ void __launch_bounds__(kBlock) kSpread(PmeSel ps, const int* __restri
         const size_t at = ((size_t) g0*g.K[1]+g1)*g.K[2]+g2;
         if (TAN)
             unsafeAtomicAdd(grid+at, (T) tile[i]);
-        else
+        else {
             atomicAdd(reinterpret_cast<unsigned long long*>(gridI+at), tileI[i]);
+            if (W == 2)
+                atomicAdd(reinterpret_cast<unsigned long long*>(gridI+ps.fixedStride+at), tileI[E3+i]);
+        }
     }
 }
 
@@ -448,7 +457,7 @@ template<class T, bool INVERSE>
 __global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ groups, int ngroups,
                                                 T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
                                                 const typename Vec2<T>::type* __restrict__ twAll, const int* skip,
-                                                const long long* __restrict__ fixedAll) {
+                                                const long long* __restrict__ fixedAll, size_t fixedStride) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
@@ -473,8 +482,8 @@ __global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ gro
             const int la = 2*(pair0+l), lb = la+1;
             T2 z;
             if (fixed) {
-                z.x = fromFixed<T>(fixed[(size_t) la*K+k]);
-                z.y = (lb < nlines ? fromFixed<T>(fixed[(size_t) lb*K+k]) : T(0));
+                z.x = fromFixed<T>(fixed+(size_t) la*K+k, fixedStride);
+                z.y = (lb < nlines ? fromFixed<T>(fixed+(size_t) lb*K+k, fixedStride) : T(0));
             }
             else {
                 z.x = real[(size_t) la*K+k];
@@ -579,7 +588,7 @@ template<class T, bool INVERSE>
 __global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ groups, int ngroups,
                                                  T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
                                                  const typename Vec2<T>::type* __restrict__ twAll, const int* skip,
-                                                 const long long* __restrict__ fixedAll) {
+                                                 const long long* __restrict__ fixedAll, size_t fixedStride) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
@@ -606,8 +615,8 @@ __global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ gr
             const int ya = 2*l, yb = ya+1;
             T2 z;
             if (fixed) {
-                z.x = fromFixed<T>(fixed[(size_t) ya*K2+k]);
-                z.y = (yb < K1 ? fromFixed<T>(fixed[(size_t) yb*K2+k]) : T(0));
+                z.x = fromFixed<T>(fixed+(size_t) ya*K2+k, fixedStride);
+                z.y = (yb < K1 ? fromFixed<T>(fixed+(size_t) yb*K2+k, fixedStride) : T(0));
             }
             else {
                 z.x = real[(size_t) ya*K2+k];
@@ -1135,7 +1144,7 @@ void launchSpread(hipStream_t st, PmeSel ps, const int* brickStart, const int* p
                   long long* fixed, const int* skip, const typename Vec4<T>::type* dipDot,
                   const typename Vec4<T>::type* disp) {
     const int E = ps.brick+ps.order-1;
-    const size_t lds = (size_t) E*E*E*sizeof(double);
+    const size_t lds = (size_t) E*E*E*sizeof(double)*(dipDot ? 1 : GridFixed<T>::words);
     dispatchOrder(ps.order, ps.table != nullptr, [&](auto P, auto B) {
         if (dipDot)
             kSpread<T, P(), true, B()><<<ps.nbrick, kBlock, lds, st>>>(ps, brickStart, pmeAtom, base, spl, dip, posq,
@@ -1171,22 +1180,23 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]) {
 template<class T>
 void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
                        const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                       int energy, int strain, double* partials, const int* skip, const long long* fixed) {
+                       int energy, int strain, double* partials, const int* skip, const long long* fixed,
+                       size_t fixedStride) {
     const FftGroup* G = L.groups;
     const int ng = L.ngroups;
     if (L.zyFused)
-        kFftZY<T, false><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip, fixed);
+        kFftZY<T, false><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip, fixed, fixedStride);
     else {
-        kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip, fixed);
+        kFftZ<T, false><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip, fixed, fixedStride);
         kFftY<T, false><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
     }
     kFftXConv<T><<<L.blocks[2], kBlock, L.lds[2], st>>>(G, ng, cplx, influence, tw, box, boxes, beta, energy, strain,
                                                         partials, skip);
     if (L.zyFused)
-        kFftZY<T, true><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip, nullptr);
+        kFftZY<T, true><<<L.blocks[3], kBlock, L.lds[3], st>>>(G, ng, real, cplx, tw, skip, nullptr, 0);
     else {
         kFftY<T, true><<<L.blocks[1], kBlock, L.lds[1], st>>>(G, ng, cplx, tw, skip);
-        kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip, nullptr);
+        kFftZ<T, true><<<L.blocks[0], kBlock, L.lds[0], st>>>(G, ng, real, cplx, tw, skip, nullptr, 0);
     }
 }
 
@@ -1255,7 +1265,7 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
     template void fftGroupShape<T>(FftGroup&, int*, size_t*);                                                       \
     template void launchPmeConvolve<T>(hipStream_t, const FftLaunch&, T*, Vec2<T>::type*, const T*,                    \
                                        const Vec2<T>::type*, BoxT<double>, const BoxT<double>*, double, int, int,     \
-                                       double*, const int*, const long long*);                                        \
+                                       double*, const int*, const long long*, size_t);                                \
     template void launchGather<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,                \
                                   const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
