@@ -289,7 +289,7 @@ def md_like(torch, E, sy, sets, steps, log, precision="mixed", step_nm=0.002, se
     for timed in (False, True):
         eng = E.Engine.for_system(sy, precision=precision)
         eng.set_history(sets)
-        eng.set_lanes(*md_like.lanes[precision])
+        apply_variant(eng, precision)
         if timed:
             eng.set_timing(True, phases=["list"])
         its, repeats = [], 0
@@ -325,7 +325,14 @@ def md_like(torch, E, sy, sets, steps, log, precision="mixed", step_nm=0.002, se
     return out
 
 
-md_like.lanes = {}
+VARIANTS = {}   # the autotuner's pair-pass choice per precision: (lanes, fused setup, on-the-fly field)
+
+
+def apply_variant(eng, precision):
+    lanes, fused, otf = VARIANTS[precision]
+    eng.set_lanes(lanes, fused)
+    if precision == "mixed":
+        eng.set_field_otf(otf)
 
 
 def mu3_variant(torch, E, sy, sets, pos_d, f_d, stream, steps, warmup, log):
@@ -335,7 +342,7 @@ def mu3_variant(torch, E, sy, sets, pos_d, f_d, stream, steps, warmup, log):
         eng = E.Engine.for_system(sy, precision=prec, predictor=1)
         eng.set_history(sets[:3])
         eng.freeze_history(True)
-        eng.set_lanes(*md_like.lanes[prec])
+        apply_variant(eng, prec)
         eng.compute(sy.pos, sy.box)
         for _ in range(warmup):
             eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
@@ -599,7 +606,7 @@ def main():
         t, f, _ = eng.compute(sy.pos, box)  # checked evaluation (host arrays)
         results[prec] = (t, f, eng.iterations())
         tuned[prec] = eng.autotune(3)
-        md_like.lanes[prec] = tuned[prec][0]
+        VARIANTS[prec] = (*tuned[prec][0], eng.field_otf())
         log(f"{prec}: {eng.iterations()} CG iterations, E = {E.total_energy(t):.6f}, pair passes {tuned[prec][0]}")
     P = engines["double"].num_pairs()
     C = engines["double"].num_candidates()
@@ -632,7 +639,8 @@ def main():
                         "dipole sets, peek 0.65; FP32 pair/PME arithmetic, FP64 accumulation and CG",
             "atoms": sy.n, "pairs_within_cutoff": P, "list_candidates": C, "cg_iterations": n_it,
             "grid": list(sy.settings["grid"]),
-            "pair_passes": {"lanes": tuned["mixed"][0][0], "fused_setup": tuned["mixed"][0][1]},
+            "pair_passes": {"lanes": tuned["mixed"][0][0], "fused_setup": tuned["mixed"][0][1],
+                            "field_on_the_fly": engines["mixed"].field_otf()},
         },
         "fp64_path": {"value": round(v64, 3), "ms_per_step": round(1000.0 * secs["double"] / a.steps, 4),
                       "cg_iterations": results["double"][2], "roofline": roof64,
