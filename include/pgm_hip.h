@@ -137,6 +137,17 @@ int pgm_launch_pending(const pgm_engine* e);
 int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* box, int new_step, double* f_dev,
                            double* strain, void* stream, pgm_terms* terms);
 
+/* No counterpart in the reference; for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): pgm_compute_device_f64 (new_step 1) that
+ * makes no hipMalloc, hipFree or other allocation.  Every buffer of the evaluation is carved from the library's
+ * static device scratch of 128 MiB (one per GPU, held until the process ends), so the engine holds at most 4096
+ * atoms (and a pair list that fits; an error says when it does not).  Nothing outlives the call: the pair list is
+ * built, the solve starts from alpha E (the predictor history is neither read nor written).  The calls on one
+ * GPU share its scratch, which keeps the last call's contents: calls that overlap on one GPU race.  One system
+ * (not a batch), the engine's own pair list.  The setup calls (pgm_create, pgm_set_*) allocate as usual. */
+int pgm_compute_device_f64_noalloc(pgm_engine* e, const double* pos_dev, const double* box, double* f_dev,
+                                   double* strain, void* stream, pgm_terms* terms);
+
 /* iterations, residual, inducedDipoles, permanentDipoles, resetHistory, listBuilds, numCandidates,
  * lateListChecks (core/cuda/pgm_cuda.h:112-122). */
 int pgm_iterations(const pgm_engine* e);

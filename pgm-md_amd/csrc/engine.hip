@@ -25,6 +25,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -50,15 +51,48 @@ void hipCheck(hipError_t e, const char* what) {
 }
 #define HIPCHK(x) hipCheck((x), #x)
 
+struct DevBuf;
+
+// A bump allocator over device memory the library does not allocate (the allocation-free entry point's static
+// scratch): while one is active on the calling thread, every DevBuf reserve carves from it; the buffers it
+// served are given back their own allocations when it ends (release).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<DevBuf*> served;
+    void* carve(size_t b);
+    void release();
+};
+thread_local Arena* t_arena = nullptr;
+
 // Device allocation that only grows.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    // while an arena serves this buffer: its own allocation, given back by Arena::release
+    void* ownP = nullptr;
+    size_t ownBytes = 0;
+    bool inArena = false;
     ~DevBuf() {
-        if (p)
+        if (p && !inArena)
             (void) hipFree(p);
+        else if (ownP)
+            (void) hipFree(ownP);
     }
     void reserve(size_t b) {
+        if (t_arena) {
+            if (inArena && b <= bytes)
+                return;
+            if (!inArena) {
+                ownP = p;
+                ownBytes = bytes;
+                inArena = true;
+                t_arena->served.push_back(this);
+            }
+            p = t_arena->carve(std::max<size_t>(b, 256));
+            bytes = std::max<size_t>(b, 256);
+            return;
+        }
         if (b <= bytes)
             return;
         if (p)
@@ -78,6 +112,28 @@ struct DevBuf {
     }
 };
 
+void* Arena::carve(size_t b) {
+    const size_t at = (used+255) & ~(size_t) 255;
+    if (at+b > cap)
+        throw std::runtime_error("pgm_hip: the allocation-free evaluation needs more than the " +
+                                 std::to_string(cap >> 20)+" MiB of its static scratch (a smaller system, or the "
+                                 "allocating entry points)");
+    used = at+b;
+    return base+at;
+}
+
+void Arena::release() {
+    for (DevBuf* d : served) {
+        d->p = d->ownP;
+        d->bytes = d->ownBytes;
+        d->ownP = nullptr;
+        d->ownBytes = 0;
+        d->inArena = false;
+    }
+    served.clear();
+    used = 0;
+}
+
 // Pinned host staging that only grows.
 struct HostBuf {
     void* p = nullptr;
@@ -89,6 +145,8 @@ struct HostBuf {
     void reserve(size_t b) {
         if (b <= bytes)
             return;
+        if (t_arena)
+            throw std::logic_error("pgm_hip: the allocation-free evaluation would allocate pinned host memory");
         if (p)
             HIPCHK(hipHostFree(p));
         p = nullptr;
@@ -100,6 +158,32 @@ struct HostBuf {
         return static_cast<U*>(p);
     }
 };
+
+
+// The scratch of the allocation-free entry point (pgm_compute_device_f64_noalloc): a static device array,
+// one per GPU the library runs on, held until the process ends.  Host code uses its address from
+// hipGetSymbolAddress (scratchAddress), never the array's name, which is only a host-side stand-in.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+constexpr int kNoAllocMaxAtoms = 4096;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratchMutex;
+std::vector<void*> g_scratchPtr;   // per device ordinal
+
+// The device address of the scratch on the current device (looked up once per device, at engine creation).
+void* scratchAddress() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if ((int) g_scratchPtr.size() <= dev)
+        g_scratchPtr.resize(dev+1, nullptr);
+    if (!g_scratchPtr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratchPtr[dev] = ptr;
+    }
+    return g_scratchPtr[dev];
+}
 
 template<class T>
 BoxT<T> makeBox(const double H[3][3], const double recip[3][3]) {
@@ -201,6 +285,8 @@ public:
             listH_.reserve(64);
             cgH_.reserve(64);
             scalH_.reserve(64);
+            resH_.reserve(kResCount*sizeof(double));   // one system's sums
+            scratchPtr_ = scratchAddress();
         });
     }
     ~DeviceEngine() {
@@ -769,6 +855,60 @@ public:
         return true;
     }
 
+    // pgm_compute_device_f64 without any allocation: every device buffer the evaluation reserves is carved from
+    // the library's static scratch of this device, and nothing of it outlives the call: the pair list is built,
+    // the solve starts from alpha E (the history is neither read nor written), and the engine's next call of any
+    // kind sets up its own buffers again.  Calls on one GPU share its scratch: calls that overlap race.
+    void evaluateNoAlloc(const double* pos, const double* box, double* fDev, double* strainOut, pgm_terms* terms,
+                         hipStream_t caller) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: compute before setAtoms");
+        if (n_ > kNoAllocMaxAtoms)
+            throw std::invalid_argument("pgm_hip: the allocation-free evaluation takes at most " +
+                                        std::to_string(kNoAllocMaxAtoms)+" atoms (this engine has "+
+                                        std::to_string(n_)+")");
+        if (extList_)
+            throw std::invalid_argument("pgm_hip: the allocation-free evaluation builds its own pair list");
+        Arena arena;
+        arena.base = static_cast<char*>(scratchPtr_);
+        arena.cap = kScratchBytes;
+        const bool wasTiming = timing_;
+        auto restore = [&] {
+            t_arena = nullptr;
+            arena.release();
+            timing_ = wasTiming;
+            noHistory_ = false;
+            forgetDeviceState();
+        };
+        forgetDeviceState();
+        timing_ = false;
+        noHistory_ = true;
+        t_arena = &arena;
+        try {
+            evaluate(PosSource::DeviceF64, pos, box, true, nullptr, fDev, nullptr, 0, 0.0, strainOut, terms, caller,
+                     false);
+        }
+        catch (...) {
+            restore();
+            throw;
+        }
+        restore();
+    }
+    // Everything derived on the device from positions and parameters is to be set up again (the buffers that
+    // held it were the scratch's).
+    void forgetDeviceState() {
+        haveList_ = false;
+        layoutDirty_ = true;
+        tablesDirty_ = true;
+        for (Sys& y : sys_)
+            y.haveInfluence = false;
+        haveResults_ = false;
+        adjointReady_ = false;
+        gvdwReady_ = false;
+        preReady_ = false;
+        prevValid_ = false;
+    }
+
     // True when an evaluation finished at bitwise these positions and boxes and nothing has changed the engine's
     // state since (parameters, settings, the pair passes' timing).  Compares on the device, ordered after the
     // caller's stream; one small read back.
@@ -1094,7 +1234,9 @@ private:
     bool haveResults_ = false;
     int lanes_ = 16;
     bool fused_ = false;
-    bool fieldOtf_ = false;   // FP32: the field pass evaluates its kernels instead of reading the stored ones
+    bool fieldOtf_ = false;
+    void* scratchPtr_ = nullptr;   // this engine's device's static scratch (scratchAddress)
+    bool noHistory_ = false;       // the allocation-free evaluation: alpha E guess, no history entry   // FP32: the field pass evaluates its kernels instead of reading the stored ones
     double tuneMs_[6] = {0, 0, 0, 0, 0, 0};
     // kernel timing
     struct Timed {
@@ -1798,6 +1940,8 @@ private:
 
     GuessCoef guessCoefficients(bool newStep) const {
         GuessCoef gc{};
+        if (noHistory_)
+            return gc;
         const int h = (int) histSlots_.size();
         auto set = [&](std::initializer_list<double> c) {
             gc.nc = (int) c.size();
@@ -2202,7 +2346,7 @@ private:
             if (its >= 0)   // (else enqueued ahead: known when the launch completes)
                 iterations_ = its;
             const int h = (int) histSlots_.size();
-            const bool push = !frozen_ && (newStep || h == 0);
+            const bool push = !frozen_ && !noHistory_ && (newStep || h == 0);
             if (push)
                 pushSlot = freeSlot();
             tb(kPhCg);
@@ -2608,6 +2752,16 @@ int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* b
         requireSingle(g);
         g.evaluate(PosSource::DeviceF64, pos_dev, box, new_step != 0, nullptr, f_dev, nullptr, 0, 0.0, strain, terms,
                    static_cast<hipStream_t>(stream), false);
+    });
+}
+
+int pgm_compute_device_f64_noalloc(pgm_engine* e, const double* pos_dev, const double* box, double* f_dev,
+                                   double* strain, void* stream, pgm_terms* terms) {
+    return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
+        if (!pos_dev || !box)
+            throw std::invalid_argument("pgm_hip: compute needs positions and a box");
+        g.withDevice([&] { g.evaluateNoAlloc(pos_dev, box, f_dev, strain, terms, static_cast<hipStream_t>(stream)); });
     });
 }
 

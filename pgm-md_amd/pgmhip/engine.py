@@ -66,7 +66,7 @@ EXPORTS = [
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
     "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device", "pgm_launch_pending", "pgm_set_field_otf",
-    "pgm_field_otf",
+    "pgm_field_otf", "pgm_compute_device_f64_noalloc",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -103,6 +103,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_launch_device.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]
     lib.pgm_finish_device.argtypes = [vp, C.POINTER(Terms), ip]
     lib.pgm_compute_device_f64.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, C.POINTER(Terms)]
+    lib.pgm_compute_device_f64_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(Terms)]
     lib.pgm_iterations.argtypes = [vp]
     lib.pgm_residual.argtypes = [vp]
     lib.pgm_residual.restype = C.c_double
@@ -254,6 +255,14 @@ class Engine:
         t = Terms()
         self._check(self.lib.pgm_compute_device_f64(self.h, pos_ptr, _ptr(box), int(bool(new_step)), f_ptr, None,
                                                     stream, C.byref(t)))  # fmt: skip
+        return t.as_dict()
+
+    def compute_device_f64_noalloc(self, pos_ptr: int, box, f_ptr: int | None = None, stream: int = 0):
+        """pgm_compute_device_f64_noalloc: one evaluation with no device allocation (static scratch)."""
+        box = _f64(box, (3, 3))
+        t = Terms()
+        self._check(self.lib.pgm_compute_device_f64_noalloc(self.h, pos_ptr, _ptr(box), f_ptr, None, stream,
+                                                            C.byref(t)))  # fmt: skip
         return t.as_dict()
 
     def compute_device(self, posq_ptr: int, box, new_step, fixed_ptr: int, stride: int, scale: float, stream: int = 0):
