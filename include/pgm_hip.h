@@ -148,6 +148,12 @@ int pgm_compute_device_f64(pgm_engine* e, const double* pos_dev, const double* b
 int pgm_compute_device_f64_noalloc(pgm_engine* e, const double* pos_dev, const double* box, double* f_dev,
                                    double* strain, void* stream, pgm_terms* terms);
 
+/* pgmcore::Engine::setGeometry + field (core/pgm_core.h:76-79): the field [e/nm^2] at every atom (field, 3n,
+ * caller's order, assigned) of the charges q (n) and dipoles d (3n, e nm) given here, direct + reciprocal + self,
+ * at host positions pos (3n) and box; the engine's own charges, covalent and induced dipoles play no part.  The
+ * pair list is checked (rebuilt if stale) as for an evaluation; the results of the last evaluation are gone. */
+int pgm_field(pgm_engine* e, const double* pos, const double* box, const double* q, const double* d, double* field);
+
 /* iterations, residual, inducedDipoles, permanentDipoles, resetHistory, listBuilds, numCandidates,
  * lateListChecks (core/cuda/pgm_cuda.h:112-122). */
 int pgm_iterations(const pgm_engine* e);

@@ -139,6 +139,11 @@ public:
     void inducedDipoles(double* mu) const { check(pgm_induced_dipoles(e_, mu)); }
     void permanentDipoles(double* p) const { check(pgm_permanent_dipoles(e_, p)); }
     void resetHistory() { check(pgm_reset_history(e_)); }
+    /* No counterpart in pgmcuda::Engine (pgmcore::Engine::field, core/pgm_core.h:76-77): the field of charges q and
+     * dipoles d at every atom at positions pos (pgm_field). */
+    void field(const double* pos, const double box[3][3], const double* q, const double* d, double* out) {
+        check(pgm_field(e_, pos, &box[0][0], q, d, out));
+    }
     long listBuilds() const { return pgm_list_builds(e_); }
     int numCandidates() const { return (int) pgm_num_candidates(e_); }
     long lateListChecks() const { return pgm_late_list_checks(e_); }
@@ -249,9 +254,32 @@ public:
         lastPos_ = pos;
         lastKey_ = key_;
         havePos_ = true;
+        setGeometry(pos, box);
         return terms_.total();
     }
     void setHistoryKey(long long key) { key_ = key; }
+    /* Field [e/nm^2] of charges q and dipoles d at every atom (direct + reciprocal + self), at the positions of
+     * the last setGeometry or compute (core/pgm_core.h:76-79). */
+    std::vector<V3> field(const std::vector<double>& q, const std::vector<V3>& d) {
+        const int n = (int) top_.q.size();
+        if (!haveGeometry_)
+            throw std::logic_error("pgm_hip: field before setGeometry or compute");
+        if ((int) q.size() != n || (int) d.size() != n)
+            throw std::invalid_argument("pgm_hip: field operands do not match the topology");
+        std::vector<V3> out(n);
+        eng_->field(&geoPos_[0][0], geoBox_, q.data(), &d[0][0], &out[0][0]);
+        return out;
+    }
+    /* The positions and box of the next field() (core/pgm_core.h:78-79); compute() sets them too. */
+    void setGeometry(const std::vector<V3>& pos, const V3 box[3]) {
+        if (pos.size() != top_.q.size())
+            throw std::invalid_argument("pgm_hip: positions do not match the topology");
+        geoPos_ = pos;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                geoBox_[r][c] = box[r][c];
+        haveGeometry_ = true;
+    }
     const std::vector<V3>& permanentDipoles() {
         perm_.resize(top_.q.size());
         eng_->permanentDipoles(&perm_[0][0]);
@@ -288,6 +316,9 @@ private:
     bool havePos_ = false;
     EnergyTerms terms_;
     std::array<double, 9> lastStrain_{};
+    std::vector<V3> geoPos_;
+    double geoBox_[3][3] = {{0}};
+    bool haveGeometry_ = false;
 };
 
 }   // namespace core

@@ -909,6 +909,60 @@ public:
         prevValid_ = false;
     }
 
+    // pgmcore::Engine::setGeometry + field (core/pgm_core.h:76-79, core/pgm_core.cpp:539-560): the field [e/nm^2]
+    // at every atom of the charges q and dipoles d given here (caller's order; the engine's own charges, covalent
+    // and induced dipoles play no part), direct + reciprocal + self, at positions pos and box: the pair list checked
+    // (or built), the atoms prepared, then the setup pair pass and a PME solve of (q, d) as an evaluation's field of
+    // the permanent multipoles.  out: 3n, caller's order, assigned.
+    void field(const double* pos, const double* boxes, const double* q, const double* d, double* out) {
+        if (n_ <= 0)
+            throw std::logic_error("pgm_hip: field before setAtoms");
+        if (!pos || !boxes || !q || !d || !out)
+            throw std::invalid_argument("pgm_hip: field needs positions, a box, charges, dipoles and an output");
+        adjointReady_ = false;
+        haveResults_ = false;
+        prevValid_ = false;
+        withDevice([&] {
+            HIPCHK(hipStreamSynchronize(st_));
+            setBoxes(boxes);
+            stagePositions(PosSource::HostF64, pos);
+            const bool mixed = s_.precision == PGM_MIXED;
+            if (haveList_ && !extList_) {
+                startListCheck();
+                if (listStale())
+                    buildList();
+            }
+            else if (!haveList_)
+                buildList();
+            mixed ? fieldOf<float>(q, d, out) : fieldOf<double>(q, d, out);
+        });
+    }
+
+    template<class T>
+    void fieldOf(const double* q, const double* d, double* out) {
+        typedef typename Vec4<T>::type T4;
+        const int n = n_;
+        prep<T>();
+        fqD_.reserve(n*sizeof(double));
+        fdD_.reserve(3*n*sizeof(double));
+        HIPCHK(hipMemcpyAsync(fqD_.p, q, n*sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(fdD_.p, d, 3*n*sizeof(double), hipMemcpyHostToDevice, st_));
+        const bool f32 = sizeof(T) == sizeof(float);
+        launchFieldOperands<T>(st_, n, orderD_.as<int>(), fqD_.as<double>(), fdD_.as<double>(), rgT<T>(),
+                               posDD_.as<double4>(), f32 ? posT<T>() : nullptr, permD_.as<double>(), permTD_.as<T4>());
+        for (DevBuf* b : {&edD_, &erD_, &bD_, &tdD_})
+            b->reserve(3*sizeof(double)*n);
+        nbD_.reserve(n*sizeof(int));
+        pairSetup<T>(lanes_, fused_, nullptr);
+        pmeSolve<T>(permTD_.as<T4>(), true, posT<T>(), 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
+        const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
+        launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
+        tmpD_.reserve(3*sizeof(double)*n);
+        launchUnsort3(st_, n, rankD_.as<int>(), bD_.as<double>(), tmpD_.as<double>());
+        HIPCHK(hipMemcpyAsync(out, tmpD_.p, 3*sizeof(double)*n, hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
     // True when an evaluation finished at bitwise these positions and boxes and nothing has changed the engine's
     // state since (parameters, settings, the pair passes' timing).  Compares on the device, ordered after the
     // caller's stream; one small read back.
@@ -1319,6 +1373,7 @@ private:
     DevBuf permD_, permTD_, edD_, erD_, bD_, xD_, rD_, zD_, pD_, apD_, tdD_, trD_, pTD_, dD_, dTD_, etotD_;
     DevBuf fpairD_, frecD_, fcovD_, forcesOutD_, tmpD_;
     DevBuf brickOfD_, pslotD_, brickCountD_, brickStartD_, pmeAtomD_, baseD_, splD_;
+    DevBuf fqD_, fdD_;   // the charges and dipoles of a field evaluation (caller's order)
     DevBuf gridD_, gridFixedD_, cplxD_, inflD_, twD_, bmodD_;
     DevBuf cgPartD_, partD_, resD_, cgD_, doneD_, histD_, maxBitsD_;
     DevBuf perAtomD_, chunkSumD_;
@@ -2762,6 +2817,13 @@ int pgm_compute_device_f64_noalloc(pgm_engine* e, const double* pos_dev, const d
         if (!pos_dev || !box)
             throw std::invalid_argument("pgm_hip: compute needs positions and a box");
         g.withDevice([&] { g.evaluateNoAlloc(pos_dev, box, f_dev, strain, terms, static_cast<hipStream_t>(stream)); });
+    });
+}
+
+int pgm_field(pgm_engine* e, const double* pos, const double* box, const double* q, const double* d, double* field) {
+    return guarded(e, [&](DeviceEngine& g) {
+        requireSingle(g);
+        g.field(pos, box, q, d, field);
     });
 }
 

@@ -447,6 +447,11 @@ void launchReducePartials(hipStream_t st, const double* partials, int nblk, int 
 void launchRangeSums(hipStream_t st, const double* rows, int width, const int* starts, int nr, const int* outIndex,
                      int outStride, double* out);
 // In the caller's order: out[i] = v[rank[i]] (3-vectors).
+// The operands of a field evaluation of given charges q and dipoles d (caller's order) into the sorted arrays:
+// posD.w and, when set, posT.w = q; dS, dT = d (dT.w = rgw).
+template<class T>
+void launchFieldOperands(hipStream_t st, int n, const int* order, const double* q, const double* d, const T* rgw,
+                         double4* posD, typename Vec4<T>::type* posT, double* dS, typename Vec4<T>::type* dT);
 void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out);
 
 // -------------------------------------------------------------------------------------------- parameter gradients

@@ -481,6 +481,29 @@ __global__ void __launch_bounds__(256) kRangeSums(const double* rows, int width,
     }
 }
 
+// The operands of a field evaluation of given charges q and dipoles d (caller's order): the sorted charges into
+// the positions' w (FP64 and, when set, the pair type), the sorted dipoles as FP64 3-vectors and as the pair type
+// with w = the radius parameter.
+template<class T>
+__global__ void kFieldOperands(int n, const int* order, const double* q, const double* d, const T* rgw,
+                               double4* posD, typename Vec4<T>::type* posT, double* dS, typename Vec4<T>::type* dT) {
+    const int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= n)
+        return;
+    const int i = order[s];
+    posD[s].w = q[i];
+    if (posT)
+        posT[s].w = (T) q[i];
+    typename Vec4<T>::type o;
+    o.x = (T) d[3*i];
+    o.y = (T) d[3*i+1];
+    o.z = (T) d[3*i+2];
+    o.w = rgw[s];
+    dT[s] = o;
+    for (int c = 0; c < 3; c++)
+        dS[3*s+c] = d[3*i+c];
+}
+
 __global__ void kUnsort3(int n, const int* rank, const double* v, double* out) {
     const int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= 3*n)
@@ -591,11 +614,19 @@ void launchRangeSums(hipStream_t st, const double* rows, int width, const int* s
         kRangeSums<<<nr, 256, 0, st>>>(rows, width, starts, outIndex, outStride, out);
 }
 
+template<class T>
+void launchFieldOperands(hipStream_t st, int n, const int* order, const double* q, const double* d, const T* rgw,
+                         double4* posD, typename Vec4<T>::type* posT, double* dS, typename Vec4<T>::type* dT) {
+    kFieldOperands<T><<<nblk(n, 256), 256, 0, st>>>(n, order, q, d, rgw, posD, posT, dS, dT);
+}
+
 void launchUnsort3(hipStream_t st, int n, const int* rank, const double* v, double* out) {
     kUnsort3<<<nblk(3LL*n, 256), 256, 0, st>>>(n, rank, v, out);
 }
 
 #define PGM_INSTANTIATE_SOLVER(T)                                                                                     \
+    template void launchFieldOperands<T>(hipStream_t, int, const int*, const double*, const double*, const T*,      \
+                                         double4*, Vec4<T>::type*, double*, Vec4<T>::type*);                          \
     template void launchToVec4<T>(hipStream_t, int, const double*, Vec4<T>::type*, const T*);                       \
     template void launchCgInit<T>(hipStream_t, int, const double*, const double*, const double*, const double*,     \
                                   const double*, double, double, int, double*, double*, double*, Vec4<T>::type*,     \

@@ -66,7 +66,7 @@ EXPORTS = [
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
     "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device", "pgm_launch_pending", "pgm_set_field_otf",
-    "pgm_field_otf", "pgm_compute_device_f64_noalloc",
+    "pgm_field_otf", "pgm_field", "pgm_compute_device_f64_noalloc",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -117,6 +117,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_late_list_checks.argtypes = [vp]
     lib.pgm_late_list_checks.restype = C.c_long
     lib.pgm_launch_pending.argtypes = [vp]
+    lib.pgm_field.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.pgm_set_field_otf.argtypes = [vp, C.c_int]
     lib.pgm_field_otf.argtypes = [vp]
     lib.pgm_set_history.argtypes = [vp, C.c_int, vp]
@@ -264,6 +265,14 @@ class Engine:
         self._check(self.lib.pgm_compute_device_f64_noalloc(self.h, pos_ptr, _ptr(box), f_ptr, None, stream,
                                                             C.byref(t)))  # fmt: skip
         return t.as_dict()
+
+    def field(self, pos, box, q, d):
+        """pgm_field: the field (n, 3) of charges q and dipoles d at every atom at these positions."""
+        pos = _f64(pos).reshape(-1, 3)
+        q, d = _f64(q), _f64(d).reshape(-1, 3)
+        out = np.zeros((self.n, 3))
+        self._check(self.lib.pgm_field(self.h, _ptr(pos), _ptr(_f64(box, (3, 3))), _ptr(q), _ptr(d), _ptr(out)))
+        return out
 
     def compute_device(self, posq_ptr: int, box, new_step, fixed_ptr: int, stride: int, scale: float, stream: int = 0):
         box = _f64(box, (3, 3))

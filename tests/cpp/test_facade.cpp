@@ -78,6 +78,28 @@ int main() {
     printf("core::Engine FP64: E %.10f oracle %.10f (%s); max|dF| %.2e (%s); max|dS| %.2e (%s); %d iterations\n", e,
            eo, okE ? "ok" : "FAIL", df, okF ? "ok" : "FAIL", ds, okS ? "ok" : "FAIL", gpu.iterations());
     fails += !okE+!okF+!okS;
+    // pgmcore::Engine::field of other charges and dipoles at the geometry of the last compute
+    {
+        std::vector<double> q2(n);
+        std::vector<pgmhip::core::V3> d2(n);
+        for (int i = 0; i < n; i++) {
+            q2[i] = 0.3*std::sin(1.7*i);
+            for (int k = 0; k < 3; k++)
+                d2[i][k] = 0.01*std::cos(0.9*i+2.1*k);
+        }
+        const std::vector<pgmhip::core::V3> Eg = gpu.field(q2, d2);
+        std::vector<double> Eo(3*(size_t) n);
+        orc_field(o, q2.data(), &d2[0][0], Eo.data());
+        double dE = 0, Emax = 0;
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) {
+                dE = std::max(dE, std::fabs(Eg[i][k]-Eo[3*i+k]));
+                Emax = std::max(Emax, std::fabs(Eo[3*i+k]));
+            }
+        const bool okField = dE <= 1e-11*Emax;
+        printf("core::Engine::field: max|dE| %.2e of %.2e (%s)\n", dE, Emax, okField ? "ok" : "FAIL");
+        fails += !okField;
+    }
     // same key: starts from the solution, no iterations
     gpu.setHistoryKey(7);
     gpu.compute(pos, box, nullptr);
