@@ -417,16 +417,17 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
     const float4 pi = pos[s];
     const float rgi = dip[s].w;
     const int start = candStart[s], cnt = nb[s];
+    // the U entries of a step summed in FP32 (the matrix-vector product is FP32), the steps in FP64
     double ex = 0, ey = 0, ez = 0;
-    auto term = [&](const float4& pjv, const float4& d, int v) {
+    auto term = [&](const float4& pjv, const float4& d, int v, float& fx, float& fy, float& fz) {
         float x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
         pairImage(x, y, z, box, v, bs.ext);
         float G[3];
         pairKernelsF<3>(rgi+d.w, beta, x*x+y*y+z*z, G);
         const float c2 = (d.x*x+d.y*y+d.z*z)*G[2];
-        ex += (double) (c2*x-d.x*G[1]);
-        ey += (double) (c2*y-d.y*G[1]);
-        ez += (double) (c2*z-d.z*G[1]);
+        fx += c2*x-d.x*G[1];
+        fy += c2*y-d.y*G[1];
+        fz += c2*z-d.z*G[1];
     };
     int k = lane;
     for (; k+(U-1)*L < cnt; k += U*L) {
@@ -441,13 +442,21 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
             pjv[u] = pos[j];
             d[u] = dip[j];
         }
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
         for (int u = 0; u < U; u++)
-            term(pjv[u], d[u], v[u]);
+            term(pjv[u], d[u], v[u], fx, fy, fz);
+        ex += (double) fx;
+        ey += (double) fy;
+        ez += (double) fz;
     }
     for (; k < cnt; k += L) {
         const int v = pj[start+k], j = entryAtom(v, bs.ext);
-        term(pos[j], dip[j], v);
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        term(pos[j], dip[j], v, fx, fy, fz);
+        ex += (double) fx;
+        ey += (double) fy;
+        ez += (double) fz;
     }
     ex = groupSum<L>(ex);
     ey = groupSum<L>(ey);
@@ -463,10 +472,10 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
 // from gv = (a, c, b), b_ij = (rg_i + rg_j)^-1/2): energy into eg, force on i and, STRAIN, the strain into f and
 // acc in units of ke (invKe = 1/ke), as the electrostatic terms (core/cuda/pgm_cuda.cu:398-414).  sign = -1
 // removes exactly what sign = +1 added: the skipped pairs.
-template<class T, bool STRAIN>
+template<class T, bool STRAIN, class F>
 __device__ __forceinline__ void gvdwTerm(T x, T y, T z, T r2, const typename Vec4<T>::type& gvi,
                                          const typename Vec4<T>::type& gvj, T rgSum, T rvdw2, bool slater,
-                                         T invKe, double sign, double& eg, double* acc, double* f) {
+                                         T invKe, double sign, double& eg, double* acc, F* f) {
     const T A = gvi.x*gvj.x, C6 = gvi.y*gvj.y;
     if (!(r2 < rvdw2) || (A == T(0) && C6 == T(0)))
         return;
@@ -474,9 +483,9 @@ __device__ __forceinline__ void gvdwTerm(T x, T y, T z, T r2, const typename Vec
     const T u = gvdwPair<T>(sqrt(r2), rsqrtT(rgSum), A, C6, T(0.5)*(gvi.z+gvj.z), slater, dEdR);
     const T fs = dEdR*invKe;
     eg += sign*(double) u;
-    f[0] += sign*(double) (fs*x);
-    f[1] += sign*(double) (fs*y);
-    f[2] += sign*(double) (fs*z);
+    f[0] += (F) (sign*(double) (fs*x));
+    f[1] += (F) (sign*(double) (fs*y));
+    f[2] += (F) (sign*(double) (fs*z));
     if (STRAIN) {
         const T xs[3] = {x, y, z};
 #pragma unroll
@@ -489,12 +498,12 @@ __device__ __forceinline__ void gvdwTerm(T x, T y, T z, T r2, const typename Vec
 
 // One pair of the final pass at fixed total dipoles: energy, dE/dx, field at i and, STRAIN, the strain
 // (core/pgm_core.cpp:422-440); GVDW adds the pair's GVDW term.
-template<class T, bool STRAIN, bool GVDW>
+template<class T, bool STRAIN, bool GVDW, class A>
 __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, const typename Vec4<T>::type& di,
                                           const typename Vec4<T>::type& pjv, const typename Vec4<T>::type& dj,
                                           const typename Vec2<T>::type& ga, const typename Vec2<T>::type& gb,
-                                          T rgSum, T beta, const BoxT<T>& box, int v, int ext, double* acc, double* f,
-                                          double* e, const typename Vec4<T>::type& gvi,
+                                          T rgSum, T beta, const BoxT<T>& box, int v, int ext, double* acc, A* f,
+                                          A* e, A& u0, const typename Vec4<T>::type& gvi,
                                           const typename Vec4<T>::type& gvj, const GvdwArgs<T>& gva, double& eg) {
     T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
     pairImage(x, y, z, box, v, ext);
@@ -513,14 +522,14 @@ __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, cons
     const T qi = pi.w, qj = pjv.w;
     const T dix = di.x*x+di.y*y+di.z*z, djx = dj.x*x+dj.y*y+dj.z*z, didj = di.x*dj.x+di.y*dj.y+di.z*dj.z;
     const T qd = qi*djx-qj*dix;
-    acc[0] += (double) (qi*qj*G0+(qd+didj)*G1-dix*djx*G2);
+    u0 += (A) (qi*qj*G0+(qd+didj)*G1-dix*djx*G2);
     const T sc = -qi*qj*G1-(qd+didj)*G2+dix*djx*G3;
     const T gx = sc*x+(qi*dj.x-qj*di.x)*G1-(djx*di.x+dix*dj.x)*G2;
     const T gy = sc*y+(qi*dj.y-qj*di.y)*G1-(djx*di.y+dix*dj.y)*G2;
     const T gz = sc*z+(qi*dj.z-qj*di.z)*G1-(djx*di.z+dix*dj.z)*G2;
-    f[0] -= (double) gx;
-    f[1] -= (double) gy;
-    f[2] -= (double) gz;
+    f[0] -= (A) gx;
+    f[1] -= (A) gy;
+    f[2] -= (A) gz;
     if (STRAIN) {
         acc[1] += (double) (gx*x);
         acc[2] += (double) (gx*y);
@@ -533,9 +542,9 @@ __device__ __forceinline__ void finalTerm(const typename Vec4<T>::type& pi, cons
         acc[9] += (double) (gz*z);
     }
     const T c = qj*G1+djx*G2;
-    e[0] += (double) (c*x-dj.x*G1);
-    e[1] += (double) (c*y-dj.y*G1);
-    e[2] += (double) (c*z-dj.z*G1);
+    e[0] += (A) (c*x-dj.x*G1);
+    e[1] += (A) (c*y-dj.y*G1);
+    e[2] += (A) (c*z-dj.z*G1);
 }
 
 // perAtom: [n][10] (energy, strain row-major) with STRAIN, else [n] (energy), each half of the atom's pair sums;
@@ -589,10 +598,19 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
                 rgj[u] = (kStoreG03<T> && !GVDW) ? T(0) : dj[u].w;
                 gvj[u] = GVDW ? gva.gv[j[u]] : dj[u];
             }
+            // the U entries of a step summed in T (FP32: one FP64 addition per step instead of per entry), the
+            // steps in FP64
+            T fu[3] = {T(0), T(0), T(0)}, eu[3] = {T(0), T(0), T(0)}, uu = T(0);
 #pragma unroll
             for (int u = 0; u < U; u++)
                 finalTerm<T, STRAIN, GVDW>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, v[u], bs.ext,
-                                           acc, f, e, gvi, gvj[u], gva, eg);
+                                           acc, fu, eu, uu, gvi, gvj[u], gva, eg);
+            acc[0] += (double) uu;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                f[c] += (double) fu[c];
+                e[c] += (double) eu[c];
+            }
         }
         for (; k < cnt; k += L) {
             const int v = pj[start+k], j = entryAtom(v, bs.ext);
@@ -602,7 +620,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             const T4 dj = dip[j];
             const T4 gvj = GVDW ? gva.gv[j] : dj;
             finalTerm<T, STRAIN, GVDW>(pi, di, pos[j], dj, g12[start+k], gb, (kStoreG03<T> && !GVDW) ? T(0) : rgi+dj.w,
-                                       beta, box, v, bs.ext, acc, f, e, gvi, gvj, gva, eg);
+                                       beta, box, v, bs.ext, acc, f, e, acc[0], gvi, gvj, gva, eg);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
