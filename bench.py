@@ -157,17 +157,15 @@ def traffic_table(path=TRAFFIC_FILE):
     return out
 
 
-def roofline(dom, live, P, C, N, precision, lanes, setup):
+def roofline(dom, live, P, C, N, precision, lanes, fused):
     """The pair pass `dom` (the one with the most time per evaluation) against the VALU rate, the bound of the
     path: its algorithmic flops per launch (PAIR_FLOPS_PER_P) over its live launch average `live` in the timed
     steps, against the MI355X FP32 (packed, 157.3 TFLOP/s) or FP64 (78.6) peak.  `traffic` is the HBM bytes per
     launch measured by the counters; `stored_list_bytes` the part of them that is this implementation's stored
     list and kernels (per entry of the full list and per candidate), which is no algorithmic traffic."""
     E = 2 * P
-    fused = setup != "separate"
     table = dict(PAIR_BYTES, pair_setup=PAIR_BYTES_FUSED_SETUP) if fused else PAIR_BYTES
-    setup_kernel = {"fused": "kCompactSetup", "staged": "kSetupStaged"}.get(setup, "kPairSetup")
-    kernels = dict(KERNEL_OF_PHASE, pair_setup=setup_kernel)
+    kernels = dict(KERNEL_OF_PHASE, pair_setup="kCompactSetup") if fused else KERNEL_OF_PHASE
     per_launch = {ph: b[precision][0] * E + b[precision][1] * N + b[precision][2] * C for ph, b in table.items()}
     stored = {ph: b[precision][0] * E + b[precision][2] * C for ph, b in table.items()}
     t = live["launch_avg_us"] * 1e-6
@@ -218,10 +216,10 @@ def gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, steps, warmup, log):
         eng.freeze_history(True)
         eng.set_gvdw(0, sy.settings["cutoff"], sa, sc, b, si, sj)
         t, forces[prec], _ = eng.compute(sy.pos, sy.box)
-        (lanes, setup), _ = eng.autotune(3)
+        (lanes, fused), _ = eng.autotune(3)
         dt, live = timed_steps(torch, eng, pos_d, f_d, sy.box, steps, warmup, stream, "pair_final")
         out[prec] = {"evals_per_s": round(steps / dt, 3), "pair_final_ms": round(live["ms_per_eval"], 4),
-                     "gvdw_kj_mol": t["gvdw"], "lanes": lanes, "setup": setup}  # fmt: skip
+                     "gvdw_kj_mol": t["gvdw"], "lanes": lanes, "fused_setup": fused}  # fmt: skip
         log(f"gvdw {prec}: {steps / dt:.2f} evals/s")
     out["force_relrms_mixed_vs_fp64"] = E.rel_rms(forces["mixed"], forces["double"])
     out["energy_rel_mixed_vs_fp64"] = out["mixed"]["gvdw_kj_mol"] / out["double"]["gvdw_kj_mol"] - 1
@@ -249,7 +247,7 @@ def large_box(torch, E, S, steps, warmup, log):
         eng.freeze_history(True)
         t, f, _ = eng.compute(sy.pos, sy.box)
         used = (free0 - torch.cuda.mem_get_info()[0]) / 2**30
-        (lanes, setup), _ = eng.autotune(3)
+        (lanes, fused), _ = eng.autotune(3)
         for _ in range(warmup):
             eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
         torch.cuda.synchronize()
@@ -261,7 +259,7 @@ def large_box(torch, E, S, steps, warmup, log):
         fs = g["forces_sample"]
         err = float(np.sqrt(((f[g["sample"]] - fs) ** 2).sum() / (fs**2).sum()))
         out[prec] = {"evals_per_s": round(steps / dt, 3), "cg_iterations": eng.iterations(), "device_gib": round(used, 2),
-                     "lanes": lanes, "setup": setup, "force_relrms_vs_oracle_tau1e-10": err,
+                     "lanes": lanes, "fused_setup": fused, "force_relrms_vs_oracle_tau1e-10": err,
                      "energy_rel_vs_oracle_tau1e-10": E.total_energy(t) / float(g["terms"].sum()) - 1}  # fmt: skip
         log(f"999k {prec}: {steps / dt:.2f} evals/s, {used:.1f} GiB, force error {err:.2e}")
         del eng
@@ -327,12 +325,12 @@ def md_like(torch, E, sy, sets, steps, log, precision="mixed", step_nm=0.002, se
     return out
 
 
-VARIANTS = {}   # the autotuner's pair-pass choice per precision: (lanes, setup variant, on-the-fly field)
+VARIANTS = {}   # the autotuner's pair-pass choice per precision: (lanes, fused setup, on-the-fly field)
 
 
 def apply_variant(eng, precision):
-    lanes, setup, otf = VARIANTS[precision]
-    eng.set_lanes(lanes, setup)
+    lanes, fused, otf = VARIANTS[precision]
+    eng.set_lanes(lanes, fused)
     if precision == "mixed":
         eng.set_field_otf(otf)
 
@@ -397,44 +395,7 @@ def batch_engine(E, b, precision, tol):
     return eng
 
 
-def gap_report(path, pause_ns=50_000_000):
-    """Kernel trace (rocprofv3 --kernel-trace, *_kernel_trace.csv) of a --batch-only run, split at every pause
-    longer than 50 ms: per segment, the union of the kernels' [start, end) intervals (both streams) is GPU-busy
-    time, the rest of its span is launch gaps and host waits.  In a --batch-only run the last two segments are
-    the FP64 and the mixed timed steps."""
-    import csv
-
-    rows = []
-    with open(path) as fh:
-        for r in csv.DictReader(fh):
-            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
-    rows.sort()
-    segments, cur = [], [rows[0]]
-    reach = rows[0][1]
-    for a, b in rows[1:]:
-        if a-reach > pause_ns:
-            segments.append(cur)
-            cur = []
-        cur.append((a, b))
-        reach = max(reach, b)
-    segments.append(cur)
-    out = []
-    for seg in segments:
-        busy, c0, c1 = 0, seg[0][0], seg[0][1]
-        for a, b in seg[1:]:
-            if a > c1:
-                busy += c1-c0
-                c0, c1 = a, b
-            else:
-                c1 = max(c1, b)
-        busy += c1-c0
-        span = max(b for _, b in seg)-seg[0][0]
-        out.append({"dispatches": len(seg), "span_ms": round(span / 1e6, 3), "busy_ms": round(busy / 1e6, 3),
-                    "idle_share": round(1.0-busy / span, 4) if span > 0 else 0.0})  # fmt: skip
-    return out
-
-
-def adjoint_batch(torch, E, S, reps, log, pause=False):
+def adjoint_batch(torch, E, S, reps, log):
     """Force-matching gradients over the 64-snapshot batch in one engine: per step a forward solve from the
     alpha E guess, the same-step re-evaluation and the adjoint solve, FP64 and mixed, tau 1e-8, the pair-pass
     lanes picked by the autotuner.  Unit: snapshot-gradients/s."""
@@ -458,7 +419,7 @@ def adjoint_batch(torch, E, S, reps, log, pause=False):
         eng = batch_engine(E, b, prec, 1e-8)
         _, f = eng.compute_batch(b["pos"], b["boxes"])
         P_batch = eng.num_pairs()
-        (lanes, setup), tune = eng.autotune(3)
+        (lanes, fused), tune = eng.autotune(3)
         fwd_it = eng.iterations()
         g = eng.backward_batch(b["pos"], b["boxes"], g_d.cpu().numpy(), None)
         if ref is None:
@@ -480,15 +441,11 @@ def adjoint_batch(torch, E, S, reps, log, pause=False):
 
         step()
         torch.cuda.synchronize()
-        if pause:   # a gap a kernel trace can find: the timed steps follow it
-            time.sleep(0.1)
         t0 = time.perf_counter()
         for _ in range(reps):
             step()
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
-        if pause:   # and after them, so the phase-bracketed step below is a segment of its own
-            time.sleep(0.1)
         # one more step with every kernel group bracketed (in line, so not the timed figure): where a batch goes
         eng.set_timing(True)
         step()
@@ -505,9 +462,9 @@ def adjoint_batch(torch, E, S, reps, log, pause=False):
             "roofline": {"bound": "valu", "flops_per_batch": fl, "achieved": round(fl / dt / 1e12, 3), "peak": peak,
                          "unit": "TFLOP/s", "frac": round(fl / dt / 1e12 / peak, 4)},
             "list_builds": eng.list_builds(),
-            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "setup": setup,
+            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "fused_setup": fused,
             "phases_ms_per_batch_in_line": phases,
-            "autotune_ms": {f"{k[0]}{k[1][0]}": round(v, 4) for k, v in tune.items()},
+            "autotune_ms": {f"{k[0]}{'f' if k[1] else 's'}": round(v, 4) for k, v in tune.items()},
             "force_relrms_vs_fp64": E.rel_rms(f, ref[0]),
             "gradient_maxrel_vs_fp64": float(np.abs(gq - gq0).max() / np.abs(gq0).max()),
         }  # fmt: skip
@@ -586,12 +543,6 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 999,000-atom box")
     ap.add_argument("--no-md", action="store_true", help="skip the MD-like steps and the mu3 variant")
     ap.add_argument("--md-steps", type=int, default=100)
-    ap.add_argument("--batch-only", action="store_true",
-                    help="profiler pass over the 64-snapshot adjoint batch alone: the timed steps follow a 100 ms pause "
-                         "so a kernel trace can find them (see --gap-report); prints the batch figures")  # fmt: skip
-    ap.add_argument("--gap-report", metavar="KERNEL_TRACE_CSV",
-                    help="from a rocprofv3 kernel trace of a --batch-only run: the share of the timed steps' span in "
-                         "which no kernel runs on the GPU (launch gaps and host waits)")  # fmt: skip
     ap.add_argument("--large-only", action="store_true",
                     help="profiler pass over the 999,000-atom box alone (both precisions, --steps timed "
                          "evaluations each); prints its figures, not the JSON line")  # fmt: skip
@@ -613,12 +564,6 @@ def main():
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    if a.gap_report:
-        print(json.dumps(gap_report(a.gap_report)))
-        return
-    if a.batch_only:
-        print(json.dumps(adjoint_batch(torch, E, S, a.batch_reps, log, pause=True)))
-        return
     if a.large_only:
         print(json.dumps(large_box(torch, E, S, a.steps, a.warmup, log)))
         return
@@ -694,12 +639,12 @@ def main():
                         "dipole sets, peek 0.65; FP32 pair/PME arithmetic, FP64 accumulation and CG",
             "atoms": sy.n, "pairs_within_cutoff": P, "list_candidates": C, "cg_iterations": n_it,
             "grid": list(sy.settings["grid"]),
-            "pair_passes": {"lanes": tuned["mixed"][0][0], "setup": tuned["mixed"][0][1],
+            "pair_passes": {"lanes": tuned["mixed"][0][0], "fused_setup": tuned["mixed"][0][1],
                             "field_on_the_fly": engines["mixed"].field_otf()},
         },
         "fp64_path": {"value": round(v64, 3), "ms_per_step": round(1000.0 * secs["double"] / a.steps, 4),
                       "cg_iterations": results["double"][2], "roofline": roof64,
-                      "pair_passes": {"lanes": tuned["double"][0][0], "setup": tuned["double"][0][1]}},
+                      "pair_passes": {"lanes": tuned["double"][0][0], "fused_setup": tuned["double"][0][1]}},
         "mixed_over_fp64": round(value / v64, 3),
         "accuracy": {
             "force_relrms_mixed_vs_fp64_gpu": E.rel_rms(fm, fd),

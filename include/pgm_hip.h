@@ -220,15 +220,13 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
 
 /* No counterpart in the reference: the autotuner of the pair passes.  After an evaluation, times the
  * compaction + setup, (iterations + 1) matrix-vector and final pair passes on the engine's current lists for
- * three setup variants (1: compaction fused into the setup pass; 0: compaction as a pass of its own; 2: fused,
- * with each cell's neighbour runs staged in LDS, for a single system's own list) and 16, 32 and 64 lanes per
- * atom in the other passes (reps repetitions each; in FP32 both forms of the field pass), keeps the fastest for
- * later evaluations and returns it in *lanes, *setup (each may be NULL) and the nine timings in ms[9] (ms per
- * evaluation, fused 16/32/64, separate 16/32/64, staged 16/32/64, 1e300 where a variant does not apply; may be
- * NULL).  pgm_set_lanes sets them directly (lanes 16, 32 or 64; setup 0, 1 or 2, the staged setup falling back
- * to the fused one where the list cannot stage; 16 and separate passes by default). */
-int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* setup, double* ms);
-int pgm_set_lanes(pgm_engine* e, int lanes, int setup);
+ * two variants (compaction fused into the setup pass, or as a pass of its own) and 16, 32 and 64 lanes per
+ * atom (reps repetitions each), keeps the fastest for later evaluations and returns it in *lanes, *fused
+ * (each may be NULL) and the six timings in ms[6] (ms per evaluation, fused 16/32/64 then separate
+ * 16/32/64, each the faster of the two field-pass forms in FP32; may be NULL).  pgm_set_lanes sets them
+ * directly (lanes 16, 32 or 64; 16 and separate passes by default). */
+int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms);
+int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
 /* No counterpart in the reference: the FP32 (PGM_MIXED) field pass of the CG products with its pair kernels
  * evaluated on the fly (1) or read from the kernels the setup pass stored (0); pgm_autotune times both and keeps
  * the faster.  pgm_field_otf: the current choice. */

@@ -43,9 +43,6 @@ constexpr int kNHist = 5;                     // 4 predictor entries + the slot 
 // (~2,400 bricks of 12) 2,184 -> 2,293 snapshot-gradients/s; the 96k box (216 bricks of 12, 729 of 8) keeps 8,
 // its spread being 30% slower with 12.
 constexpr int kBrickSmall = 8, kBrickLarge = 12, kMinBricks = 1024;
-// the setup pass variants: compaction as a pass of its own, compaction fused into it, fused and cell-staged
-constexpr int kSetupSeparate = 0, kSetupFused = 1, kSetupStaged = 2;
-constexpr size_t kStagedLdsBudget = 48*1024;
 constexpr int kChunkAtoms = 2048;             // rows per first-stage block of the per-system sums
 
 void hipCheck(hipError_t e, const char* what) {
@@ -956,7 +953,7 @@ public:
         for (DevBuf* b : {&edD_, &erD_, &bD_, &tdD_})
             b->reserve(3*sizeof(double)*n);
         nbD_.reserve(n*sizeof(int));
-        pairSetup<T>(lanes_, setupVariant_, nullptr);
+        pairSetup<T>(lanes_, fused_, nullptr);
         pmeSolve<T>(permTD_.as<T4>(), true, posT<T>(), 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
@@ -1091,37 +1088,33 @@ public:
             throw std::logic_error("pgm_hip: autotune needs an evaluation first (the lists of the workload)");
         if (reps < 1)
             throw std::invalid_argument("pgm_hip: autotune needs reps >= 1");
-        int best = lanes_, bestVariant = setupVariant_;
-        bool bestOtf = fieldOtf_;
+        int best = lanes_;
+        bool bestFused = fused_, bestOtf = fieldOtf_;
         withDevice([&] {
             double bestMs = 1e300;
             const bool mixed = s_.precision == PGM_MIXED;
-            // setup variants: fused, separate, cell-staged (where the list allows it)
-            const int variants[3] = {kSetupFused, kSetupSeparate, kSetupStaged};
-            for (int v = 0; v < 3; v++) {
-                const int variant = variants[v];
+            for (int v = 0; v < 2; v++) {
+                const bool fused = v == 0;
                 for (int c = 0; c < 3; c++) {
                     const int lanes = kPairLaneChoices[c];
                     tuneMs_[3*v+c] = 1e300;
-                    if (variant == kSetupStaged && (runCap_ <= 0 || extList_))
-                        continue;
                     // FP32 also tries the field pass with its kernels evaluated on the fly
                     for (int o = 0; o < (mixed ? 2 : 1); o++) {
                         const bool otf = o == 1;
-                        const double ms = mixed ? timePairPasses<float>(lanes, variant, reps, otf)
-                                                : timePairPasses<double>(lanes, variant, reps, false);
+                        const double ms = mixed ? timePairPasses<float>(lanes, fused, reps, otf)
+                                                : timePairPasses<double>(lanes, fused, reps, false);
                         tuneMs_[3*v+c] = std::min(tuneMs_[3*v+c], ms);
                         if (ms < bestMs) {
                             bestMs = ms;
                             best = lanes;
-                            bestVariant = variant;
+                            bestFused = fused;
                             bestOtf = otf;
                         }
                     }
                 }
             }
             lanes_ = best;
-            setupVariant_ = bestVariant;
+            fused_ = bestFused;
             fieldOtf_ = bestOtf;
         });
         return best;
@@ -1133,14 +1126,12 @@ public:
         fieldOtf_ = on;
     }
     int lanes() const { return lanes_; }
-    int setupVariant() const { return setupVariant_; }
-    void setLanes(int lanes, int variant) {
+    bool fused() const { return fused_; }
+    void setLanes(int lanes, bool fused) {
         if (lanes != 16 && lanes != 32 && lanes != 64)
             throw std::invalid_argument("pgm_hip: pair-pass lanes must be 16, 32 or 64");
-        if (variant < kSetupSeparate || variant > kSetupStaged)
-            throw std::invalid_argument("pgm_hip: setup variant must be 0 (separate), 1 (fused) or 2 (cell-staged)");
         lanes_ = lanes;
-        setupVariant_ = variant;
+        fused_ = fused;
     }
     const double* tuneTimes() const { return tuneMs_; }
 
@@ -1296,14 +1287,11 @@ private:
     bool frozen_ = false;
     bool haveResults_ = false;
     int lanes_ = 16;
-    int setupVariant_ = kSetupSeparate;   // a staged choice falls back to fused where the list cannot stage
-    bool fieldOtf_ = false;        // FP32: the field pass evaluates its kernels instead of reading the stored ones
+    bool fused_ = false;
+    bool fieldOtf_ = false;
     void* scratchPtr_ = nullptr;   // this engine's device's static scratch (scratchAddress)
-    bool noHistory_ = false;       // the allocation-free evaluation: alpha E guess, no history entry
-    // the cell-staged setup (a single system's own list): its blocks and the LDS room of a run, 0 if unavailable
-    DevBuf stagedD_;
-    int nStaged_ = 0, runCap_ = 0;
-    double tuneMs_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool noHistory_ = false;       // the allocation-free evaluation: alpha E guess, no history entry   // FP32: the field pass evaluates its kernels instead of reading the stored ones
+    double tuneMs_[6] = {0, 0, 0, 0, 0, 0};
     // kernel timing
     struct Timed {
         int phase, iteration;
@@ -2001,44 +1989,8 @@ private:
         HIPCHK(hipMemcpyAsync(posBuildD_.p, posDev_, 3*n*sizeof(double), hipMemcpyDeviceToDevice, st_));
         for (Sys& y : sys_)
             std::memcpy(y.buildBox, y.H, sizeof(y.H));
-        buildStagedBlocks();
         haveList_ = true;
         listBuilds_++;
-    }
-
-    // The cell-staged setup's blocks (up to kStagedAtoms atoms of one cell each) and the largest run of
-    // consecutive neighbour cells along z, from the cell ranges of a single system's list; runCap = 0 (the variant
-    // unavailable) for a batch, or where a run would not fit the LDS budget.
-    void buildStagedBlocks() {
-        runCap_ = 0;
-        nStaged_ = 0;
-        if (batch_ || sys_.size() != 1)
-            return;
-        const CellGrid& g = sys_[0].cells;
-        std::vector<int> cs(g.ncell+1);
-        HIPCHK(hipMemcpyAsync(cs.data(), cellStartD_.as<int>()+sys_[0].cellOff, (g.ncell+1)*sizeof(int),
-                              hipMemcpyDeviceToHost, st_));
-        HIPCHK(hipStreamSynchronize(st_));
-        std::vector<StagedBlock> blocks;
-        for (int c = 0; c < g.ncell; c++)
-            for (int a = cs[c]; a < cs[c+1]; a += kStagedAtoms)
-                blocks.push_back(StagedBlock{c, a, std::min(kStagedAtoms, cs[c+1]-a), 0});
-        int cap = 0;
-        for (int c = 0; c < g.ncell; c++) {
-            const int c2 = c%g.nc[2], c01 = c/g.nc[2];
-            int run = 0;
-            for (int d = 0; d < g.noff[2]; d++) {
-                const int cell = c01*g.nc[2]+(c2+g.off[2][d]+g.nc[2])%g.nc[2];
-                run += cs[cell+1]-cs[cell];
-            }
-            cap = std::max(cap, run);
-        }
-        const size_t perAtom = 3*(s_.precision == PGM_MIXED ? sizeof(float4) : sizeof(double4));
-        if (blocks.empty() || (size_t) cap*perAtom > kStagedLdsBudget)
-            return;
-        upload(stagedD_, blocks.data(), blocks.size()*sizeof(StagedBlock));
-        nStaged_ = (int) blocks.size();
-        runCap_ = std::max(cap, 1);
     }
 
     GuessCoef guessCoefficients(bool newStep) const {
@@ -2273,22 +2225,11 @@ private:
     // The candidates within rc, their kernels and the field of (q, perm) (and of dual): one fused pass or
     // compaction + setup.
     template<class T>
-    void pairSetup(int lanes, int variant, const typename Vec4<T>::type* dual) {
+    void pairSetup(int lanes, bool fused, const typename Vec4<T>::type* dual) {
         typedef typename Vec4<T>::type T4;
         typedef typename Vec2<T>::type T2;
         const BoxSel<T> bt = boxSel<T>();
         const T rc2 = (T) (s_.cutoff*s_.cutoff);
-        if (variant == kSetupStaged && runCap_ > 0 && !extList_) {
-            tb(kPhPairSetup);
-            launchSetupStaged<T>(st_, nStaged_, stagedD_.as<StagedBlock>(), sys_[0].cells,
-                                 cellStartD_.as<int>()+sys_[0].cellOff, runCap_, posT<T>(), rgT<T>(),
-                                 permTD_.as<T4>(), dual, candStartD_.as<int>(), candD_.as<int>(), nbD_.as<int>(),
-                                 pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt.box, rc2, (T) s_.beta,
-                                 edD_.as<double>(), tdD_.as<double>());
-            te();
-            return;
-        }
-        const bool fused = variant != kSetupSeparate;
         if (fused) {
             tb(kPhPairSetup);
             launchPairCompactSetup<T>(st_, n_, posT<T>(), rgT<T>(), permTD_.as<T4>(), dual, candStartD_.as<int>(),
@@ -2425,7 +2366,7 @@ private:
         // passes, the autotuner's choice) + PME
         PmeStreamScope pmeScope(*this, overlapPme());
         fork();
-        pairSetup<T>(lanes_, setupVariant_, dual ? pTD_.as<T4>() : nullptr);
+        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
         preReady_ = false;
         if (induce && preRadius_ > 0) {   // the preconditioner's short list, from the stored kernels
             srCountD_.reserve(n*sizeof(int));
@@ -2524,7 +2465,7 @@ private:
 
     // Milliseconds per evaluation spent in the pair passes at the given lane count (current lists and dipoles).
     template<class T>
-    double timePairPasses(int lanes, int variant, int reps, bool otf) {
+    double timePairPasses(int lanes, bool fused, int reps, bool otf) {
         typedef typename Vec4<T>::type T4;
         typedef typename Vec2<T>::type T2;
         const int n = n_;
@@ -2535,7 +2476,7 @@ private:
         timing_ = false;
         adjointReady_ = false;   // the timed passes overwrite the evaluation's pair-pass outputs
         auto once = [&] {
-            pairSetup<T>(lanes, variant, pTD_.as<T4>());
+            pairSetup<T>(lanes, fused, pTD_.as<T4>());
             for (int k = 0; k <= iterations_; k++)
                 launchPairField<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                                    g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes, otf, (T) s_.beta);
@@ -2984,20 +2925,20 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
     });
 }
 
-int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* setup, double* ms) {
+int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms) {
     return guarded(e, [&](DeviceEngine& g) {
         const int best = g.autotune(reps);
         if (lanes)
             *lanes = best;
-        if (setup)
-            *setup = g.setupVariant();
+        if (fused)
+            *fused = g.fused() ? 1 : 0;
         if (ms)
-            std::copy(g.tuneTimes(), g.tuneTimes()+9, ms);
+            std::copy(g.tuneTimes(), g.tuneTimes()+6, ms);
     });
 }
 
-int pgm_set_lanes(pgm_engine* e, int lanes, int setup) {
-    return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes, setup); });
+int pgm_set_lanes(pgm_engine* e, int lanes, int fused) {
+    return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes, fused != 0); });
 }
 
 int pgm_set_field_otf(pgm_engine* e, int on) {

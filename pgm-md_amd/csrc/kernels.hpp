@@ -164,19 +164,6 @@ void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type*
                             const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
                             typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, double* field,
                             double* fieldDual, int lanes);
-// The cell-staged compaction + setup: one block per StagedBlock (up to kStagedAtoms atoms of one cell, kStagedLanes
-// lanes each); runCap: LDS room in atoms for the largest run of consecutive neighbour cells.
-constexpr int kStagedLanes = 16;
-constexpr int kStagedAtoms = 16;
-struct StagedBlock {
-    int cell, first, count, pad;
-};
-template<class T>
-void launchSetupStaged(hipStream_t st, int nblocks, const StagedBlock* blocks, CellGrid g, const int* cellStart,
-                       int runCap, const typename Vec4<T>::type* pos, const T* rg, const typename Vec4<T>::type* dip,
-                       const typename Vec4<T>::type* dual, const int* candStart, const int* cand, int* nb, int* pj,
-                       typename Vec2<T>::type* g12, typename Vec2<T>::type* g03, BoxT<T> box, T rc2, T beta,
-                       double* field, double* fieldDual);
 // The dipole part of a CG matrix-vector product from the stored G1, G2 or, otf (FP32 only), with the kernels
 // evaluated on the fly from the positions and the radius parameters in the dipoles' w.
 template<class T>

@@ -256,142 +256,6 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
     }
 }
 
-// Compaction and setup with the partners staged in LDS, one block per up to kStagedAtoms atoms of one cell.  The
-// candidate list of every atom runs through the neighbour cells in the stencil order of kCandidates: for each
-// (x, y) offset, a run of up to five cells along z, which are consecutive in the sorted order (two pieces where
-// the run wraps the box).  The block copies each run's positions, permanent dipoles and dual vectors into LDS
-// with contiguous loads, and every atom of the block tests and evaluates its candidates in that run from there:
-// the partner gathers of the per-atom passes (a cache line per few lanes, from L2) become LDS reads.  The
-// candidates of a run are a prefix of an atom's remaining list, so each group of L lanes keeps a cursor and takes
-// L candidates at a time while they lie in the run.  The compacted entries and kernels go to the same segments
-// as kCompactSetup writes them, in the same order.
-template<class T, bool DUAL>
-__global__ void __launch_bounds__(kPairBlock) kSetupStaged(const StagedBlock* __restrict__ blocks, CellGrid g,
-                                                           const int* __restrict__ cellStart, int runCap,
-                                                           const typename Vec4<T>::type* __restrict__ pos,
-                                                           const T* __restrict__ rg,
-                                                           const typename Vec4<T>::type* __restrict__ dip,
-                                                           const typename Vec4<T>::type* __restrict__ dual,
-                                                           const int* __restrict__ candStart,
-                                                           const int* __restrict__ cand, int* __restrict__ nb,
-                                                           int* __restrict__ pj,
-                                                           typename Vec2<T>::type* __restrict__ g12,
-                                                           typename Vec2<T>::type* __restrict__ g03, BoxT<T> box,
-                                                           T rc2, T beta, double* __restrict__ field,
-                                                           double* __restrict__ fieldDual) {
-    typedef typename Vec4<T>::type T4;
-    typedef typename Vec2<T>::type T2;
-    constexpr int L = kStagedLanes;
-    extern __shared__ unsigned char smemRaw[];
-    T4* sPos = reinterpret_cast<T4*>(smemRaw);
-    T4* sDip = sPos+runCap;
-    T4* sDual = sDip+runCap;
-    const StagedBlock B = blocks[xcdBlock()];
-    const int group = threadIdx.x/L, lane = threadIdx.x%L;
-    const bool active = group < B.count;
-    const int s = B.first+(active ? group : 0);
-    const int shift = (threadIdx.x%64)/L*L;
-    const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
-    const unsigned long long gmask = (1ull << L)-1ull;
-    const int c2 = B.cell%g.nc[2], c1 = (B.cell/g.nc[2])%g.nc[1], c0 = B.cell/(g.nc[2]*g.nc[1]);
-    const T4 pi = pos[s];
-    const T rgi = rg[s];
-    int kc = candStart[s];
-    const int cEnd = candStart[s+1];
-    int ko = kc;
-    double e[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-    for (int a = 0; a < g.noff[0]; a++) {
-        const int n0 = (c0+g.off[0][a]+g.nc[0])%g.nc[0];
-        for (int b = 0; b < g.noff[1]; b++) {
-            const int n1 = (c1+g.off[1][b]+g.nc[1])%g.nc[1];
-            // the run's cells in list order, merged into pieces of consecutive atoms: [lo, hi) at LDS slot at
-            int lo[5], hi[5], at[5];
-            int np = 0, total = 0;
-            for (int d = 0; d < g.noff[2]; d++) {
-                const int cell = (n0*g.nc[1]+n1)*g.nc[2]+(c2+g.off[2][d]+g.nc[2])%g.nc[2];
-                const int l = cellStart[cell], h = cellStart[cell+1];
-                if (np > 0 && hi[np-1] == l)
-                    hi[np-1] = h;
-                else {
-                    lo[np] = l;
-                    hi[np] = h;
-                    at[np] = total;
-                    np++;
-                }
-                total += h-l;
-            }
-            __syncthreads();   // the previous run's LDS reads are done
-            for (int t = threadIdx.x; t < total; t += blockDim.x) {
-                int q = 0;
-                while (q+1 < np && t >= at[q+1])
-                    q++;
-                const int j = lo[q]+(t-at[q]);
-                sPos[t] = pos[j];
-                sDip[t] = dip[j];
-                if (DUAL)
-                    sDual[t] = dual[j];
-            }
-            __syncthreads();
-            if (!active)
-                continue;
-            while (true) {
-                const int k = kc+lane;
-                int j = -1, slot = -1;
-                if (k < cEnd) {
-                    j = cand[k];
-                    for (int q = 0; q < np; q++)
-                        if (j >= lo[q] && j < hi[q])
-                            slot = at[q]+(j-lo[q]);
-                }
-                const int nIn = __popcll((__ballot(slot >= 0) >> shift) & gmask);   // a prefix of the group
-                bool in = false;
-                T x = T(0), y = T(0), z = T(0);
-                if (slot >= 0) {
-                    const T4 pjv = sPos[slot];
-                    x = pi.x-pjv.x;
-                    y = pi.y-pjv.y;
-                    z = pi.z-pjv.z;
-                    minImage(x, y, z, box);
-                    in = x*x+y*y+z*z < rc2;
-                }
-                const unsigned long long m = (__ballot(in) >> shift) & gmask;
-                if (in) {
-                    const int out = ko+__popcll(m & below);
-                    const T4 d = sDip[slot];
-                    const T4 u = DUAL ? sDual[slot] : d;
-                    T2 ga, gb;
-                    setupTermX<T, DUAL>(x, y, z, sPos[slot].w, rgi+d.w, d, u, beta, ga, gb, e, f);
-                    pj[out] = j;
-                    g12[out] = ga;
-                    if (kStoreG03<T>)
-                        g03[out] = gb;
-                }
-                ko += __popcll(m);
-                kc += nIn;
-                if (nIn < L)
-                    break;
-            }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        e[c] = groupSum<L>(e[c]);
-        if (DUAL)
-            f[c] = groupSum<L>(f[c]);
-    }
-    if (active && lane == 0) {
-        nb[s] = ko-candStart[s];
-        field[3*s] = e[0];
-        field[3*s+1] = e[1];
-        field[3*s+2] = e[2];
-        if (DUAL) {
-            fieldDual[3*s] = f[0];
-            fieldDual[3*s+1] = f[1];
-            fieldDual[3*s+2] = f[2];
-        }
-    }
-}
-
 // Kernels and fields over the compacted pairs: every lane busy, U entries in flight per lane.
 template<class T, int L, int U, bool DUAL, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
@@ -1069,23 +933,6 @@ void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type*
 }
 
 template<class T>
-void launchSetupStaged(hipStream_t st, int nblocks, const StagedBlock* blocks, CellGrid g, const int* cellStart,
-                       int runCap, const typename Vec4<T>::type* pos, const T* rg, const typename Vec4<T>::type* dip,
-                       const typename Vec4<T>::type* dual, const int* candStart, const int* cand, int* nb, int* pj,
-                       typename Vec2<T>::type* g12, typename Vec2<T>::type* g03, BoxT<T> box, T rc2, T beta,
-                       double* field, double* fieldDual) {
-    const size_t lds = (size_t) runCap*(dual ? 3 : 2)*sizeof(typename Vec4<T>::type);
-    if (dual)
-        kSetupStaged<T, true><<<nblocks, kPairBlock, lds, st>>>(blocks, g, cellStart, runCap, pos, rg, dip, dual,
-                                                               candStart, cand, nb, pj, g12, g03, box, rc2, beta,
-                                                               field, fieldDual);
-    else
-        kSetupStaged<T, false><<<nblocks, kPairBlock, lds, st>>>(blocks, g, cellStart, runCap, pos, rg, dip, dip,
-                                                                candStart, cand, nb, pj, g12, g03, box, rc2, beta,
-                                                                field, nullptr);
-}
-
-template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
                      BoxSel<T> box, double* field, const int* skip, int lanes, bool otf, T beta) {
@@ -1183,10 +1030,6 @@ void launchShortList(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
     template void launchPairCompactSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,  \
                                             const Vec4<T>::type*, const int*, const int*, int*, int*, Vec2<T>::type*, \
                                             Vec2<T>::type*, BoxSel<T>, T, T, double*, double*, int);                  \
-    template void launchSetupStaged<T>(hipStream_t, int, const StagedBlock*, CellGrid, const int*, int,              \
-                                       const Vec4<T>::type*, const T*, const Vec4<T>::type*, const Vec4<T>::type*,    \
-                                       const int*, const int*, int*, int*, Vec2<T>::type*, Vec2<T>::type*, BoxT<T>,   \
-                                       T, T, double*, double*);                                                       \
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
                                      const int*, const int*, const Vec2<T>::type*, BoxSel<T>, double*, const int*,    \
                                      int, bool, T);                                                                   \

@@ -18,7 +18,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpgm_hip.so")
 
 PGM_DOUBLE, PGM_MIXED = 0, 1
-SETUP_NAMES = ["separate", "fused", "staged"]   # pgm_set_lanes / pgm_autotune setup variants 0, 1, 2
 
 
 class PgmError(RuntimeError):
@@ -415,20 +414,15 @@ class Engine:
         self._check(rc)
 
     def autotune(self, reps: int = 3):
-        """((lanes, setup) chosen, {(lanes, setup): ms per evaluation in the pair passes}) (pgm_autotune); setup is
-        "fused", "separate" or "staged" (the cell-staged setup; absent from the table where it does not apply)."""
-        lanes, setup = C.c_int(0), C.c_int(0)
-        ms = np.zeros(9)
-        self._check(self.lib.pgm_autotune(self.h, int(reps), C.byref(lanes), C.byref(setup), _ptr(ms)))
-        table = {(L, name): float(ms[3 * v + k]) for v, name in enumerate(("fused", "separate", "staged"))
-                 for k, L in enumerate((16, 32, 64)) if ms[3 * v + k] < 1e299}  # fmt: skip
-        return (lanes.value, SETUP_NAMES[setup.value]), table
+        """((lanes, fused) chosen, {(lanes, fused): ms per evaluation in the pair passes}) (pgm_autotune)."""
+        lanes, fused = C.c_int(0), C.c_int(0)
+        ms = np.zeros(6)
+        self._check(self.lib.pgm_autotune(self.h, int(reps), C.byref(lanes), C.byref(fused), _ptr(ms)))
+        table = {(L, f): float(ms[3 * (0 if f else 1) + k]) for f in (True, False) for k, L in enumerate((16, 32, 64))}
+        return (lanes.value, bool(fused.value)), table
 
-    def set_lanes(self, lanes: int, setup="fused"):
-        """Pair-pass lanes and setup variant ("fused", "separate", "staged"; True/False: fused/separate)."""
-        if isinstance(setup, bool):
-            setup = "fused" if setup else "separate"
-        self._check(self.lib.pgm_set_lanes(self.h, int(lanes), SETUP_NAMES.index(setup)))
+    def set_lanes(self, lanes: int, fused: bool = True):
+        self._check(self.lib.pgm_set_lanes(self.h, int(lanes), int(bool(fused))))
 
     def set_field_otf(self, on: bool = True):
         """FP32 field pass with its kernels evaluated on the fly (pgm_set_field_otf)."""
