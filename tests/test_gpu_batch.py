@@ -127,8 +127,9 @@ def test_batch_errors_and_reset():
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_autotune_keeps_results(precision):
-    """Every pair-pass variant (fused or separate compaction, 16/32/64 lanes) gives the same results (FP64: to
-    round-off; mixed: to FP32 summation order); the autotuner reports its timings and picks the fastest."""
+    """Every pair-pass variant (fused or separate compaction, 16/32/64 lanes, in FP32 the field pass on the fly or
+    from stored kernels) gives the same results (FP64: to round-off; mixed: to FP32 summation order); the
+    autotuner reports its timings and picks the fastest."""
     sy = S.config3_snapshot(41)
     sy.settings.update(TIGHT)
     eng = E.Engine.for_system(sy, precision=precision)
@@ -136,10 +137,13 @@ def test_autotune_keeps_results(precision):
     tol_e, tol_f = (1e-13, 1e-10) if precision == "double" else (1e-7, 1e-5)
     for lanes in (16, 32, 64):
         for fused in (True, False):
-            eng.set_lanes(lanes, fused)
-            t, f, _ = eng.compute(sy.pos, sy.box, new_step=False)
-            assert abs(E.total_energy(t) / E.total_energy(t0) - 1) < tol_e, (lanes, fused)
-            assert np.abs(f - f0).max() < tol_f * np.abs(f0).max(), (lanes, fused)
+            for otf in ((False, True) if precision == "mixed" else (False,)):
+                eng.set_lanes(lanes, fused)
+                if precision == "mixed":
+                    eng.set_field_otf(otf)
+                t, f, _ = eng.compute(sy.pos, sy.box, new_step=False)
+                assert abs(E.total_energy(t) / E.total_energy(t0) - 1) < tol_e, (lanes, fused, otf)
+                assert np.abs(f - f0).max() < tol_f * np.abs(f0).max(), (lanes, fused, otf)
     best, ms = eng.autotune(2)
     print(f"autotune {precision} (ms per evaluation in the pair passes):", ms, "->", best)
     assert best[0] in (16, 32, 64) and all(v > 0 for v in ms.values())
