@@ -267,11 +267,12 @@ def large_box(torch, E, S, steps, warmup, log):
     return out
 
 
-def md_like(torch, E, sy, sets, steps, log, precision="mixed", step_nm=0.002, seed=31):
+def md_like(torch, E, sy, steps, log, precision="mixed", step_nm=0.002, seed=31):
     """MD-like steps of the 96k workload through the asynchronous device boundary (pgm_launch_device +
     pgm_finish_device, float4 positions, fixed-point forces, as an OpenMM caller): the atoms move every step along
     a seeded velocity field (step_nm per step for the mean atom), every step is a new step (the mu4 history
-    follows the trajectory from the 4 prior sets), the pair list is rebuilt when the displacement check asks.
+    starts from the FP64 solutions of the 4 steps before along the same field and then follows the trajectory),
+    the pair list is rebuilt when the displacement check asks.
     Reports evals/s with the rebuilds in the timed region, the builds, repeats (valid = 0) and the mean CG
     iterations, and what one list build costs (the list phase of a separate pass with every rebuild bracketed)."""
     n = sy.n
@@ -282,6 +283,7 @@ def md_like(torch, E, sy, sets, steps, log, precision="mixed", step_nm=0.002, se
     posq = torch.zeros((steps + 1, n, 4), dtype=torch.float32, device=dev)
     for k in range(steps + 1):
         posq[k, :, :3] = torch.from_numpy((sy.pos + k * step_nm * v).astype(np.float32)).to(dev)
+    sets = prior_dipole_sets(sy, E, step=step_nm, seed=seed)   # the same field: its 4 steps before
     stride, scale = n, 2.0**32
     fixed = torch.zeros(3 * stride, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
@@ -395,7 +397,44 @@ def batch_engine(E, b, precision, tol):
     return eng
 
 
-def adjoint_batch(torch, E, S, reps, log):
+def gap_report(path, pause_ns=50_000_000):
+    """Kernel trace (rocprofv3 --kernel-trace, *_kernel_trace.csv) of a --batch-only run, split at every pause
+    longer than 50 ms: per segment, the union of the kernels' [start, end) intervals (both streams) is GPU-busy
+    time, the rest of its span is launch gaps and host waits.  In a --batch-only run the segments that follow
+    each precision's autotune are its timed steps (then one phase-bracketed step)."""
+    import csv
+
+    rows = []
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    rows.sort()
+    segments, cur = [], [rows[0]]
+    reach = rows[0][1]
+    for a, b in rows[1:]:
+        if a - reach > pause_ns:
+            segments.append(cur)
+            cur = []
+        cur.append((a, b))
+        reach = max(reach, b)
+    segments.append(cur)
+    out = []
+    for seg in segments:
+        busy, c0, c1 = 0, seg[0][0], seg[0][1]
+        for a, b in seg[1:]:
+            if a > c1:
+                busy += c1 - c0
+                c0, c1 = a, b
+            else:
+                c1 = max(c1, b)
+        busy += c1 - c0
+        span = max(b for _, b in seg) - seg[0][0]
+        out.append({"dispatches": len(seg), "span_ms": round(span / 1e6, 3), "busy_ms": round(busy / 1e6, 3),
+                    "idle_share": round(1.0 - busy / span, 4) if span > 0 else 0.0})  # fmt: skip
+    return out
+
+
+def adjoint_batch(torch, E, S, reps, log, pause=False):
     """Force-matching gradients over the 64-snapshot batch in one engine: per step a forward solve from the
     alpha E guess, the same-step re-evaluation and the adjoint solve, FP64 and mixed, tau 1e-8, the pair-pass
     lanes picked by the autotuner.  Unit: snapshot-gradients/s."""
@@ -441,11 +480,15 @@ def adjoint_batch(torch, E, S, reps, log):
 
         step()
         torch.cuda.synchronize()
+        if pause:   # a gap a kernel trace can find: the timed steps follow it
+            time.sleep(0.1)
         t0 = time.perf_counter()
         for _ in range(reps):
             step()
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
+        if pause:   # and after them, so the phase-bracketed step below is a segment of its own
+            time.sleep(0.1)
         # one more step with every kernel group bracketed (in line, so not the timed figure): where a batch goes
         eng.set_timing(True)
         step()
@@ -546,11 +589,20 @@ def main():
     ap.add_argument("--large-only", action="store_true",
                     help="profiler pass over the 999,000-atom box alone (both precisions, --steps timed "
                          "evaluations each); prints its figures, not the JSON line")  # fmt: skip
+    ap.add_argument("--batch-only", action="store_true",
+                    help="the adjoint batch alone, with 0.1 s pauses around each precision's timed steps so a "
+                         "kernel trace splits at them; prints its figures, not the JSON line")  # fmt: skip
+    ap.add_argument("--gap-report", metavar="KERNEL_TRACE_CSV",
+                    help="read a kernel trace of a --batch-only run and print, per segment between pauses, "
+                         "dispatches, span, GPU-busy time and the idle (launch-gap) share; no GPU")  # fmt: skip
     ap.add_argument("--profile", type=int, default=0, metavar="N",
                     help="profiler pass: N evaluations of the workload per precision (each engine autotuned "
                          "first) and one calibration copy of "
                          f"{CAL_COPY_BYTES >> 20} MiB, no timing, no JSON line")  # fmt: skip
     a = ap.parse_args()
+    if a.gap_report:
+        print(json.dumps(gap_report(a.gap_report)))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs one system on one GPU; --gpus must be 1 (multi-GPU sharding "
                                    "is a design note, not a code path)"}))  # fmt: skip
@@ -566,6 +618,9 @@ def main():
 
     if a.large_only:
         print(json.dumps(large_box(torch, E, S, a.steps, a.warmup, log)))
+        return
+    if a.batch_only:
+        print(json.dumps(adjoint_batch(torch, E, S, a.batch_reps, log, pause=True)))
         return
 
     sy = S.config2()
@@ -670,7 +725,7 @@ def main():
     else:
         out["cpu_baseline"] = None
     if not a.no_md:
-        out["md_like"] = md_like(torch, E, sy, sets, a.md_steps, log)
+        out["md_like"] = md_like(torch, E, sy, a.md_steps, log)
         out["mu3"] = mu3_variant(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)
     if not a.no_gvdw:
         out["gvdw"] = gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)

@@ -561,10 +561,30 @@ public:
             throw std::logic_error("pgm_hip: setCovalent before setAtoms");
         if (ncov < 0 || (ncov > 0 && (!atom || !partner || !strength)))
             throw std::invalid_argument("pgm_hip: setCovalent: bad arrays");
+        // the same entries with new strengths (a fitting step): only the strengths change on the device
+        if (ncov == covN_ && covBuiltN_ == n_ && (int) covEntryOfH_.size() == ncov &&
+            std::equal(atom, atom+ncov, covAtomH_.begin()) && std::equal(partner, partner+ncov, covPartnerH_.begin())) {
+            std::vector<double> str(ncov);
+            for (int m = 0; m < ncov; m++)
+                str[covEntryOfH_[m]] = strength[m];
+            withDevice([&] {
+                HIPCHK(hipStreamSynchronize(st_));
+                upload(covStrengthD_, str.data(), ncov*sizeof(double));
+            });
+            haveResults_ = false;
+            return;
+        }
+        std::vector<int> sysOfAtom;
+        if (sys_.size() > 1) {
+            sysOfAtom.resize(n_);
+            for (size_t y = 0; y < sys_.size(); y++)
+                for (int i = sys_[y].a0; i < sys_[y].a1; i++)
+                    sysOfAtom[i] = (int) y;
+        }
         for (int m = 0; m < ncov; m++) {
             if (atom[m] < 0 || atom[m] >= n_ || partner[m] < 0 || partner[m] >= n_ || atom[m] == partner[m])
                 throw std::invalid_argument("pgm_hip: setCovalent: atom index out of range");
-            if (systemOf(atom[m]) != systemOf(partner[m]))
+            if (!sysOfAtom.empty() && sysOfAtom[atom[m]] != sysOfAtom[partner[m]])
                 throw std::invalid_argument("pgm_hip: setCovalent: a covalent dipole joins two systems of the batch");
         }
         withDevice([&] {
@@ -1255,10 +1275,10 @@ private:
     // the PME solves of an evaluation run on st2_ beside the pair passes on st_ (fork / join by events)
     hipStream_t st2_ = nullptr, pmeSt_ = nullptr;
     hipEvent_t evFork_ = nullptr, evJoin_ = nullptr;
-    int n_ = 0, nGlobal_ = 0, covN_ = -1;
+    int n_ = 0, nGlobal_ = 0, covN_ = -1, covBuiltN_ = -1;   // covBuiltN_: the atom count of the covalent CSR
     bool polarizable_ = false;
     std::vector<double> qH_, radH_, alphaH_, forcesH_;
-    std::vector<int> gidxH_, covAtomH_, covPartnerH_;
+    std::vector<int> gidxH_, covAtomH_, covPartnerH_, covEntryOfH_;   // covEntryOfH_: input entry -> CSR slot
     // systems and the PME layout
     std::vector<Sys> sys_;
     std::vector<Group> groups_;
@@ -1497,7 +1517,9 @@ private:
         upload(covInputD_, inputOf.data(), ncov*sizeof(int));
         covAtomH_.assign(atom, atom+ncov);
         covPartnerH_.assign(partner, partner+ncov);
+        covEntryOfH_ = entryOf;
         covN_ = ncov;
+        covBuiltN_ = n_;
         haveResults_ = false;
     }
 

@@ -64,20 +64,23 @@ def test_reorder_keeps_list_and_results(precision):
             assert rel_rms(fm_back, fp) < 1e-6
 
 
-def test_new_parameters_keep_list():
-    """set_atoms with new charges, radii and polarizabilities for the same atoms (a fitting step): no list
-    build, and the results of a fresh engine with those parameters."""
+@pytest.mark.parametrize("new_strengths", [False, True])
+def test_new_parameters_keep_list(new_strengths):
+    """set_atoms with new charges, radii and polarizabilities for the same atoms (a fitting step), and
+    set_covalent with the same entries (new strengths: only those are uploaded): no list build, and the results
+    of a fresh engine with those parameters."""
     sy = S.config0()
     eng = E.Engine.for_system(sy, precision="double")
     eng.compute(sy.pos, sy.box)
     builds = eng.list_builds()
     q2, r2, a2 = sy.q * 1.05, sy.radius * 0.97, sy.alpha * 1.1
+    s2 = sy.cov_strength * np.linspace(0.9, 1.1, len(sy.cov_strength)) if new_strengths else sy.cov_strength
     eng.set_atoms(q2, r2, a2)
-    eng.set_covalent(sy.cov_atom, sy.cov_partner, sy.cov_strength)
+    eng.set_covalent(sy.cov_atom, sy.cov_partner, s2)
     eng.reset_history()   # the same guess as the fresh engine's
     t, f, _ = eng.compute(sy.pos, sy.box)
     assert eng.list_builds() == builds
-    sy2 = S.System(sy.name, sy.pos, sy.box, q2, r2, a2, sy.cov_atom, sy.cov_partner, sy.cov_strength, sy.settings)
+    sy2 = S.System(sy.name, sy.pos, sy.box, q2, r2, a2, sy.cov_atom, sy.cov_partner, s2, sy.settings)
     fresh = E.Engine.for_system(sy2, precision="double")
     tf, ff, _ = fresh.compute(sy.pos, sy.box)
     assert abs(E.total_energy(t) / E.total_energy(tf) - 1) < 1e-12
