@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -185,6 +186,44 @@ void* scratchAddress() {
     return g_scratchPtr[dev];
 }
 
+// The PME stream of the engines on the current device, shared by them and destroyed with the last one.  HIP
+// deals ordinary streams out over a small pool of hardware queues (GPU_MAX_HW_QUEUES, 4 by default); two
+// streams on one queue run one after the other, and an engine's two streams did land on one queue once a
+// process held more than four streams.  A stream created with a CU mask gets a hardware queue of its own, so
+// this one is, with every CU in the mask (bits past the device's CUs are ignored).  It synchronizes with the
+// null stream, as a stream created without flags does.
+std::mutex g_pmeStreamMutex;
+std::vector<hipStream_t> g_pmeStream;   // per device ordinal
+std::vector<int> g_pmeStreamUsers;
+
+hipStream_t acquirePmeStream() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_pmeStreamMutex);
+    if ((int) g_pmeStream.size() <= dev) {
+        g_pmeStream.resize(dev+1, nullptr);
+        g_pmeStreamUsers.resize(dev+1, 0);
+    }
+    if (!g_pmeStream[dev]) {
+        std::vector<uint32_t> mask(32, 0xffffffffu);   // 1,024 CUs
+        HIPCHK(hipExtStreamCreateWithCUMask(&g_pmeStream[dev], (uint32_t) mask.size(), mask.data()));
+    }
+    g_pmeStreamUsers[dev]++;
+    return g_pmeStream[dev];
+}
+
+// with the device current, after the caller's work on the stream has finished
+void releasePmeStream() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return;
+    std::lock_guard<std::mutex> lock(g_pmeStreamMutex);
+    if (dev >= (int) g_pmeStream.size() || !g_pmeStream[dev] || --g_pmeStreamUsers[dev] > 0)
+        return;
+    (void) hipStreamDestroy(g_pmeStream[dev]);
+    g_pmeStream[dev] = nullptr;
+}
+
 template<class T>
 BoxT<T> makeBox(const double H[3][3], const double recip[3][3]) {
     BoxT<T> b;
@@ -275,10 +314,12 @@ public:
             HIPCHK(hipEventCreateWithFlags(&evOut_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evSync_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evList_, hipEventDisableTiming));
-            HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+            st2_ = acquirePmeStream();
             HIPCHK(hipEventCreateWithFlags(&evFork_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evJoin_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evDone_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evCov_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evTab_, hipEventDisableTiming));
             pmeSt_ = st_;
             // pinned words read back by the evaluations (allocated here: a launch makes no allocation that
             // could wait for the GPU)
@@ -302,13 +343,13 @@ public:
             (void) hipEventDestroy(evIn_);
         if (evOut_)
             (void) hipEventDestroy(evOut_);
-        if (st2_)
+        if (st2_) {
             (void) hipStreamSynchronize(st2_);
-        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_, evDone_})
+            releasePmeStream();
+        }
+        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_, evDone_, evCov_, evTab_})
             if (ev)
                 (void) hipEventDestroy(ev);
-        if (st2_)
-            (void) hipStreamDestroy(st2_);
         if (st_)
             (void) hipStreamDestroy(st_);
         if (prev >= 0)
@@ -343,9 +384,12 @@ public:
         if (nGlobal < n && gidx)
             throw std::invalid_argument("pgm_hip: setAtoms: n_global < n");
         const int ng = gidx ? nGlobal : n;
-        std::vector<int> gi(n);
+        std::vector<int>& gi = giTmp_;   // members, so that a fitting step's call allocates nothing on the host
+        gi.resize(n);
+        bool identity = true;
         for (int i = 0; i < n; i++) {
             gi[i] = gidx ? gidx[i] : i;
+            identity = identity && gi[i] == i;
             if (gi[i] < 0 || gi[i] >= ng)
                 throw std::invalid_argument("pgm_hip: setAtoms: global index out of range");
             if (!(radius[i] > 0))
@@ -361,9 +405,13 @@ public:
             }
             // The same atoms again (new parameters, or another order named by the global indices): the list over
             // the spatially sorted atoms is kept and renamed (core/cuda/pgm_cuda.cu:2519-2533,2586-2632).
-            // newOf[old caller index] = new caller index of the same global atom.
-            std::vector<int> newOf;
-            if (!resized && haveList_ && !extList_ && (int) gidxH_.size() == n) {
+            // newOf[old caller index] = new caller index of the same global atom; the same identity order as
+            // before (new parameters only) needs no map.
+            std::vector<int>& newOf = newOfTmp_;
+            newOf.clear();
+            const bool keep = !resized && haveList_ && !extList_ && (int) gidxH_.size() == n;
+            const bool sameOrder = keep && identity && gidxIdentity_;
+            if (keep && !sameOrder) {
                 std::vector<int> localOf(ng, -1);
                 for (int i = 0; i < n; i++)
                     localOf[gi[i]] = i;
@@ -399,23 +447,32 @@ public:
             n_ = n;
             nGlobal_ = ng;
             qH_.assign(q, q+n);
-            radH_.assign(radius, radius+n);
-            alphaH_.resize(n);
+            refreshCharges();
+            // q, the radius parameter 2R^2 and alpha through pinned staging, copied without a wait (a fitting
+            // step sets them at every step): the stream was idle at entry, so the staging is free, and every
+            // kernel that reads them is enqueued on the same stream after the copies
+            paramH_.reserve(3*sizeof(double)*n);
+            double* stage = paramH_.as<double>();
+            std::memcpy(stage, q, n*sizeof(double));
             polarizable_ = false;
             for (int i = 0; i < n; i++) {
-                alphaH_[i] = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
-                polarizable_ = polarizable_ || alphaH_[i] > 0;
+                stage[n+i] = 2.0*radius[i]*radius[i];
+                const double a = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
+                stage[2*n+i] = a;
+                polarizable_ = polarizable_ || a > 0;
             }
-            refreshCharges();
-            gidxH_ = gi;
-            std::vector<double> rg(n);
-            for (int i = 0; i < n; i++)
-                rg[i] = 2.0*radius[i]*radius[i];
-            upload(qD_, qH_.data(), n*sizeof(double));
-            upload(rgD_, rg.data(), n*sizeof(double));
-            upload(alphaD_, alphaH_.data(), n*sizeof(double));
-            upload(gidxD_, gi.data(), n*sizeof(int));
-            if (newOf.empty())
+            DevBuf* dst[3] = {&qD_, &rgD_, &alphaD_};
+            for (int k = 0; k < 3; k++) {
+                dst[k]->reserve(n*sizeof(double));
+                HIPCHK(hipMemcpyAsync(dst[k]->p, stage+k*n, n*sizeof(double), hipMemcpyHostToDevice, st_));
+            }
+            if (resized || !(identity && gidxIdentity_) || gidxH_ != gi)
+                upload(gidxD_, gi.data(), n*sizeof(int));
+            gidxH_.swap(gi);
+            gidxIdentity_ = identity;
+            if (sameOrder)
+                permuteParameters();
+            else if (newOf.empty())
                 haveList_ = false;
             else
                 renameList(newOf);
@@ -564,12 +621,17 @@ public:
         // the same entries with new strengths (a fitting step): only the strengths change on the device
         if (ncov == covN_ && covBuiltN_ == n_ && (int) covEntryOfH_.size() == ncov &&
             std::equal(atom, atom+ncov, covAtomH_.begin()) && std::equal(partner, partner+ncov, covPartnerH_.begin())) {
-            std::vector<double> str(ncov);
-            for (int m = 0; m < ncov; m++)
-                str[covEntryOfH_[m]] = strength[m];
+            // through pinned staging, copied without a wait: the copy is ordered on the stream before every
+            // kernel that reads the strengths, and the staging is written again only after its last copy
             withDevice([&] {
-                HIPCHK(hipStreamSynchronize(st_));
-                upload(covStrengthD_, str.data(), ncov*sizeof(double));
+                HIPCHK(hipEventSynchronize(evCov_));
+                covH_.reserve(std::max(ncov, 1)*sizeof(double));
+                double* str = covH_.as<double>();
+                for (int m = 0; m < ncov; m++)
+                    str[covEntryOfH_[m]] = strength[m];
+                if (ncov)
+                    HIPCHK(hipMemcpyAsync(covStrengthD_.p, str, ncov*sizeof(double), hipMemcpyHostToDevice, st_));
+                HIPCHK(hipEventRecord(evCov_, st_));
             });
             haveResults_ = false;
             return;
@@ -1272,13 +1334,16 @@ private:
     pgm_settings s_;
     hipStream_t st_ = nullptr;
     hipEvent_t evIn_ = nullptr, evOut_ = nullptr, evSync_ = nullptr, evList_ = nullptr;
-    // the PME solves of an evaluation run on st2_ beside the pair passes on st_ (fork / join by events)
+    // the PME solves of an evaluation run on st2_ beside the pair passes on st_ (fork / join by events); st2_ is
+    // the device's PME stream, shared by its engines (acquirePmeStream)
     hipStream_t st2_ = nullptr, pmeSt_ = nullptr;
     hipEvent_t evFork_ = nullptr, evJoin_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1, covBuiltN_ = -1;   // covBuiltN_: the atom count of the covalent CSR
     bool polarizable_ = false;
-    std::vector<double> qH_, radH_, alphaH_, forcesH_;
+    std::vector<double> qH_, forcesH_;
     std::vector<int> gidxH_, covAtomH_, covPartnerH_, covEntryOfH_;   // covEntryOfH_: input entry -> CSR slot
+    std::vector<int> giTmp_, newOfTmp_;   // setAtoms' working arrays
+    bool gidxIdentity_ = false;           // gidxH_[i] == i for every i
     // systems and the PME layout
     std::vector<Sys> sys_;
     std::vector<Group> groups_;
@@ -1408,6 +1473,8 @@ private:
     int adjIterations_ = 0, lastAdjIterations_ = 0;
     double adjResidual_ = 0;
     HostBuf resH_, stageH_, scalH_, listH_, cmpH_;
+    HostBuf paramH_, covH_, tabH_;   // staging of setAtoms' parameters, setCovalent's strengths, the batch tables
+    hipEvent_t evCov_ = nullptr, evTab_ = nullptr;   // after the last copy out of covH_ / tabH_
     // the last evaluation's positions and boxes, for a backward at the same positions
     DevBuf lastPosD_, cmpPosD_, cmpFlagD_;
     std::vector<double> lastBoxes_;
@@ -1444,9 +1511,41 @@ private:
             HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
     }
+    // Small host -> device copies through pinned staging, without a wait: the copies are ordered on the stream
+    // before every later kernel, and the staging is written again only after the event of its last copies.  The
+    // allocation-free evaluation, which may not allocate pinned memory, takes the plain copies.
+    struct Staged {
+        DevBuf* dst;
+        const void* src;
+        size_t bytes;
+    };
+    void uploadStaged(std::initializer_list<Staged> parts) {
+        if (t_arena) {
+            for (const Staged& s : parts)
+                upload(*s.dst, s.src, s.bytes);
+            return;
+        }
+        size_t total = 0;
+        for (const Staged& s : parts)
+            total += (s.bytes+255) & ~(size_t) 255;
+        HIPCHK(hipEventSynchronize(evTab_));
+        tabH_.reserve(std::max<size_t>(total, 256));
+        char* h = tabH_.as<char>();
+        size_t off = 0;
+        for (const Staged& s : parts) {
+            s.dst->reserve(s.bytes);
+            if (s.bytes) {
+                std::memcpy(h+off, s.src, s.bytes);
+                HIPCHK(hipMemcpyAsync(s.dst->p, h+off, s.bytes, hipMemcpyHostToDevice, st_));
+            }
+            off += (s.bytes+255) & ~(size_t) 255;
+        }
+        HIPCHK(hipEventRecord(evTab_, st_));
+    }
 
     // The list kept over a renaming of the caller's indices (newOf: old -> new): order, rank and the build
     // positions renamed, the sorted parameters gathered again from the new arrays.
+    // Nothing here waits for the stream.
     void renameList(const std::vector<int>& newOf) {
         const int n = n_;
         upload(newOfD_, newOf.data(), n*sizeof(int));
@@ -1458,11 +1557,14 @@ private:
         orderD_.swap(order2D_);
         rankD_.swap(rank2D_);
         posBuildD_.swap(build2D_);
-        launchPermute(st_, n, orderD_.as<int>(), qD_.as<double>(), qSD_.as<double>());
-        launchPermute(st_, n, orderD_.as<int>(), alphaD_.as<double>(), alphaSD_.as<double>());
-        launchPermute(st_, n, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
-        HIPCHK(hipStreamSynchronize(st_));
         listRenames_++;
+        permuteParameters();
+    }
+    // the sorted copies of q, alpha and the radius parameter, gathered again from the caller-order arrays
+    void permuteParameters() {
+        launchPermute(st_, n_, orderD_.as<int>(), qD_.as<double>(), qSD_.as<double>());
+        launchPermute(st_, n_, orderD_.as<int>(), alphaD_.as<double>(), alphaSD_.as<double>());
+        launchPermute(st_, n_, orderD_.as<int>(), rgD_.as<double>(), rgSD_.as<double>());
     }
 
     int systemOf(int atom) const {
@@ -1755,11 +1857,9 @@ private:
         for (const Group& G : groups_)
             for (int m : G.members)
                 gb.push_back(bd[m]);
-        upload(boxTabD_, bd.data(), S*sizeof(BoxT<double>));
-        upload(boxTabFD_, bf.data(), S*sizeof(BoxT<float>));
-        upload(pmeTabD_, ps.data(), S*sizeof(PmeSys));
-        upload(groupBoxD_, gb.data(), gb.size()*sizeof(BoxT<double>));
-        upload(qBackD_, qb.data(), S*sizeof(double));
+        uploadStaged({{&boxTabD_, bd.data(), S*sizeof(BoxT<double>)}, {&boxTabFD_, bf.data(), S*sizeof(BoxT<float>)},
+                      {&pmeTabD_, ps.data(), S*sizeof(PmeSys)}, {&groupBoxD_, gb.data(), gb.size()*sizeof(BoxT<double>)},
+                      {&qBackD_, qb.data(), S*sizeof(double)}});
         tablesDirty_ = false;
     }
 
