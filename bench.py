@@ -248,18 +248,29 @@ def large_box(torch, E, S, steps, warmup, log):
         t, f, _ = eng.compute(sy.pos, sy.box)
         used = (free0 - torch.cuda.mem_get_info()[0]) / 2**30
         (lanes, fused), _ = eng.autotune(3)
-        for _ in range(warmup):
-            eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
+        beside = eng.pme_overlap()
+
+        def timed():
+            for _ in range(warmup):
+                eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                eng.compute_device_f64(pos_d.data_ptr(), sy.box, True, f_d.data_ptr(), stream)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        dt = timed()
+        eng.set_pme_overlap(not beside)   # the autotuner's other choice, for comparison
+        dt_other = timed()
+        eng.set_pme_overlap(beside)
         fs = g["forces_sample"]
         err = float(np.sqrt(((f[g["sample"]] - fs) ** 2).sum() / (fs**2).sum()))
         out[prec] = {"evals_per_s": round(steps / dt, 3), "cg_iterations": eng.iterations(), "device_gib": round(used, 2),
-                     "lanes": lanes, "fused_setup": fused, "force_relrms_vs_oracle_tau1e-10": err,
+                     "lanes": lanes, "fused_setup": fused, "pme_beside": beside,
+                     "cg_product_ms_pme_after_beside": [round(v, 4) for v in eng.pme_overlap_times()],
+                     "evals_per_s_other_pme_choice": round(steps / dt_other, 3),
+                     "force_relrms_vs_oracle_tau1e-10": err,
                      "energy_rel_vs_oracle_tau1e-10": E.total_energy(t) / float(g["terms"].sum()) - 1}  # fmt: skip
         log(f"999k {prec}: {steps / dt:.2f} evals/s, {used:.1f} GiB, force error {err:.2e}")
         del eng
@@ -327,12 +338,13 @@ def md_like(torch, E, sy, steps, log, precision="mixed", step_nm=0.002, seed=31)
     return out
 
 
-VARIANTS = {}   # the autotuner's pair-pass choice per precision: (lanes, fused setup, on-the-fly field)
+VARIANTS = {}   # the autotuner's choice per precision: (lanes, fused setup, on-the-fly field, PME beside)
 
 
 def apply_variant(eng, precision):
-    lanes, fused, otf = VARIANTS[precision]
+    lanes, fused, otf, beside = VARIANTS[precision]
     eng.set_lanes(lanes, fused)
+    eng.set_pme_overlap(beside)
     if precision == "mixed":
         eng.set_field_otf(otf)
 
@@ -506,6 +518,8 @@ def adjoint_batch(torch, E, S, reps, log, pause=False):
                          "unit": "TFLOP/s", "frac": round(fl / dt / 1e12 / peak, 4)},
             "list_builds": eng.list_builds(),
             "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "fused_setup": fused,
+            "pme_beside": eng.pme_overlap(),
+            "cg_product_ms_pme_after_beside": [round(v, 4) for v in eng.pme_overlap_times()],
             "phases_ms_per_batch_in_line": phases,
             "autotune_ms": {f"{k[0]}{'f' if k[1] else 's'}": round(v, 4) for k, v in tune.items()},
             "force_relrms_vs_fp64": E.rel_rms(f, ref[0]),
@@ -661,7 +675,7 @@ def main():
         t, f, _ = eng.compute(sy.pos, box)  # checked evaluation (host arrays)
         results[prec] = (t, f, eng.iterations())
         tuned[prec] = eng.autotune(3)
-        VARIANTS[prec] = (*tuned[prec][0], eng.field_otf())
+        VARIANTS[prec] = (*tuned[prec][0], eng.field_otf(), eng.pme_overlap())
         log(f"{prec}: {eng.iterations()} CG iterations, E = {E.total_energy(t):.6f}, pair passes {tuned[prec][0]}")
     P = engines["double"].num_pairs()
     C = engines["double"].num_candidates()
@@ -695,11 +709,15 @@ def main():
             "atoms": sy.n, "pairs_within_cutoff": P, "list_candidates": C, "cg_iterations": n_it,
             "grid": list(sy.settings["grid"]),
             "pair_passes": {"lanes": tuned["mixed"][0][0], "fused_setup": tuned["mixed"][0][1],
-                            "field_on_the_fly": engines["mixed"].field_otf()},
+                            "field_on_the_fly": engines["mixed"].field_otf(), "pme_beside": engines["mixed"].pme_overlap(),
+                            "cg_product_ms_pme_after_beside": [round(v, 4) for v in engines["mixed"].pme_overlap_times()]},
         },
         "fp64_path": {"value": round(v64, 3), "ms_per_step": round(1000.0 * secs["double"] / a.steps, 4),
                       "cg_iterations": results["double"][2], "roofline": roof64,
-                      "pair_passes": {"lanes": tuned["double"][0][0], "fused_setup": tuned["double"][0][1]}},
+                      "pair_passes": {"lanes": tuned["double"][0][0], "fused_setup": tuned["double"][0][1],
+                                      "pme_beside": engines["double"].pme_overlap(),
+                                      "cg_product_ms_pme_after_beside": [round(v, 4) for v in
+                                                                         engines["double"].pme_overlap_times()]}},
         "mixed_over_fp64": round(value / v64, 3),
         "accuracy": {
             "force_relrms_mixed_vs_fp64_gpu": E.rel_rms(fm, fd),

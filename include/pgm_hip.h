@@ -223,8 +223,9 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
  * two variants (compaction fused into the setup pass, or as a pass of its own) and 16, 32 and 64 lanes per
  * atom (reps repetitions each), keeps the fastest for later evaluations and returns it in *lanes, *fused
  * (each may be NULL) and the six timings in ms[6] (ms per evaluation, fused 16/32/64 then separate
- * 16/32/64, each the faster of the two field-pass forms in FP32; may be NULL).  pgm_set_lanes sets them
- * directly (lanes 16, 32 or 64; 16 and separate passes by default). */
+ * 16/32/64, each the faster of the two field-pass forms in FP32; may be NULL).  It then times the PME beside
+ * or after the pair passes (pgm_set_pme_overlap) and keeps the faster too.  pgm_set_lanes sets the pair
+ * passes directly (lanes 16, 32 or 64; 16 and separate passes by default). */
 int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms);
 int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
 /* No counterpart in the reference: the FP32 (PGM_MIXED) field pass of the CG products with its pair kernels
@@ -232,6 +233,14 @@ int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
  * the faster.  pgm_field_otf: the current choice. */
 int pgm_set_field_otf(pgm_engine* e, int on);
 int pgm_field_otf(const pgm_engine* e);
+/* No counterpart in the reference (whose PME solves run in line): the PME solves of an evaluation on a second
+ * stream, with a hardware queue of its own, beside the pair pass they are independent of (1, the default), or
+ * after it on the engine stream (0).  Results are the same bit for bit either way.  pgm_autotune times one CG
+ * product (field pass + PME solve) both ways and keeps the faster; pgm_pme_overlap_times gives those two
+ * timings, ms[0] after, ms[1] beside (ms per product; zeros before an autotune). */
+int pgm_set_pme_overlap(pgm_engine* e, int on);
+int pgm_pme_overlap(const pgm_engine* e);
+int pgm_pme_overlap_times(pgm_engine* e, double* ms);
 
 /* No counterpart in the reference (whose engine prints host wall clocks under PGM_CUDA_TIMING,
  * core/cuda/pgm_cuda.cu:1814-1817): HIP-event timing of the kernel groups of every evaluation, for

@@ -206,7 +206,11 @@ hipStream_t acquirePmeStream() {
     }
     if (!g_pmeStream[dev]) {
         std::vector<uint32_t> mask(32, 0xffffffffu);   // 1,024 CUs
-        HIPCHK(hipExtStreamCreateWithCUMask(&g_pmeStream[dev], (uint32_t) mask.size(), mask.data()));
+        if (hipExtStreamCreateWithCUMask(&g_pmeStream[dev], (uint32_t) mask.size(), mask.data()) != hipSuccess) {
+            (void) hipGetLastError();   // an ordinary stream then (it may share a queue with the engine's)
+            g_pmeStream[dev] = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&g_pmeStream[dev], hipStreamNonBlocking));
+        }
     }
     g_pmeStreamUsers[dev]++;
     return g_pmeStream[dev];
@@ -1198,9 +1202,18 @@ public:
             lanes_ = best;
             fused_ = bestFused;
             fieldOtf_ = bestOtf;
+            // then the PME solves beside the pair passes or after them on the engine stream, timed over a CG
+            // product (field pass + PME solve) with the chosen pair passes.  Beside, the two share the CUs and
+            // the L2; at 999k atoms in FP64, where each fills the chip, that was the slower way.
+            for (int o = 0; o < 2; o++)
+                overlapMs_[o] = mixed ? timeProduct<float>(o == 1, reps) : timeProduct<double>(o == 1, reps);
+            pmeOverlap_ = overlapMs_[1] <= overlapMs_[0];
         });
         return best;
     }
+    bool pmeOverlap() const { return pmeOverlap_; }
+    void setPmeOverlap(bool on) { pmeOverlap_ = on; }
+    const double* overlapTimes() const { return overlapMs_; }
     bool fieldOtf() const { return fieldOtf_; }
     void setFieldOtf(bool on) {
         if (on && s_.precision != PGM_MIXED)
@@ -1337,6 +1350,8 @@ private:
     // the PME solves of an evaluation run on st2_ beside the pair passes on st_ (fork / join by events); st2_ is
     // the device's PME stream, shared by its engines (acquirePmeStream)
     hipStream_t st2_ = nullptr, pmeSt_ = nullptr;
+    bool pmeOverlap_ = true;              // PME on st2_ beside the pair passes (the autotuner measures it)
+    double overlapMs_[2] = {0.0, 0.0};    // the autotuner's CG product: PME after, beside the field pass
     hipEvent_t evFork_ = nullptr, evJoin_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1, covBuiltN_ = -1;   // covBuiltN_: the atom count of the covalent CSR
     bool polarizable_ = false;
@@ -1454,6 +1469,7 @@ private:
     DevBuf covStartD_, covPartnerD_, covStrengthD_, covAtomOfD_, revStartD_, revEntryD_, covInputD_;
     DevBuf cellOfD_, slotD_, cellCountD_, cellStartD_, orderD_, rankD_, sortedCellD_, scratchD_;
     DevBuf candCountD_, candStartD_, candD_, nbD_, pjD_, g12D_, g03D_, subOfD_;
+    DevBuf listScanD_, brickScanD_;   // the tile sums of the long scans (list build, PME bricks)
     DevBuf qSD_, rgSD_, alphaSD_, posDD_, posFD_;
     DevBuf permD_, permTD_, edD_, erD_, bD_, xD_, rD_, zD_, pD_, apD_, tdD_, trD_, pTD_, dD_, dTD_, etotD_;
     DevBuf fpairD_, frecD_, fcovD_, forcesOutD_, tmpD_;
@@ -1546,6 +1562,14 @@ private:
     // The list kept over a renaming of the caller's indices (newOf: old -> new): order, rank and the build
     // positions renamed, the sorted parameters gathered again from the new arrays.
     // Nothing here waits for the stream.
+    // the tile-sum scratch of a scan of m elements in b (null for a short scan, which runs in one block)
+    int* scanScratch(DevBuf& b, int m) {
+        const int k = scanScratchInts(m);
+        if (!k)
+            return nullptr;
+        b.reserve(k*sizeof(int));
+        return b.as<int>();
+    }
     void renameList(const std::vector<int>& newOf) {
         const int n = n_;
         upload(newOfD_, newOf.data(), n*sizeof(int));
@@ -2059,7 +2083,7 @@ private:
             int* cellStart = cellStartD_.as<int>()+y.cellOff;
             launchCellAssign(st_, ns, posDev_+3*(size_t) y.a0, bd, y.cells, cellOfD_.as<int>()+y.a0,
                              slotD_.as<int>()+y.a0, cellCount, subOfD_.as<int>()+y.a0);
-            launchScan(st_, cellCount, cellStart, y.cells.ncell, nullptr);
+            launchScan(st_, cellCount, cellStart, y.cells.ncell, scanScratch(listScanD_, y.cells.ncell));
             launchCellScatter(st_, ns, cellOfD_.as<int>()+y.a0, slotD_.as<int>()+y.a0, cellStart,
                               orderD_.as<int>()+y.a0, y.a0);
             launchCellSort(st_, y.cells.ncell, cellStart, subOfD_.as<int>(), orderD_.as<int>()+y.a0);
@@ -2085,10 +2109,10 @@ private:
             launchCandCount(st_, y.a1-y.a0, posDD_.as<double4>()+y.a0, sortedCellD_.as<int>()+y.a0,
                             cellStartD_.as<int>()+y.cellOff, y.cells, bd, y.rl*y.rl, candCountD_.as<int>()+y.a0);
         }
-        launchScan(st_, candCountD_.as<int>(), candStartD_.as<int>(), n, nullptr);
+        launchScan(st_, candCountD_.as<int>(), candStartD_.as<int>(), n, scanScratch(listScanD_, n));
         scalH_.reserve(64);
         HIPCHK(hipMemcpyAsync(scalH_.p, candStartD_.as<int>()+n, sizeof(int), hipMemcpyDeviceToHost, st_));
-        HIPCHK(hipStreamSynchronize(st_));
+        waitStream();   // a spin on an event: no interrupt wake-up in this round trip
         int total;
         std::memcpy(&total, scalH_.p, sizeof(int));
         if (total < 0)
@@ -2154,7 +2178,7 @@ private:
     // PME beside the pair passes unless PME phases are being timed (their brackets are on the engine stream).
     bool overlapPme() const {
         constexpr unsigned pmeMask = (1u << kPhSpread) | (1u << kPhFft) | (1u << kPhGather);
-        return !(timing_ && (phaseMask_ & pmeMask));
+        return pmeOverlap_ && !(timing_ && (phaseMask_ & pmeMask));
     }
     // The PME stream of one evaluation, back to the engine stream when the scope ends (also on an error).
     struct PmeStreamScope {
@@ -2442,7 +2466,7 @@ private:
         const PmeSel ps = pmeSel();
         launchPmeBin<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                         brickCountD_.as<int>());
-        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_, nullptr);
+        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_, scanScratch(brickScanD_, brickTotal_));
         launchPmePlace<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                           brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
         te();
@@ -2583,6 +2607,29 @@ private:
         histSlots_.push_front(slot);
         while (histSlots_.size() > 4)
             histSlots_.pop_back();
+    }
+
+    // Milliseconds per CG product (field pass + PME solve of the total dipoles) with the PME beside the pair pass
+    // or after it on the engine stream.
+    template<class T>
+    double timeProduct(bool overlap, int reps) {
+        typedef typename Vec4<T>::type T4;
+        hipEvent_t a = nextEvent(), b = nextEvent();
+        const bool wasTiming = timing_;
+        timing_ = false;
+        adjointReady_ = false;   // the products overwrite the evaluation's CG buffers
+        PmeStreamScope scope(*this, overlap);
+        matvec<T>(posT<T>(), dTD_.as<T4>(), nullptr);   // warm-up
+        HIPCHK(hipEventRecord(a, st_));
+        for (int r = 0; r < reps; r++)
+            matvec<T>(posT<T>(), dTD_.as<T4>(), nullptr);
+        HIPCHK(hipEventRecord(b, st_));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        evUsed_ -= 2;
+        timing_ = wasTiming;
+        return ms/reps;
     }
 
     // Milliseconds per evaluation spent in the pair passes at the given lane count (current lists and dipoles).
@@ -3067,6 +3114,18 @@ int pgm_set_field_otf(pgm_engine* e, int on) {
     return guarded(e, [&](DeviceEngine& g) { g.setFieldOtf(on != 0); });
 }
 int pgm_field_otf(const pgm_engine* e) { return (e && e->eng) ? (e->eng->fieldOtf() ? 1 : 0) : -1; }
+
+int pgm_set_pme_overlap(pgm_engine* e, int on) {
+    return guarded(e, [&](DeviceEngine& g) { g.setPmeOverlap(on != 0); });
+}
+int pgm_pme_overlap(const pgm_engine* e) { return (e && e->eng) ? (e->eng->pmeOverlap() ? 1 : 0) : -1; }
+int pgm_pme_overlap_times(pgm_engine* e, double* ms) {
+    return guarded(e, [&](DeviceEngine& g) {
+        if (!ms)
+            throw std::invalid_argument("pgm_hip: pgm_pme_overlap_times needs ms[2]");
+        std::copy(g.overlapTimes(), g.overlapTimes()+2, ms);
+    });
+}
 
 int pgm_set_timing(pgm_engine* e, int on) {
     return guarded(e, [&](DeviceEngine& g) { g.setTiming(on != 0); });

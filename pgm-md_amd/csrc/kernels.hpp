@@ -72,6 +72,9 @@ struct CellGrid {
 void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
                       int* cellCount, int* subOf);
 // Exclusive scan of m ints into out[0..m] (out[m] = total); scratch: >= 1024 ints.
+// Exclusive scan of m ints into out[0..m] (out[m] = the total).  Above 4,096 elements it runs over tiles, with
+// scratch of scanScratchInts(m) ints (null: one block, slower on long arrays).
+int scanScratchInts(int m);
 void launchScan(hipStream_t st, const int* in, int* out, int m, int* scratch);
 // Sorted order: atoms grouped by cell, inside a cell by (2x2x2 sub-cell, original index).
 // base: added to the atom indices written (the system's first atom in a batch).

@@ -68,19 +68,91 @@ __global__ void __launch_bounds__(1024) kScan(const int* in, int* out, int m) {
         out[m] = (int) part[1023];
 }
 
+// Multi-block exclusive scan for long arrays, tiles of kScanTile: (1) every tile's sum, (2) the single-block
+// scan above over the tile sums, (3) every tile scanned and offset by its tile's prefix.
+constexpr int kScanTile = 1024;   // 256 threads x 4 consecutive elements
+
+__device__ inline int blockExclusiveScan256(int v, int* lds, int& total) {
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int u = t >= o ? lds[t-o] : 0;
+        __syncthreads();
+        lds[t] += u;
+        __syncthreads();
+    }
+    total = lds[255];
+    const int incl = lds[t];
+    __syncthreads();
+    return incl-v;
+}
+
+__global__ void __launch_bounds__(256) kTileSums(const int* in, int m, int* sums) {
+    __shared__ int lds[256];
+    const int base = blockIdx.x*kScanTile+4*threadIdx.x;
+    int s = 0;
+    for (int k = 0; k < 4; k++)
+        s += base+k < m ? in[base+k] : 0;
+    int total;
+    (void) blockExclusiveScan256(s, lds, total);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256) kTileScan(const int* in, int* out, int m, const int* sumStart) {
+    __shared__ int lds[256];
+    const int base = blockIdx.x*kScanTile+4*threadIdx.x;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = base+k < m ? in[base+k] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = sumStart[blockIdx.x]+blockExclusiveScan256(s, lds, total);
+    for (int k = 0; k < 4; k++) {
+        if (base+k < m)
+            out[base+k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x-1 && threadIdx.x == 0)
+        out[m] = sumStart[gridDim.x];
+}
+
 __global__ void kCellScatter(int n, const int* cellOf, const int* slot, const int* cellStart, int* order, int base) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i < n)
         order[cellStart[cellOf[i]]+slot[i]] = i+base;
 }
 
-// Inside every cell: ascending (sub-cell, atom index) (the atomic slots above are not deterministic).
-__global__ void kCellSort(int ncell, const int* cellStart, const int* subOf, int* order) {
-    const int c = blockIdx.x*blockDim.x+threadIdx.x;
-    if (c >= ncell)
+// Inside every cell: ascending (sub-cell, atom index) (the atomic slots above are not deterministic).  One
+// wavefront per cell: the cell's keys (unique: the atom index is in them) go to LDS and every key's place is the
+// number of keys below it.  A cell of more than kSortCap atoms (far above any physical density at these cell
+// sizes) is sorted by insertion in one lane.
+constexpr int kSortCap = 512;
+
+__global__ void __launch_bounds__(64) kCellSort(const int* cellStart, const int* subOf, int* order) {
+    __shared__ long long key[kSortCap];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int lo = cellStart[c], m = cellStart[c+1]-lo;
+    if (m <= kSortCap) {
+        for (int k = t; k < m; k += 64) {
+            const int v = order[lo+k];
+            key[k] = ((long long) subOf[v] << 32) | (unsigned) v;
+        }
+        __syncthreads();
+        for (int k = t; k < m; k += 64) {
+            const long long kv = key[k];
+            int below = 0;
+            for (int j = 0; j < m; j++)
+                below += key[j] < kv ? 1 : 0;
+            order[lo+below] = (int) (unsigned) (kv & 0xffffffffll);
+        }
         return;
-    const int lo = cellStart[c], hi = cellStart[c+1];
-    for (int k = lo+1; k < hi; k++) {
+    }
+    if (t != 0)
+        return;
+    for (int k = lo+1; k < lo+m; k++) {
         const int v = order[k];
         const long long kv = ((long long) subOf[v] << 32) | (unsigned) v;
         int j = k-1;
@@ -317,8 +389,21 @@ void launchCellAssign(hipStream_t st, int n, const double* pos, BoxT<double> box
     kCellAssign<<<blocksFor(n, 256), 256, 0, st>>>(n, pos, box, g, cellOf, slot, cellCount, subOf);
 }
 
-void launchScan(hipStream_t st, const int* in, int* out, int m, int*) {
-    kScan<<<1, 1024, 0, st>>>(in, out, m);
+int scanScratchInts(int m) {
+    return m > 4*kScanTile ? 2*((m+kScanTile-1)/kScanTile+1) : 0;
+}
+
+void launchScan(hipStream_t st, const int* in, int* out, int m, int* scratch) {
+    if (m <= 4*kScanTile || !scratch) {
+        kScan<<<1, 1024, 0, st>>>(in, out, m);
+        return;
+    }
+    const int nb = (m+kScanTile-1)/kScanTile;
+    int* sums = scratch;
+    int* sumStart = scratch+nb+1;
+    kTileSums<<<nb, 256, 0, st>>>(in, m, sums);
+    kScan<<<1, 1024, 0, st>>>(sums, sumStart, nb);
+    kTileScan<<<nb, 256, 0, st>>>(in, out, m, sumStart);
 }
 
 void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot, const int* cellStart, int* order,
@@ -327,7 +412,8 @@ void launchCellScatter(hipStream_t st, int n, const int* cellOf, const int* slot
 }
 
 void launchCellSort(hipStream_t st, int ncell, const int* cellStart, const int* subOf, int* order) {
-    kCellSort<<<blocksFor(ncell, 64), 64, 0, st>>>(ncell, cellStart, subOf, order);
+    if (ncell > 0)
+        kCellSort<<<ncell, 64, 0, st>>>(cellStart, subOf, order);
 }
 
 void launchRank(hipStream_t st, int n, const int* order, int* rank, int* sortedCell, const int* cellOf, int base) {

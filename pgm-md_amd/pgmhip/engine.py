@@ -66,7 +66,8 @@ EXPORTS = [
     "pgm_compute_batch", "pgm_compute_batch_device", "pgm_backward_batch", "pgm_backward_batch_device",
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
     "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device", "pgm_launch_pending", "pgm_set_field_otf",
-    "pgm_field_otf", "pgm_field", "pgm_compute_device_f64_noalloc",
+    "pgm_field_otf", "pgm_field", "pgm_compute_device_f64_noalloc", "pgm_set_pme_overlap", "pgm_pme_overlap",
+    "pgm_pme_overlap_times",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -120,6 +121,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_field.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.pgm_set_field_otf.argtypes = [vp, C.c_int]
     lib.pgm_field_otf.argtypes = [vp]
+    lib.pgm_set_pme_overlap.argtypes = [vp, C.c_int]
+    lib.pgm_pme_overlap.argtypes = [vp]
+    lib.pgm_pme_overlap_times.argtypes = [vp, vp]
     lib.pgm_set_history.argtypes = [vp, C.c_int, vp]
     lib.pgm_freeze_history.argtypes = [vp, C.c_int]
     lib.pgm_set_timing.argtypes = [vp, C.c_int]
@@ -430,6 +434,19 @@ class Engine:
 
     def field_otf(self) -> bool:
         return self.lib.pgm_field_otf(self.h) == 1
+
+    def set_pme_overlap(self, on: bool = True):
+        """PME solves beside the pair passes on a stream of their own (pgm_set_pme_overlap)."""
+        self._check(self.lib.pgm_set_pme_overlap(self.h, int(bool(on))))
+
+    def pme_overlap(self) -> bool:
+        return self.lib.pgm_pme_overlap(self.h) == 1
+
+    def pme_overlap_times(self) -> tuple:
+        """The autotuner's ms per CG product: (PME after the field pass, PME beside it)."""
+        ms = np.zeros(2)
+        self._check(self.lib.pgm_pme_overlap_times(self.h, _ptr(ms)))
+        return float(ms[0]), float(ms[1])
 
     def adjoint_iterations(self) -> int:
         return self.lib.pgm_adjoint_iterations(self.h)

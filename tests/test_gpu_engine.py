@@ -268,12 +268,15 @@ def test_large_water_properties():
 def test_evaluations_are_bitwise_reproducible(precision, maker):
     """The PME grids accumulate in 64-bit fixed point (integer atomics commute) and every other sum runs in a
     fixed order, so evaluations are reproducible bit for bit: fresh engines, and the PME beside the pair passes
-    or in line with them (timed phases), give the same energy terms, forces, strain and induced dipoles."""
+    or in line with them (timed phases, or pgm_set_pme_overlap 0), give the same energy terms, forces, strain and
+    induced dipoles."""
     sy = maker()
     out = []
-    for timed in (False, False, True):
+    for timed, beside in ((False, True), (False, True), (True, True), (False, False)):
         eng = E.Engine.for_system(sy, precision=precision)
         eng.set_timing(timed)
+        eng.set_pme_overlap(beside)
+        assert eng.pme_overlap() == beside
         t, f, S3 = eng.compute(sy.pos, sy.box, strain=True)
         out.append((t, f, S3, eng.induced_dipoles(), eng.iterations()))
     t0, f0, S0, m0, i0 = out[0]
@@ -309,3 +312,25 @@ def test_pme_beside_pair_passes_same_results(precision):
         np.testing.assert_array_equal(S3, S0)
         for k in gr:
             assert np.abs(gr[k] - g0[k]).max() <= tol * np.abs(g0[k]).max(), k
+
+
+def test_autotune_picks_pme_overlap():
+    """The autotuner times a CG product with the PME beside the field pass and after it and keeps the faster;
+    the evaluations after it give the same results with either choice, and those of the oracle."""
+    sy = S.config1()
+    sy.settings.update(predictor=0)   # every evaluation from the alpha E guess
+    eng = E.Engine.for_system(sy, precision="double")
+    eng.compute(sy.pos, sy.box)
+    assert eng.pme_overlap_times() == (0.0, 0.0)
+    eng.autotune(2)
+    after, beside = eng.pme_overlap_times()
+    assert after > 0 and beside > 0
+    assert eng.pme_overlap() == (beside <= after)
+    t0, f0, _ = eng.compute(sy.pos, sy.box)
+    eng.set_pme_overlap(not eng.pme_overlap())
+    t1, f1, _ = eng.compute(sy.pos, sy.box)
+    assert t1 == t0
+    np.testing.assert_array_equal(f1, f0)
+    eo, fo = O.Oracle(sy).compute(sy.pos, sy.box)
+    assert abs(E.total_energy(t0) / eo - 1) < 1e-10
+    assert rel_rms(f0, fo) < 1e-9
