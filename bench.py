@@ -278,14 +278,15 @@ def large_box(torch, E, S, steps, warmup, log):
     return out
 
 
-def md_like(torch, E, sy, steps, log, precision="mixed", step_nm=0.002, seed=31):
+def md_like(torch, E, sy, steps, log, precision="mixed", step_nm=0.002, seed=31, tune=False):
     """MD-like steps of the 96k workload through the asynchronous device boundary (pgm_launch_device +
     pgm_finish_device, float4 positions, fixed-point forces, as an OpenMM caller): the atoms move every step along
     a seeded velocity field (step_nm per step for the mean atom), every step is a new step (the mu4 history
     starts from the FP64 solutions of the 4 steps before along the same field and then follows the trajectory),
     the pair list is rebuilt when the displacement check asks.
     Reports evals/s with the rebuilds in the timed region, the builds, repeats (valid = 0) and the mean CG
-    iterations, and what one list build costs (the list phase of a separate pass with every rebuild bracketed)."""
+    iterations, and what one list build costs (the list phase of a separate pass with every rebuild bracketed).
+    tune: autotune the engine after step 0 instead of taking the main workload's choice."""
     n = sy.n
     rng = np.random.default_rng(seed)
     v = rng.normal(0.0, 1.0, sy.pos.shape)
@@ -302,25 +303,30 @@ def md_like(torch, E, sy, steps, log, precision="mixed", step_nm=0.002, seed=31)
     for timed in (False, True):
         eng = E.Engine.for_system(sy, precision=precision)
         eng.set_history(sets)
-        apply_variant(eng, precision)
+        if not tune:
+            apply_variant(eng, precision)
         if timed:
             eng.set_timing(True, phases=["list"])
-        its, repeats = [], 0
+        its, repeats, stale_repeats = [], 0, 0
         builds0 = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps + 1):
             if k == 1:   # step 0 builds the list and takes the first (host-checked) solve: outside the timed region
+                if tune:
+                    eng.autotune(3)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 builds0 = eng.list_builds()
             while True:
                 fixed.zero_()
+                b0 = eng.list_builds()
                 eng.launch_device(posq[k].data_ptr(), sy.box, True, fixed.data_ptr(), stride, scale, stream)
                 _, valid = eng.finish_device()
                 if valid:
                     break
                 repeats += 1
+                stale_repeats += eng.list_builds() > b0   # else the solve needed more iterations than enqueued
             its.append(eng.iterations())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
@@ -332,10 +338,28 @@ def md_like(torch, E, sy, steps, log, precision="mixed", step_nm=0.002, seed=31)
             builds = eng.list_builds() - builds0
             out.update({"evals_per_s": round(steps / dt, 3), "steps": steps, "step_nm": step_nm,
                         "list_builds": builds, "repeated_launches": repeats,
+                        "repeats_stale_list": stale_repeats, "repeats_short_solve": repeats - stale_repeats,
                         "late_list_checks": eng.late_list_checks(), "mean_cg_iterations": round(float(np.mean(its[1:])), 3)})  # fmt: skip
             log(f"md-like {precision}: {steps / dt:.2f} evals/s, {builds} list builds in {steps} steps, {repeats} repeats")
         del eng
     return out
+
+
+def liquid98k(torch, E, S, steps, log):
+    """The reference's closest published configuration: 32,768 waters (98,304 atoms) at liquid density, a 9.945 nm
+    box, PME 112^3 order 6, rc 0.9 nm, tau 1e-4, mu4 and peek 0.65 (docs/openmm.md:177: 24.0 ns/day of MD at
+    1 fs with the reference's CUDA engine under OpenMM on one RTX PRO 6000, about 278 steps/s, integration,
+    constraints and LJ included).  Here: MD-like steps of the electrostatics alone (md_like, autotuned), mixed."""
+    L = 21.54 * (32768 / 333000) ** (1.0 / 3.0)
+    sy = S.water_box(32768, L, 32, 5, 0.03, "water98k-liquid", S.base_settings(grid=(112, 112, 112), tol=1e-4))
+    out = md_like(torch, E, sy, steps, log, tune=True)
+    return {"workload": f"32,768 rigid waters (98,304 atoms), {L:.3f} nm cubic box (liquid density), rc 0.9 nm, "
+                        "PME 112^3 order 6, tau 1e-4, mu4 (history along the trajectory), peek 0.65; MD-like steps "
+                        "through launch/finish, mixed precision",
+            "mixed": out,
+            "reference_published": {"steps_per_s": 278, "ns_per_day_at_1fs": 24.0, "gpu": "NVIDIA RTX PRO 6000 Blackwell",
+                                    "what": "MD steps of OpenMM with the reference's CUDA pGM engine (integration, "
+                                            "constraints and LJ included)", "source": "docs/openmm.md:177"}}  # fmt: skip
 
 
 VARIANTS = {}   # the autotuner's choice per precision: (lanes, fused setup, on-the-fly field, PME beside)
@@ -744,6 +768,7 @@ def main():
         out["cpu_baseline"] = None
     if not a.no_md:
         out["md_like"] = md_like(torch, E, sy, a.md_steps, log)
+        out["liquid98k_vs_reference"] = liquid98k(torch, E, S, a.md_steps, log)
         out["mu3"] = mu3_variant(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)
     if not a.no_gvdw:
         out["gvdw"] = gvdw_cost(torch, E, sy, sets, pos_d, f_d, stream, max(10, a.steps // 2), a.warmup, log)

@@ -366,6 +366,29 @@ __device__ __forceinline__ V groupSum(V v) {
     return v;
 }
 
+// slot = atomicAdd(&count[key], 1) for every lane with valid set, with one atomic per distinct key in the
+// wavefront: spatially sorted atoms share a key across most of a wavefront, and atomics on one address
+// serialize.  Every lane of the wavefront calls it (valid = false where there is no item); the slots within a
+// key follow lane order, and those of different wavefronts the order of their atomics.
+__device__ __forceinline__ int waveSlot(int* count, int key, bool valid) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(valid);
+    int slot = 0;
+    while (pending != 0ull) {
+        const int leader = __ffsll((unsigned long long) pending)-1;
+        const int lk = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(valid && key == lk) & pending;
+        int base = 0;
+        if (lane == leader)
+            base = atomicAdd(&count[lk], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull)
+            slot = base+__popcll(same & ((1ull << lane)-1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
 // Block-wide sum of a double (blockDim.x a multiple of 64, <= 1024); the result is valid in thread 0.
 __device__ __forceinline__ double blockSum(double v, double* smem) {
     v = groupSum<64>(v);

@@ -318,7 +318,6 @@ public:
             HIPCHK(hipEventCreateWithFlags(&evOut_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evSync_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evList_, hipEventDisableTiming));
-            st2_ = acquirePmeStream();
             HIPCHK(hipEventCreateWithFlags(&evFork_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evJoin_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evDone_, hipEventDisableTiming));
@@ -332,6 +331,7 @@ public:
             scalH_.reserve(64);
             resH_.reserve(kResCount*sizeof(double));   // one system's sums
             scratchPtr_ = scratchAddress();
+            st2_ = acquirePmeStream();   // last: released by the destructor, which runs only for a built engine
         });
     }
     ~DeviceEngine() {
@@ -1448,7 +1448,8 @@ private:
         timed_.push_back(Timed{openPhase_, openIter_, openEv_, b});
         openPhase_ = -1;
     }
-    // After the evaluation's final synchronization: accumulate the launches that did work.
+    // Accumulate the launches that did work.  Each bracket's end event is waited for: most were passed by the
+    // evaluation's final synchronization, but a list rebuilt by pgm_finish_device is enqueued after it.
     void collectTiming() {
         if (!timing_)
             return;
@@ -1456,6 +1457,7 @@ private:
             if (t.iteration >= iterations_)
                 continue;   // enqueued ahead, returned at once
             float ms = 0;
+            HIPCHK(hipEventSynchronize(t.b));
             HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
             phaseMs_[t.phase] += ms;
             phaseCount_[t.phase]++;

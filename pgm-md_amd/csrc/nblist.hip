@@ -31,14 +31,18 @@ __device__ __forceinline__ void fracCell(double x, double y, double z, const Box
 __global__ void kCellAssign(int n, const double* pos, BoxT<double> box, CellGrid g, int* cellOf, int* slot,
                             int* cellCount, int* subOf) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i >= n)
-        return;
-    int c[3], sub;
-    fracCell(pos[3*i], pos[3*i+1], pos[3*i+2], box, g, c, sub);
-    const int cell = (c[0]*g.nc[1]+c[1])*g.nc[2]+c[2];
-    cellOf[i] = cell;
-    subOf[i] = sub;
-    slot[i] = atomicAdd(&cellCount[cell], 1);
+    const bool valid = i < n;   // no early return: the whole wavefront takes part in waveSlot
+    int cell = 0;
+    if (valid) {
+        int c[3], sub;
+        fracCell(pos[3*i], pos[3*i+1], pos[3*i+2], box, g, c, sub);
+        cell = (c[0]*g.nc[1]+c[1])*g.nc[2]+c[2];
+        cellOf[i] = cell;
+        subOf[i] = sub;
+    }
+    const int k = waveSlot(cellCount, cell, valid);
+    if (valid)
+        slot[i] = k;
 }
 
 // Single-block exclusive scan: 1024 threads, each over a contiguous chunk.

@@ -252,17 +252,21 @@ __device__ __forceinline__ const PmeSys* sysEntry(const PmeSel& ps, int sys) {
 template<class T, bool BATCH>
 __global__ void kPmeBin(int n, const double4* posD, PmeSel ps, int* brickOf, int* slot, int* brickCount) {
     const int s = blockIdx.x*blockDim.x+threadIdx.x;
-    if (s >= n)
-        return;
-    const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.sysOf[s] : 0);
-    const PmeGrid& g = BATCH ? se->g : ps.g;
-    const BoxT<double>& box = BATCH ? se->box : ps.box;
-    int base[3];
-    T fr[3];
-    fracGrid<T>(posD[s], box, g, base, fr);
-    const int b = ((base[0]/g.brick)*g.nb[1]+base[1]/g.brick)*g.nb[2]+base[2]/g.brick+(BATCH ? se->brickOff : 0);
-    brickOf[s] = b;
-    slot[s] = atomicAdd(&brickCount[b], 1);
+    const bool valid = s < n;   // no early return: the whole wavefront takes part in waveSlot
+    int b = 0;
+    if (valid) {
+        const PmeSys* se = sysEntry<BATCH>(ps, BATCH ? ps.sysOf[s] : 0);
+        const PmeGrid& g = BATCH ? se->g : ps.g;
+        const BoxT<double>& box = BATCH ? se->box : ps.box;
+        int base[3];
+        T fr[3];
+        fracGrid<T>(posD[s], box, g, base, fr);
+        b = ((base[0]/g.brick)*g.nb[1]+base[1]/g.brick)*g.nb[2]+base[2]/g.brick+(BATCH ? se->brickOff : 0);
+        brickOf[s] = b;
+    }
+    const int k = waveSlot(brickCount, b, valid);
+    if (valid)
+        slot[s] = k;
 }
 
 template<class T, int P, bool BATCH>

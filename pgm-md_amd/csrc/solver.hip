@@ -9,6 +9,8 @@
 //               (core/pgm_core.cpp:488-534).
 #include "kernels.hpp"
 
+#include <stdexcept>
+
 #include <algorithm>
 
 namespace pgmhip {
@@ -465,19 +467,40 @@ __global__ void __launch_bounds__(256) kReducePartials(const double* partials, i
     }
 }
 
+// Every column of a range of rows in one pass: per-thread sums of its rows, then one wavefront reduction per
+// column and one barrier (a reduction per column, each with its barriers, made the final phase's sums slow).
+constexpr int kMaxSumWidth = 12;
+
 __global__ void __launch_bounds__(256) kRangeSums(const double* rows, int width, const int* starts,
                                                   const int* outIndex, int outStride, double* out) {
-    __shared__ double smem[40];
+    __shared__ double smem[4][kMaxSumWidth];
     const int r = blockIdx.x;
     const int lo = starts[r], hi = starts[r+1];
     const int orow = outIndex ? outIndex[r] : r;
-    for (int q = 0; q < width; q++) {
-        double v = 0;
-        for (int k = lo+threadIdx.x; k < hi; k += blockDim.x)
-            v += rows[(size_t) k*width+q];
-        v = blockSum(v, smem);
-        if (threadIdx.x == 0)
-            out[(size_t) orow*outStride+q] = v;
+    double acc[kMaxSumWidth];
+#pragma unroll
+    for (int q = 0; q < kMaxSumWidth; q++)
+        acc[q] = 0.0;
+    for (int k = lo+threadIdx.x; k < hi; k += blockDim.x) {
+        const double* row = rows+(size_t) k*width;
+#pragma unroll
+        for (int q = 0; q < kMaxSumWidth; q++)
+            if (q < width)
+                acc[q] += row[q];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kMaxSumWidth; q++) {
+        const double v = groupSum<64>(acc[q]);
+        if (lane == 0 && q < width)
+            smem[w][q] = v;
+    }
+    __syncthreads();
+    if ((int) threadIdx.x < width) {
+        double v = 0.0;
+        for (int k = 0; k < (int) (blockDim.x >> 6); k++)
+            v += smem[k][threadIdx.x];
+        out[(size_t) orow*outStride+threadIdx.x] = v;
     }
 }
 
@@ -610,6 +633,8 @@ void launchReducePartials(hipStream_t st, const double* partials, int nblk_, int
 
 void launchRangeSums(hipStream_t st, const double* rows, int width, const int* starts, int nr, const int* outIndex,
                      int outStride, double* out) {
+    if (width < 1 || width > kMaxSumWidth)
+        throw std::logic_error("pgm_hip: range sums of more than 12 columns");
     if (nr > 0)
         kRangeSums<<<nr, 256, 0, st>>>(rows, width, starts, outIndex, outStride, out);
 }

@@ -334,3 +334,41 @@ def test_autotune_picks_pme_overlap():
     eo, fo = O.Oracle(sy).compute(sy.pos, sy.box)
     assert abs(E.total_energy(t0) / eo - 1) < 1e-10
     assert rel_rms(f0, fo) < 1e-9
+
+
+def test_engines_sharing_the_pme_stream():
+    """Every engine on a device runs its PME solves on one shared stream (a hardware queue of its own).  Two
+    engines of different systems and precisions, evaluating alternately on the caller's stream, give the same
+    bits as each engine evaluating alone (every evaluation from the alpha E guess); the stream outlives the
+    first engine's destruction, and the engines made after the last one was destroyed make it again."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    systems = [(S.config0(), "double"), (S.config1(), "mixed")]
+    for sy, _ in systems:
+        sy.settings.update(predictor=0)
+    alone = []
+    for sy, prec in systems:
+        eng = E.Engine.for_system(sy, precision=prec)
+        t, f, _ = eng.compute(sy.pos, sy.box)
+        alone.append((t, f))
+        del eng
+    engines = [E.Engine.for_system(sy, precision=prec) for sy, prec in systems]
+    pos = [torch.from_numpy(sy.pos.copy()).to(dev) for sy, _ in systems]
+    forces = [torch.zeros((sy.n, 3), dtype=torch.float64, device=dev) for sy, _ in systems]
+    terms = [None, None]
+    for _ in range(3):
+        for k, (sy, _) in enumerate(systems):
+            forces[k].zero_()
+            terms[k] = engines[k].compute_device_f64(pos[k].data_ptr(), sy.box, True, forces[k].data_ptr(), stream)
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert terms[k] == alone[k][0]
+        np.testing.assert_array_equal(forces[k].cpu().numpy(), alone[k][1])
+    engines[0] = None   # destroyed; the other engine keeps the shared stream
+    sy, _ = systems[1]
+    forces[1].zero_()
+    t = engines[1].compute_device_f64(pos[1].data_ptr(), sy.box, True, forces[1].data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert t == alone[1][0]
+    np.testing.assert_array_equal(forces[1].cpu().numpy(), alone[1][1])

@@ -123,6 +123,25 @@ def test_stale_list_found_at_finish_then_repeat(torch, precision):
     _same(t, dev.forces(), tr, fr, precision)
 
 
+def test_stale_list_at_finish_with_phase_timing(torch):
+    """The same stale list found at finish, with the list phase timed: the rebuild that finish enqueues is
+    bracketed too, and its timing is read once it has run (it used to be read too early: 'device not ready')."""
+    sy = S.config0()
+    dev = Device(torch, sy.n)
+    pos1 = sy.pos.copy()
+    pos1[5] += [0.09, 0.0, 0.0]
+    eng = E.Engine.for_system(sy, precision="mixed")
+    eng.set_timing(True, phases=["list"])
+    for pos, want in ((sy.pos, True), (pos1, False), (pos1, True)):
+        dev.load(pos)
+        dev.launch(eng, sy.box)
+        _, valid = eng.finish_device()
+        assert valid == want
+    assert eng.list_builds() == 2
+    ms, count = eng.timing()["list"]
+    assert ms > 0 and count >= 2   # both builds bracketed (the first launch's and the one finish enqueued)
+
+
 def test_short_solve_repeats(torch):
     """A launch enqueues the most iterations of the last 16 solves; after the history is reset the guess needs
     more, finish says valid = 0, and the repeat (which checks convergence on the host) is valid."""
