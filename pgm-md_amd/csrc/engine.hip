@@ -789,8 +789,8 @@ public:
     // launchDevice (core/cuda/pgm_cuda.cu:2754-2774): the whole evaluation enqueued behind the caller's stream, no
     // host wait.  The pair list is checked on the device and read by the completion; before the launch the host
     // predicts from the last step's displacement and largest move whether the list will hold and rebuilds it if
-    // not (core/cuda/pgm_cuda.cu:1912-1928).  The CG iterations enqueued are the most that the last 16 solves
-    // needed, each behind its device flag; the first launch of an engine (no such count yet), and a repeat after
+    // not (core/cuda/pgm_cuda.cu:1912-1928).  The CG iterations enqueued are one more than the most that the last
+    // 16 solves needed, each behind its device flag; the first launch of an engine (no such count yet), and a repeat after
     // a solve that needed more, wait on the host between chunks of iterations instead.  The forces are added to
     // the fixed-point buffer in any case; the completion says whether they are valid.
     void launchDevice(const void* posq4, const double* boxes, bool newStep, long long* fixed, int stride,
@@ -806,8 +806,10 @@ public:
             HIPCHK(hipStreamWaitEvent(st_, evIn_, 0));
             setBoxes(boxes);
             stagePositions(PosSource::DeviceF4, posq4);
-            // (a repeat is at the positions of the launch it repeats: no move ahead to predict)
-            const double ahead = repeatNext_ ? 0.0 : 2.0*std::sqrt(lastS2_);
+            // (a repeat is at the positions of the launch it repeats: no move ahead to predict, and none to
+            // measure, so the last step's move is kept for the next launch)
+            const bool repeat = repeatNext_;
+            const double ahead = repeat ? 0.0 : 2.0*std::sqrt(lastS2_);
             repeatNext_ = false;
             bool built = false;
             if (!haveList_ || (!extList_ && staleFor(lastD2_, ahead))) {
@@ -817,7 +819,7 @@ public:
                 built = true;
             }
             pendingChecking_ = !extList_ && !built;
-            pendingPrev_ = pendingChecking_ && prevValid_;
+            pendingPrev_ = pendingChecking_ && prevValid_ && !repeat;
             if (pendingChecking_)
                 startListCheck(pendingPrev_);
             posPrevD_.reserve(3*sizeof(double)*n_);
@@ -1250,6 +1252,17 @@ public:
         return total/2;
     }
     long lateListChecks() const { return lateChecks_; }
+    // the launch-time list prediction's inputs: displacement since the build at the last completed check, the
+    // largest move of the last step, the smallest skin, the launches repeated
+    void listCheckState(double* out) const {
+        double skin = 1e300;
+        for (const Sys& y : sys_)
+            skin = std::min(skin, y.skinNow);
+        out[0] = std::sqrt(lastD2_);
+        out[1] = std::sqrt(lastS2_);
+        out[2] = skin;
+        out[3] = (double) repeats_;
+    }
     long repeatedLaunches() const { return repeats_; }
     // The caller's stream waits on the GPU for the last evaluation's outputs (no host wait).
     void waitOn(hipStream_t stream) {
@@ -2340,17 +2353,19 @@ private:
         te();
     }
 
-    // The iterations the last 16 solves needed: their maximum is what an asynchronous launch enqueues.
+    // The iterations the last 16 solves needed: an asynchronous launch enqueues one more than their maximum.
     void noteIterations(int need) {
         recentIts_[recentPos_] = need;
         recentPos_ = (recentPos_+1)%16;
         numRecent_ = std::min(numRecent_+1, 16);
     }
+    // One more than that maximum: an iteration enqueued but not needed returns at once on the device (a few
+    // microseconds per kernel), a solve that needs more than were enqueued repeats the whole evaluation.
     int predictedIterations() const {
         int m = 1;
         for (int k = 0; k < numRecent_; k++)
             m = std::max(m, recentIts_[k]);
-        return m;
+        return std::min(m+1, std::max(s_.max_iter, 1));
     }
 
     // Per-system sums of per-atom rows [n][width] into res[s][slot..slot+width).
@@ -3023,6 +3038,13 @@ long long pgm_num_pairs(pgm_engine* e) {
 }
 
 long pgm_late_list_checks(const pgm_engine* e) { return (e && e->eng) ? e->eng->lateListChecks() : -1; }
+
+int pgm_list_check_state(const pgm_engine* e, double* out) {
+    if (!e || !e->eng || !out)
+        return -2;
+    e->eng->listCheckState(out);
+    return 0;
+}
 
 int pgm_set_history(pgm_engine* e, int nset, const double* mu_sets) {
     return guarded(e, [&](DeviceEngine& g) { g.setHistory(nset, mu_sets); });

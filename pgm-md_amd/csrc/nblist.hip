@@ -326,8 +326,8 @@ __global__ void kPrepAtoms(PrepArgs a) {
         a.zeroI[s] = 0;
     if (s < a.nZeroD)
         a.zeroD[s] = 0.0;
-    if (s == 0 && a.maxBits)
-        *a.maxBits = 0u;
+    if (s < 2 && a.maxBits)   // both words of the next list check: displacement since the build, last move
+        a.maxBits[s] = 0u;
     if (s >= a.n)
         return;
     const BoxT<double> box = selectBox<BATCH>(a.box, s);

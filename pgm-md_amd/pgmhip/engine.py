@@ -67,6 +67,7 @@ EXPORTS = [
     "pgm_autotune", "pgm_set_lanes", "pgm_eval_erfc_f32", "pgm_eval_pair_kernels", "pgm_set_gvdw",
     "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device", "pgm_launch_pending", "pgm_set_field_otf",
     "pgm_field_otf", "pgm_field", "pgm_compute_device_f64_noalloc", "pgm_set_pme_overlap", "pgm_pme_overlap",
+    "pgm_list_check_state",
     "pgm_pme_overlap_times",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
@@ -115,6 +116,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_list_builds.restype = C.c_long
     lib.pgm_num_candidates.argtypes = [vp]
     lib.pgm_num_candidates.restype = C.c_longlong
+    lib.pgm_list_check_state.argtypes = [vp, vp]
     lib.pgm_late_list_checks.argtypes = [vp]
     lib.pgm_late_list_checks.restype = C.c_long
     lib.pgm_launch_pending.argtypes = [vp]
@@ -310,6 +312,13 @@ class Engine:
 
     def late_list_checks(self) -> int:
         return self.lib.pgm_late_list_checks(self.h)
+
+    def list_check_state(self) -> dict:
+        """pgm_list_check_state: the inputs of the next launch's list prediction (test hook)."""
+        out = np.zeros(4)
+        self._check(self.lib.pgm_list_check_state(self.h, _ptr(out)))
+        return {"displacement": float(out[0]), "last_move": float(out[1]), "skin": float(out[2]),
+                "repeats": int(out[3])}
 
     def backward(self, pos, box, dl_dforces=None, dl_denergy=0.0) -> dict:
         """Gradients of a loss L(E, F) with respect to the per-atom parameters (pgm_backward).

@@ -143,8 +143,9 @@ def test_stale_list_at_finish_with_phase_timing(torch):
 
 
 def test_short_solve_repeats(torch):
-    """A launch enqueues the most iterations of the last 16 solves; after the history is reset the guess needs
-    more, finish says valid = 0, and the repeat (which checks convergence on the host) is valid."""
+    """A launch enqueues one more than the most iterations of the last 16 solves; with a bad history (four sets
+    of random dipoles, the same on both engines) the guess needs more, finish says valid = 0, and the repeat
+    (which checks convergence on the host) is valid."""
     sy = S.config0()
     dev = Device(torch, sy.n)
     eng = E.Engine.for_system(sy, precision="double")
@@ -159,11 +160,12 @@ def test_short_solve_repeats(torch):
         _, valid = eng.finish_device()
         assert valid
         its.append(eng.iterations())
-    eng.reset_history()
-    ref.reset_history()
+    bad = np.random.default_rng(8).normal(0.0, 0.01, (4, sy.n, 3))   # |mu| is about 0.003 e nm
+    eng.set_history(bad)
+    ref.set_history(bad)
     dev.load(traj[18])
     tr, fr = dev.compute(ref, sy.box)
-    assert ref.iterations() > max(its[-16:]), "the workload does not exercise a short solve"
+    assert ref.iterations() > max(its[-16:]) + 1, "the workload does not exercise a short solve"
     dev.load(traj[18])
     dev.launch(eng, sy.box)
     _, valid = eng.finish_device()
@@ -201,3 +203,41 @@ def test_wait_without_finish_counts_late(torch):
     fresh = E.Engine.for_system(sy, precision="double", tol=1e-12)
     tf, ff, _ = fresh.compute(moved, sy.box)
     assert abs(E.total_energy(t) / E.total_energy(tf) - 1) < 1e-6
+
+
+def test_launch_predicts_rebuilds_along_a_trajectory(torch):
+    """Atoms moving at constant velocity: every launch predicts from the last check's displacement and the last
+    step's largest move (both measured on the device, reported by pgm_list_check_state) whether the list holds,
+    and rebuilds before it goes stale, so no launch is repeated for a stale list.  The largest move is measured
+    afresh at each launch (its word was once left uncleared, so it held whatever the device memory held and
+    the prediction never fired)."""
+    sy = S.config0()
+    dev = Device(torch, sy.n)
+    rng = np.random.default_rng(5)
+    v = rng.normal(0.0, 1.0, sy.pos.shape)
+    v /= np.sqrt((v * v).sum(1)).max()   # the fastest atom moves 0.01 nm per step: skin/2 in 7.5 steps
+    step = 0.01
+    eng = E.Engine.for_system(sy, precision="mixed")
+    builds0, stale_repeats = None, 0
+    prev = None
+    for k in range(24):
+        pos = sy.pos + k * step * v
+        while True:
+            dev.load(pos)
+            b0 = eng.list_builds()
+            dev.launch(eng, sy.box)
+            _, valid = eng.finish_device()
+            if valid:
+                break
+            stale_repeats += eng.list_builds() > b0
+        st = eng.list_check_state()
+        assert np.isfinite(st["last_move"]) and np.isfinite(st["displacement"])
+        if prev is not None and k >= 2:
+            p32, q32 = pos.astype(np.float32).astype(float), prev.astype(np.float32).astype(float)
+            true_move = np.sqrt(((p32 - q32) ** 2).sum(1)).max()
+            assert abs(st["last_move"] - true_move) < 1e-4, (k, st, true_move)
+        prev = pos
+        if k == 0:
+            builds0 = eng.list_builds()
+    assert eng.list_builds() - builds0 >= 2   # the list was rebuilt along the way
+    assert stale_repeats == 0
