@@ -114,8 +114,8 @@ int pgm_compute_device(pgm_engine* e, const void* posq4, const double* box, int 
  * other work on the GPU in between.  pgm_launch_device enqueues the whole evaluation behind the work so far on
  * stream and returns without waiting for the GPU: the pair list is checked on the device, the host predicting
  * from the last step's displacement and largest move whether it will hold (rebuilt before the launch if not);
- * the CG iterations enqueued are one more than the most the last 16 solves needed, each skipped on the device
- * once the solve has converged.  Exceptions that wait: a list build, and the first launch of an engine or a repeat after a
+ * the CG iterations enqueued are the most the last 16 solves needed (one more if their counts varied), each
+ * skipped on the device once the solve has converged.  Exceptions that wait: a list build, and the first launch of an engine or a repeat after a
  * short solve, whose CG checks convergence on the host between chunks of iterations.  The forces are added to
  * force_fixed in every case.  pgm_finish_device waits for the launch and sets valid = 0 when its list turned
  * out to be stale for these positions (the list is rebuilt now) or its solve needed more iterations than were
@@ -164,9 +164,10 @@ int pgm_reset_history(pgm_engine* e);
 long pgm_list_builds(const pgm_engine* e);
 long long pgm_num_candidates(const pgm_engine* e);
 long pgm_late_list_checks(const pgm_engine* e);
-/* No counterpart in the reference: a test hook.  out[4] = what the next pgm_launch_device predicts the list's
- * staleness from: the largest displacement since the build at the last completed check [nm], the largest move
- * of the last step [nm], the skin [nm], and the number of launches repeated so far. */
+/* No counterpart in the reference: a test hook.  out[5]: what the next pgm_launch_device predicts the list's
+ * staleness from, the largest displacement since the build at the last completed check [nm], the largest move
+ * of the last step [nm] and the skin [nm]; the launches repeated so far; the synchronous evaluations redone
+ * because their solve needed more iterations than were enqueued ahead. */
 int pgm_list_check_state(const pgm_engine* e, double* out);
 /* No counterpart in the reference: pairs within the cutoff at the last evaluation, each counted once. */
 long long pgm_num_pairs(pgm_engine* e);

@@ -739,31 +739,73 @@ public:
                 te();
                 mixed ? prep<float>() : prep<double>();
             }
-            if (mixed)
-                run<float>(newStep, strainOut != nullptr);
-            else
-                run<double>(newStep, strainOut != nullptr);
-            // forces out, caller's order
-            tb(kPhFinal);
-            if (fDev || fixed)
-                launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
-                                fcovD_.as<double>(), s_.ke, fDev, fixed, stride, fixedScale);
-            if (fHost || keepForces) {
-                forcesOutD_.reserve(3*sizeof(double)*n_);
-                launchZero<double>(st_, forcesOutD_.as<double>(), 3*(size_t) n_, nullptr);
-                launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
-                                fcovD_.as<double>(), s_.ke, forcesOutD_.as<double>(), nullptr, 0, 0.0);
-                stageH_.reserve(3*sizeof(double)*n_);
-                HIPCHK(hipMemcpyAsync(stageH_.p, forcesOutD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost, st_));
+            // Once the engine has solved before, the CG is enqueued ahead as a launch's is (the most iterations
+            // the last 16 solves needed, one more if they varied, each skipped on the device once converged): no
+            // host wait between the solve and the final stage.  The forces are added only if those iterations sufficed (the
+            // solve's device flag); a solve that needed more is evaluated again, from the same guess (the history
+            // is advanced only by a complete solve), with its convergence checked on the host.
+            bool ahead = polarizable_ && s_.polarizable && numRecent_ > 0;
+            for (;;) {
+                asyncEval_ = asyncCg_ = ahead;
+                asyncPredict_ = predictedIterations();
+                asyncStop_ = -1;
+                pendingPush_ = -1;
+                try {
+                    if (mixed)
+                        run<float>(newStep, strainOut != nullptr);
+                    else
+                        run<double>(newStep, strainOut != nullptr);
+                }
+                catch (...) {
+                    asyncEval_ = asyncCg_ = false;
+                    throw;
+                }
+                asyncEval_ = asyncCg_ = false;
+                const int* guard = (ahead && asyncStop_ >= 0 && asyncStop_ < s_.max_iter)
+                                       ? doneD_.as<int>()+asyncStop_ : nullptr;
+                // forces out, caller's order
+                tb(kPhFinal);
+                if (fDev || fixed)
+                    launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
+                                    fcovD_.as<double>(), s_.ke, fDev, fixed, stride, fixedScale, guard);
+                if (fHost || keepForces) {
+                    forcesOutD_.reserve(3*sizeof(double)*n_);
+                    launchZero<double>(st_, forcesOutD_.as<double>(), 3*(size_t) n_, nullptr);
+                    launchForcesOut(st_, n_, rankD_.as<int>(), fpairD_.as<double>(), frecD_.as<double>(),
+                                    fcovD_.as<double>(), s_.ke, forcesOutD_.as<double>(), nullptr, 0, 0.0);
+                    stageH_.reserve(3*sizeof(double)*n_);
+                    HIPCHK(hipMemcpyAsync(stageH_.p, forcesOutD_.p, 3*sizeof(double)*n_, hipMemcpyDeviceToHost,
+                                          st_));
+                }
+                te();
+                // the evaluation's positions, for a backward at the same positions (which then skips evaluating)
+                lastPosD_.reserve(3*sizeof(double)*n_);
+                HIPCHK(hipMemcpyAsync(lastPosD_.p, posDev_, 3*sizeof(double)*n_, hipMemcpyDeviceToDevice, st_));
+                const size_t resBytes = sys_.size()*kResCount*sizeof(double);
+                resH_.reserve(resBytes);
+                HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, resBytes, hipMemcpyDeviceToHost, st_));
+                waitStream();
+                if (!ahead || asyncStop_ < 0)
+                    break;
+                CgState cs;
+                int done;
+                std::memcpy(&cs, cgH_.p, sizeof(CgState));
+                std::memcpy(&done, cgH_.as<char>()+sizeof(CgState), sizeof(int));
+                const bool shortSolve = !done && asyncStop_ < s_.max_iter;
+                noteIterations(shortSolve ? asyncStop_+1 : cs.iterations);
+                if (!shortSolve) {
+                    commitPush(pendingPush_);
+                    iterations_ = lastIterations_ = cs.iterations;
+                    residual_ = cs.err;
+                    break;
+                }
+                ahead = false;
+                shortSolves_++;
+                timed_.clear();   // (the brackets of the pass given up have all run: the wait above)
+                evUsed_ = 0;
             }
-            te();
-            // the evaluation's positions, for a backward at the same positions (which then skips evaluating)
-            lastPosD_.reserve(3*sizeof(double)*n_);
-            HIPCHK(hipMemcpyAsync(lastPosD_.p, posDev_, 3*sizeof(double)*n_, hipMemcpyDeviceToDevice, st_));
-            const size_t resBytes = sys_.size()*kResCount*sizeof(double);
-            resH_.reserve(resBytes);
-            HIPCHK(hipMemcpyAsync(resH_.p, resD_.p, resBytes, hipMemcpyDeviceToHost, st_));
-            waitStream();
+            pendingPush_ = -1;
+            asyncStop_ = -1;
             lastBoxes_.assign(boxes, boxes+9*sys_.size());
             adjointReady_ = true;
             collectTiming();
@@ -789,8 +831,8 @@ public:
     // launchDevice (core/cuda/pgm_cuda.cu:2754-2774): the whole evaluation enqueued behind the caller's stream, no
     // host wait.  The pair list is checked on the device and read by the completion; before the launch the host
     // predicts from the last step's displacement and largest move whether the list will hold and rebuilds it if
-    // not (core/cuda/pgm_cuda.cu:1912-1928).  The CG iterations enqueued are one more than the most that the last
-    // 16 solves needed, each behind its device flag; the first launch of an engine (no such count yet), and a repeat after
+    // not (core/cuda/pgm_cuda.cu:1912-1928).  The CG iterations enqueued are the most that the last 16 solves
+    // needed (one more if their counts varied), each behind its device flag; the first launch of an engine (no such count yet), and a repeat after
     // a solve that needed more, wait on the host between chunks of iterations instead.  The forces are added to
     // the fixed-point buffer in any case; the completion says whether they are valid.
     void launchDevice(const void* posq4, const double* boxes, bool newStep, long long* fixed, int stride,
@@ -1252,8 +1294,9 @@ public:
         return total/2;
     }
     long lateListChecks() const { return lateChecks_; }
+    long shortSolves() const { return shortSolves_; }
     // the launch-time list prediction's inputs: displacement since the build at the last completed check, the
-    // largest move of the last step, the smallest skin, the launches repeated
+    // largest move of the last step, the smallest skin; the launches repeated, the synchronous evaluations redone
     void listCheckState(double* out) const {
         double skin = 1e300;
         for (const Sys& y : sys_)
@@ -1262,6 +1305,7 @@ public:
         out[1] = std::sqrt(lastS2_);
         out[2] = skin;
         out[3] = (double) repeats_;
+        out[4] = (double) shortSolves_;
     }
     long repeatedLaunches() const { return repeats_; }
     // The caller's stream waits on the GPU for the last evaluation's outputs (no host wait).
@@ -1386,6 +1430,7 @@ private:
     bool inFlight_ = false, asyncEval_ = false, asyncCg_ = false, syncCgNext_ = false, repeatNext_ = false;
     bool pendingChecking_ = false, pendingPrev_ = false, prevValid_ = false;
     int asyncPredict_ = 0, asyncStop_ = 0, pendingPush_ = -1;
+    long shortSolves_ = 0;   // synchronous evaluations whose solve needed more than was enqueued ahead
     double lastD2_ = 0, lastS2_ = 0;   // squared: displacement since the build at the last step, the last move
     long lateChecks_ = 0, repeats_ = 0;
     int recentIts_[16] = {0}, numRecent_ = 0, recentPos_ = 0;
@@ -2353,19 +2398,23 @@ private:
         te();
     }
 
-    // The iterations the last 16 solves needed: an asynchronous launch enqueues one more than their maximum.
+    // The iterations the last 16 solves needed: the CG enqueued ahead takes their maximum (one more if they
+    // varied).
     void noteIterations(int need) {
         recentIts_[recentPos_] = need;
         recentPos_ = (recentPos_+1)%16;
         numRecent_ = std::min(numRecent_+1, 16);
     }
-    // One more than that maximum: an iteration enqueued but not needed returns at once on the device (a few
-    // microseconds per kernel), a solve that needs more than were enqueued repeats the whole evaluation.
+    // That maximum, and one more when the counts varied: a solve that needs more than were enqueued repeats the
+    // whole evaluation, while an iteration enqueued but not needed still dispatches every workgroup of its
+    // kernels (each returns at once), about 40 us at 96k atoms, which a steady count need not pay.
     int predictedIterations() const {
-        int m = 1;
-        for (int k = 0; k < numRecent_; k++)
-            m = std::max(m, recentIts_[k]);
-        return std::min(m+1, std::max(s_.max_iter, 1));
+        int hi = 1, lo = 1 << 30;
+        for (int k = 0; k < numRecent_; k++) {
+            hi = std::max(hi, recentIts_[k]);
+            lo = std::min(lo, recentIts_[k]);
+        }
+        return std::min(hi+(numRecent_ > 0 && lo < hi ? 1 : 0), std::max(s_.max_iter, 1));
     }
 
     // Per-system sums of per-atom rows [n][width] into res[s][slot..slot+width).

@@ -441,8 +441,10 @@ void launchCovalentForces(hipStream_t st, int n, const int* rank, const double* 
                           const int* covPartner, const double* covStrength, const int* revStart, const int* revEntry,
                           const int* covAtomOf, const double* etot, BoxSel<double> box, double* fcov,
                           double* perAtom);
+// guard (may be null): the forces are added only where *guard is nonzero (a solve's convergence flag).
 void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair, const double* frec,
-                     const double* fcov, double ke, double* fOut, long long* fixedOut, int stride, double fixedScale);
+                     const double* fcov, double ke, double* fOut, long long* fixedOut, int stride, double fixedScale,
+                     const int* guard = nullptr);
 // Column sums of partials [nblk][width] into out[0..width).
 void launchReducePartials(hipStream_t st, const double* partials, int nblk, int width, double* out);
 // Column sums of row ranges: out[outRow(r)*outStride + c] = sum of rows[k][c], k in [starts[r], starts[r+1]),

@@ -439,9 +439,10 @@ __global__ void __launch_bounds__(kCgBlock) kCovalent(int n, const int* rank, co
 }
 
 __global__ void kForcesOut(int n, const int* rank, const double* fpair, const double* frec, const double* fcov,
-                           double ke, double* fOut, long long* fixedOut, int stride, double fixedScale) {
+                           double ke, double* fOut, long long* fixedOut, int stride, double fixedScale,
+                           const int* guard) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i >= n)
+    if (i >= n || (guard && *guard == 0))
         return;
     const int s = rank[i];
 #pragma unroll
@@ -623,8 +624,10 @@ void launchCovalentForces(hipStream_t st, int n, const int* rank, const double* 
 }
 
 void launchForcesOut(hipStream_t st, int n, const int* rank, const double* fpair, const double* frec,
-                     const double* fcov, double ke, double* fOut, long long* fixedOut, int stride, double fixedScale) {
-    kForcesOut<<<nblk(n, 256), 256, 0, st>>>(n, rank, fpair, frec, fcov, ke, fOut, fixedOut, stride, fixedScale);
+                     const double* fcov, double ke, double* fOut, long long* fixedOut, int stride, double fixedScale,
+                     const int* guard) {
+    kForcesOut<<<nblk(n, 256), 256, 0, st>>>(n, rank, fpair, frec, fcov, ke, fOut, fixedOut, stride, fixedScale,
+                                             guard);
 }
 
 void launchReducePartials(hipStream_t st, const double* partials, int nblk_, int width, double* out) {

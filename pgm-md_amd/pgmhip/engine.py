@@ -315,10 +315,10 @@ class Engine:
 
     def list_check_state(self) -> dict:
         """pgm_list_check_state: the inputs of the next launch's list prediction (test hook)."""
-        out = np.zeros(4)
+        out = np.zeros(5)
         self._check(self.lib.pgm_list_check_state(self.h, _ptr(out)))
         return {"displacement": float(out[0]), "last_move": float(out[1]), "skin": float(out[2]),
-                "repeats": int(out[3])}
+                "repeats": int(out[3]), "sync_short_solves": int(out[4])}
 
     def backward(self, pos, box, dl_dforces=None, dl_denergy=0.0) -> dict:
         """Gradients of a loss L(E, F) with respect to the per-atom parameters (pgm_backward).

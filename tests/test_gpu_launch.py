@@ -143,9 +143,9 @@ def test_stale_list_at_finish_with_phase_timing(torch):
 
 
 def test_short_solve_repeats(torch):
-    """A launch enqueues one more than the most iterations of the last 16 solves; with a bad history (four sets
-    of random dipoles, the same on both engines) the guess needs more, finish says valid = 0, and the repeat
-    (which checks convergence on the host) is valid."""
+    """A launch enqueues the most iterations of the last 16 solves (one more if they varied); with a bad history
+    (four sets of random dipoles, the same on both engines) the guess needs more, finish says valid = 0, and the
+    repeat (which checks convergence on the host) is valid."""
     sy = S.config0()
     dev = Device(torch, sy.n)
     eng = E.Engine.for_system(sy, precision="double")
@@ -241,3 +241,31 @@ def test_launch_predicts_rebuilds_along_a_trajectory(torch):
             builds0 = eng.list_builds()
     assert eng.list_builds() - builds0 >= 2   # the list was rebuilt along the way
     assert stale_repeats == 0
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_sync_evaluation_with_cg_enqueued_ahead(torch, precision):
+    """pgm_compute_device enqueues the CG ahead once the engine has solved before (no host wait between the solve
+    and the final stage) and adds the forces only if the enqueued iterations sufficed.  Along a trajectory no
+    solve falls short (the trajectory tests against the oracle cover these evaluations' results); after a bad
+    history it does, the evaluation is redone from the same guess with its convergence checked on the host, the
+    forces are added once, and the results equal those of a fresh engine (host-checked: it has not solved
+    before) with the same history."""
+    sy = S.config0()
+    dev = Device(torch, sy.n)
+    eng = E.Engine.for_system(sy, precision=precision)
+    for pos in S.rigid_moves(sy, 6, 0.001, 8):
+        dev.load(pos)
+        dev.compute(eng, sy.box)
+    assert eng.list_check_state()["sync_short_solves"] == 0
+    bad = np.random.default_rng(8).normal(0.0, 0.01, (4, sy.n, 3))   # |mu| is about 0.003 e nm
+    eng.set_history(bad)
+    ref = E.Engine.for_system(sy, precision=precision)
+    ref.set_history(bad)
+    dev.load(sy.pos)
+    t, f = dev.compute(eng, sy.box)
+    dev.load(sy.pos)
+    tr, fr = dev.compute(ref, sy.box)
+    assert eng.list_check_state()["sync_short_solves"] == 1
+    assert eng.iterations() == ref.iterations()
+    _same(t, f, tr, fr, precision)
