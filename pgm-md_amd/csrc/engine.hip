@@ -741,9 +741,9 @@ public:
             }
             // Once the engine has solved before, the CG is enqueued ahead as a launch's is (the most iterations
             // the last 16 solves needed, one more if they varied, each skipped on the device once converged): no
-            // host wait between the solve and the final stage.  The forces are added only if those iterations sufficed (the
-            // solve's device flag); a solve that needed more is evaluated again, from the same guess (the history
-            // is advanced only by a complete solve), with its convergence checked on the host.
+            // host wait between the solve and the final stage.  The forces are added only if those iterations
+            // sufficed (the solve's device flag); a solve that needed more is evaluated again, from the same guess
+            // (the history is advanced only by a complete solve), with its convergence checked on the host.
             bool ahead = polarizable_ && s_.polarizable && numRecent_ > 0;
             for (;;) {
                 asyncEval_ = asyncCg_ = ahead;
@@ -832,8 +832,9 @@ public:
     // host wait.  The pair list is checked on the device and read by the completion; before the launch the host
     // predicts from the last step's displacement and largest move whether the list will hold and rebuilds it if
     // not (core/cuda/pgm_cuda.cu:1912-1928).  The CG iterations enqueued are the most that the last 16 solves
-    // needed (one more if their counts varied), each behind its device flag; the first launch of an engine (no such count yet), and a repeat after
-    // a solve that needed more, wait on the host between chunks of iterations instead.  The forces are added to
+    // needed (one more if their counts varied), each behind its device flag; the first launch of an engine (no
+    // such count yet), and a repeat after a solve that needed more, wait on the host between chunks of iterations
+    // instead.  The forces are added to
     // the fixed-point buffer in any case; the completion says whether they are valid.
     void launchDevice(const void* posq4, const double* boxes, bool newStep, long long* fixed, int stride,
                       double fixedScale, hipStream_t caller) {
@@ -1446,9 +1447,9 @@ private:
     bool haveResults_ = false;
     int lanes_ = 16;
     bool fused_ = false;
-    bool fieldOtf_ = false;
+    bool fieldOtf_ = false;        // FP32: the field pass evaluates its kernels instead of reading the stored ones
     void* scratchPtr_ = nullptr;   // this engine's device's static scratch (scratchAddress)
-    bool noHistory_ = false;       // the allocation-free evaluation: alpha E guess, no history entry   // FP32: the field pass evaluates its kernels instead of reading the stored ones
+    bool noHistory_ = false;       // the allocation-free evaluation: alpha E guess, no history entry
     double tuneMs_[6] = {0, 0, 0, 0, 0, 0};
     // kernel timing
     struct Timed {
@@ -1723,7 +1724,8 @@ private:
                 for (int c = 0; c < 3; c++)
                     H[r][c] = b[3*r+c];
             if (H[0][1] != 0 || H[0][2] != 0 || H[1][2] != 0)
-                throw std::invalid_argument("pgm_hip: the box must be in reduced triclinic form (a = (ax,0,0), b = (bx,by,0))");
+                throw std::invalid_argument("pgm_hip: the box must be in reduced triclinic form "
+                                            "(a = (ax,0,0), b = (bx,by,0))");
             if (!(H[0][0] > 0 && H[1][1] > 0 && H[2][2] > 0))
                 throw std::invalid_argument("pgm_hip: box diagonal must be positive");
             changed = changed || std::memcmp(H, y.H, sizeof(H)) != 0;
@@ -1764,8 +1766,11 @@ private:
             for (int a = 0; a < 3; a++)
                 gridsFit = gridsFit && y.grid[a] >= kBrickLarge;
         }
-        const long long tileLarge = (long long) (kBrickLarge+s_.order-1)*(kBrickLarge+s_.order-1)*(kBrickLarge+s_.order-1);
-        const bool largeFits = tileLarge*(long long) sizeof(double) <= 64*1024;   // the spread's FP64 LDS tile
+        const long long tileEdge = kBrickLarge+s_.order-1;
+        const long long tileLarge = tileEdge*tileEdge*tileEdge;
+        // the spread's LDS tile holds 8 bytes per point and grid word (one word in FP32, two in FP64: 77 KiB at
+        // order 6, within the 160 KiB of LDS a gfx950 workgroup may take); higher orders keep the small brick
+        const bool largeFits = tileLarge*(long long) sizeof(double) <= 64*1024;
         brick_ = (largeFits && gridsFit && largeBricks >= kMinBricks) ? kBrickLarge : kBrickSmall;
         for (size_t s = 0; s < sys_.size(); s++) {
             Sys& y = sys_[s];
@@ -1941,8 +1946,10 @@ private:
         for (const Group& G : groups_)
             for (int m : G.members)
                 gb.push_back(bd[m]);
-        uploadStaged({{&boxTabD_, bd.data(), S*sizeof(BoxT<double>)}, {&boxTabFD_, bf.data(), S*sizeof(BoxT<float>)},
-                      {&pmeTabD_, ps.data(), S*sizeof(PmeSys)}, {&groupBoxD_, gb.data(), gb.size()*sizeof(BoxT<double>)},
+        uploadStaged({{&boxTabD_, bd.data(), S*sizeof(BoxT<double>)},
+                      {&boxTabFD_, bf.data(), S*sizeof(BoxT<float>)},
+                      {&pmeTabD_, ps.data(), S*sizeof(PmeSys)},
+                      {&groupBoxD_, gb.data(), gb.size()*sizeof(BoxT<double>)},
                       {&qBackD_, qb.data(), S*sizeof(double)}});
         tablesDirty_ = false;
     }
@@ -2306,7 +2313,8 @@ private:
         fork();
         tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                           g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_, fieldOtf_, (T) s_.beta);
+                           g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_, fieldOtf_,
+                           (T) s_.beta);
         te();
         pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
         join();
@@ -2532,7 +2540,8 @@ private:
         const PmeSel ps = pmeSel();
         launchPmeBin<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                         brickCountD_.as<int>());
-        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_, scanScratch(brickScanD_, brickTotal_));
+        launchScan(st_, brickCountD_.as<int>(), brickStartD_.as<int>(), brickTotal_,
+                   scanScratch(brickScanD_, brickTotal_));
         launchPmePlace<T>(st_, n, posDD_.as<double4>(), ps, brickOfD_.as<int>(), pslotD_.as<int>(),
                           brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>());
         te();
