@@ -1038,6 +1038,8 @@ public:
         gvdwReady_ = false;
         preReady_ = false;
         prevValid_ = false;
+        fixedClean_ = nullptr;
+        fixedCleanWords_ = 0;
     }
 
     // pgmcore::Engine::setGeometry + field (core/pgm_core.h:76-79, core/pgm_core.cpp:539-560): the field [e/nm^2]
@@ -1537,6 +1539,8 @@ private:
     DevBuf brickOfD_, pslotD_, brickCountD_, brickStartD_, pmeAtomD_, baseD_, splD_;
     DevBuf fqD_, fdD_;   // the charges and dipoles of a field evaluation (caller's order)
     DevBuf gridD_, gridFixedD_, cplxD_, inflD_, twD_, bmodD_;
+    const long long* fixedClean_ = nullptr;   // gridFixedD_ as last left all zero, over fixedCleanWords_ words
+    size_t fixedCleanWords_ = 0;
     DevBuf cgPartD_, partD_, resD_, cgD_, doneD_, histD_, maxBitsD_;
     DevBuf perAtomD_, chunkSumD_;
     DevBuf fftGroupsD_;     // the merged transform launches' group table
@@ -2276,9 +2280,15 @@ private:
         // the charge/dipole grids through the fixed-point accumulator (order-independent), a tangent density
         // (dipDot) straight into grid
         long long* fixed = dipDot ? nullptr : gridFixedD_.as<long long>();
+        const size_t fixedWords = (size_t) realTotal_*GridFixed<T>::words;
         tb(kPhSpread, it);
-        if (fixed)
-            launchZero<long long>(pmeSt_, fixed, (size_t) realTotal_*GridFixed<T>::words, skip);
+        if (fixed) {
+            // the forward z transform zeroes every word it reads, so the grid is clean for the next spread; it is
+            // cleared here only when that is not known (new buffer, a larger layout, a solve cut short)
+            if (fixed != fixedClean_ || fixedWords > fixedCleanWords_)
+                launchZero<long long>(pmeSt_, fixed, fixedWords, nullptr);
+            fixedClean_ = nullptr;
+        }
         else
             launchZero<T>(pmeSt_, grid, (size_t) realTotal_, skip);
         launchSpread<T>(pmeSt_, ps, brickStartD_.as<int>(), pmeAtomD_.as<int>(), baseD_.as<int>(), splD_.as<T>(), dip,
@@ -2289,6 +2299,10 @@ private:
         launchPmeConvolve<T>(pmeSt_, fftL_, grid, cplxD_.as<T2>(), inflD_.as<T>(), twD_.as<T2>(), one,
                              batch_ ? groupBoxD_.as<BoxT<double>>() : nullptr, s_.beta, energyMode, strain ? 1 : 0,
                              partD_.as<double>(), skip, fixed, (size_t) realTotal_);
+        if (fixed) {   // enqueued: the spread's words are read and zeroed before anything else uses the grid
+            fixedClean_ = fixed;
+            fixedCleanWords_ = fixedWords;
+        }
         if (energyMode)
             launchRangeSums(pmeSt_, partD_.as<double>(), 10, recStartD_.as<int>(), nRecRanges_, recSysD_.as<int>(),
                             kResCount, res+kResRec);

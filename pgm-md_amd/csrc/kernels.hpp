@@ -317,6 +317,15 @@ __device__ __forceinline__ T fromFixed(const long long* p, size_t stride) {
     else
         return (T) ((double) p[0]*GridFixed<T>::inv);
 }
+// The same, leaving the words zero for the next spread (the forward z transform reads every point once).
+template<class T>
+__device__ __forceinline__ T takeFixed(long long* p, size_t stride) {
+    const T v = fromFixed<T>(p, stride);
+    p[0] = 0;
+    if constexpr (GridFixed<T>::words == 2)
+        p[stride] = 0;
+    return v;
+}
 
 // One grid shape's share of a merged transform launch: every shape of a batch goes through each stage (z, y,
 // x + convolution, y, z) in one launch; a block finds its shape by its block index.
@@ -344,11 +353,12 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]);
 // grids of all groups; real: the stacked real grids, cplx/influence/tw: the stacked complex arrays.  boxes: per
 // stacked grid (nullptr: box for all).  partials: [.][10] (energy, strain 0..8) per x block, the blocks of a
 // group's grid k being rows partOff + [k, k+1) x blocks per grid.  fixed: the forward transforms read the
-// fixed-point grids (as the spread left them) instead of real; the inverse transforms write real.
+// fixed-point grids (as the spread left them) instead of real, and leave them zero; the inverse transforms
+// write real.
 template<class T>
 void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
                        const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                       int energy, int strain, double* partials, const int* skip, const long long* fixed,
+                       int energy, int strain, double* partials, const int* skip, long long* fixed,
                        size_t fixedStride);
 // Blocks of the x pass per grid (the rows of its energy partials) when ngrid grids of shape g are stacked.
 template<class T>

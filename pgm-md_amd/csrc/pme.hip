@@ -456,12 +456,13 @@ __device__ __forceinline__ int fftGroupOf(const FftGroup* groups, int ngroups, i
 }
 
 // z: two real lines per complex FFT, unpacked into their half spectra.
-// Forward: from the fixed-point grid when fixedAll is set, else from the real grid.
+// Forward: from the fixed-point grid when fixedAll is set (each word zeroed once read: the grid is clean for
+// the next spread), else from the real grid.
 template<class T, bool INVERSE>
 __global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ groups, int ngroups,
                                                 T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
                                                 const typename Vec2<T>::type* __restrict__ twAll, const int* skip,
-                                                const long long* __restrict__ fixedAll, size_t fixedStride) {
+                                                long long* __restrict__ fixedAll, size_t fixedStride) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
@@ -480,14 +481,14 @@ __global__ void __launch_bounds__(kBlock) kFftZ(const FftGroup* __restrict__ gro
     const int pair0 = blk*pairsPerBlock;
     const int npairs = min(pairsPerBlock, (nlines+1)/2-pair0);
     if (!INVERSE) {
-        const long long* fixed = fixedAll ? fixedAll+grp.realOff : nullptr;
+        long long* fixed = fixedAll ? fixedAll+grp.realOff : nullptr;
         for (int i = threadIdx.x; i < npairs*K; i += blockDim.x) {
             const int l = i/K, k = i-l*K;
             const int la = 2*(pair0+l), lb = la+1;
             T2 z;
             if (fixed) {
-                z.x = fromFixed<T>(fixed+(size_t) la*K+k, fixedStride);
-                z.y = (lb < nlines ? fromFixed<T>(fixed+(size_t) lb*K+k, fixedStride) : T(0));
+                z.x = takeFixed<T>(fixed+(size_t) la*K+k, fixedStride);
+                z.y = (lb < nlines ? takeFixed<T>(fixed+(size_t) lb*K+k, fixedStride) : T(0));
             }
             else {
                 z.x = real[(size_t) la*K+k];
@@ -592,7 +593,7 @@ template<class T, bool INVERSE>
 __global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ groups, int ngroups,
                                                  T* __restrict__ realAll, typename Vec2<T>::type* __restrict__ cplxAll,
                                                  const typename Vec2<T>::type* __restrict__ twAll, const int* skip,
-                                                 const long long* __restrict__ fixedAll, size_t fixedStride) {
+                                                 long long* __restrict__ fixedAll, size_t fixedStride) {
     typedef typename Vec2<T>::type T2;
     extern __shared__ unsigned char smemRaw[];
     if (skip && *skip)
@@ -613,14 +614,14 @@ __global__ void __launch_bounds__(kBlock) kFftZY(const FftGroup* __restrict__ gr
     loadTwiddles<T>(twY, twAll+grp.twOff+g.K[0], K1);
     loadTwiddles<T>(twZ, twAll+grp.twOff+g.K[0]+K1, K2);
     if (!INVERSE) {
-        const long long* fixed = fixedAll ? fixedAll+grp.realOff+(size_t) plane*K1*K2 : nullptr;
+        long long* fixed = fixedAll ? fixedAll+grp.realOff+(size_t) plane*K1*K2 : nullptr;
         for (int i = threadIdx.x; i < npairs*K2; i += blockDim.x) {
             const int l = i/K2, k = i-l*K2;
             const int ya = 2*l, yb = ya+1;
             T2 z;
             if (fixed) {
-                z.x = fromFixed<T>(fixed+(size_t) ya*K2+k, fixedStride);
-                z.y = (yb < K1 ? fromFixed<T>(fixed+(size_t) yb*K2+k, fixedStride) : T(0));
+                z.x = takeFixed<T>(fixed+(size_t) ya*K2+k, fixedStride);
+                z.y = (yb < K1 ? takeFixed<T>(fixed+(size_t) yb*K2+k, fixedStride) : T(0));
             }
             else {
                 z.x = real[(size_t) ya*K2+k];
@@ -1184,7 +1185,7 @@ void fftGroupShape(FftGroup& G, int blocks[4], size_t lds[4]) {
 template<class T>
 void launchPmeConvolve(hipStream_t st, const FftLaunch& L, T* real, typename Vec2<T>::type* cplx, const T* influence,
                        const typename Vec2<T>::type* tw, BoxT<double> box, const BoxT<double>* boxes, double beta,
-                       int energy, int strain, double* partials, const int* skip, const long long* fixed,
+                       int energy, int strain, double* partials, const int* skip, long long* fixed,
                        size_t fixedStride) {
     const FftGroup* G = L.groups;
     const int ng = L.ngroups;
@@ -1269,7 +1270,7 @@ void launchZero(hipStream_t st, T* p, size_t count, const int* skip) {
     template void fftGroupShape<T>(FftGroup&, int*, size_t*);                                                       \
     template void launchPmeConvolve<T>(hipStream_t, const FftLaunch&, T*, Vec2<T>::type*, const T*,                    \
                                        const Vec2<T>::type*, BoxT<double>, const BoxT<double>*, double, int, int,     \
-                                       double*, const int*, const long long*, size_t);                                \
+                                       double*, const int*, long long*, size_t);                                      \
     template void launchGather<T>(hipStream_t, PmeSel, int, const int*, const int*, const T*, const T*,                \
                                   const Vec4<T>::type*, const Vec4<T>::type*, int, double*, double*, const int*);      \
     template void launchInfluence<T>(hipStream_t, PmeGrid, const double*, BoxT<double>, double, double, T*);           \
