@@ -39,12 +39,14 @@ __device__ __forceinline__ void minImage(T& x, T& y, T& z, const BoxT<T>& b) {
 // The gfx950 dispatcher deals workgroups to its 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each
 // XCD has an L2 of its own.  This renumbering gives XCD x a contiguous range of logical blocks, so the kernels
 // over spatially sorted atoms gather their partners from one slab of the system per L2 instead of from all of
-// it.  A bijection of [0, gridDim.x) for any grid size.
-__device__ __forceinline__ int xcdBlock() {
+// it.  A bijection of [0, gridDim.x) for any grid size.  reverse: each XCD walks its range from the end (the
+// part of the data that the pass before, walking forward, touched last comes first).
+__device__ __forceinline__ int xcdBlock(bool reverse = false) {
     constexpr int kXcd = 8;
     const int nb = (int) gridDim.x, b = (int) blockIdx.x;
     const int per = nb/kXcd, rem = nb%kXcd, x = b%kXcd, local = b/kXcd;
-    return x < rem ? x*(per+1)+local : rem*(per+1)+(x-rem)*per+local;
+    const int l = reverse ? (x < rem ? per : per-1)-local : local;
+    return x < rem ? x*(per+1)+l : rem*(per+1)+(x-rem)*per+l;
 }
 
 // Precision traits of the pair kernels: series length below a r = 1 and the damping cut beyond which the

@@ -459,8 +459,10 @@ public:
             double* stage = paramH_.as<double>();
             std::memcpy(stage, q, n*sizeof(double));
             polarizable_ = false;
+            rgMax_ = 0.0;
             for (int i = 0; i < n; i++) {
                 stage[n+i] = 2.0*radius[i]*radius[i];
+                rgMax_ = std::max(rgMax_, stage[n+i]);
                 const double a = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
                 stage[2*n+i] = a;
                 polarizable_ = polarizable_ || a > 0;
@@ -1415,6 +1417,8 @@ private:
     hipEvent_t evFork_ = nullptr, evJoin_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1, covBuiltN_ = -1;   // covBuiltN_: the atom count of the covalent CSR
     bool polarizable_ = false;
+    double rgMax_ = 0.0;   // the largest radius parameter 2R^2 (orders the fused setup's segments)
+    bool listReverse_ = false;   // the block order of the next pass over the compacted list
     std::vector<double> qH_, forcesH_;
     std::vector<int> gidxH_, covAtomH_, covPartnerH_, covEntryOfH_;   // covEntryOfH_: input entry -> CSR slot
     std::vector<int> giTmp_, newOfTmp_;   // setAtoms' working arrays
@@ -2328,7 +2332,7 @@ private:
         tb(kPhPairField, it);
         launchPairField<T>(st_, n_, posT, dipT, candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
                            g12D_.as<typename Vec2<T>::type>(), boxSel<T>(), tdD_.as<double>(), skip, lanes_, fieldOtf_,
-                           (T) s_.beta);
+                           (T) s_.beta, nextListDirection());
         te();
         pmeSolve<T>(dipT, false, posT, 0, false, trD_.as<double>(), nullptr, skip, nullptr, it);
         join();
@@ -2456,6 +2460,14 @@ private:
         return sizeof(T) == sizeof(float) ? scratchD_.as<T>() : rgSD_.as<T>();
     }
 
+    // The block order of the next pass over the compacted list, alternated pass by pass: each pass starts on the
+    // part of the list (and of the partner data) that the pass before reached last.
+    bool nextListDirection() {
+        const bool r = listReverse_;
+        listReverse_ = !r;
+        return r;
+    }
+
     // The candidates within rc, their kernels and the field of (q, perm) (and of dual): one fused pass or
     // compaction + setup.
     template<class T>
@@ -2464,12 +2476,13 @@ private:
         typedef typename Vec2<T>::type T2;
         const BoxSel<T> bt = boxSel<T>();
         const T rc2 = (T) (s_.cutoff*s_.cutoff);
+        listReverse_ = true;   // the setup writes the list forward: the next pass reads it from the end
         if (fused) {
             tb(kPhPairSetup);
             launchPairCompactSetup<T>(st_, n_, posT<T>(), rgT<T>(), permTD_.as<T4>(), dual, candStartD_.as<int>(),
                                       candD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(),
-                                      g03D_.as<T2>(), bt, rc2, (T) s_.beta, edD_.as<double>(), tdD_.as<double>(),
-                                      lanes);
+                                      g03D_.as<T2>(), bt, rc2, (T) s_.beta, (T) rgMax_, edD_.as<double>(),
+                                      tdD_.as<double>(), lanes);
             te();
             return;
         }
@@ -2662,7 +2675,7 @@ private:
         const GvdwArgs<T> ga = gvdwArgs<T>();
         launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(), nbD_.as<int>(),
                            pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt, fpairD_.as<double>(),
-                           edD_.as<double>(), perAtomD_.as<double>(), wantStrain, ga, lanes_);
+                           edD_.as<double>(), perAtomD_.as<double>(), wantStrain, ga, lanes_, nextListDirection());
         if (ga.gv && nSkip_ > 0)
             launchGvdwSkip<T>(st_, n, orderD_.as<int>(), rankD_.as<int>(), pos, rgT<T>(), skipStartD_.as<int>(),
                               skipPartD_.as<int>(), bt, (T) (s_.cutoff*s_.cutoff), wantStrain, ga,
@@ -2737,11 +2750,12 @@ private:
             pairSetup<T>(lanes, fused, pTD_.as<T4>());
             for (int k = 0; k <= iterations_; k++)
                 launchPairField<T>(st_, n, pos, dTD_.as<T4>(), candStartD_.as<int>(), nbD_.as<int>(), pjD_.as<int>(),
-                                   g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes, otf, (T) s_.beta);
+                                   g12D_.as<T2>(), bt, tdD_.as<double>(), nullptr, lanes, otf, (T) s_.beta,
+                                   nextListDirection());
             launchPairFinal<T>(st_, n, pos, dTD_.as<T4>(), rgT<T>(), (T) s_.beta, candStartD_.as<int>(),
                                nbD_.as<int>(), pjD_.as<int>(), g12D_.as<T2>(), g03D_.as<T2>(), bt,
                                fpairD_.as<double>(), edD_.as<double>(), perAtomD_.as<double>(), false,
-                               gvdwArgs<T>(), lanes);
+                               gvdwArgs<T>(), lanes, nextListDirection());
         };
         once();   // warm-up
         HIPCHK(hipEventRecord(a, st_));

@@ -160,19 +160,23 @@ void launchPairSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                      const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual, const int* candStart,
                      const int* nb, const int* pj, typename Vec2<T>::type* g12, typename Vec2<T>::type* g03,
                      BoxSel<T> box, T beta, double* field, double* fieldDual, int lanes);
-// launchPairCompact + launchPairSetup in one pass (compacted pairs buffered in LDS per atom).
+// launchPairCompact + launchPairSetup in one pass (compacted pairs buffered in LDS per atom); the pairs with
+// r^2 < rg_i + rgMax (rgMax: the largest rg, or any value: it orders the segment, nothing else) come last in it.
 template<class T>
 void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                             const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual,
                             const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
-                            typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, double* field,
+                            typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, T rgMax, double* field,
                             double* fieldDual, int lanes);
 // The dipole part of a CG matrix-vector product from the stored G1, G2 or, otf (FP32 only), with the kernels
-// evaluated on the fly from the positions and the radius parameters in the dipoles' w.
+// evaluated on the fly from the positions and the radius parameters in the dipoles' w.  reverse (here and in
+// launchPairFinal): the atoms in the opposite block order (the same results; the engine alternates the
+// direction from one pass over the list to the next, so that a pass starts on what the one before touched last).
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     BoxSel<T> box, double* field, const int* skip, int lanes, bool otf = false, T beta = T(0));
+                     BoxSel<T> box, double* field, const int* skip, int lanes, bool otf = false, T beta = T(0),
+                     bool reverse = false);
 // The CG preconditioner's short-range list: per atom, its compacted pairs within the radius (srJ partner, srG the
 // stored G1, G2; at most kPreCap, counted in srCount); *over set when an atom has more (cleared by the caller).
 template<class T>
@@ -197,7 +201,7 @@ void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                      const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
                      const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
                      double* force, double* field, double* perAtom, bool strain, const GvdwArgs<T>& gvdw,
-                     int lanes);
+                     int lanes, bool reverse = false);
 // After launchPairFinal with GVDW: the GVDW terms of the skipped pairs (per atom CSR of partners in the caller's
 // order, both directions) subtracted from the atoms' forces, GVDW energies and (strain) strain rows.
 template<class T>

@@ -27,6 +27,15 @@ constexpr int kPairBlock = 256;
 template<class T>
 constexpr bool kStoreG03 = sizeof(T) == sizeof(double);
 
+// The separate setup pass writes the stored kernels (FP64: 32 bytes per pair, 2.8 GB per evaluation at 96k atoms)
+// with non-temporal stores: no pass reads them again soon enough to find them in a cache, and the data written
+// normally, the compacted list that every field pass reads again, keeps its place there.
+template<class V>
+__device__ __forceinline__ void streamStore(V* p, const V& v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // Compaction: every atom's candidates within rc, in place in its segment (pj), and the count (nb).
 template<class T, int L, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kCompact(int n, const typename Vec4<T>::type* __restrict__ pos,
@@ -133,6 +142,14 @@ __device__ __forceinline__ void waveSync() {
 // entries; whenever a group of the wavefront has L entries buffered, every group evaluates up to L of its own
 // (kernels, fields) and writes (j, G1, G2) as contiguous entries of its atom's segment.  No compacted index list goes through HBM, the partner
 // positions are gathered once, and the transcendental part runs with every lane busy.
+//
+// Close pairs (a r < 1, r^2 < rg_i + rg_j: bonded neighbours and close contacts, about 2 per atom in water) take
+// the series form of the kernels, which every lane of a wavefront steps through when one of its lanes holds such
+// a pair.  Candidates with r^2 < rg_i + rgMax (rgMax the largest rg, so a superset of the close pairs) are held
+// apart in a stash of L entries per group and evaluated together after the others, as the last entries of the
+// segment: the evaluations here and the passes over the list meet the series form in the last row of a segment
+// instead of in a few rows spread over it.  A stash that is full leaves further ones in the stream (the order
+// is a matter of speed only; every entry is evaluated by the same code whatever its place).
 template<class T, int L, bool DUAL, bool BATCH>
 __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typename Vec4<T>::type* __restrict__ pos,
                                                             const T* __restrict__ rg,
@@ -143,30 +160,32 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
                                                             int* __restrict__ pj,
                                                             typename Vec2<T>::type* __restrict__ g12,
                                                             typename Vec2<T>::type* __restrict__ g03, BoxSel<T> bs,
-                                                            T rc2, T beta, double* __restrict__ field,
+                                                            T rc2, T beta, T rgMax, double* __restrict__ field,
                                                             double* __restrict__ fieldDual) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     __shared__ int bufJ[2*kPairBlock];
     __shared__ T4 bufX[2*kPairBlock];
+    __shared__ int closeJ[kPairBlock];
+    __shared__ T4 closeX[kPairBlock];
     const int gid = xcdBlock()*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
     const int g0 = 2*(threadIdx.x-lane);   // this group's 2L buffer slots
+    const int c0 = threadIdx.x-lane;       // and its L stash slots
     const BoxT<T> box = selectBox<BATCH>(bs, s);
     const int shift = (threadIdx.x%64)/L*L;
     const unsigned long long below = (lane == 0 ? 0ull : ((1ull << lane)-1ull));
     const unsigned long long gmask = (L == 64 ? ~0ull : ((1ull << (L & 63))-1ull));
     const T4 pi = pos[s];
     const T rgi = rg[s];
+    const T closeR2 = rgi+rgMax;
     const int start = candStart[s], cnt = candStart[s+1]-start;
     double e[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-    int kept = 0, buffered = 0;
-    auto evaluate = [&](int slot, int out) {
-        const int v = bufJ[slot];   // raw entry (stored as is); j its atom
+    int kept = 0, buffered = 0, stashed = 0;
+    auto evaluate = [&](int v, const T4& xq, int out) {   // v the raw entry (stored as is); j its atom
         const int j = entryAtom(v, bs.ext);
-        const T4 xq = bufX[slot];
         const T4 d = dip[j];
         const T4 u = DUAL ? dual[j] : d;
         T2 ga, gb;
@@ -194,14 +213,27 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         if (k+2*L < cnt)
             jAfter = cand[start+k+2*L];
         T4 xq;
+        bool close = false;
         if (k < cnt) {
             T x = pi.x-pjv.x, y = pi.y-pjv.y, z = pi.z-pjv.z;
             pairImage(x, y, z, box, j, bs.ext);
-            in = x*x+y*y+z*z < rc2;
+            const T r2 = x*x+y*y+z*z;
+            in = r2 < rc2;
+            close = in && r2 < closeR2;
             xq.x = x;
             xq.y = y;
             xq.z = z;
             xq.w = pjv.w;
+        }
+        const unsigned long long mc = (__ballot(close) >> shift) & gmask;
+        if (mc != 0ull) {   // group-uniform
+            const int rank = __popcll(mc & below);
+            if (close && stashed+rank < L) {
+                closeJ[c0+stashed+rank] = j;
+                closeX[c0+stashed+rank] = xq;
+                in = false;
+            }
+            stashed = min(stashed+__popcll(mc), L);
         }
         const unsigned long long m = (__ballot(in) >> shift) & gmask;
         if (in) {
@@ -216,7 +248,7 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         if (__ballot(lane == 0 && buffered >= L) != 0ull) {
             const int nflush = min(buffered, L);
             if (lane < nflush)
-                evaluate(g0+lane, start+kept+lane);
+                evaluate(bufJ[g0+lane], bufX[g0+lane], start+kept+lane);
             kept += nflush;
             const int rest = buffered-nflush;
             int tj = 0;
@@ -235,8 +267,11 @@ __global__ void __launch_bounds__(kPairBlock) kCompactSetup(int n, const typenam
         }
     }
     if (lane < buffered)
-        evaluate(g0+lane, start+kept+lane);
+        evaluate(bufJ[g0+lane], bufX[g0+lane], start+kept+lane);
     kept += buffered;
+    if (lane < stashed)
+        evaluate(closeJ[c0+lane], closeX[c0+lane], start+kept+lane);
+    kept += stashed;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         e[c] = groupSum<L>(e[c]);
@@ -298,9 +333,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
         for (int q = 0; q < U; q++) {
             T2 ga, gb;
             setupTerm<T, DUAL>(pi, rgi, pjv[q], d[q].w, d[q], u[q], box, v[q], bs.ext, beta, ga, gb, e, f);
-            g12[start+k+q*L] = ga;
+            streamStore(g12+start+k+q*L, ga);
             if (kStoreG03<T>)
-                g03[start+k+q*L] = gb;
+                streamStore(g03+start+k+q*L, gb);
         }
     }
     for (; k < cnt; k += L) {
@@ -309,9 +344,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairSetup(int n, const typename V
         const T4 u = DUAL ? dual[j] : pos[j];
         const T4 dj = dip[j];
         setupTerm<T, DUAL>(pi, rgi, pos[j], dj.w, dj, u, box, v, bs.ext, beta, ga, gb, e, f);
-        g12[start+k] = ga;
+        streamStore(g12+start+k, ga);
         if (kStoreG03<T>)
-            g03[start+k] = gb;
+            streamStore(g03+start+k, gb);
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -350,12 +385,13 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
                                                          const int* __restrict__ candStart,
                                                          const int* __restrict__ nb, const int* __restrict__ pj,
                                                          const typename Vec2<T>::type* __restrict__ g12, BoxSel<T> bs,
-                                                         double* __restrict__ field, const int* skip) {
+                                                         double* __restrict__ field, const int* skip,
+                                                         bool reverse) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
     if (skip && *skip)
         return;
-    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock(reverse)*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
@@ -406,10 +442,10 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
                                                             const int* __restrict__ candStart,
                                                             const int* __restrict__ nb, const int* __restrict__ pj,
                                                             BoxSel<float> bs, float beta, double* __restrict__ field,
-                                                            const int* skip) {
+                                                            const int* skip, bool reverse) {
     if (skip && *skip)
         return;
-    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock(reverse)*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     if (s >= n)
         return;
@@ -558,10 +594,11 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
                                                          const typename Vec2<T>::type* __restrict__ g12,
                                                          const typename Vec2<T>::type* __restrict__ g03, BoxSel<T> bs,
                                                          double* __restrict__ force, double* __restrict__ field,
-                                                         double* __restrict__ perAtom, GvdwArgs<T> gva) {
+                                                         double* __restrict__ perAtom, GvdwArgs<T> gva,
+                                                         bool reverse) {
     typedef typename Vec4<T>::type T4;
     typedef typename Vec2<T>::type T2;
-    const int gid = xcdBlock()*blockDim.x+threadIdx.x;
+    const int gid = xcdBlock(reverse)*blockDim.x+threadIdx.x;
     const int s = gid/L, lane = gid%L;
     constexpr int NACC = STRAIN ? 10 : 1;
     double eg = 0;
@@ -917,35 +954,36 @@ template<class T>
 void launchPairCompactSetup(hipStream_t st, int n, const typename Vec4<T>::type* pos, const T* rg,
                             const typename Vec4<T>::type* dip, const typename Vec4<T>::type* dual,
                             const int* candStart, const int* cand, int* nb, int* pj, typename Vec2<T>::type* g12,
-                            typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, double* field,
+                            typename Vec2<T>::type* g03, BoxSel<T> box, T rc2, T beta, T rgMax, double* field,
                             double* fieldDual, int lanes) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
         const int blocks = groupsBlocks(n, L());
         if (dual)
             kCompactSetup<T, L(), true, B()><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dual, candStart, cand, nb,
-                                                                            pj, g12, g03, box, rc2, beta, field,
-                                                                            fieldDual);
+                                                                            pj, g12, g03, box, rc2, beta, rgMax,
+                                                                            field, fieldDual);
         else
             kCompactSetup<T, L(), false, B()><<<blocks, kPairBlock, 0, st>>>(n, pos, rg, dip, dip, candStart, cand,
-                                                                             nb, pj, g12, g03, box, rc2, beta, field,
-                                                                             nullptr);
+                                                                             nb, pj, g12, g03, box, rc2, beta, rgMax,
+                                                                             field, nullptr);
     });
 }
 
 template<class T>
 void launchPairField(hipStream_t st, int n, const typename Vec4<T>::type* pos, const typename Vec4<T>::type* dip,
                      const int* candStart, const int* nb, const int* pj, const typename Vec2<T>::type* g12,
-                     BoxSel<T> box, double* field, const int* skip, int lanes, bool otf, T beta) {
+                     BoxSel<T> box, double* field, const int* skip, int lanes, bool otf, T beta, bool reverse) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
         if constexpr (sizeof(T) == sizeof(float)) {
             if (otf) {
                 kPairFieldOtf<L(), 4, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb,
-                                                                                       pj, box, beta, field, skip);
+                                                                                       pj, box, beta, field, skip,
+                                                                                       reverse);
                 return;
             }
         }
         kPairField<T, L(), 4, B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(n, pos, dip, candStart, nb, pj, g12,
-                                                                                box, field, skip);
+                                                                                box, field, skip, reverse);
     });
 }
 
@@ -954,11 +992,11 @@ void launchPairFinal(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                      const T* rg, T beta, const int* candStart, const int* nb, const int* pj,
                      const typename Vec2<T>::type* g12, const typename Vec2<T>::type* g03, BoxSel<T> box,
                      double* force, double* field, double* perAtom, bool strain, const GvdwArgs<T>& gvdw,
-                     int lanes) {
+                     int lanes, bool reverse) {
     dispatchLanes(lanes, box.table != nullptr, [&](auto L, auto B) {
         auto go = [&](auto S, auto G) {
             kPairFinal<T, L(), 2, S(), G(), B()><<<groupsBlocks(n, L()), kPairBlock, 0, st>>>(
-                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom, gvdw);
+                n, pos, dip, rg, beta, candStart, nb, pj, g12, g03, box, force, field, perAtom, gvdw, reverse);
         };
         const bool g = gvdw.gv != nullptr;
         if (strain)
@@ -1029,13 +1067,13 @@ void launchShortList(hipStream_t st, int n, const typename Vec4<T>::type* pos, c
                                      Vec2<T>::type*, BoxSel<T>, T, double*, double*, int);                            \
     template void launchPairCompactSetup<T>(hipStream_t, int, const Vec4<T>::type*, const T*, const Vec4<T>::type*,  \
                                             const Vec4<T>::type*, const int*, const int*, int*, int*, Vec2<T>::type*, \
-                                            Vec2<T>::type*, BoxSel<T>, T, T, double*, double*, int);                  \
+                                            Vec2<T>::type*, BoxSel<T>, T, T, T, double*, double*, int);               \
     template void launchPairField<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const int*,       \
                                      const int*, const int*, const Vec2<T>::type*, BoxSel<T>, double*, const int*,    \
-                                     int, bool, T);                                                                   \
+                                     int, bool, T, bool);                                                             \
     template void launchPairFinal<T>(hipStream_t, int, const Vec4<T>::type*, const Vec4<T>::type*, const T*, T,      \
                                      const int*, const int*, const int*, const Vec2<T>::type*, const Vec2<T>::type*,  \
-                                     BoxSel<T>, double*, double*, double*, bool, const GvdwArgs<T>&, int);            \
+                                     BoxSel<T>, double*, double*, double*, bool, const GvdwArgs<T>&, int, bool);      \
     template void launchGvdwSkip<T>(hipStream_t, int, const int*, const int*, const Vec4<T>::type*, const T*,         \
                                     const int*, const int*, BoxSel<T>, T, bool, const GvdwArgs<T>&, double*, double*); \
     template void launchPairTangent<T>(hipStream_t, int, const Vec4<T>::type*, const T*, T, const Vec4<T>::type*,    \
