@@ -111,8 +111,8 @@ void launchPermDipoles(hipStream_t st, int n, const int* order, const double* po
 
 // The per-evaluation preparation of the sorted atoms in one pass: the wrapped positions with charges (FP64
 // and, when posF is set, FP32 with the FP32 radius parameters rgF), the permanent dipoles (FP64 and as the
-// pair/PME vector type permT, w = the radius parameter), and the clearing of the counters and sums the kernels after it accumulate
-// into (zeroI: nZeroI ints, zeroD: nZeroD doubles, maxBits: the next list check's maximum).
+// pair/PME vector type permT, w = the radius parameter), and the clearing of the counters and sums the kernels
+// after it accumulate into (zeroI: nZeroI ints, zeroD: nZeroD doubles, maxBits: the next list check's maximum).
 struct PrepArgs {
     int n;
     const int* order;

@@ -140,8 +140,9 @@ __device__ __forceinline__ void waveSync() {
 // Compaction and setup in one pass.  Each group of L lanes tests L candidates per step (one per lane) and
 // ballot-compacts those within rc, with their minimum-image displacement and charge, into its LDS buffer of 2L
 // entries; whenever a group of the wavefront has L entries buffered, every group evaluates up to L of its own
-// (kernels, fields) and writes (j, G1, G2) as contiguous entries of its atom's segment.  No compacted index list goes through HBM, the partner
-// positions are gathered once, and the transcendental part runs with every lane busy.
+// (kernels, fields) and writes (j, G1, G2) as contiguous entries of its atom's segment.  No compacted index list
+// goes through HBM, the partner positions are gathered once, and the transcendental part runs with every lane
+// busy.
 //
 // Close pairs (a r < 1, r^2 < rg_i + rg_j: bonded neighbours and close contacts, about 2 per atom in water) take
 // the series form of the kernels, which every lane of a wavefront steps through when one of its lanes holds such

@@ -104,7 +104,9 @@ __host__ __device__ constexpr double cosTwoPi(int R, int k) {
 __host__ __device__ constexpr double sinTwoPi(int R, int k) {
     const int q = k <= R/2 ? k : R-k;
     const double s = R == 5 ? (q == 1 ? 0.95105651629515357212 : 0.58778525229247312917)
-                            : (q == 1 ? 0.78183148246802980871 : q == 2 ? 0.97492791218182360702 : 0.43388373911755812048);
+                            : (q == 1   ? 0.78183148246802980871
+                               : q == 2 ? 0.97492791218182360702
+                                        : 0.43388373911755812048);
     return k <= R/2 ? s : -s;
 }
 
