@@ -323,6 +323,7 @@ public:
             HIPCHK(hipEventCreateWithFlags(&evDone_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evCov_, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&evTab_, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&evParam_, hipEventDisableTiming));
             pmeSt_ = st_;
             // pinned words read back by the evaluations (allocated here: a launch makes no allocation that
             // could wait for the GPU)
@@ -351,7 +352,7 @@ public:
             (void) hipStreamSynchronize(st2_);
             releasePmeStream();
         }
-        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_, evDone_, evCov_, evTab_})
+        for (hipEvent_t ev : {evSync_, evList_, evFork_, evJoin_, evDone_, evCov_, evTab_, evParam_})
             if (ev)
                 (void) hipEventDestroy(ev);
         if (st_)
@@ -400,6 +401,22 @@ public:
                 throw std::invalid_argument("pgm_hip: setAtoms: radii must be positive");
         }
         withDevice([&] {
+            // q, the radius parameter 2R^2 and alpha into pinned staging before the stream is drained: a fitting
+            // step sets them at every step, and this loop then runs while the GPU finishes the step before.  The
+            // staging is free once its last copies have run.
+            HIPCHK(hipEventSynchronize(evParam_));
+            paramH_.reserve(3*sizeof(double)*n);
+            double* stage = paramH_.as<double>();
+            std::memcpy(stage, q, n*sizeof(double));
+            bool polarizable = false;
+            double rgMax = 0.0;
+            for (int i = 0; i < n; i++) {
+                stage[n+i] = 2.0*radius[i]*radius[i];
+                rgMax = std::max(rgMax, stage[n+i]);
+                const double a = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
+                stage[2*n+i] = a;
+                polarizable = polarizable || a > 0;
+            }
             HIPCHK(hipStreamSynchronize(st_));
             const bool resized = n != n_ || ng != nGlobal_;
             if (resized) {
@@ -452,26 +469,16 @@ public:
             nGlobal_ = ng;
             qH_.assign(q, q+n);
             refreshCharges();
-            // q, the radius parameter 2R^2 and alpha through pinned staging, copied without a wait (a fitting
-            // step sets them at every step): the stream was idle at entry, so the staging is free, and every
-            // kernel that reads them is enqueued on the same stream after the copies
-            paramH_.reserve(3*sizeof(double)*n);
-            double* stage = paramH_.as<double>();
-            std::memcpy(stage, q, n*sizeof(double));
-            polarizable_ = false;
-            rgMax_ = 0.0;
-            for (int i = 0; i < n; i++) {
-                stage[n+i] = 2.0*radius[i]*radius[i];
-                rgMax_ = std::max(rgMax_, stage[n+i]);
-                const double a = alpha[i] > kMinPolarizability ? alpha[i] : 0.0;
-                stage[2*n+i] = a;
-                polarizable_ = polarizable_ || a > 0;
-            }
+            polarizable_ = polarizable;
+            rgMax_ = rgMax;
+            // the staged parameters copied without a wait: every kernel that reads them is enqueued on the same
+            // stream after the copies
             DevBuf* dst[3] = {&qD_, &rgD_, &alphaD_};
             for (int k = 0; k < 3; k++) {
                 dst[k]->reserve(n*sizeof(double));
                 HIPCHK(hipMemcpyAsync(dst[k]->p, stage+k*n, n*sizeof(double), hipMemcpyHostToDevice, st_));
             }
+            HIPCHK(hipEventRecord(evParam_, st_));
             if (resized || !(identity && gidxIdentity_) || gidxH_ != gi)
                 upload(gidxD_, gi.data(), n*sizeof(int));
             gidxH_.swap(gi);
@@ -1560,6 +1567,7 @@ private:
     HostBuf resH_, stageH_, scalH_, listH_, cmpH_;
     HostBuf paramH_, covH_, tabH_;   // staging of setAtoms' parameters, setCovalent's strengths, the batch tables
     hipEvent_t evCov_ = nullptr, evTab_ = nullptr;   // after the last copy out of covH_ / tabH_
+    hipEvent_t evParam_ = nullptr;                   // after the last copies out of paramH_
     // the last evaluation's positions and boxes, for a backward at the same positions
     DevBuf lastPosD_, cmpPosD_, cmpFlagD_;
     std::vector<double> lastBoxes_;
