@@ -73,24 +73,28 @@ __device__ __forceinline__ double expT(double x) { return exp(x); }
 __device__ __forceinline__ float rsqrtT(float x) { return __frsqrt_rn(x); }
 __device__ __forceinline__ double rsqrtT(double x) { return rsqrt(x); }
 
-// erfc(x) for x >= 0 in FP32: the Chebyshev fit of Numerical Recipes in C, 2nd ed. (Press et al.), sec. 6.2
-// (erfcc), erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2), fractional error below 1.2e-7 for the fit and
-// below 1.3e-6 evaluated in FP32 for x <= 4.5 (the range of the pair kernels).  v_rcp / v_exp instead of the
-// library erfcf and its divisions.
-__device__ __forceinline__ float erfcFast(float x) {
-    const float t = __builtin_amdgcn_rcpf(1.0f+0.5f*x);
-    float p = 0.17087277f;
-    p = fmaf(t, p, -0.82215223f);
-    p = fmaf(t, p, 1.48851587f);
-    p = fmaf(t, p, -1.13520398f);
-    p = fmaf(t, p, 0.27886807f);
-    p = fmaf(t, p, -0.18628806f);
-    p = fmaf(t, p, 0.09678418f);
-    p = fmaf(t, p, 0.37409196f);
-    p = fmaf(t, p, 1.00002368f);
-    p = fmaf(t, p, -1.26551223f);
-    return t*__expf(fmaf(-x, x, p));
+// exp(x^2) erfc(x) for 0 <= x <= 4.5 in FP32 (the range of the pair kernels): t Q(t) with t = 1/(1 + x/2) and
+// Q a degree-8 polynomial fitted to exp(x^2) erfc(x)/t on that range (minimax relative error by iteratively
+// reweighted least squares, pgm-md_amd/csrc/erfcx_fit.py; 1.6e-8 for the fit, below 3e-7 evaluated in FP32).
+// erfc(x) is this times exp(-x^2), the factor the kernels' Gaussian term needs too, so that a kernel evaluation
+// takes one exp per width instead of two.  v_rcp instead of a division.
+__device__ __forceinline__ float erfcxFast(float x) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+    float p = -4.658861831e-02f;
+    p = fmaf(t, p, 2.203130424e-01f);
+    p = fmaf(t, p, -3.455681503e-01f);
+    p = fmaf(t, p, 1.038070619e-01f);
+    p = fmaf(t, p, 1.041026637e-01f);
+    p = fmaf(t, p, 1.415342391e-01f);
+    p = fmaf(t, p, 2.605046332e-01f);
+    p = fmaf(t, p, 2.796235681e-01f);
+    p = fmaf(t, p, 2.822715342e-01f);
+    return t*p;
 }
+
+// erfc(x) for 0 <= x <= 4.5 in FP32: erfcxFast(x) exp(-x^2); relative error below 2.5e-6 in FP32 (the rounding
+// of x^2 in the exponent dominates near x = 4.5).
+__device__ __forceinline__ float erfcFast(float x) { return erfcxFast(x)*__expf(-x*x); }
 
 // B_n of erf(a r)/r, n < NMAX, by the series in s = (a r)^2 (valid for s < 1; no cancellation at short
 // range): B_n = (2a/sqrt(pi)) (2a^2)^n sum_m (-s)^m / (m! (2m+2n+1)) (core/pgm_math.h:31-49, docs/spec.md Sec. 4).
@@ -149,7 +153,8 @@ __device__ __forceinline__ void erfcKernels(T a, T r, T ir2, T* B) {
     }
 }
 
-// FP32 forms of erfKernels / erfcKernels: 1/r from v_rsq, erfc by erfcFast, no divisions.
+// FP32 forms of erfKernels / erfcKernels: 1/r from v_rsq, erfc(x) from erfcxFast(x) and the exp(-x^2) of the
+// Gaussian term, no divisions.
 template<int NMAX>
 __device__ __forceinline__ void erfKernelsF(float a, float r, float rinv, float* B) {
     const float x = a*r, s = x*x;
@@ -157,9 +162,10 @@ __device__ __forceinline__ void erfKernelsF(float a, float r, float rinv, float*
         erfSeries<float, NMAX>(a, s, B);
         return;
     }
-    const float g = float(kTwoOverSqrtPi)*a*__expf(-s);
+    const float e = __expf(-s);
+    const float g = float(kTwoOverSqrtPi)*a*e;
     const float ir2 = rinv*rinv;
-    B[0] = (1.0f-erfcFast(x))*rinv;
+    B[0] = (1.0f-erfcxFast(x)*e)*rinv;
     float scale = 1.0f;
     const float twoA2 = 2.0f*a*a;
 #pragma unroll
@@ -172,9 +178,10 @@ __device__ __forceinline__ void erfKernelsF(float a, float r, float rinv, float*
 template<int NMAX>
 __device__ __forceinline__ void erfcKernelsF(float a, float r, float rinv, float* B) {
     const float x = a*r;
-    const float g = float(kTwoOverSqrtPi)*a*__expf(-x*x);
+    const float e = __expf(-x*x);   // shared by erfc(x) and the Gaussian term
+    const float g = float(kTwoOverSqrtPi)*a*e;
     const float ir2 = rinv*rinv;
-    B[0] = erfcFast(x)*rinv;
+    B[0] = erfcxFast(x)*e*rinv;
     float scale = 1.0f;
     const float twoA2 = 2.0f*a*a;
 #pragma unroll

@@ -462,9 +462,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
         float G[3];
         pairKernelsF<3>(rgi+d.w, beta, x*x+y*y+z*z, G);
         const float c2 = (d.x*x+d.y*y+d.z*z)*G[2];
-        fx += c2*x-d.x*G[1];
-        fy += c2*y-d.y*G[1];
-        fz += c2*z-d.z*G[1];
+        fx = fmaf(c2, x, fmaf(-d.x, G[1], fx));
+        fy = fmaf(c2, y, fmaf(-d.y, G[1], fy));
+        fz = fmaf(c2, z, fmaf(-d.z, G[1], fz));
     };
     int k = lane;
     for (; k+(U-1)*L < cnt; k += U*L) {
@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
             pjv[u] = pos[j];
             d[u] = dip[j];
         }
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        float fx = -0.0f, fy = -0.0f, fz = -0.0f;   // -0: adding it is exact for every x, so it folds away
 #pragma unroll
         for (int u = 0; u < U; u++)
             term(pjv[u], d[u], v[u], fx, fy, fz);
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
     }
     for (; k < cnt; k += L) {
         const int v = pj[start+k], j = entryAtom(v, bs.ext);
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        float fx = -0.0f, fy = -0.0f, fz = -0.0f;   // -0: adding it is exact for every x, so it folds away
         term(pos[j], dip[j], v, fx, fy, fz);
         ex += (double) fx;
         ey += (double) fy;
@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             }
             // the U entries of a step summed in T (FP32: one FP64 addition per step instead of per entry), the
             // steps in FP64
-            T fu[3] = {T(0), T(0), T(0)}, eu[3] = {T(0), T(0), T(0)}, uu = T(0);
+            T fu[3] = {-T(0), -T(0), -T(0)}, eu[3] = {-T(0), -T(0), -T(0)}, uu = -T(0);   // -0: folds away
 #pragma unroll
             for (int u = 0; u < U; u++)
                 finalTerm<T, STRAIN, GVDW>(pi, di, pjv[u], dj[u], ga[u], gb[u], rgi+rgj[u], beta, box, v[u], bs.ext,

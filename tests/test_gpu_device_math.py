@@ -1,10 +1,10 @@
 """The pair passes' own device math, evaluated on host arrays through the C ABI test hooks.
 
-- FP32 erfc (`erfcFast`): the Chebyshev fit of Press et al., Numerical Recipes 2nd ed. 6.2 (`erfcc`), whose
-  fractional error is below 1.2e-7 in exact arithmetic. In FP32 the argument of the exponential, -x^2 + P(t),
-  carries rounding of order |x^2| 2^-24 (1.2e-6 at x = 4.5, the largest argument the kernels pass: the damping
-  cut a r = 4.5; beta r stays below 3.2), and v_rcp_f32 / exp add a few ulp: the bound checked against
-  math.erfc is 3e-6 relative on [0, 4.5].
+- FP32 erfc (`erfcFast` = `erfcxFast(x)` exp(-x^2)): exp(x^2) erfc(x) as t Q(t), t = 1/(1 + x/2), Q a degree-8
+  polynomial fitted to it (pgm-md_amd/csrc/erfcx_fit.py), whose relative error is 1.6e-8 in exact arithmetic.
+  In FP32 the exponent -x^2 carries rounding of order |x^2| 2^-24 (1.2e-6 at x = 4.5, the largest argument the
+  kernels pass: the damping cut a r = 4.5; beta r stays below 3.2), and v_rcp_f32 / exp and the Horner steps
+  add a few ulp: the bound checked against math.erfc is 3e-6 relative on [0, 4.5].
 - FP64 pair kernels G0..G3 against the oracle's (the restatement of core/pgm_math.h, pinned to mpmath
   derivatives of erf(ar)/r in tests/test_oracle.py), with the damping cut the device kernels apply: beyond
   a r = 6 (FP64) or 4.5 (FP32) the erfc(a r) part is dropped, as the reference's GPU kernels do
