@@ -157,7 +157,7 @@ def traffic_table(path=TRAFFIC_FILE):
     return out
 
 
-def roofline(dom, live, P, C, N, precision, lanes, fused):
+def roofline(dom, live, P, C, N, precision, lanes, fused, setup_lanes=None):
     """The pair pass `dom` (the one with the most time per evaluation) against the VALU rate, the bound of the
     path: its algorithmic flops per launch (PAIR_FLOPS_PER_P) over its live launch average `live` in the timed
     steps, against the MI355X FP32 (packed, 157.3 TFLOP/s) or FP64 (78.6) peak.  `traffic` is the HBM bytes per
@@ -173,6 +173,8 @@ def roofline(dom, live, P, C, N, precision, lanes, fused):
     achieved = flops / t / 1e12
     peak = PEAK_TFLOPS[precision]
     real = "float" if precision == "mixed" else "double"
+    if dom in ("pair_setup", "pair_compact") and setup_lanes:
+        lanes = setup_lanes
     prefix = f"{kernels[dom]}<{real}, {lanes},"
     traffic = next((v for k, v in traffic_table().items() if k.startswith(prefix)), None)
     return {
@@ -267,7 +269,8 @@ def large_box(torch, E, S, steps, warmup, log):
         fs = g["forces_sample"]
         err = float(np.sqrt(((f[g["sample"]] - fs) ** 2).sum() / (fs**2).sum()))
         out[prec] = {"evals_per_s": round(steps / dt, 3), "cg_iterations": eng.iterations(), "device_gib": round(used, 2),
-                     "lanes": lanes, "fused_setup": fused, "pme_beside": beside,
+                     "lanes": lanes, "setup_lanes": eng.setup_lanes(), "fused_setup": fused,
+                     "pme_beside": beside,
                      "cg_product_ms_pme_after_beside": [round(v, 4) for v in eng.pme_overlap_times()],
                      "evals_per_s_other_pme_choice": round(steps / dt_other, 3),
                      "force_relrms_vs_oracle_tau1e-10": err,
@@ -362,12 +365,13 @@ def liquid98k(torch, E, S, steps, log):
                                             "constraints and LJ included)", "source": "docs/openmm.md:177"}}  # fmt: skip
 
 
-VARIANTS = {}   # the autotuner's choice per precision: (lanes, fused setup, on-the-fly field, PME beside)
+VARIANTS = {}   # the autotuner's choice per precision: (lanes, fused setup, on-the-fly field, PME beside, setup lanes)
 
 
 def apply_variant(eng, precision):
-    lanes, fused, otf, beside = VARIANTS[precision]
+    lanes, fused, otf, beside, setup_lanes = VARIANTS[precision]
     eng.set_lanes(lanes, fused)
+    eng.set_setup_lanes(setup_lanes)
     eng.set_pme_overlap(beside)
     if precision == "mixed":
         eng.set_field_otf(otf)
@@ -541,7 +545,8 @@ def adjoint_batch(torch, E, S, reps, log, pause=False):
             "roofline": {"bound": "valu", "flops_per_batch": fl, "achieved": round(fl / dt / 1e12, 3), "peak": peak,
                          "unit": "TFLOP/s", "frac": round(fl / dt / 1e12 / peak, 4)},
             "list_builds": eng.list_builds(),
-            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes, "fused_setup": fused,
+            "cg_iterations_forward": fwd_it, "cg_iterations_adjoint": adj_it, "lanes": lanes,
+            "setup_lanes": eng.setup_lanes(), "fused_setup": fused,
             "pme_beside": eng.pme_overlap(),
             "cg_product_ms_pme_after_beside": [round(v, 4) for v in eng.pme_overlap_times()],
             "phases_ms_per_batch_in_line": phases,
@@ -699,7 +704,7 @@ def main():
         t, f, _ = eng.compute(sy.pos, box)  # checked evaluation (host arrays)
         results[prec] = (t, f, eng.iterations())
         tuned[prec] = eng.autotune(3)
-        VARIANTS[prec] = (*tuned[prec][0], eng.field_otf(), eng.pme_overlap())
+        VARIANTS[prec] = (*tuned[prec][0], eng.field_otf(), eng.pme_overlap(), eng.setup_lanes())
         log(f"{prec}: {eng.iterations()} CG iterations, E = {E.total_energy(t):.6f}, pair passes {tuned[prec][0]}")
     P = engines["double"].num_pairs()
     C = engines["double"].num_candidates()
@@ -715,8 +720,10 @@ def main():
                                              dom[prec])  # fmt: skip
         log(f"{prec}: {a.steps / secs[prec]:.2f} evals/s")
     n_it = results["mixed"][2]
-    roof = roofline(dom["mixed"], live["mixed"], P, C, sy.n, "mixed", *tuned["mixed"][0])
-    roof64 = roofline(dom["double"], live["double"], P, C, sy.n, "double", *tuned["double"][0])
+    roof = roofline(dom["mixed"], live["mixed"], P, C, sy.n, "mixed", *tuned["mixed"][0],
+                    engines["mixed"].setup_lanes())
+    roof64 = roofline(dom["double"], live["double"], P, C, sy.n, "double", *tuned["double"][0],
+                      engines["double"].setup_lanes())
 
     value = a.steps / secs["mixed"]
     v64 = a.steps / secs["double"]
@@ -732,13 +739,15 @@ def main():
                         "dipole sets, peek 0.65; FP32 pair/PME arithmetic, FP64 accumulation and CG",
             "atoms": sy.n, "pairs_within_cutoff": P, "list_candidates": C, "cg_iterations": n_it,
             "grid": list(sy.settings["grid"]),
-            "pair_passes": {"lanes": tuned["mixed"][0][0], "fused_setup": tuned["mixed"][0][1],
+            "pair_passes": {"lanes": tuned["mixed"][0][0], "setup_lanes": engines["mixed"].setup_lanes(),
+                            "fused_setup": tuned["mixed"][0][1],
                             "field_on_the_fly": engines["mixed"].field_otf(), "pme_beside": engines["mixed"].pme_overlap(),
                             "cg_product_ms_pme_after_beside": [round(v, 4) for v in engines["mixed"].pme_overlap_times()]},
         },
         "fp64_path": {"value": round(v64, 3), "ms_per_step": round(1000.0 * secs["double"] / a.steps, 4),
                       "cg_iterations": results["double"][2], "roofline": roof64,
-                      "pair_passes": {"lanes": tuned["double"][0][0], "fused_setup": tuned["double"][0][1],
+                      "pair_passes": {"lanes": tuned["double"][0][0], "setup_lanes": engines["double"].setup_lanes(),
+                                      "fused_setup": tuned["double"][0][1],
                                       "pme_beside": engines["double"].pme_overlap(),
                                       "cg_product_ms_pme_after_beside": [round(v, 4) for v in
                                                                          engines["double"].pme_overlap_times()]}},

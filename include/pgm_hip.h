@@ -233,6 +233,12 @@ int pgm_backward_batch_device(pgm_engine* e, const double* pos_dev, const double
  * passes directly (lanes 16, 32 or 64; 16 and separate passes by default). */
 int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms);
 int pgm_set_lanes(pgm_engine* e, int lanes, int fused);
+/* No counterpart in the reference: the lanes per atom of the pair setup pass.  pgm_autotune, once it has chosen
+ * the pair passes, times the setup pass alone at 16, 32 and 64 lanes and keeps the fastest for it (the setup
+ * tests every candidate, the other passes stream the compacted pairs); pgm_set_lanes sets it with the others,
+ * pgm_set_setup_lanes alone (16, 32 or 64, after pgm_set_lanes). */
+int pgm_setup_lanes(const pgm_engine* e);
+int pgm_set_setup_lanes(pgm_engine* e, int lanes);
 /* No counterpart in the reference: the FP32 (PGM_MIXED) field pass of the CG products with its pair kernels
  * evaluated on the fly (1) or read from the kernels the setup pass stored (0); pgm_autotune times both and keeps
  * the faster.  pgm_field_otf: the current choice. */

@@ -1095,7 +1095,7 @@ public:
         for (DevBuf* b : {&edD_, &erD_, &bD_, &tdD_})
             b->reserve(3*sizeof(double)*n);
         nbD_.reserve(n*sizeof(int));
-        pairSetup<T>(lanes_, fused_, nullptr);
+        pairSetup<T>(setupLanes(), fused_, nullptr);
         pmeSolve<T>(permTD_.as<T4>(), true, posT<T>(), 0, false, erD_.as<double>(), nullptr, nullptr, nullptr);
         const double cself = 4.0*s_.beta*s_.beta*s_.beta/(3.0*kSqrtPi);
         launchCombineB(st_, n, edD_.as<double>(), erD_.as<double>(), permD_.as<double>(), cself, bD_.as<double>());
@@ -1258,6 +1258,19 @@ public:
             lanes_ = best;
             fused_ = bestFused;
             fieldOtf_ = bestOtf;
+            // the setup pass on its own: its best lane count need not be the other passes' (it tests every
+            // candidate, they stream the compacted pairs)
+            double bestSetup = 1e300;
+            setupLanes_ = 0;
+            for (int c = 0; c < 3; c++) {
+                const int lanes = kPairLaneChoices[c];
+                const double ms = mixed ? timeSetup<float>(lanes, fused_, reps)
+                                        : timeSetup<double>(lanes, fused_, reps);
+                if (ms < bestSetup) {
+                    bestSetup = ms;
+                    setupLanes_ = lanes;
+                }
+            }
             // then the PME solves beside the pair passes or after them on the engine stream, timed over a CG
             // product (field pass + PME solve) with the chosen pair passes.  Beside, the two share the CUs and
             // the L2; at 999k atoms in FP64, where each fills the chip, that was the slower way.
@@ -1277,11 +1290,19 @@ public:
         fieldOtf_ = on;
     }
     int lanes() const { return lanes_; }
+    // the setup pass's lanes: the autotuner's own choice for it, else the other passes'
+    int setupLanes() const { return setupLanes_ ? setupLanes_ : lanes_; }
+    void setSetupLanes(int lanes) {
+        if (lanes != 16 && lanes != 32 && lanes != 64)
+            throw std::invalid_argument("pgm_hip: pair-pass lanes must be 16, 32 or 64");
+        setupLanes_ = lanes;
+    }
     bool fused() const { return fused_; }
     void setLanes(int lanes, bool fused) {
         if (lanes != 16 && lanes != 32 && lanes != 64)
             throw std::invalid_argument("pgm_hip: pair-pass lanes must be 16, 32 or 64");
         lanes_ = lanes;
+        setupLanes_ = 0;
         fused_ = fused;
     }
     const double* tuneTimes() const { return tuneMs_; }
@@ -1423,6 +1444,7 @@ private:
     double overlapMs_[2] = {0.0, 0.0};    // the autotuner's CG product: PME after, beside the field pass
     hipEvent_t evFork_ = nullptr, evJoin_ = nullptr;
     int n_ = 0, nGlobal_ = 0, covN_ = -1, covBuiltN_ = -1;   // covBuiltN_: the atom count of the covalent CSR
+    int setupLanes_ = 0;   // the setup pass's lanes when the autotuner chose them apart (0: lanes_)
     bool polarizable_ = false;
     double rgMax_ = 0.0;   // the largest radius parameter 2R^2 (orders the fused setup's segments)
     bool listReverse_ = false;   // the block order of the next pass over the compacted list
@@ -2622,7 +2644,7 @@ private:
         // passes, the autotuner's choice) + PME
         PmeStreamScope pmeScope(*this, overlapPme());
         fork();
-        pairSetup<T>(lanes_, fused_, dual ? pTD_.as<T4>() : nullptr);
+        pairSetup<T>(setupLanes(), fused_, dual ? pTD_.as<T4>() : nullptr);
         preReady_ = false;
         if (induce && preRadius_ > 0) {   // the preconditioner's short list, from the stored kernels
             srCountD_.reserve(n*sizeof(int));
@@ -2733,6 +2755,27 @@ private:
         HIPCHK(hipEventRecord(a, st_));
         for (int r = 0; r < reps; r++)
             matvec<T>(posT<T>(), dTD_.as<T4>(), nullptr);
+        HIPCHK(hipEventRecord(b, st_));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        evUsed_ -= 2;
+        timing_ = wasTiming;
+        return ms/reps;
+    }
+
+    // Milliseconds per evaluation spent in the setup pass (with its compaction) at the given lane count.
+    template<class T>
+    double timeSetup(int lanes, bool fused, int reps) {
+        typedef typename Vec4<T>::type T4;
+        hipEvent_t a = nextEvent(), b = nextEvent();
+        const bool wasTiming = timing_;
+        timing_ = false;
+        adjointReady_ = false;   // the timed passes overwrite the evaluation's pair-pass outputs
+        pairSetup<T>(lanes, fused, pTD_.as<T4>());   // warm-up
+        HIPCHK(hipEventRecord(a, st_));
+        for (int r = 0; r < reps; r++)
+            pairSetup<T>(lanes, fused, pTD_.as<T4>());
         HIPCHK(hipEventRecord(b, st_));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0;
@@ -3226,6 +3269,11 @@ int pgm_autotune(pgm_engine* e, int reps, int* lanes, int* fused, double* ms) {
 
 int pgm_set_lanes(pgm_engine* e, int lanes, int fused) {
     return guarded(e, [&](DeviceEngine& g) { g.setLanes(lanes, fused != 0); });
+}
+
+int pgm_setup_lanes(const pgm_engine* e) { return (e && e->eng) ? e->eng->setupLanes() : -1; }
+int pgm_set_setup_lanes(pgm_engine* e, int lanes) {
+    return guarded(e, [&](DeviceEngine& g) { g.setSetupLanes(lanes); });
 }
 
 int pgm_set_field_otf(pgm_engine* e, int on) {

@@ -68,7 +68,7 @@ EXPORTS = [
     "pgm_set_preconditioner", "pgm_set_candidates", "pgm_wait_device", "pgm_launch_pending", "pgm_set_field_otf",
     "pgm_field_otf", "pgm_field", "pgm_compute_device_f64_noalloc", "pgm_set_pme_overlap", "pgm_pme_overlap",
     "pgm_list_check_state",
-    "pgm_pme_overlap_times",
+    "pgm_pme_overlap_times", "pgm_setup_lanes", "pgm_set_setup_lanes",
 ]  # fmt: skip
 PHASES = ["list", "prep", "pair_compact", "pair_setup", "pair_field", "pme_spread", "pme_fft", "pme_gather", "cg",
           "pair_final", "final"]  # fmt: skip
@@ -123,6 +123,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.pgm_field.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.pgm_set_field_otf.argtypes = [vp, C.c_int]
     lib.pgm_field_otf.argtypes = [vp]
+    lib.pgm_setup_lanes.argtypes = [vp]
+    lib.pgm_set_setup_lanes.argtypes = [vp, C.c_int]
     lib.pgm_set_pme_overlap.argtypes = [vp, C.c_int]
     lib.pgm_pme_overlap.argtypes = [vp]
     lib.pgm_pme_overlap_times.argtypes = [vp, vp]
@@ -436,6 +438,13 @@ class Engine:
 
     def set_lanes(self, lanes: int, fused: bool = True):
         self._check(self.lib.pgm_set_lanes(self.h, int(lanes), int(bool(fused))))
+
+    def setup_lanes(self) -> int:
+        """Lanes per atom of the pair setup pass (pgm_setup_lanes)."""
+        return self.lib.pgm_setup_lanes(self.h)
+
+    def set_setup_lanes(self, lanes: int):
+        self._check(self.lib.pgm_set_setup_lanes(self.h, int(lanes)))
 
     def set_field_otf(self, on: bool = True):
         """FP32 field pass with its kernels evaluated on the fly (pgm_set_field_otf)."""

@@ -145,8 +145,20 @@ def test_autotune_keeps_results(precision):
                 assert abs(E.total_energy(t) / E.total_energy(t0) - 1) < tol_e, (lanes, fused, otf)
                 assert np.abs(f - f0).max() < tol_f * np.abs(f0).max(), (lanes, fused, otf)
     best, ms = eng.autotune(2)
-    print(f"autotune {precision} (ms per evaluation in the pair passes):", ms, "->", best)
+    print(f"autotune {precision} (ms per evaluation in the pair passes):", ms, "->", best, "setup", eng.setup_lanes())
     assert best[0] in (16, 32, 64) and all(v > 0 for v in ms.values())
     assert ms[best] == min(ms.values())
+    assert eng.setup_lanes() in (16, 32, 64)
     t, f, _ = eng.compute(sy.pos, sy.box, new_step=False)
     assert np.abs(f - f0).max() < tol_f * np.abs(f0).max()
+    # the setup pass at a lane count of its own, and set_lanes setting both again
+    other = 64 if best[0] != 64 else 16
+    eng.set_setup_lanes(other)
+    assert eng.setup_lanes() == other
+    t, f, _ = eng.compute(sy.pos, sy.box, new_step=False)
+    assert abs(E.total_energy(t) / E.total_energy(t0) - 1) < tol_e
+    assert np.abs(f - f0).max() < tol_f * np.abs(f0).max()
+    eng.set_lanes(best[0], best[1])
+    assert eng.setup_lanes() == best[0]
+    with pytest.raises(E.PgmError):
+        eng.set_setup_lanes(48)
