@@ -434,6 +434,16 @@ __global__ void __launch_bounds__(kPairBlock) kPairField(int n, const typename V
     }
 }
 
+// A buffer resource over n float4 records (the gfx9 raw-buffer data word): a gather then addresses a record with
+// one 32-bit offset, the same for the position and the dipole of a partner, instead of two 64-bit addresses.  An
+// offset past the records reads zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t float4Records(const float4* p, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short) 0, n*(int) sizeof(float4), 0x00020000);
+}
+__device__ __forceinline__ float4 loadRecord(__amdgpu_buffer_rsrc_t r, int offset) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, offset, 0, 0));
+}
+
 // The FP32 field pass with the kernels evaluated on the fly (no stored kernels read): per entry the partner's
 // index, position and dipole (w = its radius parameter rg_j; rg_i rides in the atom's own dipole w), G1 and G2
 // from r^2 and rg_i + rg_j (pairKernelsF, as the setup pass evaluates them).
@@ -453,6 +463,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
     const BoxT<float> box = selectBox<BATCH>(bs, s);
     const float4 pi = pos[s];
     const float rgi = dip[s].w;
+    const __amdgpu_buffer_rsrc_t posR = float4Records(pos, n), dipR = float4Records(dip, n);
     const int start = candStart[s], cnt = nb[s];
     // the U entries of a step summed in FP32 (the matrix-vector product is FP32), the steps in FP64
     double ex = 0, ey = 0, ez = 0;
@@ -475,9 +486,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
             v[u] = pj[start+k+u*L];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int j = entryAtom(v[u], bs.ext);
-            pjv[u] = pos[j];
-            d[u] = dip[j];
+            const int o = entryAtom(v[u], bs.ext)*(int) sizeof(float4);
+            pjv[u] = loadRecord(posR, o);
+            d[u] = loadRecord(dipR, o);
         }
         float fx = -0.0f, fy = -0.0f, fz = -0.0f;   // -0: adding it is exact for every x, so it folds away
 #pragma unroll
@@ -488,9 +499,9 @@ __global__ void __launch_bounds__(kPairBlock) kPairFieldOtf(int n, const float4*
         ez += (double) fz;
     }
     for (; k < cnt; k += L) {
-        const int v = pj[start+k], j = entryAtom(v, bs.ext);
+        const int v = pj[start+k], o = entryAtom(v, bs.ext)*(int) sizeof(float4);
         float fx = -0.0f, fy = -0.0f, fz = -0.0f;   // -0: adding it is exact for every x, so it folds away
-        term(pos[j], dip[j], v, fx, fy, fz);
+        term(loadRecord(posR, o), loadRecord(dipR, o), v, fx, fy, fz);
         ex += (double) fx;
         ey += (double) fy;
         ez += (double) fz;
