@@ -625,6 +625,20 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
         const T rgi = (kStoreG03<T> && !GVDW) ? T(0) : rg[s];
         const T4 gvi = GVDW ? gva.gv[s] : di;
         const int start = candStart[s], cnt = nb[s];
+        // FP32: the partner gathers through buffer resources (as in kPairFieldOtf); unused for FP64
+        const __amdgpu_buffer_rsrc_t posR = float4Records(reinterpret_cast<const float4*>(pos), n);
+        const __amdgpu_buffer_rsrc_t dipR = float4Records(reinterpret_cast<const float4*>(dip), n);
+        auto gather = [&](int jj, T4& p, T4& d) {
+            if constexpr (sizeof(T) == sizeof(float)) {
+                const int o = jj*(int) sizeof(float4);
+                p = loadRecord(posR, o);
+                d = loadRecord(dipR, o);
+            }
+            else {
+                p = pos[jj];
+                d = dip[jj];
+            }
+        };
         double f[3] = {0, 0, 0}, e[3] = {0, 0, 0};
         int k = lane;
         for (; k+(U-1)*L < cnt; k += U*L) {
@@ -642,8 +656,7 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                pjv[u] = pos[j[u]];
-                dj[u] = dip[j[u]];
+                gather(j[u], pjv[u], dj[u]);
                 rgj[u] = (kStoreG03<T> && !GVDW) ? T(0) : dj[u].w;
                 gvj[u] = GVDW ? gva.gv[j[u]] : dj[u];
             }
@@ -666,9 +679,10 @@ __global__ void __launch_bounds__(kPairBlock) kPairFinal(int n, const typename V
             T2 gb{};
             if (kStoreG03<T>)
                 gb = g03[start+k];
-            const T4 dj = dip[j];
+            T4 pjr, dj;
+            gather(j, pjr, dj);
             const T4 gvj = GVDW ? gva.gv[j] : dj;
-            finalTerm<T, STRAIN, GVDW>(pi, di, pos[j], dj, g12[start+k], gb, (kStoreG03<T> && !GVDW) ? T(0) : rgi+dj.w,
+            finalTerm<T, STRAIN, GVDW>(pi, di, pjr, dj, g12[start+k], gb, (kStoreG03<T> && !GVDW) ? T(0) : rgi+dj.w,
                                        beta, box, v, bs.ext, acc, f, e, acc[0], gvi, gvj, gva, eg);
         }
 #pragma unroll
