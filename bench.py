@@ -416,7 +416,7 @@ def snapshot_batch(S, nsnap=64):
 
 
 def batch_flops(b, P, n, m):
-    """Sum over the snapshots of F_adj (SURVEY's adjoint count), with the batch's total half pairs P split over
+    """Sum over the snapshots of F_adj (adjoint_flops), with the batch's total half pairs P split over
     the snapshots in proportion to their atoms (all snapshots have the same density)."""
     N = int(b["off"][-1])
     total = 0.0
